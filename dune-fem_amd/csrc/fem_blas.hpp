@@ -1,0 +1,71 @@
+// BLAS-1 kernels and deterministic reductions on device DoF vectors.
+#pragma once
+
+#include "fem_common.hpp"
+
+namespace femamd {
+
+// wavefront (64-lane) sum followed by an LDS tree over the block's waves;
+// result valid in thread 0
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T *lds /* >= blockDim/64 */)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) lds[wid] = v;
+    __syncthreads();
+    T s = 0;
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int i = 0; i < nw; ++i) s += lds[i];
+    }
+    return s;
+}
+
+// Deterministic finish of a fixed-grid reduction: every block stores its
+// partial, the last block to arrive sums the partials in block order and
+// writes result[slot].  The counter is reset by that block.
+// Returns true in thread 0 of the last block (result written).
+__device__ __forceinline__ bool finish_reduction(double partial, double *partials,
+                                                 unsigned *counter, double *result, int nslots,
+                                                 const double *own /* [nslots] in thread 0 */)
+{
+    __shared__ bool amLast;
+    if (threadIdx.x == 0) {
+        for (int s = 0; s < nslots; ++s) partials[(size_t)s * gridDim.x + blockIdx.x] = own[s];
+        __threadfence();
+        unsigned prev = atomicAdd(counter, 1u);
+        amLast = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    (void)partial;
+    if (!amLast) return false;
+    __threadfence();
+    __shared__ double red[RED_THREADS / 64];
+    for (int s = 0; s < nslots; ++s) {
+        double v = 0;
+        // fixed order: thread t sums blocks t, t+256, ...
+        for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x)
+            v += __builtin_nontemporal_load(&partials[(size_t)s * gridDim.x + b]);
+        double tot = block_sum(v, red);
+        if (threadIdx.x == 0) result[s] = tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *counter = 0;
+    return true;
+}
+
+// host-side launchers (fem_blas.hip)
+void vec_fill(Vec &v, double a);
+void vec_copy(Vec &dst, const Vec &src);
+void vec_axpy(Vec &y, double a, const Vec &x);
+void vec_scal(Vec &x, double a);
+// device result in ctx.red_result[slot]
+void vec_dot_async(const Vec &x, const Vec &y, int slot);
+double vec_dot(const Vec &x, const Vec &y);
+// raw-pointer variants used by the solvers
+void dot_async(Context &c, int64_t n, const double *x, const double *y, double *result, unsigned *counter, double *partials);
+void dot_async(Context &c, int64_t n, const float *x, const float *y, double *result, unsigned *counter, double *partials);
+
+}  // namespace femamd

@@ -1,0 +1,448 @@
+// extern "C" boundary: opaque handles, int status codes, no exceptions
+// across the ABI (the reference throws Dune exceptions; non-convergence is a
+// negative iteration count, not an error).
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+#include "fem_blas.hpp"
+#include "fem_op.hpp"
+#include "fem_solver.hpp"
+
+using namespace femamd;
+
+static thread_local std::string g_last_error;
+
+template <typename F>
+static int guarded(F &&f)
+{
+    try {
+        f();
+        return FEM_OK;
+    } catch (const Error &e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return FEM_ERR_ARG;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return FEM_ERR_ARG;
+    }
+}
+
+extern "C" {
+
+const char *fem_last_error(void) { return g_last_error.c_str(); }
+int fem_version(void) { return 10000; }
+
+int fem_ctx_create(int device, fem_ctx **out)
+{
+    return guarded([&] {
+        FEM_CHECK(out, FEM_ERR_ARG, "null output handle");
+        int n = 0;
+        FEM_HIP(hipGetDeviceCount(&n));
+        FEM_CHECK(device >= 0 && device < n, FEM_ERR_ARG, "no such HIP device");
+        FEM_HIP(hipSetDevice(device));
+        fem_ctx *h = new fem_ctx();
+        Context &c = h->c;
+        c.device = device;
+        try {
+            FEM_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+            FEM_HIP(hipMalloc(&c.red_partial, sizeof(double) * RED_BLOCKS * RED_MAX_SLOTS));
+            FEM_HIP(hipMalloc(&c.red_counter, sizeof(unsigned) * RED_MAX_SLOTS));
+            FEM_HIP(hipMemset(c.red_counter, 0, sizeof(unsigned) * RED_MAX_SLOTS));
+            FEM_HIP(hipMalloc(&c.red_result, sizeof(double) * RED_MAX_SLOTS));
+            FEM_HIP(hipHostMalloc(&c.host_result, sizeof(double) * RED_MAX_SLOTS, hipHostMallocDefault));
+            FEM_HIP(hipEventCreate(&c.ev0));
+            FEM_HIP(hipEventCreate(&c.ev1));
+            FEM_HIP(hipDeviceSynchronize());
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int fem_ctx_destroy(fem_ctx *h)
+{
+    return guarded([&] {
+        if (!h) return;
+        Context &c = h->c;
+        (void)hipSetDevice(c.device);
+        if (c.stream) (void)hipStreamSynchronize(c.stream);
+        if (c.red_partial) (void)hipFree(c.red_partial);
+        if (c.red_counter) (void)hipFree(c.red_counter);
+        if (c.red_result) (void)hipFree(c.red_result);
+        if (c.host_result) (void)hipHostFree(c.host_result);
+        if (c.ev0) (void)hipEventDestroy(c.ev0);
+        if (c.ev1) (void)hipEventDestroy(c.ev1);
+        if (c.stream) (void)hipStreamDestroy(c.stream);
+        delete h;
+    });
+}
+
+int fem_ctx_synchronize(fem_ctx *h)
+{
+    return guarded([&] {
+        FEM_CHECK(h, FEM_ERR_ARG, "null context");
+        FEM_HIP(hipStreamSynchronize(h->c.stream));
+    });
+}
+
+int fem_ctx_mark(fem_ctx *h, int which)
+{
+    return guarded([&] {
+        FEM_CHECK(h, FEM_ERR_ARG, "null context");
+        FEM_HIP(hipEventRecord(which ? h->c.ev1 : h->c.ev0, h->c.stream));
+    });
+}
+
+int fem_ctx_elapsed_ms(fem_ctx *h, float *ms)
+{
+    return guarded([&] {
+        FEM_CHECK(h && ms, FEM_ERR_ARG, "null argument");
+        FEM_HIP(hipEventSynchronize(h->c.ev1));
+        FEM_HIP(hipEventElapsedTime(ms, h->c.ev0, h->c.ev1));
+    });
+}
+
+// ---- vectors -----------------------------------------------------------------
+int fem_vec_create(fem_ctx *ctx, int64_t n, int precision, fem_vec **out)
+{
+    return guarded([&] {
+        FEM_CHECK(ctx && out, FEM_ERR_ARG, "null argument");
+        FEM_CHECK(n >= 0, FEM_ERR_ARG, "negative size");
+        FEM_CHECK(precision == FEM_F64 || precision == FEM_F32, FEM_ERR_ARG, "vector precision must be FEM_F64 or FEM_F32");
+        fem_vec *h = new fem_vec();
+        h->v.ctx = &ctx->c;
+        h->v.n = n;
+        h->v.prec = precision;
+        if (n > 0) {
+            hipError_t e = hipMalloc(&h->v.data, h->v.bytes());
+            if (e != hipSuccess) {
+                delete h;
+                throw Error(FEM_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+            }
+            FEM_HIP(hipMemsetAsync(h->v.data, 0, h->v.bytes(), ctx->c.stream));
+        }
+        *out = h;
+    });
+}
+
+int fem_vec_destroy(fem_vec *v)
+{
+    return guarded([&] {
+        if (!v) return;
+        if (v->v.data && v->v.owner) {
+            (void)hipStreamSynchronize(v->v.ctx->stream);
+            (void)hipFree(v->v.data);
+        }
+        delete v;
+    });
+}
+
+int64_t fem_vec_size(const fem_vec *v) { return v ? v->v.n : -1; }
+int fem_vec_precision(const fem_vec *v) { return v ? v->v.prec : -1; }
+void *fem_vec_device_ptr(fem_vec *v) { return v ? v->v.data : nullptr; }
+
+int fem_vec_upload(fem_vec *v, const double *host)
+{
+    return guarded([&] {
+        FEM_CHECK(v && host, FEM_ERR_ARG, "null argument");
+        Vec &x = v->v;
+        if (x.n == 0) return;
+        if (x.prec == FEM_F64) {
+            FEM_HIP(hipMemcpyAsync(x.data, host, x.bytes(), hipMemcpyHostToDevice, x.ctx->stream));
+        } else {
+            std::vector<float> tmp(x.n);
+            for (int64_t i = 0; i < x.n; ++i) tmp[i] = (float)host[i];
+            FEM_HIP(hipMemcpyAsync(x.data, tmp.data(), x.bytes(), hipMemcpyHostToDevice, x.ctx->stream));
+            FEM_HIP(hipStreamSynchronize(x.ctx->stream));
+            return;
+        }
+        FEM_HIP(hipStreamSynchronize(x.ctx->stream));
+    });
+}
+
+int fem_vec_upload_f32(fem_vec *v, const float *host)
+{
+    return guarded([&] {
+        FEM_CHECK(v && host, FEM_ERR_ARG, "null argument");
+        Vec &x = v->v;
+        FEM_CHECK(x.prec == FEM_F32, FEM_ERR_ARG, "upload_f32 needs an FP32 vector");
+        if (x.n == 0) return;
+        FEM_HIP(hipMemcpyAsync(x.data, host, x.bytes(), hipMemcpyHostToDevice, x.ctx->stream));
+        FEM_HIP(hipStreamSynchronize(x.ctx->stream));
+    });
+}
+
+int fem_vec_download(const fem_vec *v, double *host)
+{
+    return guarded([&] {
+        FEM_CHECK(v && host, FEM_ERR_ARG, "null argument");
+        const Vec &x = v->v;
+        if (x.n == 0) return;
+        if (x.prec == FEM_F64) {
+            FEM_HIP(hipMemcpyAsync(host, x.data, x.bytes(), hipMemcpyDeviceToHost, x.ctx->stream));
+            FEM_HIP(hipStreamSynchronize(x.ctx->stream));
+        } else {
+            std::vector<float> tmp(x.n);
+            FEM_HIP(hipMemcpyAsync(tmp.data(), x.data, x.bytes(), hipMemcpyDeviceToHost, x.ctx->stream));
+            FEM_HIP(hipStreamSynchronize(x.ctx->stream));
+            for (int64_t i = 0; i < x.n; ++i) host[i] = tmp[i];
+        }
+    });
+}
+
+int fem_vec_fill(fem_vec *v, double a)
+{
+    return guarded([&] {
+        FEM_CHECK(v, FEM_ERR_ARG, "null vector");
+        if (v->v.n) vec_fill(v->v, a);
+    });
+}
+
+int fem_vec_copy(fem_vec *d, const fem_vec *s)
+{
+    return guarded([&] {
+        FEM_CHECK(d && s, FEM_ERR_ARG, "null vector");
+        if (d->v.n) vec_copy(d->v, s->v);
+    });
+}
+
+int fem_vec_axpy(fem_vec *y, double a, const fem_vec *x)
+{
+    return guarded([&] {
+        FEM_CHECK(y && x, FEM_ERR_ARG, "null vector");
+        if (y->v.n) vec_axpy(y->v, a, x->v);
+    });
+}
+
+int fem_vec_scal(fem_vec *x, double a)
+{
+    return guarded([&] {
+        FEM_CHECK(x, FEM_ERR_ARG, "null vector");
+        if (x->v.n) vec_scal(x->v, a);
+    });
+}
+
+int fem_vec_dot(const fem_vec *x, const fem_vec *y, double *out)
+{
+    return guarded([&] {
+        FEM_CHECK(x && y && out, FEM_ERR_ARG, "null argument");
+        *out = vec_dot(x->v, y->v);
+    });
+}
+
+int fem_vec_norm2sq(const fem_vec *x, double *out)
+{
+    return guarded([&] {
+        FEM_CHECK(x && out, FEM_ERR_ARG, "null argument");
+        *out = vec_dot(x->v, x->v);
+    });
+}
+
+// ---- operators -----------------------------------------------------------------
+int fem_op_create(fem_ctx *ctx, const fem_op_desc *desc, fem_op **out)
+{
+    return guarded([&] {
+        FEM_CHECK(ctx && desc && out, FEM_ERR_ARG, "null argument");
+        fem_op *h = new fem_op();
+        try {
+            h->o.init(&ctx->c, *desc);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int fem_op_destroy(fem_op *op)
+{
+    return guarded([&] {
+        if (op) {
+            (void)hipStreamSynchronize(op->o.ctx->stream);
+            delete op;
+        }
+    });
+}
+
+int64_t fem_op_ndofs(const fem_op *op) { return op ? op->o.N : -1; }
+
+int fem_op_set_kappa_qp(fem_op *op, const double *kvol, const double *kx, const double *ky, const double *kz)
+{
+    return guarded([&] {
+        FEM_CHECK(op, FEM_ERR_ARG, "null operator");
+        op->o.set_kappa_qp(kvol, kx, ky, kz);
+    });
+}
+
+int fem_op_set_kappa_cell(fem_op *op, const double *kc)
+{
+    return guarded([&] {
+        FEM_CHECK(op, FEM_ERR_ARG, "null operator");
+        op->o.set_kappa_cell(kc);
+    });
+}
+
+int fem_op_apply(fem_op *op, const fem_vec *u, fem_vec *w)
+{
+    return guarded([&] {
+        FEM_CHECK(op && u && w, FEM_ERR_ARG, "null argument");
+        op->o.apply(u->v, w->v);
+    });
+}
+
+int fem_op_assemble_rhs(fem_op *op, const double *fq, fem_vec *F)
+{
+    return guarded([&] {
+        FEM_CHECK(op && fq && F, FEM_ERR_ARG, "null argument");
+        op->o.rhs(fq, F->v);
+    });
+}
+
+int fem_op_diagonal(fem_op *op, fem_vec *d)
+{
+    return guarded([&] {
+        FEM_CHECK(op && d, FEM_ERR_ARG, "null argument");
+        op->o.diagonal(d->v);
+    });
+}
+
+// ---- preconditioners -----------------------------------------------------------
+int fem_precond_create(fem_op *op, int kind, fem_precond **out)
+{
+    return guarded([&] {
+        FEM_CHECK(op && out, FEM_ERR_ARG, "null argument");
+        fem_precond *h = new fem_precond();
+        try {
+            h->p.init(&op->o, kind);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int fem_precond_destroy(fem_precond *p)
+{
+    return guarded([&] { delete p; });
+}
+
+int fem_precond_apply(fem_precond *p, const fem_vec *x, fem_vec *y)
+{
+    return guarded([&] {
+        FEM_CHECK(p && x && y, FEM_ERR_ARG, "null argument");
+        FEM_CHECK(x->v.n == p->p.op->N && y->v.n == p->p.op->N, FEM_ERR_ARG, "precond: vector size");
+        FEM_CHECK(x->v.prec == p->p.prec && y->v.prec == p->p.prec, FEM_ERR_ARG, "precond: vector precision");
+        p->p.apply_raw(x->v.data, y->v.data, nullptr);
+    });
+}
+
+// ---- solvers -------------------------------------------------------------------
+void fem_solver_default_params(fem_solver_params *p)
+{
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->method = FEM_SOLVER_GMRES;     // KrylovInverseOperator default (krylovinverseoperators.hh:103-107)
+    p->tolerance = 1e-8;              // parameter.hh:118-141
+    p->errormeasure = FEM_ERRMEASURE_ABSOLUTE;
+    p->maxiterations = 1 << 30;
+    p->restart = 20;                  // parameter.hh:197-201
+    p->verbose = 0;
+    p->refinement = 0;
+    p->inner_tolerance = 1e-3;
+    p->inner_maxiterations = 1000;
+}
+
+int fem_solver_params_set(fem_solver_params *p, const char *key, const char *value)
+{
+    return guarded([&] {
+        FEM_CHECK(p && key && value, FEM_ERR_ARG, "null argument");
+        std::string k(key), v(value);
+        for (const char *pre : { "fem.solver.", "linear.", "fem.", "" }) {
+            if (k.rfind(pre, 0) == 0 && std::strlen(pre)) { k = k.substr(std::strlen(pre)); break; }
+        }
+        if (k == "method") {
+            if (v == "cg") p->method = FEM_SOLVER_CG;
+            else if (v == "gmres") p->method = FEM_SOLVER_GMRES;
+            else if (v == "fgmres") p->method = FEM_SOLVER_FGMRES;
+            else throw Error(FEM_ERR_UNSUPPORTED, "method: " + v);
+        } else if (k == "tolerance") {
+            p->tolerance = std::stod(v);
+        } else if (k == "errormeasure") {
+            if (v == "absolute") p->errormeasure = FEM_ERRMEASURE_ABSOLUTE;
+            else if (v == "relative") p->errormeasure = FEM_ERRMEASURE_RELATIVE;
+            else if (v == "residualreduction") p->errormeasure = FEM_ERRMEASURE_REDUCTION;
+            else throw Error(FEM_ERR_ARG, "errormeasure: " + v);
+        } else if (k == "maxiterations") {
+            p->maxiterations = std::stoi(v);
+        } else if (k == "gmres.restart") {
+            p->restart = std::stoi(v);
+        } else if (k == "verbose") {
+            p->verbose = (v == "true" || v == "1") ? 1 : 0;
+        } else {
+            throw Error(FEM_ERR_ARG, std::string("unknown solver parameter: ") + key);
+        }
+    });
+}
+
+int fem_solver_create(fem_ctx *ctx, const fem_solver_params *params, fem_solver **out)
+{
+    return guarded([&] {
+        FEM_CHECK(ctx && params && out, FEM_ERR_ARG, "null argument");
+        fem_solver *h = new fem_solver();
+        try {
+            h->s.init(&ctx->c, *params);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int fem_solver_destroy(fem_solver *s)
+{
+    return guarded([&] { delete s; });
+}
+
+int fem_solver_bind(fem_solver *s, fem_op *op, fem_precond *pre, fem_op *iop, fem_precond *ipre)
+{
+    return guarded([&] {
+        FEM_CHECK(s && op, FEM_ERR_ARG, "null argument");
+        s->s.bind(&op->o, pre ? &pre->p : nullptr, iop ? &iop->o : nullptr, ipre ? &ipre->p : nullptr);
+    });
+}
+
+int fem_solver_unbind(fem_solver *s)
+{
+    return guarded([&] {
+        FEM_CHECK(s, FEM_ERR_ARG, "null solver");
+        s->s.unbind();
+    });
+}
+
+int fem_solver_solve(fem_solver *s, const fem_vec *b, fem_vec *x, fem_solver_info *info)
+{
+    return guarded([&] {
+        FEM_CHECK(s && b && x, FEM_ERR_ARG, "null argument");
+        fem_solver_info tmp;
+        s->s.solve(b->v, x->v, info ? *info : tmp);
+    });
+}
+
+int fem_gauss_points(int m, double *x, double *w)
+{
+    return guarded([&] {
+        FEM_CHECK(m >= 1 && m <= 32 && x && w, FEM_ERR_ARG, "gauss: 1 <= m <= 32");
+        gauss_legendre01(m, x, w);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,193 @@
+// Continuous Lagrange Q2 operator on structured quad meshes: matrix-free
+// apply, diagonal and load vector, for one instance or a batch of
+// independent instances in one launch.
+//
+// Reference semantics: GalerkinOperator::evaluate element loop with
+// addLocalDofs scatter (dune/fem/schemes/galerkin.hh:811-917, 929-934) and
+// DirichletWrapperOperator::operator() (dirichletwrapper.hh:101-105: w_D = u_D).
+//
+// Design: a workgroup owns a CG_TX x CG_TY tile of elements and the nodes at
+// their lower-left 2x2 node block; one thread per element of the tile plus a
+// lower halo row/column, so every contribution to an owned node is computed
+// inside the workgroup.  Element results are added into an LDS node lattice
+// in four colour phases (elements of one colour share no node), which makes
+// the summation order fixed: results are deterministic and need no atomics.
+#include "fem_cg.hpp"
+
+namespace femamd {
+
+CGTab make_cg_tab()
+{
+    CGTab t;
+    double x[3], w[3];
+    gauss_legendre01(3, x, w);
+    for (int q = 0; q < 3; ++q) {
+        t.w[q] = w[q];
+        for (int i = 0; i < 3; ++i) {
+            t.L[q][i] = lagrange2(i, x[q]);
+            t.Ld[q][i] = lagrange2_d(i, x[q]);
+        }
+    }
+    return t;
+}
+
+constexpr int CG_LX = 2 * CG_TX + 3, CG_LY = 2 * CG_TY + 3;
+
+template <typename T>
+__global__ void __launch_bounds__(CG_THREADS)
+    k_cg(int mode, const CGInst *__restrict__ insts, int ninst, const CGTab *__restrict__ tabp,
+         const T *__restrict__ kvol, const T *__restrict__ in, T *__restrict__ out, const int *__restrict__ skip)
+{
+    if (skip && *skip) return;
+    __shared__ T lat[CG_LX * CG_LY];
+    __shared__ int s_inst;
+    const unsigned bid = blockIdx.x;
+    if (threadIdx.x == 0) {
+        int lo = 0, hi = ninst - 1;
+        while (lo < hi) {  // last instance with tile_off <= bid
+            int mid = (lo + hi + 1) >> 1;
+            if ((unsigned)insts[mid].tile_off <= bid) lo = mid; else hi = mid - 1;
+        }
+        s_inst = lo;
+    }
+    for (int k = threadIdx.x; k < CG_LX * CG_LY; k += blockDim.x) lat[k] = 0;
+    __syncthreads();
+    const CGInst P = insts[s_inst];
+    const CGTab &tb = *tabp;
+    const int tile = (int)bid - P.tile_off;
+    const int ex0 = (tile % P.tiles_x) * CG_TX, ey0 = (tile / P.tiles_x) * CG_TY;
+    const int tx = threadIdx.x % (CG_TX + 1), ty = threadIdx.x / (CG_TX + 1);
+    const int ex = ex0 - 1 + tx, ey = ey0 - 1 + ty;
+    const bool valid = ex >= 0 && ey >= 0 && ex < P.nx && ey < P.ny;
+    const int NX = 2 * P.nx + 1;
+    const T *u = in + P.dof_off;
+    T wl[3][3];
+    if (valid) {
+        const double hx = 1.0 / P.nx, hy = 1.0 / P.ny, detJ = hx * hy;
+        const int64_t e = P.elem_off + (int64_t)ey * P.nx + ex;
+        if (mode == CG_MODE_APPLY) {
+            T ul[3][3];
+#pragma unroll
+            for (int iy = 0; iy < 3; ++iy)
+#pragma unroll
+                for (int ix = 0; ix < 3; ++ix) ul[iy][ix] = u[(int64_t)(2 * ey + iy) * NX + 2 * ex + ix];
+            // sum factorisation: x then y
+            T t0[3][3], t1[3][3];
+#pragma unroll
+            for (int iy = 0; iy < 3; ++iy)
+#pragma unroll
+                for (int qx = 0; qx < 3; ++qx) {
+                    T a = 0, b = 0;
+#pragma unroll
+                    for (int ix = 0; ix < 3; ++ix) {
+                        a += (T)tb.L[qx][ix] * ul[iy][ix];
+                        b += (T)tb.Ld[qx][ix] * ul[iy][ix];
+                    }
+                    t0[iy][qx] = a;
+                    t1[iy][qx] = b;
+                }
+            T fx[3][3], fy[3][3], fm[3][3];
+#pragma unroll
+            for (int qy = 0; qy < 3; ++qy)
+#pragma unroll
+                for (int qx = 0; qx < 3; ++qx) {
+                    T uq = 0, gx = 0, gy = 0;
+#pragma unroll
+                    for (int iy = 0; iy < 3; ++iy) {
+                        uq += (T)tb.L[qy][iy] * t0[iy][qx];
+                        gx += (T)tb.L[qy][iy] * t1[iy][qx];
+                        gy += (T)tb.Ld[qy][iy] * t0[iy][qx];
+                    }
+                    const T W = (T)(tb.w[qx] * tb.w[qy] * detJ);
+                    const T kap = kvol ? kvol[e * 9 + qx + 3 * qy] : (T)P.kappa_const;
+                    fx[qy][qx] = W * kap * gx * (T)(1.0 / (hx * hx));
+                    fy[qy][qx] = W * kap * gy * (T)(1.0 / (hy * hy));
+                    fm[qy][qx] = W * (T)P.c * uq;
+                }
+            // transpose: y then x
+#pragma unroll
+            for (int qx = 0; qx < 3; ++qx)
+#pragma unroll
+                for (int iy = 0; iy < 3; ++iy) {
+                    T a = 0, b = 0;
+#pragma unroll
+                    for (int qy = 0; qy < 3; ++qy) {
+                        a += (T)tb.L[qy][iy] * (fx[qy][qx]);
+                        b += (T)tb.Ld[qy][iy] * fy[qy][qx] + (T)tb.L[qy][iy] * fm[qy][qx];
+                    }
+                    t0[iy][qx] = a;   // to be contracted with Ld in x
+                    t1[iy][qx] = b;   // with L in x
+                }
+#pragma unroll
+            for (int iy = 0; iy < 3; ++iy)
+#pragma unroll
+                for (int ix = 0; ix < 3; ++ix) {
+                    T a = 0;
+#pragma unroll
+                    for (int qx = 0; qx < 3; ++qx) a += (T)tb.Ld[qx][ix] * t0[iy][qx] + (T)tb.L[qx][ix] * t1[iy][qx];
+                    wl[iy][ix] = a;
+                }
+        } else {
+#pragma unroll
+            for (int iy = 0; iy < 3; ++iy)
+#pragma unroll
+                for (int ix = 0; ix < 3; ++ix) {
+                    double a = 0;
+                    for (int qy = 0; qy < 3; ++qy)
+                        for (int qx = 0; qx < 3; ++qx) {
+                            const double W = tb.w[qx] * tb.w[qy] * detJ;
+                            const double k = kvol ? (double)kvol[e * 9 + qx + 3 * qy] : P.kappa_const;
+                            const double lx = tb.L[qx][ix], ly = tb.L[qy][iy];
+                            if (mode == CG_MODE_DIAG) {
+                                const double gx = tb.Ld[qx][ix] * ly / hx, gy = lx * tb.Ld[qy][iy] / hy;
+                                a += W * (k * (gx * gx + gy * gy) + P.c * lx * lx * ly * ly);
+                            } else {
+                                a += W * k * lx * ly;  // kvol carries f for the load vector
+                            }
+                        }
+                    wl[iy][ix] = (T)a;
+                }
+        }
+    }
+    // coloured accumulation into the LDS node lattice
+#pragma unroll
+    for (int col = 0; col < 4; ++col) {
+        if (valid && (tx & 1) == (col & 1) && (ty & 1) == (col >> 1)) {
+#pragma unroll
+            for (int iy = 0; iy < 3; ++iy)
+#pragma unroll
+                for (int ix = 0; ix < 3; ++ix) lat[(2 * ty + iy) * CG_LX + 2 * tx + ix] += wl[iy][ix];
+        }
+        __syncthreads();
+    }
+    // owned nodes
+    const int I0 = 2 * ex0, J0 = 2 * ey0;
+    const int I1 = (ex0 + CG_TX >= P.nx) ? 2 * P.nx + 1 : 2 * ex0 + 2 * CG_TX;
+    const int J1 = (ey0 + CG_TY >= P.ny) ? 2 * P.ny + 1 : 2 * ey0 + 2 * CG_TY;
+    const int wx = I1 - I0, wy = J1 - J0;
+    T *o = out + P.dof_off;
+    for (int k = threadIdx.x; k < wx * wy; k += blockDim.x) {
+        const int I = I0 + k % wx, J = J0 + k / wx;
+        const int64_t node = (int64_t)J * NX + I;
+        const bool bnd = I == 0 || J == 0 || I == 2 * P.nx || J == 2 * P.ny;
+        T val = lat[(J - J0 + 2) * CG_LX + (I - I0 + 2)];
+        if (bnd) val = mode == CG_MODE_APPLY ? u[node] : mode == CG_MODE_DIAG ? T(1) : T(0);
+        o[node] = val;
+    }
+}
+
+template <typename T>
+void cg_launch(int mode, const CGInst *d_inst, int ninst, int total_tiles, const CGTab *d_tab, const T *kvol,
+               const T *in, T *out, const int *skip, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_cg<T>, dim3(total_tiles), dim3(CG_THREADS), 0, s, mode, d_inst, ninst, d_tab, kvol, in,
+                       out, skip);
+    FEM_HIP(hipGetLastError());
+}
+
+template void cg_launch<double>(int, const CGInst *, int, int, const CGTab *, const double *, const double *,
+                                double *, const int *, hipStream_t);
+template void cg_launch<float>(int, const CGInst *, int, int, const CGTab *, const float *, const float *,
+                               float *, const int *, hipStream_t);
+
+}  // namespace femamd

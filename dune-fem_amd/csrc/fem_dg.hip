@@ -1,0 +1,700 @@
+// SIPG DG operator on a structured hexahedral mesh of [0,1]^3: matrix-free
+// apply by sum factorisation.
+//
+// Reference semantics: GalerkinOperator::evaluate / evaluateImpl
+// (dune/fem/schemes/galerkin.hh:811-917, 1460-1496) with the SIPG + upwind
+// integrands of dune/fempy/test/parallelTest.py:113-129 and
+// pydemo/advectiondiffusion.py:30-56, quadrature orders 2p/2p+1
+// (galerkin.hh:131-132) on (p+1)-point Gauss rules.
+//
+// Design (MI355X): one workgroup handles EPB elements with K*K threads per
+// element (K = p+1).  A thread holds one K-long "pencil" of an element tensor
+// in registers; contractions along the pencil are register-local, and the
+// pencil direction is changed by a write/read through LDS.  The reference's
+// dense nq x nb tables (default.hh:276-487) become K x K contractions per
+// direction.  Faces are integrated by both adjacent elements ("owner
+// computes"): each element evaluates the neighbour trace itself, so every
+// element writes only its own DoFs and no atomics or locks are needed
+// (the reference visits each face once and locks the scatter,
+// galerkin.hh:879,951-991).  Face traces of the own element come from the
+// quadrature-point values by 1D extrapolation; neighbour traces are a
+// normal-direction contraction of the neighbour's modal DoFs followed by two
+// tangential contractions.
+#include <hip/hip_bf16.h>
+
+#include "fem_dg.hpp"
+
+namespace femamd {
+
+template <int K, typename T>
+__device__ __forceinline__ void mv(const T (&M)[KMAX][KMAX], const T *in, T *out)
+{
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        T s = 0;
+#pragma unroll
+        for (int a = 0; a < K; ++a) s = fma(M[q][a], in[a], s);
+        out[q] = s;
+    }
+}
+
+template <int K, typename T>
+__device__ __forceinline__ void mvT(const T (&M)[KMAX][KMAX], const T *in, T *out)
+{
+#pragma unroll
+    for (int a = 0; a < K; ++a) {
+        T s = 0;
+#pragma unroll
+        for (int q = 0; q < K; ++q) s = fma(M[q][a], in[q], s);
+        out[a] = s;
+    }
+}
+
+template <int K, typename T>
+__device__ __forceinline__ T dotk(const T *v, const T *in)
+{
+    T s = 0;
+#pragma unroll
+    for (int q = 0; q < K; ++q) s = fma(v[q], in[q], s);
+    return s;
+}
+
+template <int K>
+struct DGShape {
+    static constexpr int K2 = K * K, K3 = K * K * K;
+    // elements per workgroup: fill one or two wavefronts
+    static constexpr int EPB = (K == 2) ? 16 : (K == 3) ? 7 : (K == 4) ? 4 : (K == 5) ? 5 : (K == 6) ? 3 : 2;
+    static constexpr int THREADS = EPB * K2;
+    // LDS per element: A (K^3), R (K^3), F (6 faces x 2 x K^2), NS and T2
+    // (2 faces x 2 x K^2 each)
+    static constexpr int SZ = 2 * K3 + 12 * K2 + 8 * K2;
+};
+
+template <typename T>
+__device__ __forceinline__ T kappa_vol(const DGArgs<T> &a, int64_t e, int q, int K3)
+{
+    if (a.kappa_mode == FEM_KAPPA_QP) return a.kvol[e * K3 + q];
+    if (a.kappa_mode == FEM_KAPPA_CELL) return a.kcell[e];
+    return a.kappa_const;
+}
+
+template <int K, typename T>
+__global__ void __launch_bounds__(DGShape<K>::THREADS)
+    k_dg_apply(const DGArgs<T> a, const T *__restrict__ u, T *__restrict__ w, const int *__restrict__ skip)
+{
+    using S = DGShape<K>;
+    constexpr int K2 = S::K2, K3 = S::K3, EPB = S::EPB;
+    if (skip && *skip) return;
+    __shared__ T smem[EPB * S::SZ];
+    const Tab1D<T> &tb = *a.tab;
+
+    const int tid = threadIdx.x;
+    const int el = tid / K2, t = tid % K2, i = t / K, j = t % K;
+    const int64_t E = (int64_t)a.n[0] * a.n[1] * a.n[2];
+    const unsigned blk = xcd_remap(blockIdx.x, gridDim.x);
+    const int64_t e_raw = (int64_t)blk * EPB + el;
+    const bool active = e_raw < E;
+    const int64_t e = active ? e_raw : 0;
+    const int ex = (int)(e % a.n[0]), ey = (int)((e / a.n[0]) % a.n[1]), ez = (int)(e / ((int64_t)a.n[0] * a.n[1]));
+    const int ec[3] = { ex, ey, ez };
+    const int64_t stride[3] = { 1, (int64_t)a.n[0], (int64_t)a.n[0] * a.n[1] };
+
+    T *A = smem + el * S::SZ;
+    T *R = A + K3;
+    T *F = R + K3;        // [6][2][K2]
+    T *NS = F + 12 * K2;  // [2][2][K2]
+    T *T2 = NS + 4 * K2;  // [2][2][K2]
+
+    const T ih[3] = { T(1) / a.h[0], T(1) / a.h[1], T(1) / a.h[2] };
+    const T detJ = a.h[0] * a.h[1] * a.h[2];
+    T v[K], tmp[K];
+
+    // ---- forward: modal -> quadrature points ------------------------------
+    const T *ue = u + e * K3;
+    // Z layout: thread (i=a, j=b) holds u[a][b][0..K)
+#pragma unroll
+    for (int c = 0; c < K; ++c) v[c] = ue[i * K2 + j * K + c];
+    mv<K>(tb.B, v, tmp);
+#pragma unroll
+    for (int q = 0; q < K; ++q) A[i * K2 + j * K + q] = tmp[q];
+    __syncthreads();
+    // Y layout: thread (i=a, j=qz)
+#pragma unroll
+    for (int b = 0; b < K; ++b) v[b] = A[i * K2 + b * K + j];
+    mv<K>(tb.B, v, tmp);
+#pragma unroll
+    for (int q = 0; q < K; ++q) A[i * K2 + q * K + j] = tmp[q];
+    __syncthreads();
+    // X layout: thread (i=qy, j=qz)
+    T uq[K];
+#pragma unroll
+    for (int b = 0; b < K; ++b) v[b] = A[b * K2 + i * K + j];
+    mv<K>(tb.B, v, uq);
+#pragma unroll
+    for (int q = 0; q < K; ++q) A[q * K2 + i * K + j] = uq[q];
+
+    // x-direction volume flux and own x-face traces
+    T Rx[K];
+    {
+        T g[K], r[K];
+        mv<K>(tb.Dt, uq, g);
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            const T W = tb.w[q] * tb.w[i] * tb.w[j] * detJ;
+            const T kap = kappa_vol(a, e, q + K * i + K2 * j, K3);
+            r[q] = W * (kap * g[q] * ih[0] * ih[0] - a.b[0] * uq[q] * ih[0]);
+        }
+        mvT<K>(tb.Dt, r, Rx);
+        if (a.c != T(0)) {
+#pragma unroll
+            for (int q = 0; q < K; ++q) Rx[q] += tb.w[q] * tb.w[i] * tb.w[j] * detJ * a.c * uq[q];
+        }
+        const int idx = i + K * j;
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            F[((0 + s) * 2 + 0) * K2 + idx] = dotk<K>(tb.E[s], uq);
+            F[((0 + s) * 2 + 1) * K2 + idx] = dotk<K>(tb.dE[s], uq);
+        }
+    }
+    __syncthreads();
+    // Y layout of u_q: thread (i=qx, j=qz)
+    {
+        T g[K], r[K];
+#pragma unroll
+        for (int q = 0; q < K; ++q) v[q] = A[i * K2 + q * K + j];
+        mv<K>(tb.Dt, v, g);
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            const T W = tb.w[i] * tb.w[q] * tb.w[j] * detJ;
+            const T kap = kappa_vol(a, e, i + K * q + K2 * j, K3);
+            r[q] = W * (kap * g[q] * ih[1] * ih[1] - a.b[1] * v[q] * ih[1]);
+        }
+        mvT<K>(tb.Dt, r, tmp);
+#pragma unroll
+        for (int q = 0; q < K; ++q) R[i * K2 + q * K + j] = tmp[q];
+        const int idx = i + K * j;
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            F[((2 + s) * 2 + 0) * K2 + idx] = dotk<K>(tb.E[s], v);
+            F[((2 + s) * 2 + 1) * K2 + idx] = dotk<K>(tb.dE[s], v);
+        }
+    }
+    __syncthreads();
+    // Z layout of u_q: thread (i=qx, j=qy)
+    {
+        T g[K], r[K];
+#pragma unroll
+        for (int q = 0; q < K; ++q) v[q] = A[i * K2 + j * K + q];
+        mv<K>(tb.Dt, v, g);
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            const T W = tb.w[i] * tb.w[j] * tb.w[q] * detJ;
+            const T kap = kappa_vol(a, e, i + K * j + K2 * q, K3);
+            r[q] = W * (kap * g[q] * ih[2] * ih[2] - a.b[2] * v[q] * ih[2]);
+        }
+        mvT<K>(tb.Dt, r, tmp);
+#pragma unroll
+        for (int q = 0; q < K; ++q) R[i * K2 + j * K + q] += tmp[q];
+        const int idx = i + K * j;
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            F[((4 + s) * 2 + 0) * K2 + idx] = dotk<K>(tb.E[s], v);
+            F[((4 + s) * 2 + 1) * K2 + idx] = dotk<K>(tb.dE[s], v);
+        }
+    }
+
+    // ---- faces: neighbour traces and numerical fluxes ----------------------
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        // (1) normal contraction of the neighbour DoFs: thread (i=m1, j=m2)
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const int nbc = ec[d] + (s ? 1 : -1);
+            T val = 0, der = 0;
+            if (nbc >= 0 && nbc < a.n[d]) {
+                const T *un = u + (e + (s ? stride[d] : -stride[d])) * K3;
+#pragma unroll
+                for (int m = 0; m < K; ++m) {
+                    const int off = d == 0 ? m * K2 + i * K + j : d == 1 ? i * K2 + m * K + j : i * K2 + j * K + m;
+                    const T x = un[off];
+                    val = fma(tb.P[1 - s][m], x, val);
+                    der = fma(tb.dP[1 - s][m], x, der);
+                }
+            }
+            NS[(s * 2 + 0) * K2 + i * K + j] = val;
+            NS[(s * 2 + 1) * K2 + i * K + j] = der;
+        }
+        __syncthreads();
+        // (2) second tangential index: thread (i=m1, j=q2)
+#pragma unroll
+        for (int sq = 0; sq < 4; ++sq) {
+#pragma unroll
+            for (int m = 0; m < K; ++m) v[m] = NS[sq * K2 + i * K + m];
+            T2[sq * K2 + i * K + j] = dotk<K>(tb.B[j], v);
+        }
+        __syncthreads();
+        // (3) first tangential index + flux: thread (i=q1, j=q2)
+        const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+        const T Wf = tb.w[i] * tb.w[j] * a.h[t1] * a.h[t2];
+        const int idx = i + K * j;
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const int f = 2 * d + s;
+            const T nrm = s ? T(1) : T(-1);
+            const int nbc = ec[d] + (s ? 1 : -1);
+            const bool interior = nbc >= 0 && nbc < a.n[d];
+            const T uin = F[(f * 2 + 0) * K2 + idx];
+            const T din = F[(f * 2 + 1) * K2 + idx] * ih[d] * nrm;
+            const T bn = a.b[d] * nrm;
+            T kin, kout, kp;
+            if (a.kappa_mode == FEM_KAPPA_QP) {
+                int cc[3] = { ec[0], ec[1], ec[2] };
+                cc[d] += s;
+                const int64_t N0 = a.n[0] + (d == 0), N1 = a.n[1] + (d == 1);
+                const int64_t fidx = cc[0] + N0 * (cc[1] + N1 * (int64_t)cc[2]);
+                kin = kout = kp = a.kface[d][fidx * K2 + idx];
+            } else if (a.kappa_mode == FEM_KAPPA_CELL) {
+                kin = a.kcell[e];
+                kout = interior ? a.kcell[e + (s ? stride[d] : -stride[d])] : kin;
+                kp = T(0.5) * (kin + kout);
+            } else {
+                kin = kout = kp = a.kappa_const;
+            }
+            T c0, c1;
+            if (interior) {
+                T nv = 0, nd = 0;
+#pragma unroll
+                for (int m = 0; m < K; ++m) {
+                    nv = fma(tb.B[i][m], T2[(s * 2 + 0) * K2 + m * K + j], nv);
+                    nd = fma(tb.B[i][m], T2[(s * 2 + 1) * K2 + m * K + j], nd);
+                }
+                const T uout = nv, dout = nd * ih[d] * nrm;
+                const T jump = uin - uout;
+                const T sigma = T(0.5) * (kin * din + kout * dout);
+                const T adv = (bn > T(0) ? bn : T(0)) * uin - (bn < T(0) ? -bn : T(0)) * uout;
+                c0 = Wf * (kp * a.beta * ih[d] * jump - sigma + adv);
+                c1 = Wf * (T(-0.5) * kin * jump);
+            } else {
+                c0 = Wf * (kin * a.beta * ih[d] * uin - kin * din + (bn > T(0) ? bn : T(0)) * uin);
+                c1 = Wf * (-kin * uin);
+            }
+            F[(f * 2 + 0) * K2 + idx] = c0;
+            F[(f * 2 + 1) * K2 + idx] = c1 * nrm * ih[d];
+        }
+    }
+    __syncthreads();
+
+    // ---- backward: quadrature points -> modal test functions ---------------
+    // X layout: thread (i=qy, j=qz)
+    T Rt[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) Rt[q] = Rx[q] + R[q * K2 + i * K + j];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const T c0 = F[((0 + s) * 2 + 0) * K2 + i + K * j], c1 = F[((0 + s) * 2 + 1) * K2 + i + K * j];
+#pragma unroll
+        for (int q = 0; q < K; ++q) Rt[q] += tb.E[s][q] * c0 + tb.dE[s][q] * c1;
+        const T ey0 = tb.E[s][i], dey = tb.dE[s][i];
+        const T ez0 = tb.E[s][j], dez = tb.dE[s][j];
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            Rt[q] += ey0 * F[((2 + s) * 2 + 0) * K2 + q + K * j] + dey * F[((2 + s) * 2 + 1) * K2 + q + K * j];
+            Rt[q] += ez0 * F[((4 + s) * 2 + 0) * K2 + q + K * i] + dez * F[((4 + s) * 2 + 1) * K2 + q + K * i];
+        }
+    }
+    mvT<K>(tb.B, Rt, tmp);
+#pragma unroll
+    for (int q = 0; q < K; ++q) A[q * K2 + i * K + j] = tmp[q];
+    __syncthreads();
+    // Y layout: thread (i=a, j=qz)
+#pragma unroll
+    for (int q = 0; q < K; ++q) v[q] = A[i * K2 + q * K + j];
+    mvT<K>(tb.B, v, tmp);
+#pragma unroll
+    for (int q = 0; q < K; ++q) A[i * K2 + q * K + j] = tmp[q];
+    __syncthreads();
+    // Z layout: thread (i=a, j=b)
+#pragma unroll
+    for (int q = 0; q < K; ++q) v[q] = A[i * K2 + j * K + q];
+    mvT<K>(tb.B, v, tmp);
+    if (active) {
+        T *we = w + e * K3;
+#pragma unroll
+        for (int c = 0; c < K; ++c) we[i * K2 + j * K + c] = tmp[c];
+    }
+}
+
+template <int K, typename T>
+static void launch_apply(const DGArgs<T> &a, const T *u, T *w, const int *skip, hipStream_t s)
+{
+    using S = DGShape<K>;
+    const int64_t E = (int64_t)a.n[0] * a.n[1] * a.n[2];
+    const unsigned nb = (unsigned)((E + S::EPB - 1) / S::EPB);
+    hipLaunchKernelGGL((k_dg_apply<K, T>), dim3(nb), dim3(S::THREADS), 0, s, a, u, w, skip);
+    FEM_HIP(hipGetLastError());
+}
+
+template <typename T>
+static void dispatch_apply(const DGArgs<T> &a, int k, const T *u, T *w, const int *skip, hipStream_t s)
+{
+    switch (k) {
+    case 2: launch_apply<2, T>(a, u, w, skip, s); break;
+    case 3: launch_apply<3, T>(a, u, w, skip, s); break;
+    case 4: launch_apply<4, T>(a, u, w, skip, s); break;
+    case 5: launch_apply<5, T>(a, u, w, skip, s); break;
+    default: throw Error(FEM_ERR_UNSUPPORTED, "DG apply: order must be 1..4");
+    }
+}
+
+void dg_apply_f64(const DGArgs<double> &a, int k, const double *u, double *w, const int *skip, hipStream_t s)
+{
+    dispatch_apply<double>(a, k, u, w, skip, s);
+}
+
+void dg_apply_f32(const DGArgs<float> &a, int k, const float *u, float *w, const int *skip, hipStream_t s)
+{
+    dispatch_apply<float>(a, k, u, w, skip, s);
+}
+
+// ---------------------------------------------------------------------------
+// load vector F_i = int_K f phi_i (f at volume quadrature points)
+// ---------------------------------------------------------------------------
+template <int K>
+__global__ void __launch_bounds__(DGShape<K>::THREADS)
+    k_dg_rhs(const DGArgs<double> a, const double *__restrict__ fq, double *__restrict__ Fv)
+{
+    using S = DGShape<K>;
+    constexpr int K2 = S::K2, K3 = S::K3, EPB = S::EPB;
+    __shared__ double smem[EPB * K3];
+    const Tab1D<double> &tb = *a.tab;
+    const int tid = threadIdx.x, el = tid / K2, t = tid % K2, i = t / K, j = t % K;
+    const int64_t E = (int64_t)a.n[0] * a.n[1] * a.n[2];
+    const int64_t e_raw = (int64_t)blockIdx.x * EPB + el;
+    const bool active = e_raw < E;
+    const int64_t e = active ? e_raw : 0;
+    double *A = smem + el * K3;
+    const double detJ = a.h[0] * a.h[1] * a.h[2];
+    double v[K], tmp[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) v[q] = tb.w[q] * tb.w[i] * tb.w[j] * detJ * fq[e * K3 + q + K * i + K2 * j];
+    mvT<K>(tb.B, v, tmp);
+#pragma unroll
+    for (int q = 0; q < K; ++q) A[q * K2 + i * K + j] = tmp[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < K; ++q) v[q] = A[i * K2 + q * K + j];
+    mvT<K>(tb.B, v, tmp);
+#pragma unroll
+    for (int q = 0; q < K; ++q) A[i * K2 + q * K + j] = tmp[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < K; ++q) v[q] = A[i * K2 + j * K + q];
+    mvT<K>(tb.B, v, tmp);
+    if (active)
+#pragma unroll
+        for (int c = 0; c < K; ++c) Fv[e * K3 + i * K2 + j * K + c] = tmp[c];
+}
+
+void dg_rhs(const DGArgs<double> &a, int k, const double *fq, double *F, hipStream_t s)
+{
+    const int64_t E = (int64_t)a.n[0] * a.n[1] * a.n[2];
+#define RHS_CASE(KK)                                                                                   \
+    case KK: {                                                                                         \
+        unsigned nb = (unsigned)((E + DGShape<KK>::EPB - 1) / DGShape<KK>::EPB);                       \
+        hipLaunchKernelGGL(k_dg_rhs<KK>, dim3(nb), dim3(DGShape<KK>::THREADS), 0, s, a, fq, F);         \
+        break;                                                                                         \
+    }
+    switch (k) {
+        RHS_CASE(2) RHS_CASE(3) RHS_CASE(4) RHS_CASE(5)
+    default: throw Error(FEM_ERR_UNSUPPORTED, "DG rhs: order must be 1..4");
+    }
+#undef RHS_CASE
+    FEM_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// operator diagonal: one thread per DoF, direct sums over quadrature points
+// ---------------------------------------------------------------------------
+__global__ void k_dg_diag(const DGArgs<double> a, int K, double *__restrict__ dout)
+{
+    const int K2 = K * K, K3 = K2 * K;
+    const int64_t E = (int64_t)a.n[0] * a.n[1] * a.n[2];
+    const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (gid >= E * K3) return;
+    const Tab1D<double> &tb = *a.tab;
+    const int64_t e = gid / K3;
+    const int li = (int)(gid % K3);
+    const int ai[3] = { li / K2, (li / K) % K, li % K };
+    const int ec[3] = { (int)(e % a.n[0]), (int)((e / a.n[0]) % a.n[1]), (int)(e / ((int64_t)a.n[0] * a.n[1])) };
+    const int64_t stride[3] = { 1, (int64_t)a.n[0], (int64_t)a.n[0] * a.n[1] };
+    const double detJ = a.h[0] * a.h[1] * a.h[2];
+    double acc = 0;
+    for (int qz = 0; qz < K; ++qz)
+        for (int qy = 0; qy < K; ++qy)
+            for (int qx = 0; qx < K; ++qx) {
+                const int qq[3] = { qx, qy, qz };
+                double phi = 1, gr[3];
+                for (int d = 0; d < 3; ++d) phi *= tb.B[qq[d]][ai[d]];
+                for (int d = 0; d < 3; ++d) {
+                    double g = tb.Bd[qq[d]][ai[d]] / a.h[d];
+                    for (int d2 = 0; d2 < 3; ++d2)
+                        if (d2 != d) g *= tb.B[qq[d2]][ai[d2]];
+                    gr[d] = g;
+                }
+                const double W = tb.w[qx] * tb.w[qy] * tb.w[qz] * detJ;
+                const double kap = kappa_vol(a, e, qx + K * qy + K2 * qz, K3);
+                acc += W * (kap * (gr[0] * gr[0] + gr[1] * gr[1] + gr[2] * gr[2]) -
+                            (a.b[0] * gr[0] + a.b[1] * gr[1] + a.b[2] * gr[2]) * phi + a.c * phi * phi);
+            }
+    for (int d = 0; d < 3; ++d) {
+        const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+        for (int s = 0; s < 2; ++s) {
+            const double nrm = s ? 1.0 : -1.0;
+            const int nbc = ec[d] + (s ? 1 : -1);
+            const bool interior = nbc >= 0 && nbc < a.n[d];
+            const double bn = a.b[d] * nrm, bp = bn > 0 ? bn : 0.0;
+            for (int q2 = 0; q2 < K; ++q2)
+                for (int q1 = 0; q1 < K; ++q1) {
+                    const double tang = tb.B[q1][ai[t1]] * tb.B[q2][ai[t2]];
+                    const double phi = tb.P[s][ai[d]] * tang;
+                    const double dn = nrm * tb.dP[s][ai[d]] * tang / a.h[d];
+                    const double Wf = tb.w[q1] * tb.w[q2] * a.h[t1] * a.h[t2];
+                    double kin, kp;
+                    const int idx = q1 + K * q2;
+                    if (a.kappa_mode == FEM_KAPPA_QP) {
+                        int cc[3] = { ec[0], ec[1], ec[2] };
+                        cc[d] += s;
+                        const int64_t N0 = a.n[0] + (d == 0), N1 = a.n[1] + (d == 1);
+                        const int64_t fidx = cc[0] + N0 * (cc[1] + N1 * (int64_t)cc[2]);
+                        kin = kp = a.kface[d][fidx * K2 + idx];
+                    } else if (a.kappa_mode == FEM_KAPPA_CELL) {
+                        kin = a.kcell[e];
+                        const double kout = interior ? a.kcell[e + (s ? stride[d] : -stride[d])] : kin;
+                        kp = 0.5 * (kin + kout);
+                    } else {
+                        kin = kp = a.kappa_const;
+                    }
+                    if (interior)
+                        acc += Wf * (kp * a.beta / a.h[d] * phi * phi - kin * phi * dn + bp * phi * phi);
+                    else
+                        acc += Wf * (kin * a.beta / a.h[d] * phi * phi - 2.0 * kin * phi * dn + bp * phi * phi);
+                }
+        }
+    }
+    dout[gid] = acc;
+}
+
+void dg_diagonal(const DGArgs<double> &a, int k, double *d, hipStream_t s)
+{
+    const int64_t N = (int64_t)a.n[0] * a.n[1] * a.n[2] * k * k * k;
+    hipLaunchKernelGGL(k_dg_diag, dim3(grid_for(N, 256)), dim3(256), 0, s, a, k, d);
+    FEM_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// exact element-block inverse for cellwise kappa without advection
+// ---------------------------------------------------------------------------
+// cyclic Jacobi eigenvalue iteration for a symmetric K x K matrix
+__device__ void sym_eig(int K, double (*A)[KMAX], double (*Q)[KMAX])
+{
+    for (int r = 0; r < K; ++r)
+        for (int c = 0; c < K; ++c) Q[r][c] = (r == c) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 50; ++sweep) {
+        double off = 0, tot = 0;
+        for (int r = 0; r < K; ++r)
+            for (int c = 0; c < K; ++c) {
+                tot += A[r][c] * A[r][c];
+                if (r != c) off += A[r][c] * A[r][c];
+            }
+        if (off <= 1e-30 * tot) break;
+        for (int p = 0; p < K - 1; ++p)
+            for (int q = p + 1; q < K; ++q) {
+                if (A[p][q] == 0.0) continue;
+                const double theta = (A[q][q] - A[p][p]) / (2.0 * A[p][q]);
+                const double tt = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                const double cs = 1.0 / sqrt(tt * tt + 1.0), sn = tt * cs;
+                for (int k = 0; k < K; ++k) {
+                    const double akp = A[k][p], akq = A[k][q];
+                    A[k][p] = cs * akp - sn * akq;
+                    A[k][q] = sn * akp + cs * akq;
+                }
+                for (int k = 0; k < K; ++k) {
+                    const double apk = A[p][k], aqk = A[q][k];
+                    A[p][k] = cs * apk - sn * aqk;
+                    A[q][k] = sn * apk + cs * aqk;
+                }
+                for (int k = 0; k < K; ++k) {
+                    const double qkp = Q[k][p], qkq = Q[k][q];
+                    Q[k][p] = cs * qkp - sn * qkq;
+                    Q[k][q] = sn * qkp + cs * qkq;
+                }
+            }
+    }
+}
+
+__global__ void k_dg_block_setup(const DGArgs<double> a, int K, double *__restrict__ Qout, double *__restrict__ lam)
+{
+    const int64_t E = (int64_t)a.n[0] * a.n[1] * a.n[2];
+    const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (gid >= E * 3) return;
+    const int64_t e = gid / 3;
+    const int d = (int)(gid % 3);
+    const Tab1D<double> &tb = *a.tab;
+    const int ec[3] = { (int)(e % a.n[0]), (int)((e / a.n[0]) % a.n[1]), (int)(e / ((int64_t)a.n[0] * a.n[1])) };
+    const int64_t stride[3] = { 1, (int64_t)a.n[0], (int64_t)a.n[0] * a.n[1] };
+    const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+    const double cd = a.h[t1] * a.h[t2];
+    double kin;
+    if (a.kappa_mode == FEM_KAPPA_CELL) kin = a.kcell[e];
+    else if (a.kappa_mode == FEM_KAPPA_QP) {
+        // element mean of the pointwise coefficient (approximate block)
+        double s = 0;
+        for (int q = 0; q < K * K * K; ++q) s += a.kvol[e * K * K * K + q];
+        kin = s / (K * K * K);
+    } else kin = a.kappa_const;
+    double M[KMAX][KMAX], Q[KMAX][KMAX];
+    for (int r = 0; r < K; ++r)
+        for (int c = 0; c < K; ++c) M[r][c] = cd * kin / a.h[d] * tb.S[r][c];
+    for (int s = 0; s < 2; ++s) {
+        const double nrm = s ? 1.0 : -1.0;
+        const int nbc = ec[d] + (s ? 1 : -1);
+        const bool interior = nbc >= 0 && nbc < a.n[d];
+        double kp;
+        if (interior) {
+            double kout;
+            if (a.kappa_mode == FEM_KAPPA_CELL) kout = a.kcell[e + (s ? stride[d] : -stride[d])];
+            else kout = kin;
+            kp = 0.5 * (kin + kout);
+        } else kp = kin;
+        const double symw = interior ? 0.5 : 1.0;
+        for (int r = 0; r < K; ++r)
+            for (int c = 0; c < K; ++c)
+                M[r][c] += cd * (kp * a.beta / a.h[d] * tb.P[s][r] * tb.P[s][c] -
+                                 symw * kin * nrm / a.h[d] * (tb.dP[s][c] * tb.P[s][r] + tb.P[s][c] * tb.dP[s][r]));
+    }
+    sym_eig(K, M, Q);
+    for (int r = 0; r < K; ++r) {
+        lam[(e * 3 + d) * K + r] = M[r][r];
+        for (int c = 0; c < K; ++c) Qout[((e * 3 + d) * K + r) * K + c] = Q[r][c];
+    }
+}
+
+void dg_block_setup(const DGArgs<double> &a, int k, double *Q, double *lam, hipStream_t s)
+{
+    const int64_t E = (int64_t)a.n[0] * a.n[1] * a.n[2];
+    hipLaunchKernelGGL(k_dg_block_setup, dim3(grid_for(E * 3, 128)), dim3(128), 0, s, a, k, Q, lam);
+    FEM_HIP(hipGetLastError());
+}
+
+// y_e = (Qx (x) Qy (x) Qz) diag(1/(lx+ly+lz)) (Qx (x) Qy (x) Qz)^T x_e
+template <int K, typename T>
+__global__ void __launch_bounds__(DGShape<K>::THREADS)
+    k_dg_block_apply(int64_t E, const T *__restrict__ Q, const T *__restrict__ lam, const T *__restrict__ x,
+                     T *__restrict__ y, const int *__restrict__ skip)
+{
+    using S = DGShape<K>;
+    constexpr int K2 = S::K2, K3 = S::K3, EPB = S::EPB;
+    if (skip && *skip) return;
+    __shared__ T smem[EPB * K3];
+    const int tid = threadIdx.x, el = tid / K2, t = tid % K2, i = t / K, j = t % K;
+    const int64_t e_raw = (int64_t)blockIdx.x * EPB + el;
+    const bool active = e_raw < E;
+    const int64_t e = active ? e_raw : 0;
+    T *A = smem + el * K3;
+    const T *Qe = Q + e * 3 * K2;
+    T v[K], tmp[K];
+    // Q_z^T along c: thread (i=a, j=b)
+#pragma unroll
+    for (int c = 0; c < K; ++c) v[c] = x[e * K3 + i * K2 + j * K + c];
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+        T s = 0;
+#pragma unroll
+        for (int c = 0; c < K; ++c) s = fma(Qe[2 * K2 + c * K + r], v[c], s);
+        tmp[r] = s;
+    }
+#pragma unroll
+    for (int r = 0; r < K; ++r) A[i * K2 + j * K + r] = tmp[r];
+    __syncthreads();
+    // Q_y^T along b: thread (i=a, j=c')
+#pragma unroll
+    for (int b = 0; b < K; ++b) v[b] = A[i * K2 + b * K + j];
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+        T s = 0;
+#pragma unroll
+        for (int b = 0; b < K; ++b) s = fma(Qe[1 * K2 + b * K + r], v[b], s);
+        tmp[r] = s;
+    }
+#pragma unroll
+    for (int r = 0; r < K; ++r) A[i * K2 + r * K + j] = tmp[r];
+    __syncthreads();
+    // Q_x^T along a, scale, Q_x: thread (i=b', j=c')
+#pragma unroll
+    for (int b = 0; b < K; ++b) v[b] = A[b * K2 + i * K + j];
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+        T s = 0;
+#pragma unroll
+        for (int b = 0; b < K; ++b) s = fma(Qe[0 * K2 + b * K + r], v[b], s);
+        tmp[r] = s / (lam[(e * 3 + 0) * K + r] + lam[(e * 3 + 1) * K + i] + lam[(e * 3 + 2) * K + j]);
+    }
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+        T s = 0;
+#pragma unroll
+        for (int b = 0; b < K; ++b) s = fma(Qe[0 * K2 + r * K + b], tmp[b], s);
+        v[r] = s;
+    }
+#pragma unroll
+    for (int r = 0; r < K; ++r) A[r * K2 + i * K + j] = v[r];
+    __syncthreads();
+    // Q_y along b: thread (i=a, j=c')
+#pragma unroll
+    for (int b = 0; b < K; ++b) v[b] = A[i * K2 + b * K + j];
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+        T s = 0;
+#pragma unroll
+        for (int b = 0; b < K; ++b) s = fma(Qe[1 * K2 + r * K + b], v[b], s);
+        tmp[r] = s;
+    }
+#pragma unroll
+    for (int r = 0; r < K; ++r) A[i * K2 + r * K + j] = tmp[r];
+    __syncthreads();
+    // Q_z along c: thread (i=a, j=b)
+#pragma unroll
+    for (int c = 0; c < K; ++c) v[c] = A[i * K2 + j * K + c];
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+        T s = 0;
+#pragma unroll
+        for (int c = 0; c < K; ++c) s = fma(Qe[2 * K2 + r * K + c], v[c], s);
+        tmp[r] = s;
+    }
+    if (active)
+#pragma unroll
+        for (int r = 0; r < K; ++r) y[e * K3 + i * K2 + j * K + r] = tmp[r];
+}
+
+template <typename T>
+void dg_block_apply(int k, int64_t E, const T *Q, const T *lam, const T *x, T *y, const int *skip, hipStream_t s)
+{
+#define BA_CASE(KK)                                                                                    \
+    case KK: {                                                                                         \
+        unsigned nb = (unsigned)((E + DGShape<KK>::EPB - 1) / DGShape<KK>::EPB);                       \
+        hipLaunchKernelGGL((k_dg_block_apply<KK, T>), dim3(nb), dim3(DGShape<KK>::THREADS), 0, s, E, Q, lam, x, y, skip); \
+        break;                                                                                         \
+    }
+    switch (k) {
+        BA_CASE(2) BA_CASE(3) BA_CASE(4) BA_CASE(5)
+    default: throw Error(FEM_ERR_UNSUPPORTED, "block Jacobi: order must be 1..4");
+    }
+#undef BA_CASE
+    FEM_HIP(hipGetLastError());
+}
+
+template void dg_block_apply<double>(int, int64_t, const double *, const double *, const double *, double *, const int *, hipStream_t);
+template void dg_block_apply<float>(int, int64_t, const float *, const float *, const float *, float *, const int *, hipStream_t);
+
+}  // namespace femamd

@@ -1,0 +1,43 @@
+// SIPG DG operator on structured hexahedral meshes: kernel arguments and launchers.
+#pragma once
+
+#include "fem_common.hpp"
+
+namespace femamd {
+
+template <typename T>
+struct DGArgs {
+    int n[3];
+    T h[3];
+    T beta;
+    T b[3];
+    T c;
+    int kappa_mode;
+    T kappa_const;
+    const T *kvol;
+    const T *kface[3];
+    const T *kcell;
+    const Tab1D<T> *tab;   // device copy of the 1D tables
+};
+
+// w = A u.  `skip` (device flag, may be null): when *skip != 0 the launch
+// does nothing (device-driven solver loops).
+void dg_apply_f64(const DGArgs<double> &a, int k, const double *u, double *w, const int *skip, hipStream_t s);
+void dg_apply_f32(const DGArgs<float> &a, int k, const float *u, float *w, const int *skip, hipStream_t s);
+// BF16 MFMA contraction variant (FP32 data and accumulation)
+void dg_apply_bf16(const DGArgs<float> &a, int k, const float *u, float *w, const int *skip, hipStream_t s);
+// operator diagonal
+void dg_diagonal(const DGArgs<double> &a, int k, double *d, hipStream_t s);
+// load vector from f at volume quadrature points
+void dg_rhs(const DGArgs<double> &a, int k, const double *fq, double *F, hipStream_t s);
+
+// Exact element-block inverse for cellwise kappa (Kronecker-sum structure of
+// the element block of the SIPG diffusion operator in the orthonormal
+// Legendre basis): per element and direction an orthogonal K x K matrix Q_d
+// and eigenvalues lam_d with A_KK = sum_d (I..A_d..I), A_d = Q_d diag(lam_d) Q_d^T.
+void dg_block_setup(const DGArgs<double> &a, int k, double *Q /*[E][3][K][K]*/, double *lam /*[E][3][K]*/,
+                    hipStream_t s);
+template <typename T>
+void dg_block_apply(int k, int64_t E, const T *Q, const T *lam, const T *x, T *y, const int *skip, hipStream_t s);
+
+}  // namespace femamd

@@ -1,0 +1,298 @@
+// Operator and preconditioner objects: set-up of coefficient data and
+// dispatch to the DG / CG kernels.
+#include <cmath>
+
+#include "fem_blas.hpp"
+#include "fem_op.hpp"
+
+namespace femamd {
+
+template <typename T>
+__global__ void k_jacobi_inv(const double *__restrict__ d, T *__restrict__ dinv, int64_t n)
+{
+    // near-zero diagonal entries become 1 (fempreconditioning.hh:99-106)
+    const double eps = 16.0 * 2.220446049250313e-16;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dinv[i] = (T)(fabs(d[i]) < eps ? 1.0 : 1.0 / d[i]);
+}
+
+template <typename T>
+__global__ void k_pointwise_mul(const T *__restrict__ a, const T *__restrict__ x, T *__restrict__ y, int64_t n,
+                                const int *__restrict__ skip)
+{
+    if (skip && *skip) return;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = a[i] * x[i];
+}
+
+template <typename T>
+static void upload_as(DevBuf &b, const double *host, size_t n, hipStream_t s)
+{
+    b.alloc(n * sizeof(T));
+    if (std::is_same<T, double>::value) {
+        FEM_HIP(hipMemcpyAsync(b.p, host, n * sizeof(double), hipMemcpyHostToDevice, s));
+        FEM_HIP(hipStreamSynchronize(s));
+    } else {
+        std::vector<float> tmp(n);
+        for (size_t i = 0; i < n; ++i) tmp[i] = (float)host[i];
+        FEM_HIP(hipMemcpyAsync(b.p, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice, s));
+        FEM_HIP(hipStreamSynchronize(s));
+    }
+}
+
+void Operator::init(Context *c, const fem_op_desc &d)
+{
+    ctx = c;
+    desc = d;
+    FEM_CHECK(d.precision == FEM_F64 || d.precision == FEM_F32 || d.precision == FEM_BF16MFMA, FEM_ERR_ARG,
+              "operator precision");
+    vec_prec = d.precision == FEM_F64 ? FEM_F64 : FEM_F32;
+    if (d.kind == FEM_OP_DG_SIPG_3D) {
+        FEM_CHECK(d.order >= 1 && d.order <= 4, FEM_ERR_UNSUPPORTED, "DG order must be 1..4");
+        FEM_CHECK(d.n[0] > 0 && d.n[1] > 0 && d.n[2] > 0, FEM_ERR_ARG, "mesh size");
+        FEM_CHECK(d.kappa_mode >= 0 && d.kappa_mode <= 2, FEM_ERR_ARG, "kappa mode");
+        k = d.order + 1;
+        E = (int64_t)d.n[0] * d.n[1] * d.n[2];
+        N = E * k * k * k;
+        Tab1D<double> t = make_tab1d(k);
+        Tab1D<float> tf = to_float(t);
+        tab64.alloc(sizeof(t));
+        tab32.alloc(sizeof(tf));
+        FEM_HIP(hipMemcpy(tab64.p, &t, sizeof(t), hipMemcpyHostToDevice));
+        FEM_HIP(hipMemcpy(tab32.p, &tf, sizeof(tf), hipMemcpyHostToDevice));
+        for (int i = 0; i < 3; ++i) {
+            a64.n[i] = a32.n[i] = d.n[i];
+            a64.h[i] = 1.0 / d.n[i];
+            a32.h[i] = (float)(1.0 / d.n[i]);
+            a64.b[i] = d.b[i];
+            a32.b[i] = (float)d.b[i];
+        }
+        a64.beta = d.beta;
+        a32.beta = (float)d.beta;
+        a64.c = d.c;
+        a32.c = (float)d.c;
+        a64.kappa_mode = a32.kappa_mode = d.kappa_mode == FEM_KAPPA_CONST ? FEM_KAPPA_CONST : -1;  // set on data upload
+        a64.kappa_const = d.kappa_const;
+        a32.kappa_const = (float)d.kappa_const;
+        a64.tab = tab64.as<Tab1D<double>>();
+        a32.tab = tab32.as<Tab1D<float>>();
+    } else if (d.kind == FEM_OP_CG_Q2_2D) {
+        FEM_CHECK(d.order == 2, FEM_ERR_UNSUPPORTED, "CG operator supports order 2");
+        FEM_CHECK(d.n[0] > 0 && d.n[1] > 0, FEM_ERR_ARG, "mesh size");
+        FEM_CHECK(d.kappa_mode == FEM_KAPPA_CONST || d.kappa_mode == FEM_KAPPA_QP, FEM_ERR_ARG, "kappa mode");
+        k = 3;
+        E = (int64_t)d.n[0] * d.n[1];
+        N = (int64_t)(2 * d.n[0] + 1) * (2 * d.n[1] + 1);
+        inst.nx = d.n[0];
+        inst.ny = d.n[1];
+        inst.dof_off = 0;
+        inst.elem_off = 0;
+        inst.tiles_x = (d.n[0] + CG_TX - 1) / CG_TX;
+        inst.ntiles = inst.tiles_x * ((d.n[1] + CG_TY - 1) / CG_TY);
+        inst.tile_off = 0;
+        inst.c = d.c;
+        inst.kappa_const = d.kappa_const;
+        tiles = inst.ntiles;
+        inst_dev.alloc(sizeof(CGInst));
+        FEM_HIP(hipMemcpy(inst_dev.p, &inst, sizeof(CGInst), hipMemcpyHostToDevice));
+        CGTab t = make_cg_tab();
+        cgtab.alloc(sizeof(t));
+        FEM_HIP(hipMemcpy(cgtab.p, &t, sizeof(t), hipMemcpyHostToDevice));
+    } else {
+        throw Error(FEM_ERR_ARG, "unknown operator kind");
+    }
+}
+
+void Operator::set_kappa_qp(const double *kvol, const double *kx, const double *ky, const double *kz)
+{
+    hipStream_t s = ctx->stream;
+    if (is_dg()) {
+        FEM_CHECK(desc.kappa_mode == FEM_KAPPA_QP, FEM_ERR_STATE, "operator not created with FEM_KAPPA_QP");
+        FEM_CHECK(kvol && kx && ky && kz, FEM_ERR_ARG, "null kappa array");
+        const int k2 = k * k;
+        const size_t nv = (size_t)E * k * k2;
+        const double *kf[3] = { kx, ky, kz };
+        size_t nf[3];
+        for (int d = 0; d < 3; ++d)
+            nf[d] = (size_t)(desc.n[0] + (d == 0)) * (desc.n[1] + (d == 1)) * (desc.n[2] + (d == 2)) * k2;
+        if (vec_prec == FEM_F64) {
+            upload_as<double>(kvol64, kvol, nv, s);
+            for (int d = 0; d < 3; ++d) upload_as<double>(kface64[d], kf[d], nf[d], s);
+        }
+        upload_as<float>(kvol32, kvol, nv, s);
+        for (int d = 0; d < 3; ++d) upload_as<float>(kface32[d], kf[d], nf[d], s);
+        if (vec_prec != FEM_F64) {
+            // FP64 copies are kept only for FP64 operators; the diagonal and
+            // block set-up of FP32 operators read the FP32 data widened
+            upload_as<double>(kvol64, kvol, nv, s);
+            for (int d = 0; d < 3; ++d) upload_as<double>(kface64[d], kf[d], nf[d], s);
+        }
+        a64.kvol = kvol64.as<double>();
+        a32.kvol = kvol32.as<float>();
+        for (int d = 0; d < 3; ++d) {
+            a64.kface[d] = kface64[d].as<double>();
+            a32.kface[d] = kface32[d].as<float>();
+        }
+        a64.kappa_mode = a32.kappa_mode = FEM_KAPPA_QP;
+    } else {
+        FEM_CHECK(desc.kappa_mode == FEM_KAPPA_QP, FEM_ERR_STATE, "operator not created with FEM_KAPPA_QP");
+        FEM_CHECK(kvol, FEM_ERR_ARG, "null kappa array");
+        upload_as<double>(kvol64, kvol, (size_t)E * 9, s);
+        upload_as<float>(kvol32, kvol, (size_t)E * 9, s);
+    }
+}
+
+void Operator::set_kappa_cell(const double *kc)
+{
+    FEM_CHECK(is_dg() && desc.kappa_mode == FEM_KAPPA_CELL, FEM_ERR_STATE, "operator not created with FEM_KAPPA_CELL");
+    FEM_CHECK(kc, FEM_ERR_ARG, "null kappa array");
+    upload_as<double>(kcell64, kc, (size_t)E, ctx->stream);
+    upload_as<float>(kcell32, kc, (size_t)E, ctx->stream);
+    a64.kcell = kcell64.as<double>();
+    a32.kcell = kcell32.as<float>();
+    a64.kappa_mode = a32.kappa_mode = FEM_KAPPA_CELL;
+}
+
+void Operator::apply_raw(const void *x, void *y, const int *skip) const
+{
+    hipStream_t s = ctx->stream;
+    if (is_dg()) {
+        FEM_CHECK(a64.kappa_mode >= 0, FEM_ERR_STATE, "coefficient data not set");
+        if (desc.precision == FEM_F64) dg_apply_f64(a64, k, (const double *)x, (double *)y, skip, s);
+        else if (desc.precision == FEM_F32) dg_apply_f32(a32, k, (const float *)x, (float *)y, skip, s);
+        else dg_apply_bf16(a32, k, (const float *)x, (float *)y, skip, s);
+    } else {
+        FEM_CHECK(desc.kappa_mode == FEM_KAPPA_CONST || kvol64.p, FEM_ERR_STATE, "coefficient data not set");
+        if (vec_prec == FEM_F64)
+            cg_launch<double>(CG_MODE_APPLY, inst_dev.as<CGInst>(), 1, tiles, cgtab.as<CGTab>(),
+                              desc.kappa_mode == FEM_KAPPA_QP ? kvol64.as<double>() : nullptr, (const double *)x,
+                              (double *)y, skip, s);
+        else
+            cg_launch<float>(CG_MODE_APPLY, inst_dev.as<CGInst>(), 1, tiles, cgtab.as<CGTab>(),
+                             desc.kappa_mode == FEM_KAPPA_QP ? kvol32.as<float>() : nullptr, (const float *)x,
+                             (float *)y, skip, s);
+    }
+}
+
+void Operator::apply(const Vec &x, Vec &y, const int *skip) const
+{
+    FEM_CHECK(x.n == N && y.n == N, FEM_ERR_ARG, "apply: vector size does not match the operator");
+    FEM_CHECK(x.prec == vec_prec && y.prec == vec_prec, FEM_ERR_ARG, "apply: vector precision does not match the operator");
+    FEM_CHECK(x.data != y.data, FEM_ERR_ARG, "apply: u and w must be distinct");
+    apply_raw(x.data, y.data, skip);
+}
+
+void Operator::rhs(const double *fq_host, Vec &F) const
+{
+    FEM_CHECK(F.n == N, FEM_ERR_ARG, "rhs: vector size");
+    hipStream_t s = ctx->stream;
+    DevBuf fq, out;
+    const size_t nq = is_dg() ? (size_t)E * k * k * k : (size_t)E * 9;
+    upload_as<double>(fq, fq_host, nq, s);
+    double *dst;
+    if (F.prec == FEM_F64) dst = (double *)F.data;
+    else { out.alloc(N * sizeof(double)); dst = out.as<double>(); }
+    if (is_dg()) dg_rhs(a64, k, fq.as<double>(), dst, s);
+    else cg_launch<double>(CG_MODE_RHS, inst_dev.as<CGInst>(), 1, tiles, cgtab.as<CGTab>(), fq.as<double>(), nullptr, dst, nullptr, s);
+    if (F.prec != FEM_F64) {
+        Vec tmp;
+        tmp.ctx = ctx; tmp.n = N; tmp.prec = FEM_F64; tmp.data = dst; tmp.owner = false;
+        vec_copy(F, tmp);
+    }
+    FEM_HIP(hipStreamSynchronize(s));
+}
+
+void Operator::diagonal(Vec &d) const
+{
+    FEM_CHECK(d.n == N, FEM_ERR_ARG, "diagonal: vector size");
+    hipStream_t s = ctx->stream;
+    DevBuf out;
+    double *dst;
+    if (d.prec == FEM_F64) dst = (double *)d.data;
+    else { out.alloc(N * sizeof(double)); dst = out.as<double>(); }
+    if (is_dg()) {
+        FEM_CHECK(a64.kappa_mode >= 0, FEM_ERR_STATE, "coefficient data not set");
+        dg_diagonal(a64, k, dst, s);
+    } else {
+        cg_launch<double>(CG_MODE_DIAG, inst_dev.as<CGInst>(), 1, tiles, cgtab.as<CGTab>(),
+                          desc.kappa_mode == FEM_KAPPA_QP ? kvol64.as<double>() : nullptr, nullptr, dst, nullptr, s);
+    }
+    if (d.prec != FEM_F64) {
+        Vec tmp;
+        tmp.ctx = ctx; tmp.n = N; tmp.prec = FEM_F64; tmp.data = dst; tmp.owner = false;
+        vec_copy(d, tmp);
+    }
+    FEM_HIP(hipStreamSynchronize(s));
+}
+
+void Precond::init(Operator *o, int k_)
+{
+    op = o;
+    kind = k_;
+    prec = o->vec_prec;
+    hipStream_t s = o->ctx->stream;
+    if (kind == FEM_PRE_JACOBI) {
+        DevBuf d;
+        d.alloc(o->N * sizeof(double));
+        Vec dv;
+        dv.ctx = o->ctx; dv.n = o->N; dv.prec = FEM_F64; dv.data = d.p; dv.owner = false;
+        o->diagonal(dv);
+        dinv.alloc(o->N * (prec == FEM_F64 ? 8 : 4));
+        if (prec == FEM_F64)
+            hipLaunchKernelGGL(k_jacobi_inv<double>, dim3(grid_for(o->N, 256, 8192)), dim3(256), 0, s, d.as<double>(), dinv.as<double>(), o->N);
+        else
+            hipLaunchKernelGGL(k_jacobi_inv<float>, dim3(grid_for(o->N, 256, 8192)), dim3(256), 0, s, d.as<double>(), dinv.as<float>(), o->N);
+        FEM_HIP(hipGetLastError());
+        FEM_HIP(hipStreamSynchronize(s));
+    } else if (kind == FEM_PRE_BLOCK_JACOBI) {
+        FEM_CHECK(o->is_dg(), FEM_ERR_UNSUPPORTED, "block Jacobi needs a DG operator");
+        FEM_CHECK(o->desc.b[0] == 0 && o->desc.b[1] == 0 && o->desc.b[2] == 0 && o->desc.c == 0, FEM_ERR_UNSUPPORTED,
+                  "block Jacobi: diffusion operators only (b = 0, c = 0)");
+        FEM_CHECK(o->a64.kappa_mode >= 0, FEM_ERR_STATE, "coefficient data not set");
+        const int k = o->k;
+        DevBuf Q64, l64;
+        Q64.alloc((size_t)o->E * 3 * k * k * sizeof(double));
+        l64.alloc((size_t)o->E * 3 * k * sizeof(double));
+        dg_block_setup(o->a64, k, Q64.as<double>(), l64.as<double>(), s);
+        if (prec == FEM_F64) {
+            Q.alloc(Q64.bytes); lam.alloc(l64.bytes);
+            FEM_HIP(hipMemcpyAsync(Q.p, Q64.p, Q64.bytes, hipMemcpyDeviceToDevice, s));
+            FEM_HIP(hipMemcpyAsync(lam.p, l64.p, l64.bytes, hipMemcpyDeviceToDevice, s));
+        } else {
+            Q.alloc(Q64.bytes / 2); lam.alloc(l64.bytes / 2);
+            Vec a, b;
+            a.ctx = b.ctx = o->ctx;
+            a.prec = FEM_F32; b.prec = FEM_F64; a.owner = b.owner = false;
+            a.n = b.n = (int64_t)o->E * 3 * k * k; a.data = Q.p; b.data = Q64.p;
+            vec_copy(a, b);
+            a.n = b.n = (int64_t)o->E * 3 * k; a.data = lam.p; b.data = l64.p;
+            vec_copy(a, b);
+        }
+        FEM_HIP(hipStreamSynchronize(s));
+    } else if (kind == FEM_PRE_NONE) {
+    } else {
+        throw Error(FEM_ERR_UNSUPPORTED, "preconditioner kind");
+    }
+}
+
+void Precond::apply_raw(const void *x, void *y, const int *skip) const
+{
+    hipStream_t s = op->ctx->stream;
+    const int64_t N = op->N;
+    if (kind == FEM_PRE_JACOBI) {
+        if (prec == FEM_F64)
+            hipLaunchKernelGGL(k_pointwise_mul<double>, dim3(grid_for(N, 256, 8192)), dim3(256), 0, s, dinv.as<double>(), (const double *)x, (double *)y, N, skip);
+        else
+            hipLaunchKernelGGL(k_pointwise_mul<float>, dim3(grid_for(N, 256, 8192)), dim3(256), 0, s, dinv.as<float>(), (const float *)x, (float *)y, N, skip);
+        FEM_HIP(hipGetLastError());
+    } else if (kind == FEM_PRE_BLOCK_JACOBI) {
+        if (prec == FEM_F64)
+            dg_block_apply<double>(op->k, op->E, Q.as<double>(), lam.as<double>(), (const double *)x, (double *)y, skip, s);
+        else
+            dg_block_apply<float>(op->k, op->E, Q.as<float>(), lam.as<float>(), (const float *)x, (float *)y, skip, s);
+    } else {
+        FEM_HIP(hipMemcpyAsync(y, x, N * (prec == FEM_F64 ? 8 : 4), hipMemcpyDeviceToDevice, s));
+    }
+}
+
+}  // namespace femamd

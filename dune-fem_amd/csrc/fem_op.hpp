@@ -1,0 +1,75 @@
+// Operator and preconditioner objects behind the C ABI.
+#pragma once
+
+#include "fem_cg.hpp"
+#include "fem_dg.hpp"
+
+namespace femamd {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    void alloc(size_t b)
+    {
+        free();
+        if (b) {
+            hipError_t e = hipMalloc(&p, b);
+            if (e != hipSuccess) throw Error(FEM_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+        }
+        bytes = b;
+    }
+    void free()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    ~DevBuf() { free(); }
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    template <typename T> T *as() const { return (T *)p; }
+};
+
+struct Operator {
+    Context *ctx = nullptr;
+    fem_op_desc desc{};
+    int k = 0;         // points/modes per direction (DG: p+1; CG: 3)
+    int64_t E = 0, N = 0;
+    int vec_prec = FEM_F64;  // precision of the vectors it acts on
+    // DG
+    DGArgs<double> a64{};
+    DGArgs<float> a32{};
+    DevBuf tab64, tab32;
+    DevBuf kvol64, kface64[3], kcell64, kvol32, kface32[3], kcell32;
+    // CG
+    CGInst inst{};
+    DevBuf inst_dev, cgtab;
+    int tiles = 0;
+
+    void init(Context *c, const fem_op_desc &d);
+    void set_kappa_qp(const double *kvol, const double *kx, const double *ky, const double *kz);
+    void set_kappa_cell(const double *kc);
+    // y = A x on raw device pointers of the operator's vector precision
+    void apply_raw(const void *x, void *y, const int *skip) const;
+    void apply(const Vec &x, Vec &y, const int *skip = nullptr) const;
+    void rhs(const double *fq_host, Vec &F) const;
+    void diagonal(Vec &d) const;  // FP64 or FP32 output
+    bool is_dg() const { return desc.kind == FEM_OP_DG_SIPG_3D; }
+};
+
+struct Precond {
+    Operator *op = nullptr;
+    int kind = FEM_PRE_NONE;
+    int prec = FEM_F64;
+    DevBuf dinv;       // Jacobi
+    DevBuf Q, lam;     // block Jacobi (tensor-product eigen-decomposition)
+    void init(Operator *o, int kind);
+    void apply_raw(const void *x, void *y, const int *skip) const;
+    bool pointwise() const { return kind == FEM_PRE_JACOBI; }
+};
+
+}  // namespace femamd
+
+struct fem_op { femamd::Operator o; };
+struct fem_precond { femamd::Precond p; };

@@ -1,0 +1,49 @@
+// Device-driven Krylov solvers behind the InverseOperator interface.
+#pragma once
+
+#include "fem_op.hpp"
+
+namespace femamd {
+
+constexpr int GM_MAX = 64;   // maximum GMRES restart length
+
+// scalars of one solve, resident in device memory; kernels read and update
+// them so that the iteration loop needs no host round trip per step
+struct KState {
+    double res, prev, qh, alpha, beta, tol, bnorm2, scale;
+    int it, done, maxit, criteria;
+    // GMRES
+    double H[(GM_MAX + 1) * GM_MAX];
+    double g[GM_MAX + 1], cs[GM_MAX], sn[GM_MAX], y[GM_MAX];
+    double gtol, hinv;
+    int jlast;       // inner steps done in the current cycle
+    int converged;   // cycle ended by |g| < tol
+    int cycle_done;  // stop inner loop (converged or maxit)
+};
+
+struct Solver {
+    Context *ctx = nullptr;
+    fem_solver_params prm{};
+    Operator *op = nullptr, *iop = nullptr;
+    Precond *pre = nullptr, *ipre = nullptr;
+    DevBuf state, partials, counters;
+    KState *hstate = nullptr;   // pinned mirror
+    std::vector<DevBuf *> work;
+
+    void init(Context *c, const fem_solver_params &p);
+    void bind(Operator *o, Precond *p, Operator *io, Precond *ip);
+    void unbind();
+    void solve(const Vec &b, Vec &x, fem_solver_info &info);
+    ~Solver();
+
+    // one Krylov solve in precision of `o` on raw pointers; returns signed iterations
+    int krylov(int method, Operator *o, Precond *p, const void *b, void *x, double tol, int criteria, int maxit,
+               int restart, int64_t N, int prec);
+    void *workbuf(int i, size_t bytes);
+    void fetch_state();
+    double true_residual(const Vec &b, const Vec &x, Vec &r);
+};
+
+}  // namespace femamd
+
+struct fem_solver { femamd::Solver s; };

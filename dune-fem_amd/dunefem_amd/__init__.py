@@ -1,0 +1,387 @@
+"""Python mirror of the dune-fem operator / inverse-operator interface over
+the MI355X C ABI (libdunefem_amd.so).
+
+The objects follow the reference's Python-facing contract:
+
+* ``DofVector``         -- DoF storage of a discrete function (``dofVector``,
+  ``clear``, ``assign``, ``axpy``, ``scalarProductDofs``; buffer access as in
+  dune/fempy/py/discretefunction.hh:55-167);
+* ``GalerkinOperator``  -- ``op(u, w)`` computes w = A u
+  (dune/fempy/py/operator.hh:49,215 -> galerkin.hh:1430-1433);
+* ``InverseOperator``   -- ``bind(op[, pre])``, ``inv(b, x)``, ``iterations()``,
+  ``info()`` (dune/fem/solver/inverseoperatorinterface.hh:66-166);
+* ``solve()`` returns the info dict of ``scheme.solve``
+  (dune/fempy/py/scheme.hh:156-187).
+
+There is no fallback: every call goes through the HIP library, and the module
+raises if the library is missing or no GPU is present.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libdunefem_amd.so")
+
+F64, F32, BF16MFMA = 0, 1, 2
+OP_DG_SIPG_3D, OP_CG_Q2_2D = 1, 2
+KAPPA_CONST, KAPPA_QP, KAPPA_CELL = 0, 1, 2
+PRE_NONE, PRE_JACOBI, PRE_BLOCK_JACOBI, PRE_LEARNED = 0, 1, 2, 3
+CG, GMRES, FGMRES = 0, 1, 2
+ABSOLUTE, RELATIVE, REDUCTION = 0, 1, 2
+
+# every symbol declared in include/dunefem_amd.h
+EXPORTS = [
+    "fem_last_error", "fem_version", "fem_ctx_create", "fem_ctx_destroy", "fem_ctx_synchronize",
+    "fem_ctx_mark", "fem_ctx_elapsed_ms", "fem_vec_create", "fem_vec_destroy", "fem_vec_size",
+    "fem_vec_precision", "fem_vec_device_ptr", "fem_vec_upload", "fem_vec_download",
+    "fem_vec_upload_f32", "fem_vec_fill", "fem_vec_copy", "fem_vec_axpy", "fem_vec_scal",
+    "fem_vec_dot", "fem_vec_norm2sq", "fem_op_create", "fem_op_destroy", "fem_op_ndofs",
+    "fem_op_set_kappa_qp", "fem_op_set_kappa_cell", "fem_op_apply", "fem_op_assemble_rhs",
+    "fem_op_diagonal", "fem_precond_create", "fem_precond_destroy", "fem_precond_apply",
+    "fem_solver_default_params", "fem_solver_params_set", "fem_solver_create",
+    "fem_solver_destroy", "fem_solver_bind", "fem_solver_unbind", "fem_solver_solve",
+    "fem_gauss_points",
+]
+
+
+class FemError(RuntimeError):
+    pass
+
+
+class OpDesc(C.Structure):
+    _fields_ = [("kind", C.c_int), ("n", C.c_int * 3), ("order", C.c_int), ("beta", C.c_double),
+                ("b", C.c_double * 3), ("c", C.c_double), ("kappa_mode", C.c_int),
+                ("kappa_const", C.c_double), ("precision", C.c_int)]
+
+
+class SolverParams(C.Structure):
+    _fields_ = [("method", C.c_int), ("tolerance", C.c_double), ("errormeasure", C.c_int),
+                ("maxiterations", C.c_int), ("restart", C.c_int), ("verbose", C.c_int),
+                ("refinement", C.c_int), ("inner_tolerance", C.c_double),
+                ("inner_maxiterations", C.c_int)]
+
+
+class SolverInfo(C.Structure):
+    _fields_ = [("converged", C.c_int), ("iterations", C.c_int), ("outer_iterations", C.c_int),
+                ("true_rel_residual", C.c_double), ("time_s", C.c_double)]
+
+
+_lib = None
+_dp = C.POINTER(C.c_double)
+_vp = C.c_void_p
+
+
+def load_library(path=LIBPATH):
+    """Load the C ABI library (no compute calls are made here)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FemError("libdunefem_amd.so not built (run __graft_entry__.build())")
+    L = C.CDLL(path)
+    sig = {
+        "fem_last_error": ([], C.c_char_p),
+        "fem_version": ([], C.c_int),
+        "fem_ctx_create": ([C.c_int, C.POINTER(_vp)], C.c_int),
+        "fem_ctx_destroy": ([_vp], C.c_int),
+        "fem_ctx_synchronize": ([_vp], C.c_int),
+        "fem_ctx_mark": ([_vp, C.c_int], C.c_int),
+        "fem_ctx_elapsed_ms": ([_vp, C.POINTER(C.c_float)], C.c_int),
+        "fem_vec_create": ([_vp, C.c_int64, C.c_int, C.POINTER(_vp)], C.c_int),
+        "fem_vec_destroy": ([_vp], C.c_int),
+        "fem_vec_size": ([_vp], C.c_int64),
+        "fem_vec_precision": ([_vp], C.c_int),
+        "fem_vec_device_ptr": ([_vp], _vp),
+        "fem_vec_upload": ([_vp, _dp], C.c_int),
+        "fem_vec_download": ([_vp, _dp], C.c_int),
+        "fem_vec_upload_f32": ([_vp, C.POINTER(C.c_float)], C.c_int),
+        "fem_vec_fill": ([_vp, C.c_double], C.c_int),
+        "fem_vec_copy": ([_vp, _vp], C.c_int),
+        "fem_vec_axpy": ([_vp, C.c_double, _vp], C.c_int),
+        "fem_vec_scal": ([_vp, C.c_double], C.c_int),
+        "fem_vec_dot": ([_vp, _vp, _dp], C.c_int),
+        "fem_vec_norm2sq": ([_vp, _dp], C.c_int),
+        "fem_op_create": ([_vp, C.POINTER(OpDesc), C.POINTER(_vp)], C.c_int),
+        "fem_op_destroy": ([_vp], C.c_int),
+        "fem_op_ndofs": ([_vp], C.c_int64),
+        "fem_op_set_kappa_qp": ([_vp, _dp, _dp, _dp, _dp], C.c_int),
+        "fem_op_set_kappa_cell": ([_vp, _dp], C.c_int),
+        "fem_op_apply": ([_vp, _vp, _vp], C.c_int),
+        "fem_op_assemble_rhs": ([_vp, _dp, _vp], C.c_int),
+        "fem_op_diagonal": ([_vp, _vp], C.c_int),
+        "fem_precond_create": ([_vp, C.c_int, C.POINTER(_vp)], C.c_int),
+        "fem_precond_destroy": ([_vp], C.c_int),
+        "fem_precond_apply": ([_vp, _vp, _vp], C.c_int),
+        "fem_solver_default_params": ([C.POINTER(SolverParams)], None),
+        "fem_solver_params_set": ([C.POINTER(SolverParams), C.c_char_p, C.c_char_p], C.c_int),
+        "fem_solver_create": ([_vp, C.POINTER(SolverParams), C.POINTER(_vp)], C.c_int),
+        "fem_solver_destroy": ([_vp], C.c_int),
+        "fem_solver_bind": ([_vp, _vp, _vp, _vp, _vp], C.c_int),
+        "fem_solver_unbind": ([_vp], C.c_int),
+        "fem_solver_solve": ([_vp, _vp, _vp, C.POINTER(SolverInfo)], C.c_int),
+        "fem_gauss_points": ([C.c_int, _dp, _dp], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise FemError("dunefem_amd error %d: %s" % (rc, _lib.fem_last_error().decode()))
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def gauss_points(m):
+    """m-point Gauss-Legendre rule on [0,1] from the library (host-only call)."""
+    L = load_library()
+    x = np.zeros(m)
+    w = np.zeros(m)
+    _check(L.fem_gauss_points(m, _dptr(x), _dptr(w)))
+    return x, w
+
+
+class Context:
+    """Device context: one GPU, one stream."""
+
+    def __init__(self, device=0):
+        L = load_library()
+        h = _vp()
+        _check(L.fem_ctx_create(device, C.byref(h)))
+        self.h = h
+
+    def synchronize(self):
+        _check(_lib.fem_ctx_synchronize(self.h))
+
+    def mark(self, which):
+        _check(_lib.fem_ctx_mark(self.h, which))
+
+    def elapsed_ms(self):
+        ms = C.c_float()
+        _check(_lib.fem_ctx_elapsed_ms(self.h, C.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if self.h:
+            _check(_lib.fem_ctx_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DofVector:
+    """Device DoF vector (the dofVector of a discrete function)."""
+
+    def __init__(self, ctx, n, precision=F64, data=None):
+        self.ctx = ctx
+        h = _vp()
+        _check(_lib.fem_vec_create(ctx.h, int(n), precision, C.byref(h)))
+        self.h = h
+        self.n = int(n)
+        self.precision = precision
+        if data is not None:
+            self.upload(data)
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+        assert a.size == self.n
+        _check(_lib.fem_vec_upload(self.h, _dptr(a)))
+
+    def download(self):
+        out = np.empty(self.n)
+        _check(_lib.fem_vec_download(self.h, _dptr(out)))
+        return out
+
+    def numpy(self):
+        return self.download()
+
+    def clear(self):
+        _check(_lib.fem_vec_fill(self.h, 0.0))
+
+    def fill(self, v):
+        _check(_lib.fem_vec_fill(self.h, float(v)))
+
+    def assign(self, other):
+        _check(_lib.fem_vec_copy(self.h, other.h))
+
+    def axpy(self, a, x):
+        _check(_lib.fem_vec_axpy(self.h, float(a), x.h))
+
+    def scale(self, a):
+        _check(_lib.fem_vec_scal(self.h, float(a)))
+
+    def scalarProductDofs(self, other):
+        out = C.c_double()
+        _check(_lib.fem_vec_dot(self.h, other.h, C.byref(out)))
+        return out.value
+
+    def normSquaredDofs(self):
+        out = C.c_double()
+        _check(_lib.fem_vec_norm2sq(self.h, C.byref(out)))
+        return out.value
+
+    def device_ptr(self):
+        return _lib.fem_vec_device_ptr(self.h)
+
+    def __del__(self):
+        try:
+            if self.h:
+                _lib.fem_vec_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class GalerkinOperator:
+    """Matrix-free Galerkin operator; ``op(u, w)`` overwrites w with A u."""
+
+    def __init__(self, ctx, kind, n, order, beta=0.0, b=(0.0, 0.0, 0.0), c=0.0,
+                 kappa_mode=KAPPA_CONST, kappa_const=1.0, precision=F64):
+        self.ctx = ctx
+        d = OpDesc()
+        d.kind = kind
+        nn = list(n) + [1] * (3 - len(n))
+        d.n[:] = nn
+        d.order = order
+        d.beta = beta
+        d.b[:] = list(b)
+        d.c = c
+        d.kappa_mode = kappa_mode
+        d.kappa_const = kappa_const
+        d.precision = precision
+        h = _vp()
+        _check(_lib.fem_op_create(ctx.h, C.byref(d), C.byref(h)))
+        self.h = h
+        self.desc = d
+        self.precision = precision
+        self.vec_precision = F64 if precision == F64 else F32
+        self.N = _lib.fem_op_ndofs(h)
+
+    def set_kappa_qp(self, kvol, kfaces=None):
+        kvol = np.ascontiguousarray(kvol, dtype=np.float64)
+        if kfaces is None:
+            _check(_lib.fem_op_set_kappa_qp(self.h, _dptr(kvol), None, None, None))
+        else:
+            kf = [np.ascontiguousarray(k, dtype=np.float64) for k in kfaces]
+            _check(_lib.fem_op_set_kappa_qp(self.h, _dptr(kvol), _dptr(kf[0]), _dptr(kf[1]), _dptr(kf[2])))
+
+    def set_kappa_cell(self, kcell):
+        kcell = np.ascontiguousarray(kcell, dtype=np.float64)
+        _check(_lib.fem_op_set_kappa_cell(self.h, _dptr(kcell)))
+
+    def vector(self, data=None):
+        return DofVector(self.ctx, self.N, self.vec_precision, data)
+
+    def __call__(self, u, w):
+        _check(_lib.fem_op_apply(self.h, u.h, w.h))
+
+    def rhs(self, fq, F=None):
+        fq = np.ascontiguousarray(fq, dtype=np.float64)
+        F = F if F is not None else self.vector()
+        _check(_lib.fem_op_assemble_rhs(self.h, _dptr(fq), F.h))
+        return F
+
+    def diagonal(self, d=None):
+        d = d if d is not None else self.vector()
+        _check(_lib.fem_op_diagonal(self.h, d.h))
+        return d
+
+    def nonlinear(self):
+        return False
+
+    def __del__(self):
+        try:
+            if self.h:
+                _lib.fem_op_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class Preconditioner:
+    def __init__(self, op, kind):
+        self.op = op
+        h = _vp()
+        _check(_lib.fem_precond_create(op.h, kind, C.byref(h)))
+        self.h = h
+
+    def __call__(self, x, y):
+        _check(_lib.fem_precond_apply(self.h, x.h, y.h))
+
+    def __del__(self):
+        try:
+            if self.h:
+                _lib.fem_precond_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def solver_parameters(**kw):
+    p = SolverParams()
+    load_library().fem_solver_default_params(C.byref(p))
+    for k, v in kw.items():
+        if k in ("method", "errormeasure", "maxiterations", "restart", "verbose", "refinement",
+                 "inner_maxiterations"):
+            setattr(p, k, int(v))
+        elif k in ("tolerance", "inner_tolerance"):
+            setattr(p, k, float(v))
+        else:
+            _check(_lib.fem_solver_params_set(C.byref(p), k.encode(), str(v).encode()))
+    return p
+
+
+class InverseOperator:
+    """Krylov inverse operator (CgInverseOperator / GmresInverseOperator / FGMRES)."""
+
+    def __init__(self, ctx, parameters=None, **kw):
+        self.ctx = ctx
+        p = parameters if parameters is not None else solver_parameters(**kw)
+        self.params = p
+        h = _vp()
+        _check(_lib.fem_solver_create(ctx.h, C.byref(p), C.byref(h)))
+        self.h = h
+        self._info = SolverInfo()
+        self._bound = None
+
+    def bind(self, op, pre=None, inner_op=None, inner_pre=None):
+        _check(_lib.fem_solver_bind(self.h, op.h, pre.h if pre else None,
+                                    inner_op.h if inner_op else None,
+                                    inner_pre.h if inner_pre else None))
+        self._bound = (op, pre, inner_op, inner_pre)
+
+    def unbind(self):
+        _check(_lib.fem_solver_unbind(self.h))
+        self._bound = None
+
+    def __call__(self, b, x):
+        _check(_lib.fem_solver_solve(self.h, b.h, x.h, C.byref(self._info)))
+
+    def iterations(self):
+        return self._info.iterations
+
+    def info(self):
+        i = self._info
+        return {"converged": bool(i.converged), "iterations": i.iterations,
+                "linear_iterations": i.iterations, "outer_iterations": i.outer_iterations,
+                "true_rel_residual": i.true_rel_residual, "timing": [i.time_s]}
+
+    def __del__(self):
+        try:
+            if self.h:
+                _lib.fem_solver_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass

@@ -1,0 +1,205 @@
+/*
+ * dunefem_amd.h -- C ABI of the MI355X-native matrix-free Galerkin apply and
+ * Krylov solve library.
+ *
+ * Every entry point returns an int status: 0 on success, < 0 on error (the
+ * message is available from fem_last_error()).  Nothing throws across this
+ * boundary.  Non-convergence of a solver is not an error: it is reported as
+ * a negative iteration count, as the reference does
+ * (dune/fem/solver/linear/cg.hh:116, dune/fem/solver/linear/gmres.hh:298).
+ *
+ * Handles are opaque and owned by the caller (create/destroy).  All device
+ * work of one context is ordered on that context's HIP stream; calls that
+ * return scalars to the host synchronise that stream.  A context is used by
+ * one host thread at a time.
+ */
+#ifndef DUNEFEM_AMD_H
+#define DUNEFEM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------- */
+#define FEM_OK 0
+#define FEM_ERR_ARG (-1)      /* invalid argument / shape mismatch           */
+#define FEM_ERR_HIP (-2)      /* HIP runtime error                           */
+#define FEM_ERR_STATE (-3)    /* wrong call order (e.g. solve before bind)   */
+#define FEM_ERR_NOMEM (-4)    /* allocation failed                           */
+#define FEM_ERR_UNSUPPORTED (-5)
+
+/* ---- precisions of DoF vectors and operator arithmetic -------------------- */
+#define FEM_F64 0            /* IEEE double                                  */
+#define FEM_F32 1            /* IEEE single                                  */
+#define FEM_BF16MFMA 2       /* operator only: FP32 data, BF16 MFMA contractions, FP32 accumulation */
+
+typedef struct fem_ctx fem_ctx;
+typedef struct fem_vec fem_vec;
+typedef struct fem_op fem_op;
+typedef struct fem_precond fem_precond;
+typedef struct fem_solver fem_solver;
+
+const char *fem_last_error(void);
+int fem_version(void);
+
+/* ---- context ---------------------------------------------------------------
+ * One device + stream (replaces the MPIManager thread pool as the place where
+ * parallel work runs: dune/fem/misc/mpimanager.hh:155-198). */
+int fem_ctx_create(int device, fem_ctx **out);
+int fem_ctx_destroy(fem_ctx *ctx);
+int fem_ctx_synchronize(fem_ctx *ctx);
+/* elapsed milliseconds between two event marks recorded on the context stream */
+int fem_ctx_mark(fem_ctx *ctx, int which /* 0 start, 1 stop */);
+int fem_ctx_elapsed_ms(fem_ctx *ctx, float *ms);
+
+/* ---- DoF vectors -----------------------------------------------------------
+ * Device-resident contiguous DoF array; replaces the DoF storage of
+ * AdaptiveDiscreteFunction / MutableBlockVector<DynamicArray<double>>
+ * (dune/fem/function/adaptivefunction/adaptivefunction.hh:45-61,
+ *  dune/fem/function/blockvectors/defaultblockvectors.hh:444-505). */
+int fem_vec_create(fem_ctx *ctx, int64_t n, int precision, fem_vec **out);
+int fem_vec_destroy(fem_vec *v);
+int64_t fem_vec_size(const fem_vec *v);
+int fem_vec_precision(const fem_vec *v);
+void *fem_vec_device_ptr(fem_vec *v);
+/* host <-> device copies with conversion from/to double */
+int fem_vec_upload(fem_vec *v, const double *host);
+int fem_vec_download(const fem_vec *v, double *host);
+int fem_vec_upload_f32(fem_vec *v, const float *host);
+/* DiscreteFunctionInterface BLAS-1 (dune/fem/function/common/discretefunction.hh:731-835,
+ * defaultblockvectors.hh:110-135): clear, assign, axpy, scaling, scalarProductDofs,
+ * normSquaredDofs (scalarproducts.hh:101-127).  Mixed precisions convert. */
+int fem_vec_fill(fem_vec *v, double value);               /* clear() = fill 0 */
+int fem_vec_copy(fem_vec *dst, const fem_vec *src);       /* assign()         */
+int fem_vec_axpy(fem_vec *y, double a, const fem_vec *x); /* y += a x         */
+int fem_vec_scal(fem_vec *x, double a);                   /* x *= a           */
+int fem_vec_dot(const fem_vec *x, const fem_vec *y, double *out);
+int fem_vec_norm2sq(const fem_vec *x, double *out);
+
+/* ---- operators ---------------------------------------------------------------
+ * Matrix-free Galerkin operator on a structured Cartesian mesh of the unit
+ * square/cube.  Replaces GalerkinOperator<Integrands,DF,DF>::operator()(u,w)
+ * (dune/fem/schemes/galerkin.hh:1430-1433,1460-1496; the element/intersection
+ * loop :811-917) for the integrands of the SIPG/upwind form
+ * (dune/fempy/test/parallelTest.py:113-129, pydemo/advectiondiffusion.py:30-56)
+ * and of the continuous Lagrange form with strong Dirichlet constraints
+ * (DirichletWrapperOperator, dune/fem/schemes/dirichletwrapper.hh:101-105). */
+#define FEM_OP_DG_SIPG_3D 1   /* discontinuous Legendre Q_p on hexes, weak Dirichlet  */
+#define FEM_OP_CG_Q2_2D 2     /* continuous Lagrange Q2 on quads, strong u = 0        */
+
+#define FEM_KAPPA_CONST 0     /* constant diffusion coefficient               */
+#define FEM_KAPPA_QP 1        /* pointwise: volume + face quadrature values   */
+#define FEM_KAPPA_CELL 2      /* one value per element                        */
+
+typedef struct {
+    int kind;            /* FEM_OP_*                                          */
+    int n[3];            /* cells per direction (n[2] ignored in 2D)          */
+    int order;           /* polynomial order p (CG: must be 2)                */
+    double beta;         /* SIPG penalty factor, e.g. 10 p^2                  */
+    double b[3];         /* constant advection field                          */
+    double c;            /* reaction coefficient (u v term)                   */
+    int kappa_mode;      /* FEM_KAPPA_*                                       */
+    double kappa_const;
+    int precision;       /* FEM_F64, FEM_F32 or FEM_BF16MFMA                   */
+} fem_op_desc;
+
+int fem_op_create(fem_ctx *ctx, const fem_op_desc *desc, fem_op **out);
+int fem_op_destroy(fem_op *op);
+int64_t fem_op_ndofs(const fem_op *op);
+/* coefficient data (host arrays, copied).  Layouts:
+ *  volume qp  [E][m^d], qp index dim 0 fastest
+ *  face qp    per normal direction d: faces on the element grid extended by
+ *             one in d (x fastest), [faces_d][m^(d-1)], tangential dims in
+ *             increasing order, lower fastest
+ *  cell       [E] */
+int fem_op_set_kappa_qp(fem_op *op, const double *kvol, const double *kface_x,
+                        const double *kface_y, const double *kface_z);
+int fem_op_set_kappa_cell(fem_op *op, const double *kcell);
+/* Operator::operator()(u, w): w = A u, w overwritten
+ * (dune/fem/operator/common/operator.hh:55; w.clear() at galerkin.hh:1463).
+ * u and w must have the operator's precision class (F64 for FEM_F64, F32 otherwise). */
+int fem_op_apply(fem_op *op, const fem_vec *u, fem_vec *w);
+/* Affine part: load vector F_i = int f phi_i from f at the volume quadrature
+ * points [E][m^d] (host array); Dirichlet entries are zero.  The reference's
+ * GalerkinOperator applies R(u) = A u - F (galerkin.hh:333-360, source term). */
+int fem_op_assemble_rhs(fem_op *op, const double *fq, fem_vec *F);
+/* operator diagonal, matrix-free (takes the place of assembling the Jacobian
+ * and SparseRowMatrix::extractDiagonal, dune/fem/operator/matrix/spmatrix.hh:846,
+ * used by FemPreconditioning, dune/fem/solver/fempreconditioning.hh:85-107) */
+int fem_op_diagonal(fem_op *op, fem_vec *d);
+
+/* ---- preconditioners -------------------------------------------------------- */
+#define FEM_PRE_NONE 0
+#define FEM_PRE_JACOBI 1        /* d^-1 with |d| < 16 eps -> 1 (fempreconditioning.hh:99-106) */
+#define FEM_PRE_BLOCK_JACOBI 2  /* DG element blocks A_KK^-1                        */
+#define FEM_PRE_LEARNED 3       /* CNN coarse correction + block Jacobi (FGMRES only) */
+
+int fem_precond_create(fem_op *op, int kind, fem_precond **out);
+int fem_precond_destroy(fem_precond *p);
+/* y = M^-1 x */
+int fem_precond_apply(fem_precond *p, const fem_vec *x, fem_vec *y);
+
+/* ---- inverse operators -------------------------------------------------------
+ * Replaces InverseOperatorInterface / KrylovInverseOperator
+ * (dune/fem/solver/inverseoperatorinterface.hh:33-239,
+ *  dune/fem/solver/krylovinverseoperators.hh:46-295) with the CG and GMRES
+ * recurrences of dune/fem/solver/linear/{cg,gmres}.hh and a flexible GMRES. */
+#define FEM_SOLVER_CG 0
+#define FEM_SOLVER_GMRES 1
+#define FEM_SOLVER_FGMRES 2
+
+#define FEM_ERRMEASURE_ABSOLUTE 0
+#define FEM_ERRMEASURE_RELATIVE 1
+#define FEM_ERRMEASURE_REDUCTION 2
+
+typedef struct {
+    int method;          /* fem.solver.method                                 */
+    double tolerance;    /* fem.solver.tolerance (default 1e-8)               */
+    int errormeasure;    /* fem.solver.errormeasure (default absolute)        */
+    int maxiterations;   /* fem.solver.maxiterations                          */
+    int restart;         /* fem.solver.gmres.restart (default 20)             */
+    int verbose;         /* fem.solver.verbose                                */
+    /* mixed precision: 0 = plain (solve in the operator's precision);
+     * 1 = FP64 iterative refinement around an inner solve in `inner_op`'s
+     * precision; the true residual, inner products of the outer loop and the
+     * update are FP64 */
+    int refinement;
+    double inner_tolerance;   /* relative reduction of each inner solve        */
+    int inner_maxiterations;
+} fem_solver_params;
+
+typedef struct {
+    int converged;
+    int iterations;          /* signed: < 0 not converged (reference semantics) */
+    int outer_iterations;    /* refinement steps (0 for a plain solve)         */
+    double true_rel_residual; /* ||b - A x||_2 / ||b||_2 in FP64                */
+    double time_s;
+} fem_solver_info;
+
+void fem_solver_default_params(fem_solver_params *p);
+/* key/value setter mirroring the fem.solver.* keys of SolverParameter
+ * (dune/fem/solver/parameter.hh:71-250): "fem.solver.method" (cg|gmres|fgmres),
+ * "fem.solver.tolerance", "fem.solver.errormeasure" (absolute|relative|
+ * residualreduction), "fem.solver.maxiterations", "fem.solver.gmres.restart",
+ * "fem.solver.verbose"; the "linear." prefix is accepted as well */
+int fem_solver_params_set(fem_solver_params *p, const char *key, const char *value);
+int fem_solver_create(fem_ctx *ctx, const fem_solver_params *params, fem_solver **out);
+int fem_solver_destroy(fem_solver *s);
+/* bind(op[, precond]) / unbind() (inverseoperatorinterface.hh:127-151).
+ * For refinement, inner_op is the low-precision operator (may be NULL). */
+int fem_solver_bind(fem_solver *s, fem_op *op, fem_precond *pre, fem_op *inner_op,
+                    fem_precond *inner_pre);
+int fem_solver_unbind(fem_solver *s);
+/* operator()(b, x): x is the initial guess on entry, the solution on exit */
+int fem_solver_solve(fem_solver *s, const fem_vec *b, fem_vec *x, fem_solver_info *info);
+
+/* ---- quadrature (shared with hosts that generate coefficient data) ---------- */
+/* m-point Gauss-Legendre rule on [0,1], ascending points */
+int fem_gauss_points(int m, double *x, double *w);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,184 @@
+"""GPU parity of the matrix-free applies against the CPU oracle (through the C ABI).
+
+Bar: FP64 apply within 1e-12 relative (max-norm, relative to max |w|) of the
+oracle on the same inputs; FP32 apply within 1e-5.
+"""
+import numpy as np
+import pytest
+
+import dunefem_amd as fa
+import pyoracle as ora
+from dunefem_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+TOL32 = 2e-5
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def random_qp_kappa(rng, n, p):
+    m = p + 1
+    E = n[0] * n[1] * n[2]
+    kvol = rng.uniform(0.5, 2.0, size=(E, m ** 3))
+    kf = tuple(rng.uniform(0.5, 2.0, size=((n[0] + (d == 0)) * (n[1] + (d == 1)) * (n[2] + (d == 2)), m * m))
+               for d in range(3))
+    return kvol, kf
+
+
+def make_dg(ctx, n, p, beta, b, c, kappa, precision=fa.F64):
+    if isinstance(kappa, tuple) and kappa[0] == "qp":
+        op = fa.GalerkinOperator(ctx, fa.OP_DG_SIPG_3D, n, p, beta, b, c, fa.KAPPA_QP, precision=precision)
+        op.set_kappa_qp(kappa[1], kappa[2])
+    elif isinstance(kappa, tuple) and kappa[0] == "cell":
+        op = fa.GalerkinOperator(ctx, fa.OP_DG_SIPG_3D, n, p, beta, b, c, fa.KAPPA_CELL, precision=precision)
+        op.set_kappa_cell(kappa[1])
+    else:
+        op = fa.GalerkinOperator(ctx, fa.OP_DG_SIPG_3D, n, p, beta, b, c, fa.KAPPA_CONST, kappa, precision=precision)
+    return op
+
+
+CASES = [
+    # n, p, b, c, kappa kind
+    ((1, 1, 1), 1, (0, 0, 0), 0.0, "const"),
+    ((2, 3, 4), 1, (1, .5, .25), 0.0, "qp"),
+    ((3, 2, 2), 2, (0, 0, 0), 0.0, "cell"),
+    ((4, 4, 4), 2, (1, -.5, .25), 0.3, "qp"),
+    ((5, 3, 2), 3, (1, .5, .25), 0.0, "qp"),
+    ((3, 4, 5), 3, (0, 0, 0), 0.0, "cell"),
+    ((2, 2, 3), 4, (.3, .5, -1), 0.0, "qp"),
+    ((3, 3, 3), 4, (0, 0, 0), 1.0, "cell"),
+]
+
+
+@pytest.mark.parametrize("n,p,b,c,kind", CASES)
+def test_dg_apply_fp64_matches_oracle(ctx, n, p, b, c, kind):
+    rng = np.random.default_rng(hash((n, p, kind)) % 2 ** 32)
+    E = n[0] * n[1] * n[2]
+    if kind == "qp":
+        kv, kf = random_qp_kappa(rng, n, p)
+        kappa = ("qp", kv, kf)
+    elif kind == "cell":
+        kappa = ("cell", rng.uniform(0.1, 10, size=E))
+    else:
+        kappa = 1.3
+    beta = 10.0 * p * p
+    oop = ora.DGOperator(n, p, beta, b, c, kappa=kappa)
+    gop = make_dg(ctx, n, p, beta, b, c, kappa)
+    assert gop.N == oop.N
+    u = rng.normal(size=oop.N)
+    wref = oop.apply(u)
+    U, W = gop.vector(u), gop.vector()
+    W.fill(np.nan)   # w is overwritten, not accumulated
+    gop(U, W)
+    w = W.download()
+    print("case", n, p, kind, "rel err", rel(w, wref), flush=True)
+    assert rel(w, wref) < TOL64
+    # FP32 operator on the same data
+    gop32 = make_dg(ctx, n, p, beta, b, c, kappa, precision=fa.F32)
+    U32, W32 = gop32.vector(u), gop32.vector()
+    gop32(U32, W32)
+    assert rel(W32.download(), wref) < TOL32
+
+
+def test_dg_diagonal_and_rhs_match_oracle(ctx):
+    rng = np.random.default_rng(11)
+    n, p = (3, 2, 4), 2
+    kv, kf = random_qp_kappa(rng, n, p)
+    kappa = ("qp", kv, kf)
+    oop = ora.DGOperator(n, p, 40.0, (1, .5, .25), 0.2, kappa=kappa)
+    gop = make_dg(ctx, n, p, 40.0, (1, .5, .25), 0.2, kappa)
+    # diagonal = A e_i . e_i
+    d = gop.diagonal().download()
+    N = oop.N
+    dref = np.empty(N)
+    e = np.zeros(N)
+    for i in range(N):
+        e[:] = 0
+        e[i] = 1
+        dref[i] = oop.apply(e)[i]
+    assert rel(d, dref) < TOL64
+    fq = rng.normal(size=(24, 27))
+    assert rel(gop.rhs(fq).download(), oop.rhs(fq)) < TOL64
+
+
+def test_problem1_subdomain_apply_matches_oracle(ctx):
+    # the benchmark workload generator at reduced size (16^3, Q3, pointwise kappa, b)
+    P = wl.problem1(n=16)
+    oop = ora.DGOperator(P["n"], P["p"], P["beta"], P["b"], 0.0, kappa=("qp", P["kvol"], tuple(P["kfaces"])))
+    gop = make_dg(ctx, P["n"], P["p"], P["beta"], P["b"], 0.0, ("qp", P["kvol"], tuple(P["kfaces"])))
+    wref = oop.apply(P["u"], nthreads=8)
+    U, W = gop.vector(P["u"]), gop.vector()
+    gop(U, W)
+    err = rel(W.download(), wref)
+    print("problem1 16^3 rel err", err, flush=True)
+    assert err < TOL64
+
+
+def test_problem1_full_size_properties(ctx):
+    # 64^3 Q3 (16.8M DoFs): linearity and the transpose identity <A u, v> - <u, A v>
+    # equals the advection part only (A_diff symmetric), checked with the GPU alone
+    P = wl.problem1(n=64)
+    kappa = ("qp", P["kvol"], tuple(P["kfaces"]))
+    gop = make_dg(ctx, P["n"], P["p"], P["beta"], P["b"], 0.0, kappa)
+    gdiff = make_dg(ctx, P["n"], P["p"], P["beta"], (0, 0, 0), 0.0, kappa)
+    rng = np.random.default_rng(5)
+    u = P["u"]
+    v = rng.normal(size=u.size)
+    U, V, W1, W2, W3 = gop.vector(u), gop.vector(v), gop.vector(), gop.vector(), gop.vector()
+    gop(U, W1)
+    gop(V, W2)
+    S = gop.vector(2.0 * u - 3.0 * v)
+    gop(S, W3)
+    W3.axpy(-2.0, W1)
+    W3.axpy(3.0, W2)
+    assert np.sqrt(W3.normSquaredDofs()) < 1e-12 * np.sqrt(W1.normSquaredDofs())
+    # diffusion part symmetric at full size
+    gdiff(U, W1)
+    gdiff(V, W2)
+    a, b = W1.scalarProductDofs(V), U.scalarProductDofs(W2)
+    assert abs(a - b) < 1e-11 * abs(a)
+
+
+def test_cg_q2_apply_diag_rhs_match_oracle(ctx):
+    rng = np.random.default_rng(12)
+    for n, kind in [((1, 1), "const"), ((3, 5), "qp"), ((17, 33), "qp"), ((64, 64), "const"), ((40, 31), "qp")]:
+        E = n[0] * n[1]
+        kv = rng.uniform(0.3, 3.0, size=(E, 9)) if kind == "qp" else None
+        oop = ora.CGOperator(n, c=0.7, kappa=kv if kv is not None else 1.0)
+        gop = fa.GalerkinOperator(ctx, fa.OP_CG_Q2_2D, n, 2, c=0.7,
+                                  kappa_mode=fa.KAPPA_QP if kv is not None else fa.KAPPA_CONST)
+        if kv is not None:
+            gop.set_kappa_qp(kv)
+        u = rng.normal(size=oop.N)
+        U, W = gop.vector(u), gop.vector()
+        W.fill(np.nan)
+        gop(U, W)
+        print("cg case", n, kind, rel(W.download(), oop.apply(u)), flush=True)
+        assert rel(W.download(), oop.apply(u)) < TOL64
+        assert rel(gop.diagonal().download(), oop.diag()) < TOL64
+        fq = rng.normal(size=(E, 9))
+        assert rel(gop.rhs(fq).download(), oop.rhs(fq)) < TOL64
+
+
+def test_blas_and_dot_deterministic(ctx):
+    rng = np.random.default_rng(13)
+    for n in [0, 1, 7, 1000, 123457, 3_000_001]:
+        a, b = rng.normal(size=n), rng.normal(size=n)
+        A, B = fa.DofVector(ctx, n, fa.F64, a), fa.DofVector(ctx, n, fa.F64, b)
+        if n == 0:
+            assert A.scalarProductDofs(B) == 0.0
+            continue
+        d1 = A.scalarProductDofs(B)
+        d2 = A.scalarProductDofs(B)
+        assert d1 == d2   # fixed reduction order
+        assert abs(d1 - a @ b) < 1e-12 * np.abs(a).dot(np.abs(b))
+        A.axpy(0.5, B)
+        A.scale(2.0)
+        assert np.abs(A.download() - 2 * (a + 0.5 * b)).max() < 1e-12
+        A32 = fa.DofVector(ctx, n, fa.F32)
+        A32.assign(B)
+        assert np.abs(A32.download() - b.astype(np.float32)).max() == 0
