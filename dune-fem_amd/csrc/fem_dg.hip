@@ -62,263 +62,369 @@ __device__ __forceinline__ T dotk(const T *v, const T *in)
 template <int K>
 struct DGShape {
     static constexpr int K2 = K * K, K3 = K * K * K;
-    // elements per workgroup: fill one or two wavefronts
-    static constexpr int EPB = (K == 2) ? 16 : (K == 3) ? 7 : (K == 4) ? 4 : (K == 5) ? 5 : (K == 6) ? 3 : 2;
+    // elements per workgroup: all threads of a workgroup form one wavefront,
+    // so intra-element exchanges through LDS need no workgroup barrier
+    static constexpr int EPB = 64 / K2;
     static constexpr int THREADS = EPB * K2;
-    // LDS per element: A (K^3), R (K^3), F (6 faces x 2 x K^2), NS and T2
-    // (2 faces x 2 x K^2 each)
-    static constexpr int SZ = 2 * K3 + 12 * K2 + 8 * K2;
+    // LDS tensor layout (a, b, c) -> a*S1 + b*K + swz(c, a).  For K = 4 the
+    // stride S1 = 20 plus the XOR swizzle make the x-, y- and z-pencil
+    // accesses of the 16 lanes of an element hit 16 distinct 8-byte banks.
+    static constexpr int S1 = (K == 4) ? 20 : K2 + 1;
+    static constexpr int TS = K * S1;           // one padded K^3 tensor
+    // per element: A (u at quadrature points), R (residual at quadrature
+    // points), S (neighbour-trace scratch, 4 K^2); the element stride is
+    // padded to an odd multiple of 16 doubles to separate the elements' banks
+    static constexpr int SZ0 = 2 * TS + 4 * K2;
+    static constexpr int SZ = ((SZ0 + 15) / 16) * 16 + (((SZ0 + 15) / 16) % 2 == 0 ? 16 : 0);
 };
 
-template <typename T>
-__device__ __forceinline__ T kappa_vol(const DGArgs<T> &a, int64_t e, int q, int K3)
+template <int K>
+__device__ __forceinline__ int lidx(int a, int b, int c)
 {
-    if (a.kappa_mode == FEM_KAPPA_QP) return a.kvol[e * K3 + q];
-    if (a.kappa_mode == FEM_KAPPA_CELL) return a.kcell[e];
-    return a.kappa_const;
+    using S = DGShape<K>;
+    return a * S::S1 + b * K + ((K == 4) ? (c ^ a) : c);
+}
+
+// Orders LDS accesses between the lanes of one wavefront.  LDS instructions of
+// a wavefront execute in issue order, so only the compiler must be kept from
+// moving memory operations across this point; no s_barrier and no wait on
+// outstanding global loads is needed.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Legendre-basis contractions use the parity of the basis on the symmetric
+// Gauss points, B[K-1-q][a] = (-1)^a B[q][a]: out[q] and out[K-1-q] share the
+// even and odd partial sums.
+template <int K, typename T>
+__device__ __forceinline__ void mvB(const T (&M)[KMAX][KMAX], const T *in, T *out)
+{
+#pragma unroll
+    for (int q = 0; q < K / 2; ++q) {
+        T ev = 0, od = 0;
+#pragma unroll
+        for (int a = 0; a < K; a += 2) ev = fma(M[q][a], in[a], ev);
+#pragma unroll
+        for (int a = 1; a < K; a += 2) od = fma(M[q][a], in[a], od);
+        out[q] = ev + od;
+        out[K - 1 - q] = ev - od;
+    }
+    if (K % 2) {
+        T ev = 0;
+#pragma unroll
+        for (int a = 0; a < K; a += 2) ev = fma(M[K / 2][a], in[a], ev);
+        out[K / 2] = ev;
+    }
 }
 
 template <int K, typename T>
-__global__ void __launch_bounds__(DGShape<K>::THREADS)
+__device__ __forceinline__ void mvBT(const T (&M)[KMAX][KMAX], const T *in, T *out)
+{
+    T sp[(K + 1) / 2], sm[(K + 1) / 2];
+#pragma unroll
+    for (int q = 0; q < K / 2; ++q) {
+        sp[q] = in[q] + in[K - 1 - q];
+        sm[q] = in[q] - in[K - 1 - q];
+    }
+#pragma unroll
+    for (int a = 0; a < K; ++a) {
+        T acc = (K % 2 && a % 2 == 0) ? M[K / 2][a] * in[K / 2] : T(0);
+#pragma unroll
+        for (int q = 0; q < K / 2; ++q) acc = fma(M[q][a], (a % 2) ? sm[q] : sp[q], acc);
+        out[a] = acc;
+    }
+}
+
+template <int K, typename T>
+__global__ void __launch_bounds__(64)
     k_dg_apply(const DGArgs<T> a, const T *__restrict__ u, T *__restrict__ w, const int *__restrict__ skip)
 {
     using S = DGShape<K>;
-    constexpr int K2 = S::K2, K3 = S::K3, EPB = S::EPB;
+    constexpr int K2 = S::K2, K3 = S::K3, EPB = S::EPB, TS = S::TS;
     if (skip && *skip) return;
     __shared__ T smem[EPB * S::SZ];
     const Tab1D<T> &tb = *a.tab;
 
     const int tid = threadIdx.x;
-    const int el = tid / K2, t = tid % K2, i = t / K, j = t % K;
-    const int64_t E = (int64_t)a.n[0] * a.n[1] * a.n[2];
-    const unsigned blk = xcd_remap(blockIdx.x, gridDim.x);
-    const int64_t e_raw = (int64_t)blk * EPB + el;
-    const bool active = e_raw < E;
-    const int64_t e = active ? e_raw : 0;
-    const int ex = (int)(e % a.n[0]), ey = (int)((e / a.n[0]) % a.n[1]), ez = (int)(e / ((int64_t)a.n[0] * a.n[1]));
+    const int el0 = tid / K2;
+    const bool wr = el0 < EPB;   // lanes beyond EPB*K2 (K = 3, 5) only shadow element 0
+    const int el = wr ? el0 : 0;
+    const int t = tid % K2, i = t / K, j = t % K;
+    const int nx = a.n[0], ny = a.n[1], nz = a.n[2];
+    const int E = nx * ny * nz;
+    const int blk = (int)xcd_remap(blockIdx.x, gridDim.x);
+    const int e_raw = blk * EPB + el;
+    const bool active = wr && e_raw < E;
+    const int e = active ? e_raw : 0;
+    const int ex = e % nx, ey = (e / nx) % ny, ez = e / (nx * ny);
     const int ec[3] = { ex, ey, ez };
-    const int64_t stride[3] = { 1, (int64_t)a.n[0], (int64_t)a.n[0] * a.n[1] };
+    const int stride[3] = { 1, nx, nx * ny };
 
     T *A = smem + el * S::SZ;
-    T *R = A + K3;
-    T *F = R + K3;        // [6][2][K2]
-    T *NS = F + 12 * K2;  // [2][2][K2]
-    T *T2 = NS + 4 * K2;  // [2][2][K2]
+    T *R = A + TS;
+    T *Sx = R + TS;   // [2 faces][2 quantities][K2]
 
-    const T ih[3] = { T(1) / a.h[0], T(1) / a.h[1], T(1) / a.h[2] };
+    // weights folded per thread: volume point (i, j) of a pencil, scaled by
+    // |K| / h_d^2 for the diffusion part; advection scaled by h_d
     const T detJ = a.h[0] * a.h[1] * a.h[2];
+    const T wij = tb.w[i] * tb.w[j];
     T v[K], tmp[K];
 
-    // ---- forward: modal -> quadrature points ------------------------------
-    const T *ue = u + e * K3;
-    // Z layout: thread (i=a, j=b) holds u[a][b][0..K)
+    // ---- all global loads are issued up front ------------------------------
+    // own DoFs (z pencil), the six neighbour pencils through this lane's
+    // tangential modes, and the coefficient values this lane needs; the
+    // neighbour data is contracted in the normal direction while the volume
+    // work runs, so its latency is hidden behind the volume phase
+    const T *ue = u + (size_t)e * K3;
 #pragma unroll
     for (int c = 0; c < K; ++c) v[c] = ue[i * K2 + j * K + c];
-    mv<K>(tb.B, v, tmp);
+    T nbr[3][2][K];
+    bool has[3][2];
 #pragma unroll
-    for (int q = 0; q < K; ++q) A[i * K2 + j * K + q] = tmp[q];
-    __syncthreads();
-    // Y layout: thread (i=a, j=qz)
+    for (int d = 0; d < 3; ++d)
 #pragma unroll
-    for (int b = 0; b < K; ++b) v[b] = A[i * K2 + b * K + j];
-    mv<K>(tb.B, v, tmp);
+        for (int s2 = 0; s2 < 2; ++s2) {
+            has[d][s2] = s2 ? ec[d] + 1 < a.n[d] : ec[d] > 0;
+            const T *un = u + (size_t)(has[d][s2] ? e + (s2 ? stride[d] : -stride[d]) : e) * K3;
 #pragma unroll
-    for (int q = 0; q < K; ++q) A[i * K2 + q * K + j] = tmp[q];
-    __syncthreads();
+            for (int m = 0; m < K; ++m) {
+                const int off = d == 0 ? m * K2 + i * K + j : d == 1 ? i * K2 + m * K + j : i * K2 + j * K + m;
+                nbr[d][s2][m] = un[off];
+            }
+        }
+    const bool kqp = a.kappa_mode == FEM_KAPPA_QP;
+    T kx[K], ky[K], kz[K], kf[3][2];
+    {
+        const T kel0 = a.kappa_mode == FEM_KAPPA_CELL ? a.kcell[e] : a.kappa_const;
+        const T *kv = kqp ? a.kvol + (size_t)e * K3 : nullptr;
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            kx[q] = kqp ? kv[q + K * i + K2 * j] : kel0;
+            ky[q] = kqp ? kv[i + K * q + K2 * j] : kel0;
+            kz[q] = kqp ? kv[i + K * j + K2 * q] : kel0;
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const int N0 = nx + (d == 0), N1 = ny + (d == 1);
+            const int fbase = ec[0] + N0 * (ec[1] + N1 * ec[2]);
+            const int fstep = d == 0 ? 1 : d == 1 ? nx : nx * ny;
+#pragma unroll
+            for (int s2 = 0; s2 < 2; ++s2) {
+                if (kqp) kf[d][s2] = a.kface[d][(size_t)(fbase + (s2 ? fstep : 0)) * K2 + i + K * j];
+                else if (a.kappa_mode == FEM_KAPPA_CELL)
+                    kf[d][s2] = has[d][s2] ? a.kcell[e + (s2 ? stride[d] : -stride[d])] : kel0;
+                else kf[d][s2] = kel0;
+            }
+        }
+    }
+    // normal contraction of the neighbour pencils: thread (i=m1, j=m2)
+    T nval[3][2], nder[3][2];
+
+    // ---- forward: modal -> quadrature points ------------------------------
+    // Z layout: thread (i=a, j=b) holds u[a][b][0..K)
+    mvB<K>(tb.B, v, tmp);
+    if (wr)
+#pragma unroll
+        for (int q = 0; q < K; ++q) A[lidx<K>(i, j, q)] = tmp[q];
+    wave_lds_sync();
+    // Y layout: thread (i=a, j=qz); each thread rewrites the pencil it read
+#pragma unroll
+    for (int b = 0; b < K; ++b) v[b] = A[lidx<K>(i, b, j)];
+    mvB<K>(tb.B, v, tmp);
+    wave_lds_sync();
+    if (wr)
+#pragma unroll
+        for (int q = 0; q < K; ++q) A[lidx<K>(i, q, j)] = tmp[q];
+    wave_lds_sync();
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int s2 = 0; s2 < 2; ++s2) {
+            T val = 0, der = 0;
+#pragma unroll
+            for (int m = 0; m < K; ++m) {
+                val = fma(tb.P[1 - s2][m], nbr[d][s2][m], val);
+                der = fma(tb.dP[1 - s2][m], nbr[d][s2][m], der);
+            }
+            nval[d][s2] = has[d][s2] ? val : T(0);
+            nder[d][s2] = has[d][s2] ? der : T(0);
+        }
     // X layout: thread (i=qy, j=qz)
     T uq[K];
 #pragma unroll
-    for (int b = 0; b < K; ++b) v[b] = A[b * K2 + i * K + j];
-    mv<K>(tb.B, v, uq);
+    for (int b = 0; b < K; ++b) v[b] = A[lidx<K>(b, i, j)];
+    mvB<K>(tb.B, v, uq);
+    wave_lds_sync();
+    if (wr)
 #pragma unroll
-    for (int q = 0; q < K; ++q) A[q * K2 + i * K + j] = uq[q];
+        for (int q = 0; q < K; ++q) A[lidx<K>(q, i, j)] = uq[q];
 
-    // x-direction volume flux and own x-face traces
+
+    // x-direction volume flux (kept in registers, X layout)
     T Rx[K];
     {
+        const T sc = wij * detJ / (a.h[0] * a.h[0]), bh = a.b[0] * a.h[0];
         T g[K], r[K];
         mv<K>(tb.Dt, uq, g);
 #pragma unroll
         for (int q = 0; q < K; ++q) {
-            const T W = tb.w[q] * tb.w[i] * tb.w[j] * detJ;
-            const T kap = kappa_vol(a, e, q + K * i + K2 * j, K3);
-            r[q] = W * (kap * g[q] * ih[0] * ih[0] - a.b[0] * uq[q] * ih[0]);
+            r[q] = (tb.w[q] * sc) * fma(kx[q], g[q], -bh * uq[q]);
         }
         mvT<K>(tb.Dt, r, Rx);
         if (a.c != T(0)) {
+            const T cm = wij * detJ * a.c;
 #pragma unroll
-            for (int q = 0; q < K; ++q) Rx[q] += tb.w[q] * tb.w[i] * tb.w[j] * detJ * a.c * uq[q];
-        }
-        const int idx = i + K * j;
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            F[((0 + s) * 2 + 0) * K2 + idx] = dotk<K>(tb.E[s], uq);
-            F[((0 + s) * 2 + 1) * K2 + idx] = dotk<K>(tb.dE[s], uq);
+            for (int q = 0; q < K; ++q) Rx[q] = fma(tb.w[q] * cm, uq[q], Rx[q]);
         }
     }
-    __syncthreads();
-    // Y layout of u_q: thread (i=qx, j=qz)
+    wave_lds_sync();
+    // Y layout of u_q: thread (i=qx, j=qz) -> R (y part)
     {
+        const T sc = wij * detJ / (a.h[1] * a.h[1]), bh = a.b[1] * a.h[1];
         T g[K], r[K];
 #pragma unroll
-        for (int q = 0; q < K; ++q) v[q] = A[i * K2 + q * K + j];
+        for (int q = 0; q < K; ++q) v[q] = A[lidx<K>(i, q, j)];
         mv<K>(tb.Dt, v, g);
 #pragma unroll
         for (int q = 0; q < K; ++q) {
-            const T W = tb.w[i] * tb.w[q] * tb.w[j] * detJ;
-            const T kap = kappa_vol(a, e, i + K * q + K2 * j, K3);
-            r[q] = W * (kap * g[q] * ih[1] * ih[1] - a.b[1] * v[q] * ih[1]);
+            r[q] = (tb.w[q] * sc) * fma(ky[q], g[q], -bh * v[q]);
         }
         mvT<K>(tb.Dt, r, tmp);
+        if (wr)
 #pragma unroll
-        for (int q = 0; q < K; ++q) R[i * K2 + q * K + j] = tmp[q];
-        const int idx = i + K * j;
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            F[((2 + s) * 2 + 0) * K2 + idx] = dotk<K>(tb.E[s], v);
-            F[((2 + s) * 2 + 1) * K2 + idx] = dotk<K>(tb.dE[s], v);
-        }
+            for (int q = 0; q < K; ++q) R[lidx<K>(i, q, j)] = tmp[q];
     }
-    __syncthreads();
-    // Z layout of u_q: thread (i=qx, j=qy)
+    wave_lds_sync();
+    // Z layout of u_q: thread (i=qx, j=qy) -> R += z part
     {
+        const T sc = wij * detJ / (a.h[2] * a.h[2]), bh = a.b[2] * a.h[2];
         T g[K], r[K];
 #pragma unroll
-        for (int q = 0; q < K; ++q) v[q] = A[i * K2 + j * K + q];
+        for (int q = 0; q < K; ++q) v[q] = A[lidx<K>(i, j, q)];
         mv<K>(tb.Dt, v, g);
 #pragma unroll
         for (int q = 0; q < K; ++q) {
-            const T W = tb.w[i] * tb.w[j] * tb.w[q] * detJ;
-            const T kap = kappa_vol(a, e, i + K * j + K2 * q, K3);
-            r[q] = W * (kap * g[q] * ih[2] * ih[2] - a.b[2] * v[q] * ih[2]);
+            r[q] = (tb.w[q] * sc) * fma(kz[q], g[q], -bh * v[q]);
         }
         mvT<K>(tb.Dt, r, tmp);
+        if (wr)
 #pragma unroll
-        for (int q = 0; q < K; ++q) R[i * K2 + j * K + q] += tmp[q];
-        const int idx = i + K * j;
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            F[((4 + s) * 2 + 0) * K2 + idx] = dotk<K>(tb.E[s], v);
-            F[((4 + s) * 2 + 1) * K2 + idx] = dotk<K>(tb.dE[s], v);
-        }
+            for (int q = 0; q < K; ++q) R[lidx<K>(i, j, q)] += tmp[q];
     }
 
-    // ---- faces: neighbour traces and numerical fluxes ----------------------
+    // ---- faces: neighbour traces, numerical fluxes, lifting into R ---------
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        // (1) normal contraction of the neighbour DoFs: thread (i=m1, j=m2)
+        wave_lds_sync();
+        if (wr) {
 #pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            const int nbc = ec[d] + (s ? 1 : -1);
-            T val = 0, der = 0;
-            if (nbc >= 0 && nbc < a.n[d]) {
-                const T *un = u + (e + (s ? stride[d] : -stride[d])) * K3;
-#pragma unroll
-                for (int m = 0; m < K; ++m) {
-                    const int off = d == 0 ? m * K2 + i * K + j : d == 1 ? i * K2 + m * K + j : i * K2 + j * K + m;
-                    const T x = un[off];
-                    val = fma(tb.P[1 - s][m], x, val);
-                    der = fma(tb.dP[1 - s][m], x, der);
-                }
+            for (int s = 0; s < 2; ++s) {
+                Sx[(s * 2 + 0) * K2 + i * K + j] = nval[d][s];
+                Sx[(s * 2 + 1) * K2 + i * K + j] = nder[d][s];
             }
-            NS[(s * 2 + 0) * K2 + i * K + j] = val;
-            NS[(s * 2 + 1) * K2 + i * K + j] = der;
         }
-        __syncthreads();
-        // (2) second tangential index: thread (i=m1, j=q2)
+        wave_lds_sync();
+        // (2) second tangential index, in place: thread (i=m1, j=q2)
+        T t2v[4];
 #pragma unroll
         for (int sq = 0; sq < 4; ++sq) {
 #pragma unroll
-            for (int m = 0; m < K; ++m) v[m] = NS[sq * K2 + i * K + m];
-            T2[sq * K2 + i * K + j] = dotk<K>(tb.B[j], v);
+            for (int m = 0; m < K; ++m) v[m] = Sx[sq * K2 + i * K + m];
+            t2v[sq] = dotk<K>(tb.B[j], v);
         }
-        __syncthreads();
-        // (3) first tangential index + flux: thread (i=q1, j=q2)
+        wave_lds_sync();
+        if (wr)
+#pragma unroll
+            for (int sq = 0; sq < 4; ++sq) Sx[sq * K2 + i * K + j] = t2v[sq];
+        wave_lds_sync();
+        // (3) first tangential index, own traces from u_q, flux, lift:
+        //     thread (i=q1, j=q2) owns the R pencil along d through (q1, q2)
         const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
-        const T Wf = tb.w[i] * tb.w[j] * a.h[t1] * a.h[t2];
+        const T Wf = wij * a.h[t1] * a.h[t2];
+        const T ihd = T(1) / a.h[d];
         const int idx = i + K * j;
+        int poff[K];
+#pragma unroll
+        for (int m = 0; m < K; ++m) poff[m] = d == 0 ? lidx<K>(m, i, j) : d == 1 ? lidx<K>(i, m, j) : lidx<K>(i, j, m);
+        T up[K], rp[K];
+#pragma unroll
+        for (int m = 0; m < K; ++m) {
+            up[m] = A[poff[m]];
+            rp[m] = R[poff[m]];
+        }
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
-            const int f = 2 * d + s;
             const T nrm = s ? T(1) : T(-1);
-            const int nbc = ec[d] + (s ? 1 : -1);
-            const bool interior = nbc >= 0 && nbc < a.n[d];
-            const T uin = F[(f * 2 + 0) * K2 + idx];
-            const T din = F[(f * 2 + 1) * K2 + idx] * ih[d] * nrm;
+            const bool interior = has[d][s];
+            const T uin = dotk<K>(tb.E[s], up);
+            const T din = dotk<K>(tb.dE[s], up) * (ihd * nrm);
             const T bn = a.b[d] * nrm;
+            const T bp = bn > T(0) ? bn : T(0), bm = bn < T(0) ? -bn : T(0);
             T kin, kout, kp;
-            if (a.kappa_mode == FEM_KAPPA_QP) {
-                int cc[3] = { ec[0], ec[1], ec[2] };
-                cc[d] += s;
-                const int64_t N0 = a.n[0] + (d == 0), N1 = a.n[1] + (d == 1);
-                const int64_t fidx = cc[0] + N0 * (cc[1] + N1 * (int64_t)cc[2]);
-                kin = kout = kp = a.kface[d][fidx * K2 + idx];
-            } else if (a.kappa_mode == FEM_KAPPA_CELL) {
-                kin = a.kcell[e];
-                kout = interior ? a.kcell[e + (s ? stride[d] : -stride[d])] : kin;
+            if (a.kappa_mode == FEM_KAPPA_CELL) {
+                kin = kx[0];
+                kout = kf[d][s];
                 kp = T(0.5) * (kin + kout);
             } else {
-                kin = kout = kp = a.kappa_const;
+                kin = kout = kp = kf[d][s];
             }
+            const T pen = a.beta * ihd;
             T c0, c1;
             if (interior) {
                 T nv = 0, nd = 0;
 #pragma unroll
                 for (int m = 0; m < K; ++m) {
-                    nv = fma(tb.B[i][m], T2[(s * 2 + 0) * K2 + m * K + j], nv);
-                    nd = fma(tb.B[i][m], T2[(s * 2 + 1) * K2 + m * K + j], nd);
+                    nv = fma(tb.B[i][m], Sx[(s * 2 + 0) * K2 + m * K + j], nv);
+                    nd = fma(tb.B[i][m], Sx[(s * 2 + 1) * K2 + m * K + j], nd);
                 }
-                const T uout = nv, dout = nd * ih[d] * nrm;
-                const T jump = uin - uout;
-                const T sigma = T(0.5) * (kin * din + kout * dout);
-                const T adv = (bn > T(0) ? bn : T(0)) * uin - (bn < T(0) ? -bn : T(0)) * uout;
-                c0 = Wf * (kp * a.beta * ih[d] * jump - sigma + adv);
+                const T dout = nd * (ihd * nrm);
+                const T jump = uin - nv;
+                const T sigma = T(0.5) * fma(kin, din, kout * dout);
+                c0 = Wf * (fma(kp * pen, jump, -sigma) + fma(bp, uin, -bm * nv));
                 c1 = Wf * (T(-0.5) * kin * jump);
             } else {
-                c0 = Wf * (kin * a.beta * ih[d] * uin - kin * din + (bn > T(0) ? bn : T(0)) * uin);
+                c0 = Wf * fma(fma(kin, pen, bp), uin, -kin * din);
                 c1 = Wf * (-kin * uin);
             }
-            F[(f * 2 + 0) * K2 + idx] = c0;
-            F[(f * 2 + 1) * K2 + idx] = c1 * nrm * ih[d];
+            c1 *= nrm * ihd;
+#pragma unroll
+            for (int m = 0; m < K; ++m) rp[m] = fma(tb.E[s][m], c0, fma(tb.dE[s][m], c1, rp[m]));
         }
+        wave_lds_sync();
+        if (wr)
+#pragma unroll
+            for (int m = 0; m < K; ++m) R[poff[m]] = rp[m];
     }
-    __syncthreads();
+    wave_lds_sync();
 
     // ---- backward: quadrature points -> modal test functions ---------------
     // X layout: thread (i=qy, j=qz)
     T Rt[K];
 #pragma unroll
-    for (int q = 0; q < K; ++q) Rt[q] = Rx[q] + R[q * K2 + i * K + j];
+    for (int q = 0; q < K; ++q) Rt[q] = Rx[q] + R[lidx<K>(q, i, j)];
+    mvBT<K>(tb.B, Rt, tmp);
+    if (wr)
 #pragma unroll
-    for (int s = 0; s < 2; ++s) {
-        const T c0 = F[((0 + s) * 2 + 0) * K2 + i + K * j], c1 = F[((0 + s) * 2 + 1) * K2 + i + K * j];
-#pragma unroll
-        for (int q = 0; q < K; ++q) Rt[q] += tb.E[s][q] * c0 + tb.dE[s][q] * c1;
-        const T ey0 = tb.E[s][i], dey = tb.dE[s][i];
-        const T ez0 = tb.E[s][j], dez = tb.dE[s][j];
-#pragma unroll
-        for (int q = 0; q < K; ++q) {
-            Rt[q] += ey0 * F[((2 + s) * 2 + 0) * K2 + q + K * j] + dey * F[((2 + s) * 2 + 1) * K2 + q + K * j];
-            Rt[q] += ez0 * F[((4 + s) * 2 + 0) * K2 + q + K * i] + dez * F[((4 + s) * 2 + 1) * K2 + q + K * i];
-        }
-    }
-    mvT<K>(tb.B, Rt, tmp);
-#pragma unroll
-    for (int q = 0; q < K; ++q) A[q * K2 + i * K + j] = tmp[q];
-    __syncthreads();
+        for (int q = 0; q < K; ++q) A[lidx<K>(q, i, j)] = tmp[q];
+    wave_lds_sync();
     // Y layout: thread (i=a, j=qz)
 #pragma unroll
-    for (int q = 0; q < K; ++q) v[q] = A[i * K2 + q * K + j];
-    mvT<K>(tb.B, v, tmp);
+    for (int q = 0; q < K; ++q) v[q] = A[lidx<K>(i, q, j)];
+    mvBT<K>(tb.B, v, tmp);
+    wave_lds_sync();
+    if (wr)
 #pragma unroll
-    for (int q = 0; q < K; ++q) A[i * K2 + q * K + j] = tmp[q];
-    __syncthreads();
+        for (int q = 0; q < K; ++q) A[lidx<K>(i, q, j)] = tmp[q];
+    wave_lds_sync();
     // Z layout: thread (i=a, j=b)
 #pragma unroll
-    for (int q = 0; q < K; ++q) v[q] = A[i * K2 + j * K + q];
-    mvT<K>(tb.B, v, tmp);
+    for (int q = 0; q < K; ++q) v[q] = A[lidx<K>(i, j, q)];
+    mvBT<K>(tb.B, v, tmp);
     if (active) {
-        T *we = w + e * K3;
+        T *we = w + (size_t)e * K3;
 #pragma unroll
         for (int c = 0; c < K; ++c) we[i * K2 + j * K + c] = tmp[c];
     }
@@ -330,7 +436,7 @@ static void launch_apply(const DGArgs<T> &a, const T *u, T *w, const int *skip, 
     using S = DGShape<K>;
     const int64_t E = (int64_t)a.n[0] * a.n[1] * a.n[2];
     const unsigned nb = (unsigned)((E + S::EPB - 1) / S::EPB);
-    hipLaunchKernelGGL((k_dg_apply<K, T>), dim3(nb), dim3(S::THREADS), 0, s, a, u, w, skip);
+    hipLaunchKernelGGL((k_dg_apply<K, T>), dim3(nb), dim3(64), 0, s, a, u, w, skip);
     FEM_HIP(hipGetLastError());
 }
 
@@ -415,6 +521,14 @@ void dg_rhs(const DGArgs<double> &a, int k, const double *fq, double *F, hipStre
 // ---------------------------------------------------------------------------
 // operator diagonal: one thread per DoF, direct sums over quadrature points
 // ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T kappa_vol(const DGArgs<T> &a, int64_t e, int q, int K3)
+{
+    if (a.kappa_mode == FEM_KAPPA_QP) return a.kvol[e * K3 + q];
+    if (a.kappa_mode == FEM_KAPPA_CELL) return a.kcell[e];
+    return a.kappa_const;
+}
+
 __global__ void k_dg_diag(const DGArgs<double> a, int K, double *__restrict__ dout)
 {
     const int K2 = K * K, K3 = K2 * K;
