@@ -1,0 +1,154 @@
+"""GPU Krylov solvers and preconditioners against the oracle (through the C ABI).
+
+CG follows LinearSolver::cg exactly (same recurrence and stopping rule), so the
+iteration count matches the oracle's up to rounding (+-1) and the solutions
+agree to the solver tolerance.  GMRES/FGMRES are checked against the oracle's
+restatement and a dense solve.  Every solve is also judged by its FP64 true
+relative residual.
+"""
+import numpy as np
+import pytest
+
+import dunefem_amd as fa
+import pyoracle as ora
+from dunefem_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def cg_problem(ctx, n=(64, 64)):
+    P = wl.problem0(n=n[0])
+    gop = fa.GalerkinOperator(ctx, fa.OP_CG_Q2_2D, n, 2, c=1.0)
+    oop = ora.CGOperator(n, c=1.0)
+    return P, gop, oop
+
+
+def test_cg_jacobi_problem0_matches_oracle(ctx):
+    P, gop, oop = cg_problem(ctx)
+    F = oop.rhs(P["fq"])
+    xo, ito = ora.cg(oop, F, pre=ora.jacobi(1.0 / oop.diag()), tol=1e-10, maxit=5000, criteria=1)
+    B = gop.rhs(P["fq"])
+    assert np.abs(B.download() - F).max() < 1e-12 * np.abs(F).max()
+    X = gop.vector()
+    pre = fa.Preconditioner(gop, fa.PRE_JACOBI)
+    inv = fa.InverseOperator(ctx, method=fa.CG, tolerance=1e-10, errormeasure=fa.RELATIVE, maxiterations=5000)
+    inv.bind(gop, pre)
+    inv(B, X)
+    info = inv.info()
+    print("problem0 CG+Jacobi: gpu its", info["iterations"], "oracle its", ito, "true rel res",
+          info["true_rel_residual"], "time", info["timing"], flush=True)
+    assert info["converged"] and abs(info["iterations"] - ito) <= 1
+    x = X.download()
+    assert np.abs(x - xo).max() < 1e-8 * np.abs(xo).max()
+    assert info["true_rel_residual"] < 1e-9
+
+
+def test_cg_unpreconditioned_and_maxit(ctx):
+    P, gop, oop = cg_problem(ctx, (24, 20))
+    F = oop.rhs(np.random.default_rng(1).normal(size=(480, 9)))
+    xo, ito = ora.cg(oop, F, tol=1e-9, maxit=5000, criteria=1)
+    B, X = gop.vector(F), gop.vector()
+    inv = fa.InverseOperator(ctx, method=fa.CG, tolerance=1e-9, errormeasure=fa.RELATIVE, maxiterations=5000)
+    inv.bind(gop)
+    inv(B, X)
+    assert abs(inv.iterations() - ito) <= 1
+    assert np.abs(X.download() - xo).max() < 1e-7 * np.abs(xo).max()
+    # non-convergence is a negative count, not an error (cg.hh:116)
+    inv2 = fa.InverseOperator(ctx, method=fa.CG, tolerance=1e-14, errormeasure=fa.RELATIVE, maxiterations=7)
+    inv2.bind(gop)
+    X.clear()
+    inv2(B, X)
+    assert inv2.iterations() == -7 and not inv2.info()["converged"]
+
+
+def dg_small(ctx, n, p, kappa, b=(0, 0, 0), prec=fa.F64):
+    if kappa[0] == "cell":
+        op = fa.GalerkinOperator(ctx, fa.OP_DG_SIPG_3D, n, p, 10.0 * p * p, b, 0.0, fa.KAPPA_CELL, precision=prec)
+        op.set_kappa_cell(kappa[1])
+    else:
+        op = fa.GalerkinOperator(ctx, fa.OP_DG_SIPG_3D, n, p, 10.0 * p * p, b, 0.0, fa.KAPPA_QP, precision=prec)
+        op.set_kappa_qp(kappa[1], kappa[2])
+    return op
+
+
+@pytest.mark.parametrize("method", ["gmres", "fgmres"])
+def test_gmres_matches_oracle_and_dense(ctx, method):
+    rng = np.random.default_rng(2)
+    n, p = (4, 3, 3), 2
+    E = 36
+    kc = rng.uniform(0.5, 5.0, size=E)
+    oop = ora.DGOperator(n, p, 40.0, (1, .5, .25), 0.0, kappa=("cell", kc))
+    gop = dg_small(ctx, n, p, ("cell", kc), b=(1, .5, .25))
+    b = rng.normal(size=oop.N)
+    f = ora.gmres if method == "gmres" else ora.fgmres
+    xo, ito = f(oop, b, tol=1e-10, maxit=2000, criteria=1, restart=15)
+    B, X = gop.vector(b), gop.vector()
+    inv = fa.InverseOperator(ctx, method=fa.GMRES if method == "gmres" else fa.FGMRES, tolerance=1e-10,
+                             errormeasure=fa.RELATIVE, maxiterations=2000, restart=15)
+    inv.bind(gop)
+    inv(B, X)
+    info = inv.info()
+    print(method, "gpu its", info["iterations"], "oracle its", ito, "res", info["true_rel_residual"], flush=True)
+    assert info["converged"] and abs(info["iterations"] - ito) <= 2
+    assert info["true_rel_residual"] < 1.5e-10
+    assert np.abs(X.download() - xo).max() < 1e-7 * np.abs(xo).max()
+
+
+def test_block_jacobi_is_exact_element_inverse(ctx):
+    # cellwise kappa, b = 0: the tensor-product eigen-decomposition gives A_KK^-1 exactly
+    rng = np.random.default_rng(3)
+    n, p = (3, 2, 4), 2
+    kc = 10.0 ** rng.integers(0, 5, size=24).astype(float)
+    oop = ora.DGOperator(n, p, 40.0, kappa=("cell", kc))
+    blocks = oop.block_diag()
+    gop = dg_small(ctx, n, p, ("cell", kc))
+    pre = fa.Preconditioner(gop, fa.PRE_BLOCK_JACOBI)
+    x = rng.normal(size=oop.N)
+    X, Y = gop.vector(x), gop.vector()
+    pre(X, Y)
+    nb = 27
+    ref = np.linalg.solve(blocks, x.reshape(-1, nb)[..., None])[..., 0].ravel()
+    err = np.abs(Y.download() - ref).max() / np.abs(ref).max()
+    print("block jacobi rel err", err, flush=True)
+    assert err < 1e-11
+
+
+def test_fgmres_block_jacobi_problem3_small(ctx):
+    # the high-contrast channel problem at 12^3: FGMRES(30) + block Jacobi to 1e-8
+    P = wl.problem3(n=12)
+    gop = dg_small(ctx, P["n"], P["p"], ("cell", P["kcell"]))
+    pre = fa.Preconditioner(gop, fa.PRE_BLOCK_JACOBI)
+    F = gop.rhs(P["fq"])
+    X = gop.vector()
+    inv = fa.InverseOperator(ctx, method=fa.FGMRES, tolerance=1e-8, errormeasure=fa.RELATIVE,
+                             maxiterations=5000, restart=30)
+    inv.bind(gop, pre)
+    inv(F, X)
+    info = inv.info()
+    print("problem3 12^3 FGMRES+BJ", info, flush=True)
+    assert info["converged"] and info["true_rel_residual"] < 1.01e-8
+    # the oracle reaches the same residual with the same iteration count (+-2)
+    oop = ora.DGOperator(P["n"], P["p"], P["beta"], kappa=("cell", P["kcell"]))
+    inv_blocks = np.linalg.inv(oop.block_diag())
+    xo, ito = ora.fgmres(oop, F.download(), pre=ora.block_jacobi(inv_blocks), tol=1e-8, maxit=5000,
+                         criteria=1, restart=30)
+    assert abs(info["iterations"] - ito) <= 2
+    assert np.abs(X.download() - xo).max() < 1e-6 * np.abs(xo).max()
+
+
+@pytest.mark.parametrize("method", [fa.CG, fa.FGMRES])
+def test_mixed_precision_refinement_reaches_fp64_residual(ctx, method):
+    P = wl.problem3(n=12)
+    op64 = dg_small(ctx, P["n"], P["p"], ("cell", P["kcell"]))
+    op32 = dg_small(ctx, P["n"], P["p"], ("cell", P["kcell"]), prec=fa.F32)
+    pre32 = fa.Preconditioner(op32, fa.PRE_BLOCK_JACOBI)
+    F = op64.rhs(P["fq"])
+    X = op64.vector()
+    inv = fa.InverseOperator(ctx, method=method, tolerance=1e-8, errormeasure=fa.RELATIVE, maxiterations=20000,
+                             restart=30, refinement=1, inner_tolerance=1e-3, inner_maxiterations=2000)
+    inv.bind(op64, None, op32, pre32)
+    inv(F, X)
+    info = inv.info()
+    print("IR", method, info, flush=True)
+    assert info["converged"] and info["true_rel_residual"] <= 1e-8
+    assert info["outer_iterations"] >= 2
