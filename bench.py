@@ -38,7 +38,7 @@ def parse():
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=3)
-    ap.add_argument("--solve", action="store_true")
+    ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--calib", action="store_true",
                     help="also run a plain FP64 axpy of known size (HBM counter calibration)")
@@ -58,6 +58,50 @@ def time_applies(fa, ctx, op, U, W, steps, warmup):
     wall = time.perf_counter() - t0
     ev_ms = ctx.elapsed_ms()
     return wall, ev_ms / steps
+
+
+def solve_section(fa, wl, ctx):
+    """FGMRES(30) to 1e-8 FP64 relative residual on the high-contrast channel
+    problem; plain FP64 GPU path vs FP64 refinement around FP32 solves."""
+    P = wl.problem3(n=48)
+    out = {"workload": "3D SIPG DG Q2 diffusion, 48^3 hexes (2,985,984 DoFs), kappa=10^(4H), H binary "
+                       "channels 20%, f = 16 Fourier modes, FGMRES(30), rel tol 1e-8 (FP64 true residual)"}
+
+    def dg(prec):
+        op = fa.GalerkinOperator(ctx, fa.OP_DG_SIPG_3D, P["n"], P["p"], P["beta"], (0, 0, 0), 0.0, fa.KAPPA_CELL,
+                                 precision=prec)
+        op.set_kappa_cell(P["kcell"])
+        return op
+
+    op64 = dg(fa.F64)
+    F = op64.rhs(P["fq"])
+    runs = {}
+    pre64 = fa.Preconditioner(op64, fa.PRE_BLOCK_JACOBI)
+    for name, method in (("fp64_fgmres_bj", fa.FGMRES), ("fp64_cg_bj", fa.CG)):
+        X = op64.vector()
+        inv = fa.InverseOperator(ctx, method=method, tolerance=1e-8, errormeasure=fa.RELATIVE,
+                                 maxiterations=100000, restart=30)
+        inv.bind(op64, pre64)
+        inv(F, X)
+        runs[name] = inv.info()
+    op32 = dg(fa.F32)
+    pre32 = fa.Preconditioner(op32, fa.PRE_BLOCK_JACOBI)
+    for name, method, itol in (("mixed_fp32_fgmres_bj_ir", fa.FGMRES, 1e-3), ("mixed_fp32_cg_bj_ir", fa.CG, 1e-3)):
+        X = op64.vector()
+        inv = fa.InverseOperator(ctx, method=method, tolerance=1e-8, errormeasure=fa.RELATIVE,
+                                 maxiterations=100000, restart=30, refinement=1, inner_tolerance=itol,
+                                 inner_maxiterations=20000)
+        inv.bind(op64, None, op32, pre32)
+        inv(F, X)
+        runs[name] = inv.info()
+    for k, v in runs.items():
+        runs[k] = {"converged": v["converged"], "iterations": v["iterations"],
+                   "outer_iterations": v["outer_iterations"], "time_s": round(v["timing"][0], 4),
+                   "true_rel_residual": v["true_rel_residual"]}
+    out["runs"] = runs
+    base = runs["fp64_fgmres_bj"]["time_s"]
+    out["speedup_vs_fp64_fgmres"] = {k: round(base / v["time_s"], 3) for k, v in runs.items()}
+    return out
 
 
 def main():
@@ -146,6 +190,10 @@ def main():
                        "hbm_frac": round(b2 / (kms2 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
         del op, U2, W2
     result["mixed"] = mixed
+
+    # ---- time to solve (problem 3: 48^3 SIPG Q2, kappa = 10^(4H) channels) --------
+    if not a.no_solve:
+        result["solve"] = solve_section(fa, wl, ctx)
 
     # ---- CPU baseline: the oracle (reference algorithm) on the host cores ----------
     if not a.no_cpu:

@@ -6,6 +6,7 @@
 #include <cstring>
 #include <string>
 
+#include "fem_batch.hpp"
 #include "fem_blas.hpp"
 #include "fem_op.hpp"
 #include "fem_solver.hpp"
@@ -386,6 +387,8 @@ int fem_solver_params_set(fem_solver_params *p, const char *key, const char *val
             p->restart = std::stoi(v);
         } else if (k == "verbose") {
             p->verbose = (v == "true" || v == "1") ? 1 : 0;
+        } else if (k == "residualcheck") {
+            p->residual_check = (v == "true" || v == "1") ? 1 : 0;
         } else {
             throw Error(FEM_ERR_ARG, std::string("unknown solver parameter: ") + key);
         }
@@ -434,6 +437,104 @@ int fem_solver_solve(fem_solver *s, const fem_vec *b, fem_vec *x, fem_solver_inf
         FEM_CHECK(s && b && x, FEM_ERR_ARG, "null argument");
         fem_solver_info tmp;
         s->s.solve(b->v, x->v, info ? *info : tmp);
+    });
+}
+
+// ---- batched solves --------------------------------------------------------------
+int fem_batch_create(fem_ctx *ctx, int count, const int *nx, const int *ny, double c, fem_batch **out)
+{
+    return guarded([&] {
+        FEM_CHECK(ctx && nx && ny && out, FEM_ERR_ARG, "null argument");
+        fem_batch *h = new fem_batch();
+        try {
+            h->b.init(&ctx->c, count, nx, ny, c);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int fem_batch_destroy(fem_batch *b)
+{
+    return guarded([&] {
+        if (b) {
+            (void)hipStreamSynchronize(b->b.ctx->stream);
+            delete b;
+        }
+    });
+}
+
+int64_t fem_batch_ndofs(const fem_batch *b) { return b ? b->b.N : -1; }
+
+int fem_batch_dof_offsets(const fem_batch *b, int64_t *off)
+{
+    return guarded([&] {
+        FEM_CHECK(b && off, FEM_ERR_ARG, "null argument");
+        for (int i = 0; i <= b->b.count; ++i) off[i] = b->b.dof_off[i];
+    });
+}
+
+int fem_batch_set_kappa_qp(fem_batch *b, const double *kvol)
+{
+    return guarded([&] {
+        FEM_CHECK(b, FEM_ERR_ARG, "null batch");
+        b->b.set_kappa_qp(kvol);
+    });
+}
+
+int fem_batch_apply(fem_batch *b, const fem_vec *u, fem_vec *w)
+{
+    return guarded([&] {
+        FEM_CHECK(b && u && w, FEM_ERR_ARG, "null argument");
+        FEM_CHECK(u->v.n == b->b.N && w->v.n == b->b.N && u->v.prec == w->v.prec, FEM_ERR_ARG,
+                  "batch apply: vectors of the batch size and one precision");
+        FEM_CHECK(u->v.data != w->v.data, FEM_ERR_ARG, "batch apply: u and w must be distinct");
+        b->b.apply_raw(u->v.prec, u->v.data, w->v.data, nullptr);
+    });
+}
+
+int fem_batch_assemble_rhs(fem_batch *b, const double *fq, fem_vec *F)
+{
+    return guarded([&] {
+        FEM_CHECK(b && fq && F, FEM_ERR_ARG, "null argument");
+        b->b.rhs(fq, F->v);
+    });
+}
+
+int fem_batch_autotune(fem_batch *b, int *tx, int *ty, float *ms)
+{
+    return guarded([&] {
+        FEM_CHECK(b, FEM_ERR_ARG, "null batch");
+        FEM_CHECK(b->b.kvol32.p, FEM_ERR_STATE, "batch: coefficient data not set");
+        const int sh = b->b.autotune(10);
+        if (tx) *tx = CG_SHAPES[sh][0];
+        if (ty) *ty = CG_SHAPES[sh][1];
+        if (ms)
+            for (int i = 0; i < CG_NSHAPES; ++i) ms[i] = b->b.tune_ms[i];
+    });
+}
+
+int fem_batch_solve(fem_batch *b, const fem_vec *rhs, fem_vec *x, double tol, double inner_tol, int inner_maxit,
+                    int max_outer, int use_graph, fem_batch_info *info)
+{
+    return guarded([&] {
+        FEM_CHECK(b && rhs && x, FEM_ERR_ARG, "null argument");
+        fem_batch_info tmp;
+        b->b.solve(rhs->v, x->v, tol, inner_tol, inner_maxit, max_outer, use_graph, info ? *info : tmp);
+    });
+}
+
+int fem_batch_instance_results(const fem_batch *b, int *iterations, double *rel_residual)
+{
+    return guarded([&] {
+        FEM_CHECK(b, FEM_ERR_ARG, "null batch");
+        FEM_CHECK(!b->b.last_iters.empty(), FEM_ERR_STATE, "no solve yet");
+        for (int i = 0; i < b->b.count; ++i) {
+            if (iterations) iterations[i] = b->b.last_iters[i];
+            if (rel_residual) rel_residual[i] = b->b.last_relres[i];
+        }
     });
 }
 

@@ -31,13 +31,13 @@ CGTab make_cg_tab()
     return t;
 }
 
-constexpr int CG_LX = 2 * CG_TX + 3, CG_LY = 2 * CG_TY + 3;
-
-template <typename T>
-__global__ void __launch_bounds__(CG_THREADS)
+template <int TX, int TY, typename T>
+__global__ void __launch_bounds__((TX + 1) * (TY + 1))
     k_cg(int mode, const CGInst *__restrict__ insts, int ninst, const CGTab *__restrict__ tabp,
-         const T *__restrict__ kvol, const T *__restrict__ in, T *__restrict__ out, const int *__restrict__ skip)
+         const T *__restrict__ kvol, const T *__restrict__ in, T *__restrict__ out, const int *__restrict__ skip,
+         const int *__restrict__ inst_done)
 {
+    constexpr int CG_LX = 2 * TX + 3, CG_LY = 2 * TY + 3;
     if (skip && *skip) return;
     __shared__ T lat[CG_LX * CG_LY];
     __shared__ int s_inst;
@@ -52,11 +52,12 @@ __global__ void __launch_bounds__(CG_THREADS)
     }
     for (int k = threadIdx.x; k < CG_LX * CG_LY; k += blockDim.x) lat[k] = 0;
     __syncthreads();
+    if (inst_done && inst_done[s_inst]) return;   // uniform over the block
     const CGInst P = insts[s_inst];
     const CGTab &tb = *tabp;
     const int tile = (int)bid - P.tile_off;
-    const int ex0 = (tile % P.tiles_x) * CG_TX, ey0 = (tile / P.tiles_x) * CG_TY;
-    const int tx = threadIdx.x % (CG_TX + 1), ty = threadIdx.x / (CG_TX + 1);
+    const int ex0 = (tile % P.tiles_x) * TX, ey0 = (tile / P.tiles_x) * TY;
+    const int tx = threadIdx.x % (TX + 1), ty = threadIdx.x / (TX + 1);
     const int ex = ex0 - 1 + tx, ey = ey0 - 1 + ty;
     const bool valid = ex >= 0 && ey >= 0 && ex < P.nx && ey < P.ny;
     const int NX = 2 * P.nx + 1;
@@ -162,8 +163,8 @@ __global__ void __launch_bounds__(CG_THREADS)
     }
     // owned nodes
     const int I0 = 2 * ex0, J0 = 2 * ey0;
-    const int I1 = (ex0 + CG_TX >= P.nx) ? 2 * P.nx + 1 : 2 * ex0 + 2 * CG_TX;
-    const int J1 = (ey0 + CG_TY >= P.ny) ? 2 * P.ny + 1 : 2 * ey0 + 2 * CG_TY;
+    const int I1 = (ex0 + TX >= P.nx) ? 2 * P.nx + 1 : 2 * ex0 + 2 * TX;
+    const int J1 = (ey0 + TY >= P.ny) ? 2 * P.ny + 1 : 2 * ey0 + 2 * TY;
     const int wx = I1 - I0, wy = J1 - J0;
     T *o = out + P.dof_off;
     for (int k = threadIdx.x; k < wx * wy; k += blockDim.x) {
@@ -177,17 +178,36 @@ __global__ void __launch_bounds__(CG_THREADS)
 }
 
 template <typename T>
+void cg_launch_shape(int shape, int mode, const CGInst *d_inst, int ninst, int total_tiles, const CGTab *d_tab,
+                     const T *kvol, const T *in, T *out, const int *skip, const int *inst_done, hipStream_t s)
+{
+#define CGS(ID, TX, TY)                                                                                       \
+    case ID:                                                                                                 \
+        hipLaunchKernelGGL((k_cg<TX, TY, T>), dim3(total_tiles), dim3((TX + 1) * (TY + 1)), 0, s, mode, d_inst, \
+                           ninst, d_tab, kvol, in, out, skip, inst_done);                                    \
+        break;
+    switch (shape) {
+        CGS(0, 15, 15) CGS(1, 7, 7) CGS(2, 15, 7) CGS(3, 31, 7) CGS(4, 7, 31)
+    default: throw Error(FEM_ERR_ARG, "tile shape");
+    }
+#undef CGS
+    FEM_HIP(hipGetLastError());
+}
+
+template <typename T>
 void cg_launch(int mode, const CGInst *d_inst, int ninst, int total_tiles, const CGTab *d_tab, const T *kvol,
                const T *in, T *out, const int *skip, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_cg<T>, dim3(total_tiles), dim3(CG_THREADS), 0, s, mode, d_inst, ninst, d_tab, kvol, in,
-                       out, skip);
-    FEM_HIP(hipGetLastError());
+    cg_launch_shape<T>(0, mode, d_inst, ninst, total_tiles, d_tab, kvol, in, out, skip, nullptr, s);
 }
 
 template void cg_launch<double>(int, const CGInst *, int, int, const CGTab *, const double *, const double *,
                                 double *, const int *, hipStream_t);
 template void cg_launch<float>(int, const CGInst *, int, int, const CGTab *, const float *, const float *,
                                float *, const int *, hipStream_t);
+template void cg_launch_shape<double>(int, int, const CGInst *, int, int, const CGTab *, const double *,
+                                      const double *, double *, const int *, const int *, hipStream_t);
+template void cg_launch_shape<float>(int, int, const CGInst *, int, int, const CGTab *, const float *,
+                                     const float *, float *, const int *, const int *, hipStream_t);
 
 }  // namespace femamd

@@ -18,8 +18,12 @@ struct CGInst {
     double kappa_const; // used when kvol == nullptr
 };
 
-constexpr int CG_TX = 15, CG_TY = 15;                  // owned elements per tile
+constexpr int CG_TX = 15, CG_TY = 15;                  // default tile: owned elements per tile
 constexpr int CG_THREADS = (CG_TX + 1) * (CG_TY + 1);  // + lower halo row/column
+
+// tile shapes available to the autotuner (owned elements per tile)
+constexpr int CG_NSHAPES = 5;
+constexpr int CG_SHAPES[CG_NSHAPES][2] = { { 15, 15 }, { 7, 7 }, { 15, 7 }, { 31, 7 }, { 7, 31 } };
 
 struct CGTab {
     double L[3][3];   // Q2 basis i at Gauss point q: L[q][i]
@@ -35,6 +39,11 @@ enum { CG_MODE_APPLY = 0, CG_MODE_DIAG = 1, CG_MODE_RHS = 2 };
 template <typename T>
 void cg_launch(int mode, const CGInst *d_inst, int ninst, int total_tiles, const CGTab *d_tab, const T *kvol,
                const T *in, T *out, const int *skip, hipStream_t s);
+// same with an explicit tile shape (index into CG_SHAPES) and an optional
+// per-instance mask: instances with inst_done[i] != 0 are skipped
+template <typename T>
+void cg_launch_shape(int shape, int mode, const CGInst *d_inst, int ninst, int total_tiles, const CGTab *d_tab,
+                     const T *kvol, const T *in, T *out, const int *skip, const int *inst_done, hipStream_t s);
 
 CGTab make_cg_tab();
 

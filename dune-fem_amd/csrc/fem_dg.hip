@@ -138,7 +138,87 @@ __device__ __forceinline__ void mvBT(const T (&M)[KMAX][KMAX], const T *in, T *o
     }
 }
 
-template <int K, typename T>
+// ---- BF16 MFMA contractions (v_mfma_f32_4x4x4bf16_1k, 16 blocks) --------
+// Operand layout of the 16-block 4x4x4 instruction: lane (i + 4b) supplies
+// row i of A_b and column i of B_b (4 values each) and receives column i of
+// D_b.  With A_b = M for every block, a lane's own K-pencil is contracted by
+// one instruction: D(lane) = M * pencil(lane), FP32 accumulation.
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ short to_bf16(float x)
+{
+    // round to nearest even
+    unsigned u = __float_as_uint(x);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (short)(u >> 16);
+}
+
+struct MFRows {
+    bf16x4 B, BT, D, DT;   // row (lane % 4) of B, B^T, Dt, Dt^T (4x4 leading block)
+};
+
+template <int K>
+__device__ __forceinline__ bf16x4 mf_row(const float (&M)[KMAX][KMAX], bool trans, int r)
+{
+    bf16x4 v;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float x = (r < K && k < K) ? (trans ? M[k][r] : M[r][k]) : 0.0f;
+        v[k] = to_bf16(x);
+    }
+    return v;
+}
+
+// out = M in (trans: M^T in) for a K-pencil; K <= 4 padded to 4, K = 5 as a
+// 4x4 MFMA block plus the fifth row/column on the VALU
+template <int K>
+__device__ __forceinline__ void mf_mv(const bf16x4 &A, const float (&M)[KMAX][KMAX], bool trans, const float *in,
+                                      float *out)
+{
+    bf16x4 b;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) b[k] = to_bf16(k < K ? in[k] : 0.0f);
+    const f32x4 z = { 0.f, 0.f, 0.f, 0.f };
+    const f32x4 d = __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(A, b, z, 0, 0, 0);
+#pragma unroll
+    for (int q = 0; q < (K < 4 ? K : 4); ++q) out[q] = d[q];
+    if (K == 5) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) out[q] = fmaf(trans ? M[4][q] : M[q][4], in[4], out[q]);
+        float s = 0.f;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) s = fmaf(trans ? M[k][4] : M[4][k], in[k], s);
+        out[4] = s;
+    }
+}
+
+// contraction policy: VALU in T, or BF16 MFMA with FP32 accumulation
+template <int K, typename T, bool MF>
+struct Ctr {
+    static __device__ __forceinline__ void B(const Tab1D<T> &tb, const MFRows &r, const T *in, T *out)
+    {
+        if constexpr (MF) mf_mv<K>(r.B, tb.B, false, in, out);
+        else mvB<K>(tb.B, in, out);
+    }
+    static __device__ __forceinline__ void BT(const Tab1D<T> &tb, const MFRows &r, const T *in, T *out)
+    {
+        if constexpr (MF) mf_mv<K>(r.BT, tb.B, true, in, out);
+        else mvBT<K>(tb.B, in, out);
+    }
+    static __device__ __forceinline__ void D(const Tab1D<T> &tb, const MFRows &r, const T *in, T *out)
+    {
+        if constexpr (MF) mf_mv<K>(r.D, tb.Dt, false, in, out);
+        else mv<K>(tb.Dt, in, out);
+    }
+    static __device__ __forceinline__ void DT(const Tab1D<T> &tb, const MFRows &r, const T *in, T *out)
+    {
+        if constexpr (MF) mf_mv<K>(r.DT, tb.Dt, true, in, out);
+        else mvT<K>(tb.Dt, in, out);
+    }
+};
+
+template <int K, typename T, bool MF>
 __global__ void __launch_bounds__(64)
     k_dg_apply(const DGArgs<T> a, const T *__restrict__ u, T *__restrict__ w, const int *__restrict__ skip)
 {
@@ -147,6 +227,15 @@ __global__ void __launch_bounds__(64)
     if (skip && *skip) return;
     __shared__ T smem[EPB * S::SZ];
     const Tab1D<T> &tb = *a.tab;
+    using C = Ctr<K, T, MF>;
+    MFRows mr;
+    if constexpr (MF) {
+        const int r = threadIdx.x & 3;
+        mr.B = mf_row<K>(tb.B, false, r);
+        mr.BT = mf_row<K>(tb.B, true, r);
+        mr.D = mf_row<K>(tb.Dt, false, r);
+        mr.DT = mf_row<K>(tb.Dt, true, r);
+    }
 
     const int tid = threadIdx.x;
     const int el0 = tid / K2;
@@ -225,7 +314,7 @@ __global__ void __launch_bounds__(64)
 
     // ---- forward: modal -> quadrature points ------------------------------
     // Z layout: thread (i=a, j=b) holds u[a][b][0..K)
-    mvB<K>(tb.B, v, tmp);
+    C::B(tb, mr, v, tmp);
     if (wr)
 #pragma unroll
         for (int q = 0; q < K; ++q) A[lidx<K>(i, j, q)] = tmp[q];
@@ -233,7 +322,7 @@ __global__ void __launch_bounds__(64)
     // Y layout: thread (i=a, j=qz); each thread rewrites the pencil it read
 #pragma unroll
     for (int b = 0; b < K; ++b) v[b] = A[lidx<K>(i, b, j)];
-    mvB<K>(tb.B, v, tmp);
+    C::B(tb, mr, v, tmp);
     wave_lds_sync();
     if (wr)
 #pragma unroll
@@ -256,7 +345,7 @@ __global__ void __launch_bounds__(64)
     T uq[K];
 #pragma unroll
     for (int b = 0; b < K; ++b) v[b] = A[lidx<K>(b, i, j)];
-    mvB<K>(tb.B, v, uq);
+    C::B(tb, mr, v, uq);
     wave_lds_sync();
     if (wr)
 #pragma unroll
@@ -268,12 +357,12 @@ __global__ void __launch_bounds__(64)
     {
         const T sc = wij * detJ / (a.h[0] * a.h[0]), bh = a.b[0] * a.h[0];
         T g[K], r[K];
-        mv<K>(tb.Dt, uq, g);
+        C::D(tb, mr, uq, g);
 #pragma unroll
         for (int q = 0; q < K; ++q) {
             r[q] = (tb.w[q] * sc) * fma(kx[q], g[q], -bh * uq[q]);
         }
-        mvT<K>(tb.Dt, r, Rx);
+        C::DT(tb, mr, r, Rx);
         if (a.c != T(0)) {
             const T cm = wij * detJ * a.c;
 #pragma unroll
@@ -287,12 +376,12 @@ __global__ void __launch_bounds__(64)
         T g[K], r[K];
 #pragma unroll
         for (int q = 0; q < K; ++q) v[q] = A[lidx<K>(i, q, j)];
-        mv<K>(tb.Dt, v, g);
+        C::D(tb, mr, v, g);
 #pragma unroll
         for (int q = 0; q < K; ++q) {
             r[q] = (tb.w[q] * sc) * fma(ky[q], g[q], -bh * v[q]);
         }
-        mvT<K>(tb.Dt, r, tmp);
+        C::DT(tb, mr, r, tmp);
         if (wr)
 #pragma unroll
             for (int q = 0; q < K; ++q) R[lidx<K>(i, q, j)] = tmp[q];
@@ -304,12 +393,12 @@ __global__ void __launch_bounds__(64)
         T g[K], r[K];
 #pragma unroll
         for (int q = 0; q < K; ++q) v[q] = A[lidx<K>(i, j, q)];
-        mv<K>(tb.Dt, v, g);
+        C::D(tb, mr, v, g);
 #pragma unroll
         for (int q = 0; q < K; ++q) {
             r[q] = (tb.w[q] * sc) * fma(kz[q], g[q], -bh * v[q]);
         }
-        mvT<K>(tb.Dt, r, tmp);
+        C::DT(tb, mr, r, tmp);
         if (wr)
 #pragma unroll
             for (int q = 0; q < K; ++q) R[lidx<K>(i, j, q)] += tmp[q];
@@ -405,7 +494,7 @@ __global__ void __launch_bounds__(64)
     T Rt[K];
 #pragma unroll
     for (int q = 0; q < K; ++q) Rt[q] = Rx[q] + R[lidx<K>(q, i, j)];
-    mvBT<K>(tb.B, Rt, tmp);
+    C::BT(tb, mr, Rt, tmp);
     if (wr)
 #pragma unroll
         for (int q = 0; q < K; ++q) A[lidx<K>(q, i, j)] = tmp[q];
@@ -413,7 +502,7 @@ __global__ void __launch_bounds__(64)
     // Y layout: thread (i=a, j=qz)
 #pragma unroll
     for (int q = 0; q < K; ++q) v[q] = A[lidx<K>(i, q, j)];
-    mvBT<K>(tb.B, v, tmp);
+    C::BT(tb, mr, v, tmp);
     wave_lds_sync();
     if (wr)
 #pragma unroll
@@ -422,7 +511,7 @@ __global__ void __launch_bounds__(64)
     // Z layout: thread (i=a, j=b)
 #pragma unroll
     for (int q = 0; q < K; ++q) v[q] = A[lidx<K>(i, j, q)];
-    mvBT<K>(tb.B, v, tmp);
+    C::BT(tb, mr, v, tmp);
     if (active) {
         T *we = w + (size_t)e * K3;
 #pragma unroll
@@ -430,13 +519,14 @@ __global__ void __launch_bounds__(64)
     }
 }
 
-template <int K, typename T>
+template <int K, typename T, bool MF = false>
 static void launch_apply(const DGArgs<T> &a, const T *u, T *w, const int *skip, hipStream_t s)
 {
     using S = DGShape<K>;
     const int64_t E = (int64_t)a.n[0] * a.n[1] * a.n[2];
+    FEM_CHECK(E * S::K3 < (int64_t)1 << 31, FEM_ERR_UNSUPPORTED, "DG apply: more than 2^31 DoFs");
     const unsigned nb = (unsigned)((E + S::EPB - 1) / S::EPB);
-    hipLaunchKernelGGL((k_dg_apply<K, T>), dim3(nb), dim3(64), 0, s, a, u, w, skip);
+    hipLaunchKernelGGL((k_dg_apply<K, T, MF>), dim3(nb), dim3(64), 0, s, a, u, w, skip);
     FEM_HIP(hipGetLastError());
 }
 
@@ -460,6 +550,17 @@ void dg_apply_f64(const DGArgs<double> &a, int k, const double *u, double *w, co
 void dg_apply_f32(const DGArgs<float> &a, int k, const float *u, float *w, const int *skip, hipStream_t s)
 {
     dispatch_apply<float>(a, k, u, w, skip, s);
+}
+
+void dg_apply_bf16(const DGArgs<float> &a, int k, const float *u, float *w, const int *skip, hipStream_t s)
+{
+    switch (k) {
+    case 2: launch_apply<2, float, true>(a, u, w, skip, s); break;
+    case 3: launch_apply<3, float, true>(a, u, w, skip, s); break;
+    case 4: launch_apply<4, float, true>(a, u, w, skip, s); break;
+    case 5: launch_apply<5, float, true>(a, u, w, skip, s); break;
+    default: throw Error(FEM_ERR_UNSUPPORTED, "DG apply: order must be 1..4");
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -1,11 +1,4 @@
-// BF16-MFMA variant of the SIPG DG apply (FP32 data, FP32 accumulation).
+// The BF16-MFMA DG apply is the contraction policy Ctr<K, float, true> of the
+// sum-factorised kernel in fem_dg.hip (dg_apply_bf16); this translation unit
+// is kept empty so that the build lists stay stable.
 #include "fem_dg.hpp"
-
-namespace femamd {
-
-void dg_apply_bf16(const DGArgs<float> &, int, const float *, float *, const int *, hipStream_t)
-{
-    throw Error(FEM_ERR_UNSUPPORTED, "BF16 MFMA DG apply not built yet");
-}
-
-}  // namespace femamd

@@ -34,13 +34,14 @@ __device__ __forceinline__ void fin_cg_alpha(KState *st, double qh)
     st->alpha = st->res / qh;
 }
 
-__device__ __forceinline__ void fin_cg_res(KState *st, double r)
+__device__ __forceinline__ void fin_cg_res(KState *st, double r, double rr)
 {
     st->prev = st->res;
     st->res = r;
+    st->rr = rr;
     st->beta = r / st->prev;
     st->it += 1;
-    st->done = !(r > st->tol) || st->it >= st->maxit;
+    st->done = !((st->crit_res ? rr : r) > st->tol) || st->it >= st->maxit;
 }
 
 // ---- CG kernels ---------------------------------------------------------------
@@ -55,13 +56,15 @@ __global__ void k_cg_setup(const T *__restrict__ b, const T *__restrict__ h, T *
     }
 }
 
-__global__ void k_cg_init(KState *st, double eps, int criteria, int maxit)
+__global__ void k_cg_init(KState *st, double eps, int criteria, int maxit, int crit_res)
 {
-    const double ref = criteria == FEM_ERRMEASURE_RELATIVE ? st->bnorm2 : criteria == FEM_ERRMEASURE_REDUCTION ? st->res : 1.0;
+    const double r0 = crit_res ? st->rr : st->res;
+    const double ref = criteria == FEM_ERRMEASURE_RELATIVE ? st->bnorm2 : criteria == FEM_ERRMEASURE_REDUCTION ? r0 : 1.0;
     st->tol = eps * eps * ref;
     st->it = 0;
     st->maxit = maxit;
-    st->done = !(st->res > st->tol) || maxit <= 0;
+    st->crit_res = crit_res;
+    st->done = !(r0 > st->tol) || maxit <= 0;
 }
 
 template <typename T>
@@ -86,7 +89,7 @@ __global__ void __launch_bounds__(RED_THREADS)
 {
     if (st->done) return;
     __shared__ double lds[RED_THREADS / 64];
-    double acc = 0;
+    double acc = 0, acc2 = 0;
     const double alpha = st->alpha;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         if (MODE == 0) {          // dot(q, h) -> alpha
@@ -99,35 +102,46 @@ __global__ void __launch_bounds__(RED_THREADS)
             const T sv = dinv[i] * pt;
             s[i] = sv;
             acc += (double)pt * (double)sv;
+            acc2 += (double)pt * (double)pt;
         } else if (MODE == 2) {   // general M: x += a q, p -= a h (no reduction)
             x[i] = (T)((double)x[i] + alpha * (double)q[i]);
             p[i] = (T)((double)p[i] - alpha * (double)h[i]);
         } else if (MODE == 3) {   // <p, s> after a general M
             acc += (double)p[i] * (double)s[i];
+            acc2 += (double)p[i] * (double)p[i];
         } else {                  // no preconditioner: x += a p, r += a h, <r, r>
             x[i] = (T)((double)x[i] + alpha * (double)q[i]);
             const T rn = (T)((double)r[i] + alpha * (double)h[i]);
             r[i] = rn;
             acc += (double)rn * (double)rn;
+            acc2 = acc;
         }
     }
     if (MODE == 2) return;
     double b = block_sum(acc, lds);
+    __syncthreads();
+    double b2 = block_sum(acc2, lds);
     __shared__ bool amLast;
     if (threadIdx.x == 0) {
         partials[blockIdx.x] = b;
+        partials[gridDim.x + blockIdx.x] = b2;
         __threadfence();
         amLast = atomicAdd(counter, 1u) == gridDim.x - 1;
     }
     __syncthreads();
     if (!amLast) return;
     __threadfence();
-    double v = 0;
-    for (unsigned k = threadIdx.x; k < gridDim.x; k += blockDim.x) v += __builtin_nontemporal_load(&partials[k]);
+    double v = 0, v2 = 0;
+    for (unsigned k = threadIdx.x; k < gridDim.x; k += blockDim.x) {
+        v += __builtin_nontemporal_load(&partials[k]);
+        v2 += __builtin_nontemporal_load(&partials[gridDim.x + k]);
+    }
     double tot = block_sum(v, lds);
+    __syncthreads();
+    double tot2 = block_sum(v2, lds);
     if (threadIdx.x == 0) {
         if (MODE == 0) fin_cg_alpha(st, tot);
-        else fin_cg_res(st, tot);
+        else fin_cg_res(st, tot, tot2);
         *counter = 0;
     }
 }
@@ -441,8 +455,9 @@ static int run_krylov(Solver &S, int method, Operator *o, Precond *p, const T *b
             FEM_HIP(hipMemcpyAsync(ss, q, vb, hipMemcpyDeviceToDevice, s));
         }
         dot_async(*S.ctx, N, pp, q, &st->res, cnt + 1, part);
+        dot_async(*S.ctx, N, pp, pp, &st->rr, cnt + 1, part);
         dot_async(*S.ctx, N, b, b, &st->bnorm2, cnt + 1, part);
-        hipLaunchKernelGGL(k_cg_init, dim3(1), dim3(1), 0, s, st, eps, criteria, maxit);
+        hipLaunchKernelGGL(k_cg_init, dim3(1), dim3(1), 0, s, st, eps, criteria, maxit, S.prm.residual_check);
         FEM_HIP(hipGetLastError());
         const T *dinv = (haspre && p->pointwise()) ? (p->prec == FEM_F64 ? (const T *)p->dinv.p : (const T *)p->dinv.p) : nullptr;
         int chunk = 4;

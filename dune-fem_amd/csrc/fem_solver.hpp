@@ -12,6 +12,8 @@ constexpr int GM_MAX = 64;   // maximum GMRES restart length
 struct KState {
     double res, prev, qh, alpha, beta, tol, bnorm2, scale;
     int it, done, maxit, criteria;
+    int crit_res;      // stop on ||r||^2 instead of <r, M^-1 r>
+    double rr;         // ||r||^2 of the current residual
     // GMRES
     double H[(GM_MAX + 1) * GM_MAX];
     double g[GM_MAX + 1], cs[GM_MAX], sn[GM_MAX], y[GM_MAX];

@@ -42,7 +42,9 @@ EXPORTS = [
     "fem_op_diagonal", "fem_precond_create", "fem_precond_destroy", "fem_precond_apply",
     "fem_solver_default_params", "fem_solver_params_set", "fem_solver_create",
     "fem_solver_destroy", "fem_solver_bind", "fem_solver_unbind", "fem_solver_solve",
-    "fem_gauss_points",
+    "fem_gauss_points", "fem_batch_create", "fem_batch_destroy", "fem_batch_ndofs",
+    "fem_batch_dof_offsets", "fem_batch_set_kappa_qp", "fem_batch_apply", "fem_batch_assemble_rhs",
+    "fem_batch_autotune", "fem_batch_solve", "fem_batch_instance_results",
 ]
 
 
@@ -60,7 +62,14 @@ class SolverParams(C.Structure):
     _fields_ = [("method", C.c_int), ("tolerance", C.c_double), ("errormeasure", C.c_int),
                 ("maxiterations", C.c_int), ("restart", C.c_int), ("verbose", C.c_int),
                 ("refinement", C.c_int), ("inner_tolerance", C.c_double),
-                ("inner_maxiterations", C.c_int)]
+                ("inner_maxiterations", C.c_int), ("residual_check", C.c_int)]
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [("count", C.c_int), ("converged", C.c_int), ("outer_iterations", C.c_int),
+                ("max_inner_iterations", C.c_int), ("total_inner_iterations", C.c_int),
+                ("max_true_rel_residual", C.c_double), ("time_s", C.c_double),
+                ("tile_x", C.c_int), ("tile_y", C.c_int)]
 
 
 class SolverInfo(C.Structure):
@@ -122,6 +131,18 @@ def load_library(path=LIBPATH):
         "fem_solver_unbind": ([_vp], C.c_int),
         "fem_solver_solve": ([_vp, _vp, _vp, C.POINTER(SolverInfo)], C.c_int),
         "fem_gauss_points": ([C.c_int, _dp, _dp], C.c_int),
+        "fem_batch_create": ([_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_double,
+                              C.POINTER(_vp)], C.c_int),
+        "fem_batch_destroy": ([_vp], C.c_int),
+        "fem_batch_ndofs": ([_vp], C.c_int64),
+        "fem_batch_dof_offsets": ([_vp, C.POINTER(C.c_int64)], C.c_int),
+        "fem_batch_set_kappa_qp": ([_vp, _dp], C.c_int),
+        "fem_batch_apply": ([_vp, _vp, _vp], C.c_int),
+        "fem_batch_assemble_rhs": ([_vp, _dp, _vp], C.c_int),
+        "fem_batch_autotune": ([_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float)], C.c_int),
+        "fem_batch_solve": ([_vp, _vp, _vp, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                             C.POINTER(BatchInfo)], C.c_int),
+        "fem_batch_instance_results": ([_vp, C.POINTER(C.c_int), _dp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -334,7 +355,7 @@ def solver_parameters(**kw):
     load_library().fem_solver_default_params(C.byref(p))
     for k, v in kw.items():
         if k in ("method", "errormeasure", "maxiterations", "restart", "verbose", "refinement",
-                 "inner_maxiterations"):
+                 "inner_maxiterations", "residual_check"):
             setattr(p, k, int(v))
         elif k in ("tolerance", "inner_tolerance"):
             setattr(p, k, float(v))
@@ -382,6 +403,63 @@ class InverseOperator:
         try:
             if self.h:
                 _lib.fem_solver_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many independent continuous-Q2 problems solved together (one launch per step)."""
+
+    def __init__(self, ctx, sides, c=0.0):
+        self.ctx = ctx
+        nx = np.array([s[0] for s in sides], dtype=np.int32)
+        ny = np.array([s[1] for s in sides], dtype=np.int32)
+        h = _vp()
+        _check(_lib.fem_batch_create(ctx.h, len(sides), nx.ctypes.data_as(C.POINTER(C.c_int)),
+                                     ny.ctypes.data_as(C.POINTER(C.c_int)), c, C.byref(h)))
+        self.h = h
+        self.count = len(sides)
+        self.N = _lib.fem_batch_ndofs(h)
+        off = np.zeros(self.count + 1, dtype=np.int64)
+        _check(_lib.fem_batch_dof_offsets(h, off.ctypes.data_as(C.POINTER(C.c_int64))))
+        self.offsets = off
+
+    def set_kappa_qp(self, kvol):
+        kvol = np.ascontiguousarray(kvol, dtype=np.float64)
+        _check(_lib.fem_batch_set_kappa_qp(self.h, _dptr(kvol)))
+
+    def vector(self, data=None, precision=F64):
+        return DofVector(self.ctx, self.N, precision, data)
+
+    def __call__(self, u, w):
+        _check(_lib.fem_batch_apply(self.h, u.h, w.h))
+
+    def rhs(self, fq):
+        fq = np.ascontiguousarray(fq, dtype=np.float64)
+        F = self.vector()
+        _check(_lib.fem_batch_assemble_rhs(self.h, _dptr(fq), F.h))
+        return F
+
+    def autotune(self):
+        tx, ty = C.c_int(), C.c_int()
+        ms = (C.c_float * 5)()
+        _check(_lib.fem_batch_autotune(self.h, C.byref(tx), C.byref(ty), ms))
+        return (tx.value, ty.value), list(ms)
+
+    def solve(self, b, x, tol=1e-8, inner_tol=1e-3, inner_maxit=5000, max_outer=20, use_graph=True):
+        info = BatchInfo()
+        _check(_lib.fem_batch_solve(self.h, b.h, x.h, tol, inner_tol, inner_maxit, max_outer,
+                                    1 if use_graph else 0, C.byref(info)))
+        its = np.zeros(self.count, dtype=np.int32)
+        rr = np.zeros(self.count)
+        _check(_lib.fem_batch_instance_results(self.h, its.ctypes.data_as(C.POINTER(C.c_int)), _dptr(rr)))
+        return {f: getattr(info, f) for f, _ in BatchInfo._fields_} | {"iterations": its, "rel_residual": rr}
+
+    def __del__(self):
+        try:
+            if self.h:
+                _lib.fem_batch_destroy(self.h)
                 self.h = None
         except Exception:
             pass

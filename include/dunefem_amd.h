@@ -168,6 +168,10 @@ typedef struct {
     int refinement;
     double inner_tolerance;   /* relative reduction of each inner solve        */
     int inner_maxiterations;
+    /* CG stopping rule: 0 = the reference rule <r, M^-1 r> <= eps^2 ref
+     * (dune/fem/solver/linear/cg.hh:62-70); 1 = on the unpreconditioned
+     * residual ||r||_2^2 <= eps^2 ref (GMRES always stops on ||r||_2) */
+    int residual_check;
 } fem_solver_params;
 
 typedef struct {
@@ -183,7 +187,8 @@ void fem_solver_default_params(fem_solver_params *p);
  * (dune/fem/solver/parameter.hh:71-250): "fem.solver.method" (cg|gmres|fgmres),
  * "fem.solver.tolerance", "fem.solver.errormeasure" (absolute|relative|
  * residualreduction), "fem.solver.maxiterations", "fem.solver.gmres.restart",
- * "fem.solver.verbose"; the "linear." prefix is accepted as well */
+ * "fem.solver.verbose", "fem.solver.residualcheck" (true|false); the "linear." prefix
+ * is accepted as well */
 int fem_solver_params_set(fem_solver_params *p, const char *key, const char *value);
 int fem_solver_create(fem_ctx *ctx, const fem_solver_params *params, fem_solver **out);
 int fem_solver_destroy(fem_solver *s);
@@ -194,6 +199,47 @@ int fem_solver_bind(fem_solver *s, fem_op *op, fem_precond *pre, fem_op *inner_o
 int fem_solver_unbind(fem_solver *s);
 /* operator()(b, x): x is the initial guess on entry, the solution on exit */
 int fem_solver_solve(fem_solver *s, const fem_vec *b, fem_vec *x, fem_solver_info *info);
+
+/* ---- batched solves -----------------------------------------------------------
+ * Many independent continuous-Q2 problems -div(kappa grad u) + c u = f on
+ * [0,1]^2 (strong u = 0), of different mesh sizes, solved together: every
+ * launch covers all unconverged instances.  No reference counterpart (the
+ * reference solves one problem per FemScheme::solve,
+ * dune/fem/schemes/femscheme.hh:195-253); each instance follows the CG
+ * recurrence of dune/fem/solver/linear/cg.hh with Jacobi preconditioning in
+ * FP32, inside an FP64 iterative refinement loop on the true residual.
+ * DoFs of all instances are concatenated in instance order; coefficient and
+ * forcing arrays are concatenated [sum_i E_i][9]. */
+typedef struct fem_batch fem_batch;
+
+typedef struct {
+    int count;
+    int converged;                 /* instances with FP64 ||b-Ax|| <= tol ||b||  */
+    int outer_iterations;          /* refinement steps                           */
+    int max_inner_iterations;      /* sum over steps of the slowest instance     */
+    int total_inner_iterations;
+    double max_true_rel_residual;
+    double time_s;
+    int tile_x, tile_y;            /* tile shape in use (autotuner)              */
+} fem_batch_info;
+
+int fem_batch_create(fem_ctx *ctx, int count, const int *nx, const int *ny, double c, fem_batch **out);
+int fem_batch_destroy(fem_batch *b);
+int64_t fem_batch_ndofs(const fem_batch *b);
+/* count + 1 DoF offsets */
+int fem_batch_dof_offsets(const fem_batch *b, int64_t *off);
+int fem_batch_set_kappa_qp(fem_batch *b, const double *kvol);
+/* w = A u for all instances (FP64 or FP32 vectors of fem_batch_ndofs) */
+int fem_batch_apply(fem_batch *b, const fem_vec *u, fem_vec *w);
+int fem_batch_assemble_rhs(fem_batch *b, const double *fq, fem_vec *F);
+/* time every tile shape on the batch and keep the fastest; returns its shape */
+int fem_batch_autotune(fem_batch *b, int *tile_x, int *tile_y, float *ms_per_shape /* 5 entries or NULL */);
+/* x: initial guess in, solution out (FP64).  use_graph: replay one captured
+ * inner step as a HIP graph */
+int fem_batch_solve(fem_batch *b, const fem_vec *rhs, fem_vec *x, double tol, double inner_tol, int inner_maxit,
+                    int max_outer, int use_graph, fem_batch_info *info);
+/* per-instance results of the last solve: inner iterations and FP64 relative residual */
+int fem_batch_instance_results(const fem_batch *b, int *iterations, double *rel_residual);
 
 /* ---- quadrature (shared with hosts that generate coefficient data) ---------- */
 /* m-point Gauss-Legendre rule on [0,1], ascending points */

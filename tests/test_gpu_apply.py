@@ -182,3 +182,27 @@ def test_blas_and_dot_deterministic(ctx):
         A32 = fa.DofVector(ctx, n, fa.F32)
         A32.assign(B)
         assert np.abs(A32.download() - b.astype(np.float32)).max() == 0
+
+
+@pytest.mark.parametrize("n,p,kind", [((4, 3, 5), 1, "qp"), ((3, 4, 2), 2, "cell"), ((5, 3, 4), 3, "qp"),
+                                      ((3, 3, 2), 4, "qp")])
+def test_dg_apply_bf16_mfma_error_bound(ctx, n, p, kind):
+    # BF16 inputs to each contraction, FP32 accumulation: relative L2 error
+    # vs the FP64 oracle below 1e-2 (measured values are printed)
+    rng = np.random.default_rng(21)
+    E = n[0] * n[1] * n[2]
+    if kind == "qp":
+        kv, kf = random_qp_kappa(rng, n, p)
+        kappa = ("qp", kv, kf)
+    else:
+        kappa = ("cell", rng.uniform(0.1, 10, size=E))
+    b = (1, .5, .25)
+    oop = ora.DGOperator(n, p, 10.0 * p * p, b, 0.0, kappa=kappa)
+    gop = make_dg(ctx, n, p, 10.0 * p * p, b, 0.0, kappa, precision=fa.BF16MFMA)
+    u = rng.normal(size=oop.N)
+    wref = oop.apply(u)
+    U, W = gop.vector(u), gop.vector()
+    gop(U, W)
+    err = np.linalg.norm(W.download() - wref) / np.linalg.norm(wref)
+    print("bf16 mfma", n, p, kind, "rel l2 err", err, flush=True)
+    assert err < 1e-2
