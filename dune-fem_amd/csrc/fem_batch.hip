@@ -46,10 +46,12 @@ __device__ __forceinline__ double bsum(double v, double *lds)
     return block_sum(v, lds);
 }
 
-// last block of an instance: sums that instance's partials in block order
+// last block of an instance: sums that instance's partials in a fixed order
+// (thread t takes blocks first+t, first+t+256, ...; then a block tree)
 __device__ __forceinline__ bool seg_finish(const SegTab &S, int inst, int nq, const double *own, double *tot)
 {
     __shared__ bool amLast;
+    __shared__ double lds2[SEG_THREADS / 64];
     const int nb = gridDim.x;
     if (threadIdx.x == 0) {
         for (int q = 0; q < nq; ++q) S.partials[(size_t)q * nb + blockIdx.x] = own[q];
@@ -60,15 +62,16 @@ __device__ __forceinline__ bool seg_finish(const SegTab &S, int inst, int nq, co
     __syncthreads();
     if (!amLast) return false;
     __threadfence();
-    if (threadIdx.x == 0) {
-        for (int q = 0; q < nq; ++q) {
-            double s = 0;
-            for (int b = S.first[inst]; b < S.first[inst + 1]; ++b)
-                s += __builtin_nontemporal_load(&S.partials[(size_t)q * nb + b]);
-            tot[q] = s;
-        }
-        S.counter[inst] = 0;
+    const int b0 = S.first[inst], b1 = S.first[inst + 1];
+    for (int q = 0; q < nq; ++q) {
+        double s = 0;
+        for (int b = b0 + threadIdx.x; b < b1; b += SEG_THREADS)
+            s += __builtin_nontemporal_load(&S.partials[(size_t)q * nb + b]);
+        const double t = block_sum(s, lds2);
+        if (threadIdx.x == 0) tot[q] = t;
+        __syncthreads();
     }
+    if (threadIdx.x == 0) S.counter[inst] = 0;
     return true;
 }
 

@@ -8,6 +8,8 @@
 
 #include "fem_batch.hpp"
 #include "fem_blas.hpp"
+#include "fem_cnn.hpp"
+#include "fem_learned.hpp"
 #include "fem_op.hpp"
 #include "fem_solver.hpp"
 
@@ -437,6 +439,157 @@ int fem_solver_solve(fem_solver *s, const fem_vec *b, fem_vec *x, fem_solver_inf
         FEM_CHECK(s && b && x, FEM_ERR_ARG, "null argument");
         fem_solver_info tmp;
         s->s.solve(b->v, x->v, info ? *info : tmp);
+    });
+}
+
+// ---- learned preconditioner ------------------------------------------------------
+int fem_cnn_create(fem_ctx *ctx, int nx, int ny, int nz, const int *dil, fem_cnn **out)
+{
+    return guarded([&] {
+        FEM_CHECK(ctx && out, FEM_ERR_ARG, "null argument");
+        const int n[3] = { nx, ny, nz };
+        const int d0[CNN_LAYERS] = { 1, 2, 4, 1 };
+        fem_cnn *h = new fem_cnn();
+        try {
+            h->c.init(&ctx->c, n, dil ? dil : d0);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int fem_cnn_destroy(fem_cnn *c)
+{
+    return guarded([&] {
+        if (c) {
+            (void)hipStreamSynchronize(c->c.ctx->stream);
+            delete c;
+        }
+    });
+}
+
+int64_t fem_cnn_nparams(const fem_cnn *c) { return c ? (int64_t)c->c.nparams : -1; }
+
+int fem_cnn_set_weights(fem_cnn *c, const float *w)
+{
+    return guarded([&] {
+        FEM_CHECK(c && w, FEM_ERR_ARG, "null argument");
+        c->c.set_weights(w);
+    });
+}
+
+int fem_cnn_get_weights(const fem_cnn *c, float *w)
+{
+    return guarded([&] {
+        FEM_CHECK(c && w, FEM_ERR_ARG, "null argument");
+        c->c.get_weights(w);
+    });
+}
+
+int fem_cnn_forward(fem_cnn *c, const float *x, float *y)
+{
+    return guarded([&] {
+        FEM_CHECK(c && x && y, FEM_ERR_ARG, "null argument");
+        CNN &n = c->c;
+        hipStream_t s = n.ctx->stream;
+        FEM_HIP(hipMemcpyAsync(n.x0.p, x, sizeof(float) * 2 * n.V, hipMemcpyHostToDevice, s));
+        n.forward(n.x0.as<float>(), n.y.as<float>(), s);
+        FEM_HIP(hipMemcpyAsync(y, n.y.p, sizeof(float) * n.V, hipMemcpyDeviceToHost, s));
+        FEM_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int fem_cnn_loss_grad(fem_cnn *c, const float *x, const float *t, double *loss, float *grad)
+{
+    return guarded([&] {
+        FEM_CHECK(c && x && t, FEM_ERR_ARG, "null argument");
+        CNN &n = c->c;
+        hipStream_t s = n.ctx->stream;
+        DevBuf td;
+        td.alloc(sizeof(float) * n.V);
+        FEM_HIP(hipMemcpyAsync(n.x0.p, x, sizeof(float) * 2 * n.V, hipMemcpyHostToDevice, s));
+        FEM_HIP(hipMemcpyAsync(td.p, t, sizeof(float) * n.V, hipMemcpyHostToDevice, s));
+        n.forward(n.x0.as<float>(), n.y.as<float>(), s);
+        n.zero_grad(s);
+        n.backward_accumulate(td.as<float>(), s);
+        if (loss) *loss = n.loss_value(s);
+        if (grad) FEM_HIP(hipMemcpyAsync(grad, n.grad.p, sizeof(float) * n.nparams, hipMemcpyDeviceToHost, s));
+        FEM_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int fem_cnn_set_coefficient(fem_cnn *c, const double *kcell, double beta)
+{
+    return guarded([&] {
+        FEM_CHECK(c && kcell && beta > 0, FEM_ERR_ARG, "null argument or beta <= 0");
+        cnn_set_coefficient(c->c, kcell, beta);
+    });
+}
+
+int fem_cnn_coarse_apply(fem_cnn *c, const float *x, float *y)
+{
+    return guarded([&] {
+        FEM_CHECK(c && x && y && c->c.stencil.p, FEM_ERR_ARG, "null argument or coefficient not set");
+        CNN &n = c->c;
+        hipStream_t s = n.ctx->stream;
+        DevBuf a, b;
+        a.alloc(sizeof(float) * n.V);
+        b.alloc(sizeof(float) * n.V);
+        FEM_HIP(hipMemcpyAsync(a.p, x, sizeof(float) * n.V, hipMemcpyHostToDevice, s));
+        n.coarse_apply(a.as<float>(), b.as<float>(), s);
+        FEM_HIP(hipMemcpyAsync(y, b.p, sizeof(float) * n.V, hipMemcpyDeviceToHost, s));
+        FEM_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int fem_cnn_collect_pairs(fem_cnn *c, fem_op *op64, fem_precond *bj64, const fem_vec *b, int max_pairs, int every,
+                          int *npairs)
+{
+    return guarded([&] {
+        FEM_CHECK(c && op64 && bj64 && b, FEM_ERR_ARG, "null argument");
+        const int n = cnn_collect_pairs(c->c, &op64->o, &bj64->p, b->v, max_pairs, every);
+        if (npairs) *npairs = n;
+    });
+}
+
+int fem_cnn_train(fem_cnn *c, int epochs, int batch, float lr, float val_frac, unsigned seed, double *train_loss,
+                  double *val_rel)
+{
+    return guarded([&] {
+        FEM_CHECK(c && epochs >= 1 && batch >= 1, FEM_ERR_ARG, "train arguments");
+        double tl = 0, vr = 0;
+        cnn_train(c->c, epochs, batch, lr, val_frac, seed, tl, vr);
+        if (train_loss) *train_loss = tl;
+        if (val_rel) *val_rel = vr;
+    });
+}
+
+int fem_precond_create_learned(fem_op *op, fem_cnn *cnn, int multiplicative, fem_precond **out)
+{
+    return guarded([&] {
+        FEM_CHECK(op && cnn && out, FEM_ERR_ARG, "null argument");
+        fem_precond *h = new fem_precond();
+        try {
+            h->p.op = &op->o;
+            h->p.kind = FEM_PRE_LEARNED;
+            h->p.prec = op->o.vec_prec;
+            h->p.learned = new Learned();
+            h->p.learned->init(&op->o, &cnn->c, multiplicative);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int fem_precond_set_learned_enabled(fem_precond *p, int enabled)
+{
+    return guarded([&] {
+        FEM_CHECK(p && p->p.kind == FEM_PRE_LEARNED, FEM_ERR_ARG, "not a learned preconditioner");
+        p->p.learned->enabled = enabled != 0;
     });
 }
 

@@ -3,6 +3,7 @@
 #include <cmath>
 
 #include "fem_blas.hpp"
+#include "fem_learned.hpp"
 #include "fem_op.hpp"
 
 namespace femamd {
@@ -271,15 +272,19 @@ void Precond::init(Operator *o, int k_)
         FEM_HIP(hipStreamSynchronize(s));
     } else if (kind == FEM_PRE_NONE) {
     } else {
-        throw Error(FEM_ERR_UNSUPPORTED, "preconditioner kind");
+        throw Error(FEM_ERR_UNSUPPORTED, "preconditioner kind (the learned one is made by fem_precond_create_learned)");
     }
 }
+
+Precond::~Precond() { delete learned; }
 
 void Precond::apply_raw(const void *x, void *y, const int *skip) const
 {
     hipStream_t s = op->ctx->stream;
     const int64_t N = op->N;
-    if (kind == FEM_PRE_JACOBI) {
+    if (kind == FEM_PRE_LEARNED) {
+        learned->apply_raw(x, y, skip);
+    } else if (kind == FEM_PRE_JACOBI) {
         if (prec == FEM_F64)
             hipLaunchKernelGGL(k_pointwise_mul<double>, dim3(grid_for(N, 256, 8192)), dim3(256), 0, s, dinv.as<double>(), (const double *)x, (double *)y, N, skip);
         else

@@ -58,15 +58,19 @@ struct Operator {
     bool is_dg() const { return desc.kind == FEM_OP_DG_SIPG_3D; }
 };
 
+struct Learned;
+
 struct Precond {
     Operator *op = nullptr;
     int kind = FEM_PRE_NONE;
     int prec = FEM_F64;
     DevBuf dinv;       // Jacobi
     DevBuf Q, lam;     // block Jacobi (tensor-product eigen-decomposition)
+    Learned *learned = nullptr;   // FEM_PRE_LEARNED (owned)
     void init(Operator *o, int kind);
     void apply_raw(const void *x, void *y, const int *skip) const;
     bool pointwise() const { return kind == FEM_PRE_JACOBI; }
+    ~Precond();
 };
 
 }  // namespace femamd

@@ -22,6 +22,7 @@
 #include <cstdio>
 
 #include "fem_blas.hpp"
+#include "fem_learned.hpp"
 #include "fem_solver.hpp"
 
 namespace femamd {
@@ -340,6 +341,14 @@ __global__ void k_gm_update(TU *__restrict__ u, const T *__restrict__ Z, int64_t
     }
 }
 
+// t = b - t
+template <typename T>
+__global__ void k_bmax(const T *__restrict__ b, T *__restrict__ t, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        t[i] = b[i] - t[i];
+}
+
 template <typename T>
 __global__ void k_add(T *__restrict__ u, const T *__restrict__ z, int64_t n)
 {
@@ -460,7 +469,7 @@ static int run_krylov(Solver &S, int method, Operator *o, Precond *p, const T *b
         hipLaunchKernelGGL(k_cg_init, dim3(1), dim3(1), 0, s, st, eps, criteria, maxit, S.prm.residual_check);
         FEM_HIP(hipGetLastError());
         const T *dinv = (haspre && p->pointwise()) ? (p->prec == FEM_F64 ? (const T *)p->dinv.p : (const T *)p->dinv.p) : nullptr;
-        int chunk = 4;
+        int chunk = S.snapshot ? S.snap_chunk : 4;
         while (true) {
             for (int c = 0; c < chunk; ++c) {
                 hipLaunchKernelGGL(k_cg_qupd<T>, dim3(G), dim3(256), 0, s, q, ss, pp, r, N, st, haspre ? 1 : 0);
@@ -480,7 +489,8 @@ static int run_krylov(Solver &S, int method, Operator *o, Precond *p, const T *b
             S.fetch_state();
             if (verbose) std::printf("Fem::CG it: %d : residual %g\n", S.hstate->it, std::sqrt(S.hstate->res));
             if (S.hstate->done) break;
-            if (chunk < 64) chunk *= 2;
+            if (S.snapshot) S.snapshot(x, pp);
+            else if (chunk < 64) chunk *= 2;
         }
         const int it = S.hstate->it;
         return it < maxit ? it : -it;
@@ -510,7 +520,17 @@ static int run_krylov(Solver &S, int method, Operator *o, Precond *p, const T *b
         FEM_HIP(hipMemcpyAsync(st, S.hstate, sizeof(KState), hipMemcpyHostToDevice, s));
     }
     bool first = true;
+    Learned *lp = (p && p->kind == FEM_PRE_LEARNED) ? p->learned : nullptr;
+    T *xsave = lp ? (T *)S.workbuf(4, vb) : nullptr;
+    T *rtmp = (lp && lp->restart_mode()) ? (T *)S.workbuf(5, vb) : nullptr;
     while (true) {
+        if (lp && lp->enabled) FEM_HIP(hipMemcpyAsync(xsave, x, vb, hipMemcpyDeviceToDevice, s));
+        if (lp && lp->enabled && lp->restart_mode()) {
+            // learned coarse correction of the iterate from its true residual
+            o->apply_raw(x, rtmp, nullptr);
+            hipLaunchKernelGGL(k_bmax<T>, dim3(G), dim3(256), 0, s, b, rtmp, N);
+            lp->correct(rtmp, x);
+        }
         o->apply_raw(x, V, nullptr);
         hipLaunchKernelGGL(k_gm_start<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, V, b, N, st, part, cnt, first ? 1 : 0);
         hipLaunchKernelGGL(k_gm_scale<T>, dim3(G), dim3(256), 0, s, V, N, st);
@@ -541,7 +561,16 @@ static int run_krylov(Solver &S, int method, Operator *o, Precond *p, const T *b
         }
         FEM_HIP(hipGetLastError());
         S.fetch_state();
-        if (verbose) std::printf("Fem::%sGMRES it: %d : %g\n", flex ? "F" : "", S.hstate->it, std::fabs(S.hstate->g[S.hstate->jlast]));
+        const double gres = std::fabs(S.hstate->g[S.hstate->jlast]);
+        if (verbose) std::printf("Fem::%sGMRES it: %d : %g\n", flex ? "F" : "", S.hstate->it, gres);
+        if (lp && lp->enabled && !S.hstate->converged && S.hstate->jlast > 0 && !(gres < 0.95 * S.hstate->res)) {
+            // the learned correction stagnates or broke down: undo the cycle
+            // and continue with block Jacobi only
+            FEM_HIP(hipMemcpyAsync(x, xsave, vb, hipMemcpyDeviceToDevice, s));
+            lp->enabled = false;
+            S.fallback_used = 1;
+            continue;
+        }
         if (S.hstate->converged || S.hstate->it >= maxit) break;
     }
     const int it = S.hstate->it;
@@ -573,6 +602,7 @@ void Solver::solve(const Vec &b, Vec &x, fem_solver_info &info)
     FEM_CHECK(b.n == op->N && x.n == op->N, FEM_ERR_ARG, "solve: vector size");
     FEM_CHECK(b.prec == op->vec_prec && x.prec == op->vec_prec, FEM_ERR_ARG, "solve: vector precision");
     std::memset(&info, 0, sizeof(info));
+    fallback_used = 0;
     const int64_t N = op->N;
     FEM_HIP(hipStreamSynchronize(ctx->stream));
     auto t0 = std::chrono::steady_clock::now();
@@ -629,6 +659,7 @@ void Solver::solve(const Vec &b, Vec &x, fem_solver_info &info)
     r.data = workbuf(20, x.bytes());
     r.owner = false;
     info.true_rel_residual = true_residual(b, x, r);
+    info.fallback = fallback_used;
 }
 
 }  // namespace femamd

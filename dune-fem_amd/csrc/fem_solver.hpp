@@ -1,6 +1,8 @@
 // Device-driven Krylov solvers behind the InverseOperator interface.
 #pragma once
 
+#include <functional>
+
 #include "fem_op.hpp"
 
 namespace femamd {
@@ -31,6 +33,10 @@ struct Solver {
     DevBuf state, partials, counters;
     KState *hstate = nullptr;   // pinned mirror
     std::vector<DevBuf *> work;
+    // CG: called with the device iterate and residual every snap_chunk steps
+    std::function<void(const void *x, const void *r)> snapshot;
+    int snap_chunk = 0;
+    int fallback_used = 0;
 
     void init(Context *c, const fem_solver_params &p);
     void bind(Operator *o, Precond *p, Operator *io, Precond *ip);

@@ -45,6 +45,9 @@ EXPORTS = [
     "fem_gauss_points", "fem_batch_create", "fem_batch_destroy", "fem_batch_ndofs",
     "fem_batch_dof_offsets", "fem_batch_set_kappa_qp", "fem_batch_apply", "fem_batch_assemble_rhs",
     "fem_batch_autotune", "fem_batch_solve", "fem_batch_instance_results",
+    "fem_cnn_create", "fem_cnn_destroy", "fem_cnn_nparams", "fem_cnn_set_weights", "fem_cnn_get_weights",
+    "fem_cnn_forward", "fem_cnn_loss_grad", "fem_cnn_set_coefficient", "fem_cnn_collect_pairs",
+    "fem_cnn_train", "fem_precond_create_learned", "fem_precond_set_learned_enabled", "fem_cnn_coarse_apply",
 ]
 
 
@@ -74,7 +77,7 @@ class BatchInfo(C.Structure):
 
 class SolverInfo(C.Structure):
     _fields_ = [("converged", C.c_int), ("iterations", C.c_int), ("outer_iterations", C.c_int),
-                ("true_rel_residual", C.c_double), ("time_s", C.c_double)]
+                ("true_rel_residual", C.c_double), ("time_s", C.c_double), ("fallback", C.c_int)]
 
 
 _lib = None
@@ -143,6 +146,19 @@ def load_library(path=LIBPATH):
         "fem_batch_solve": ([_vp, _vp, _vp, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
                              C.POINTER(BatchInfo)], C.c_int),
         "fem_batch_instance_results": ([_vp, C.POINTER(C.c_int), _dp], C.c_int),
+        "fem_cnn_create": ([_vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(_vp)], C.c_int),
+        "fem_cnn_destroy": ([_vp], C.c_int),
+        "fem_cnn_nparams": ([_vp], C.c_int64),
+        "fem_cnn_set_weights": ([_vp, C.POINTER(C.c_float)], C.c_int),
+        "fem_cnn_get_weights": ([_vp, C.POINTER(C.c_float)], C.c_int),
+        "fem_cnn_forward": ([_vp, C.POINTER(C.c_float), C.POINTER(C.c_float)], C.c_int),
+        "fem_cnn_loss_grad": ([_vp, C.POINTER(C.c_float), C.POINTER(C.c_float), _dp, C.POINTER(C.c_float)], C.c_int),
+        "fem_cnn_set_coefficient": ([_vp, _dp, C.c_double], C.c_int),
+        "fem_cnn_coarse_apply": ([_vp, C.POINTER(C.c_float), C.POINTER(C.c_float)], C.c_int),
+        "fem_cnn_collect_pairs": ([_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.POINTER(C.c_int)], C.c_int),
+        "fem_cnn_train": ([_vp, C.c_int, C.c_int, C.c_float, C.c_float, C.c_uint, _dp, _dp], C.c_int),
+        "fem_precond_create_learned": ([_vp, _vp, C.c_int, C.POINTER(_vp)], C.c_int),
+        "fem_precond_set_learned_enabled": ([_vp, C.c_int], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -397,7 +413,8 @@ class InverseOperator:
         i = self._info
         return {"converged": bool(i.converged), "iterations": i.iterations,
                 "linear_iterations": i.iterations, "outer_iterations": i.outer_iterations,
-                "true_rel_residual": i.true_rel_residual, "timing": [i.time_s]}
+                "true_rel_residual": i.true_rel_residual, "timing": [i.time_s],
+                "fallback": bool(i.fallback)}
 
     def __del__(self):
         try:
@@ -463,3 +480,98 @@ class Batch:
                 self.h = None
         except Exception:
             pass
+
+
+def _fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class CNN:
+    """Coarse-correction network on an element grid (BF16 MFMA kernels)."""
+
+    LAYERS = ((2, 32), (32, 32), (32, 32), (32, 1))
+
+    def __init__(self, ctx, n, dilations=(1, 2, 4, 1)):
+        self.ctx = ctx
+        self.n = tuple(int(v) for v in n)
+        d = (C.c_int * 4)(*dilations)
+        h = _vp()
+        _check(_lib.fem_cnn_create(ctx.h, self.n[0], self.n[1], self.n[2], d, C.byref(h)))
+        self.h = h
+        self.dilations = tuple(dilations)
+        self.nparams = _lib.fem_cnn_nparams(h)
+        self.V = self.n[0] * self.n[1] * self.n[2]
+
+    def split(self, flat):
+        out, off = [], 0
+        for ci, co in self.LAYERS:
+            out.append(flat[off:off + 27 * ci * co].reshape(27, ci, co))
+            off += 27 * ci * co
+        return out
+
+    def get_weights(self):
+        w = np.zeros(self.nparams, dtype=np.float32)
+        _check(_lib.fem_cnn_get_weights(self.h, _fptr(w)))
+        return w
+
+    def set_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32).ravel()
+        assert w.size == self.nparams
+        _check(_lib.fem_cnn_set_weights(self.h, _fptr(w)))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(self.V, 2)
+        y = np.zeros(self.V, dtype=np.float32)
+        _check(_lib.fem_cnn_forward(self.h, _fptr(x), _fptr(y)))
+        return y
+
+    def loss_grad(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(self.V, 2)
+        t = np.ascontiguousarray(t, dtype=np.float32).reshape(self.V)
+        loss = C.c_double()
+        g = np.zeros(self.nparams, dtype=np.float32)
+        _check(_lib.fem_cnn_loss_grad(self.h, _fptr(x), _fptr(t), C.byref(loss), _fptr(g)))
+        return loss.value, g
+
+    def set_coefficient(self, kcell, beta):
+        kcell = np.ascontiguousarray(kcell, dtype=np.float64)
+        _check(_lib.fem_cnn_set_coefficient(self.h, _dptr(kcell), float(beta)))
+
+    def coarse_apply(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32).ravel()
+        y = np.zeros(self.V, dtype=np.float32)
+        _check(_lib.fem_cnn_coarse_apply(self.h, _fptr(x), _fptr(y)))
+        return y
+
+    def collect_pairs(self, op64, bj64, b, max_pairs, every=10):
+        n = C.c_int()
+        _check(_lib.fem_cnn_collect_pairs(self.h, op64.h, bj64.h, b.h, max_pairs, every, C.byref(n)))
+        return n.value
+
+    def train(self, epochs=10, batch=8, lr=1e-3, val_frac=0.1, seed=0):
+        tl, vr = C.c_double(), C.c_double()
+        _check(_lib.fem_cnn_train(self.h, epochs, batch, lr, val_frac, seed, C.byref(tl), C.byref(vr)))
+        return tl.value, vr.value
+
+    def __del__(self):
+        try:
+            if self.h:
+                _lib.fem_cnn_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class LearnedPreconditioner(Preconditioner):
+    """Two-level preconditioner: CNN coarse correction + exact block Jacobi."""
+
+    def __init__(self, op, cnn, multiplicative=1):
+        """multiplicative: 0 additive, 1 multiplicative two-level, 2 correction at FGMRES restarts"""
+        self.op = op
+        self.cnn = cnn
+        h = _vp()
+        _check(_lib.fem_precond_create_learned(op.h, cnn.h, int(multiplicative), C.byref(h)))
+        self.h = h
+
+    def set_enabled(self, on):
+        _check(_lib.fem_precond_set_learned_enabled(self.h, 1 if on else 0))

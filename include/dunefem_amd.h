@@ -180,6 +180,8 @@ typedef struct {
     int outer_iterations;    /* refinement steps (0 for a plain solve)         */
     double true_rel_residual; /* ||b - A x||_2 / ||b||_2 in FP64                */
     double time_s;
+    int fallback;            /* 1: the learned preconditioner stalled and the
+                                solve continued with block Jacobi            */
 } fem_solver_info;
 
 void fem_solver_default_params(fem_solver_params *p);
@@ -199,6 +201,42 @@ int fem_solver_bind(fem_solver *s, fem_op *op, fem_precond *pre, fem_op *inner_o
 int fem_solver_unbind(fem_solver *s);
 /* operator()(b, x): x is the initial guess on entry, the solution on exit */
 int fem_solver_solve(fem_solver *s, const fem_vec *b, fem_vec *x, fem_solver_info *info);
+
+/* ---- learned coarse-space preconditioner ----------------------------------------
+ * A 4-layer 3D CNN (3x3x3 dilated convolutions, 32 channels, no biases, ReLU)
+ * on the element grid maps the normalised element-mean residual (and residual
+ * / kappa) to the element-mean error; BF16 weights and activations, FP32
+ * accumulation, BF16 MFMA for the hidden layers.  Used in FGMRES as a
+ * two-level preconditioner with the exact block-Jacobi smoother, falling back
+ * to block Jacobi when a restart cycle stalls.  No reference counterpart. */
+typedef struct fem_cnn fem_cnn;
+
+int fem_cnn_create(fem_ctx *ctx, int nx, int ny, int nz, const int *dilations /* 4 */, fem_cnn **out);
+int fem_cnn_destroy(fem_cnn *c);
+int64_t fem_cnn_nparams(const fem_cnn *c);
+/* weights: layer blocks [27][cin][cout], tap t = (dx+1) + 3(dy+1) + 9(dz+1) */
+int fem_cnn_set_weights(fem_cnn *c, const float *w);
+int fem_cnn_get_weights(const fem_cnn *c, float *w);
+/* host arrays: x [V][2] (voxel x fastest), y [V] */
+int fem_cnn_forward(fem_cnn *c, const float *x, float *y);
+/* MSE loss mean((y - t)^2) and its gradient w.r.t. the weights (host arrays) */
+int fem_cnn_loss_grad(fem_cnn *c, const float *x, const float *t, double *loss, float *grad);
+/* per-element kappa and the SIPG penalty factor: sets the second input
+ * channel (1 / kappa) and the Galerkin coarse operator A_c = P^T A P of
+ * element means (7-point stencil: only the penalty terms act on piecewise
+ * constants), whose energy 1/2 c^T A_c c - c^T R r is the training objective */
+int fem_cnn_set_coefficient(fem_cnn *c, const double *kcell, double beta);
+/* y = A_c x on element means (host arrays [V]) */
+int fem_cnn_coarse_apply(fem_cnn *c, const float *x, float *y);
+/* plain FP64 CG + block-Jacobi solve of op64 x = b from x = 0; stores coarse
+ * (residual, error) pairs of every `every`-th iterate (at most max_pairs) */
+int fem_cnn_collect_pairs(fem_cnn *c, fem_op *op64, fem_precond *bj64, const fem_vec *b, int max_pairs, int every,
+                          int *npairs);
+int fem_cnn_train(fem_cnn *c, int epochs, int batch, float lr, float val_frac, unsigned seed, double *train_loss,
+                  double *val_rel_mse);
+int fem_precond_create_learned(fem_op *op, fem_cnn *cnn, int multiplicative, fem_precond **out);
+/* 0: use block Jacobi only (ablation / manual fallback) */
+int fem_precond_set_learned_enabled(fem_precond *p, int enabled);
 
 /* ---- batched solves -----------------------------------------------------------
  * Many independent continuous-Q2 problems -div(kappa grad u) + c u = f on

@@ -1,0 +1,130 @@
+"""GPU CNN kernels (BF16 MFMA, FP32 accumulation) against the numpy oracle,
+and the learned preconditioner inside FGMRES."""
+import numpy as np
+import pytest
+
+import cnn_oracle as cnn
+import dunefem_amd as fa
+from dunefem_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def relf(a, b):
+    return np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-30)
+
+
+def grid_input(rng, n):
+    nx, ny, nz = n
+    x = rng.normal(size=(nz, ny, nx, 2)).astype(np.float32)
+    return cnn.bf16(x).astype(np.float32)
+
+
+@pytest.mark.parametrize("n,dil", [((10, 9, 8), (1, 2, 4, 1)), ((16, 16, 16), (1, 2, 4, 1)), ((7, 5, 3), (1, 1, 2, 1))])
+def test_cnn_forward_and_gradients_match_oracle(ctx, n, dil):
+    rng = np.random.default_rng(0)
+    net = fa.CNN(ctx, n, dil)
+    w = (rng.normal(size=net.nparams) * 0.2).astype(np.float32)
+    net.set_weights(w)
+    Ws = [x.astype(np.float64) for x in net.split(w)]
+    x = grid_input(rng, n)
+    t = rng.normal(size=(n[2], n[1], n[0], 1))
+    y = net.forward(x.reshape(-1, 2))
+    loss_o, grads_o, yo = cnn.loss_and_grads(x.astype(np.float64), t, Ws, list(dil), bf=True)
+    ef = relf(y, yo.ravel())
+    print("forward rel err", n, ef, flush=True)
+    assert ef < 2e-2
+    loss, g = net.loss_grad(x.reshape(-1, 2), t.ravel())
+    assert abs(loss - loss_o) < 2e-2 * abs(loss_o)
+    gs = net.split(g)
+    for l in range(4):
+        eg = relf(gs[l], grads_o[l])
+        print("layer", l, "grad rel err", eg, flush=True)
+        assert eg < 5e-2
+
+
+def test_cnn_training_reduces_loss_on_a_linear_target(ctx):
+    # learn t = smoothing of the first input channel: loss must drop
+    n = (12, 12, 12)
+    rng = np.random.default_rng(1)
+    net = fa.CNN(ctx, n)
+    x = grid_input(rng, n)
+    t = cnn.conv(x[..., :1].astype(np.float64), np.full((27, 1, 1), 1.0 / 27), 1)
+    l0, _ = net.loss_grad(x.reshape(-1, 2), t.ravel())
+    w = net.get_weights().astype(np.float64)
+    m = np.zeros_like(w); v = np.zeros_like(w)
+    for it in range(1, 201):   # Adam on the GPU gradients (host update keeps the test independent of the trainer)
+        loss, g = net.loss_grad(x.reshape(-1, 2), t.ravel())
+        m = 0.9 * m + 0.1 * g; v = 0.999 * v + 0.001 * g * g
+        w -= 3e-3 * (m / (1 - 0.9 ** it)) / (np.sqrt(v / (1 - 0.999 ** it)) + 1e-8)
+        net.set_weights(w.astype(np.float32))
+    l1, _ = net.loss_grad(x.reshape(-1, 2), t.ravel())
+    print("toy training loss", l0, "->", l1, flush=True)
+    assert l1 < 0.2 * l0
+
+
+def learned_setup(ctx, n=12, solves=8, pairs_per_solve=60, epochs=20):
+    P = wl.problem3(n=n)
+    op64 = fa.GalerkinOperator(ctx, fa.OP_DG_SIPG_3D, P["n"], P["p"], P["beta"], (0, 0, 0), 0.0, fa.KAPPA_CELL)
+    op64.set_kappa_cell(P["kcell"])
+    bj64 = fa.Preconditioner(op64, fa.PRE_BLOCK_JACOBI)
+    net = fa.CNN(ctx, P["n"])
+    net.set_coefficient(P["kcell"], P["beta"])
+    rng = np.random.default_rng(7)
+    m = P["p"] + 1
+    for s in range(solves):
+        modes = wl.fourier_modes(rng, 16, 3)
+        c = wl.qp_coords(n, m)
+        fq = wl.vol_layout3(wl.sine_sum_grid(modes, [c, c, c]), P["n"], m)
+        b = op64.rhs(fq)
+        net.collect_pairs(op64, bj64, b, pairs_per_solve, every=4)
+    tl, vr = net.train(epochs=epochs, batch=8, lr=1e-3)
+    return P, op64, bj64, net, tl, vr
+
+
+def test_coarse_operator_is_galerkin_projection(ctx):
+    # A_c = P^T A P of element means, checked against the oracle's DG operator
+    import pyoracle as ora
+    P = wl.problem3(n=5)
+    oop = ora.DGOperator(P["n"], P["p"], P["beta"], kappa=("cell", P["kcell"]))
+    net = fa.CNN(ctx, P["n"])
+    net.set_coefficient(P["kcell"], P["beta"])
+    rng = np.random.default_rng(3)
+    c = rng.normal(size=125)
+    u = np.zeros(oop.N)
+    u[::27] = c
+    ref = oop.apply(u)[::27]
+    got = net.coarse_apply(c)
+    assert np.abs(got - ref).max() <= 1e-5 * np.abs(ref).max()
+
+
+def test_learned_preconditioner_fgmres_ablation_and_fallback(ctx):
+    P, op64, bj64, net, tl, vr = learned_setup(ctx)
+    print("train loss", tl, "val rel mse", vr, flush=True)
+    F = op64.rhs(P["fq"])
+    res = {}
+    for name in ("bj", "learned_twolevel", "learned_restart"):
+        pre = (bj64 if name == "bj" else
+               fa.LearnedPreconditioner(op64, net, multiplicative=1 if name == "learned_twolevel" else 2))
+        inv = fa.InverseOperator(ctx, method=fa.FGMRES, tolerance=1e-8, errormeasure=fa.RELATIVE,
+                                 maxiterations=20000, restart=30)
+        inv.bind(op64, pre)
+        X = op64.vector()
+        inv(F, X)
+        res[name] = inv.info()
+        print(name, res[name], flush=True)
+        assert res[name]["converged"] and res[name]["true_rel_residual"] <= 1e-8
+    assert min(res["learned_twolevel"]["iterations"], res["learned_restart"]["iterations"]) < res["bj"]["iterations"]
+    # a network with destroyed weights must not break the solve: fallback to BJ
+    bad = fa.CNN(ctx, P["n"])
+    bad.set_coefficient(P["kcell"], P["beta"])
+    bad.set_weights(np.full(bad.nparams, 3.0, dtype=np.float32))
+    pre = fa.LearnedPreconditioner(op64, bad)
+    inv = fa.InverseOperator(ctx, method=fa.FGMRES, tolerance=1e-8, errormeasure=fa.RELATIVE,
+                             maxiterations=20000, restart=30)
+    inv.bind(op64, pre)
+    X = op64.vector()
+    inv(F, X)
+    info = inv.info()
+    print("bad network", info, flush=True)
+    assert info["converged"] and info["true_rel_residual"] <= 1e-8
