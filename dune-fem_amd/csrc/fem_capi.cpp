@@ -558,7 +558,7 @@ int fem_cnn_train(fem_cnn *c, int epochs, int batch, float lr, float val_frac, u
                   double *val_rel)
 {
     return guarded([&] {
-        FEM_CHECK(c && epochs >= 1 && batch >= 1, FEM_ERR_ARG, "train arguments");
+        FEM_CHECK(c && epochs >= 0 && batch >= 1, FEM_ERR_ARG, "train arguments");
         double tl = 0, vr = 0;
         cnn_train(c->c, epochs, batch, lr, val_frac, seed, tl, vr);
         if (train_loss) *train_loss = tl;
@@ -566,7 +566,7 @@ int fem_cnn_train(fem_cnn *c, int epochs, int batch, float lr, float val_frac, u
     });
 }
 
-int fem_precond_create_learned(fem_op *op, fem_cnn *cnn, int multiplicative, fem_precond **out)
+int fem_precond_create_learned(fem_op *op, fem_cnn *cnn, int coarse_steps, fem_precond **out)
 {
     return guarded([&] {
         FEM_CHECK(op && cnn && out, FEM_ERR_ARG, "null argument");
@@ -576,7 +576,7 @@ int fem_precond_create_learned(fem_op *op, fem_cnn *cnn, int multiplicative, fem
             h->p.kind = FEM_PRE_LEARNED;
             h->p.prec = op->o.vec_prec;
             h->p.learned = new Learned();
-            h->p.learned->init(&op->o, &cnn->c, multiplicative);
+            h->p.learned->init(&op->o, &cnn->c, coarse_steps);
         } catch (...) {
             delete h;
             throw;
@@ -585,11 +585,13 @@ int fem_precond_create_learned(fem_op *op, fem_cnn *cnn, int multiplicative, fem
     });
 }
 
-int fem_precond_set_learned_enabled(fem_precond *p, int enabled)
+int fem_precond_set_learned_mode(fem_precond *p, int mode)
 {
     return guarded([&] {
         FEM_CHECK(p && p->p.kind == FEM_PRE_LEARNED, FEM_ERR_ARG, "not a learned preconditioner");
-        p->p.learned->enabled = enabled != 0;
+        FEM_CHECK(mode == FEM_LEARNED_OFF || mode == FEM_LEARNED_NET || mode == FEM_LEARNED_COARSE_ONLY, FEM_ERR_ARG,
+                  "learned mode: FEM_LEARNED_OFF, FEM_LEARNED_NET or FEM_LEARNED_COARSE_ONLY");
+        p->p.learned->mode = mode;
     });
 }
 

@@ -493,7 +493,6 @@ void CNN::set_problem(const double *kcell, double beta)
     stencil.alloc(sizeof(float) * V * 7);
     FEM_HIP(hipMemcpy(kinv.p, k.data(), sizeof(float) * V, hipMemcpyHostToDevice));
     FEM_HIP(hipMemcpy(stencil.p, st.data(), sizeof(float) * V * 7, hipMemcpyHostToDevice));
-    stencil_host = st;
 }
 
 void CNN::coarse_apply(const float *c, float *out, hipStream_t s) const

@@ -29,13 +29,11 @@ struct CNN {
     // the coefficient feature 1/kappa per voxel
     DevBuf ds_r, ds_e, kinv;
     int ds_cap = 0, ds_n = 0;
-    float gamma0 = 1.0f;   // typical ||e_c|| / ||r_c|| of the training set
-    int objective = 1;     // 0: supervised coarse error, 1: Galerkin coarse energy
-    DevBuf feat_red, tgt;   // feature normalisation scalars, normalised target
+    float gamma0 = 1.0f;   // output scale: median ||A_c^-1 r_c|| / ||r_c|| of the training set
+    DevBuf feat_red;       // feature normalisation scalars of the current input
     // Galerkin coarse operator A_c = P^T A P of the SIPG operator on element
     // means: 7-point stencil [V][7] = (diag, -x, +x, -y, +y, -z, +z) weights
     DevBuf stencil;
-    std::vector<float> stencil_host;
     void set_problem(const double *kcell, double beta);
     void coarse_apply(const float *c, float *out, hipStream_t s) const;
 

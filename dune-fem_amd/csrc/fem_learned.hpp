@@ -7,21 +7,36 @@ namespace femamd {
 
 struct CNN;
 
+// Jacobi-preconditioned CG on the 7-point Galerkin coarse operator of a CNN
+// grid, entirely on the device (scalars stay in device memory, no host sync
+// per step); FP32 vectors, FP64 reductions with a deterministic finish.
+struct CoarseCG {
+    const CNN *cnn = nullptr;
+    int V = 0;
+    DevBuf x, r, z, p[2], Ap, sc, partials, counter;
+    int cur = 0;   // index of the current search direction buffer
+    void init(const CNN *c);
+    // x = x0, r = b - A_c x0 with b = src[v * stride]; x0 = s0 gamma0 y if
+    // y is given (s0 = red[0]), else 0
+    template <typename T>
+    void start(const T *src, int stride, const float *y, const float *red, float gamma0, hipStream_t s,
+               const int *skip);
+    void step(hipStream_t s, const int *skip);
+    // host: current (r, z) product, its start value
+    double rz(hipStream_t s) const;
+};
+
 struct Learned {
     Operator *op = nullptr;
     CNN *cnn = nullptr;
     Precond bj;                 // exact element-block inverse
-    // 0 additive, 1 multiplicative two-level preconditioner; 2 restart mode:
-    // the preconditioner is block Jacobi and the network corrects the iterate
-    // from its true residual at every FGMRES restart
-    int multiplicative = 1;
-    bool enabled = true;        // false: block Jacobi only (fallback / ablation)
+    CoarseCG ccg;
+    int coarse_steps = 5;
+    int mode = FEM_LEARNED_NET;
     DevBuf z1, t1, w1, x0, yc, red, partials, counter;
-    void init(Operator *o, CNN *c, int multiplicative);
+    void init(Operator *o, CNN *c, int coarse_steps);
     void apply_raw(const void *x, void *y, const int *skip);
-    // restart mode: x += P C(R r) for the residual r (device, operator precision)
-    void correct(const void *r, void *x);
-    bool restart_mode() const { return multiplicative == 2; }
+    bool active() const { return mode != FEM_LEARNED_OFF; }
 };
 
 void cnn_set_coefficient(CNN &c, const double *kcell, double beta);

@@ -341,14 +341,6 @@ __global__ void k_gm_update(TU *__restrict__ u, const T *__restrict__ Z, int64_t
     }
 }
 
-// t = b - t
-template <typename T>
-__global__ void k_bmax(const T *__restrict__ b, T *__restrict__ t, int64_t n)
-{
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        t[i] = b[i] - t[i];
-}
-
 template <typename T>
 __global__ void k_add(T *__restrict__ u, const T *__restrict__ z, int64_t n)
 {
@@ -522,15 +514,8 @@ static int run_krylov(Solver &S, int method, Operator *o, Precond *p, const T *b
     bool first = true;
     Learned *lp = (p && p->kind == FEM_PRE_LEARNED) ? p->learned : nullptr;
     T *xsave = lp ? (T *)S.workbuf(4, vb) : nullptr;
-    T *rtmp = (lp && lp->restart_mode()) ? (T *)S.workbuf(5, vb) : nullptr;
     while (true) {
-        if (lp && lp->enabled) FEM_HIP(hipMemcpyAsync(xsave, x, vb, hipMemcpyDeviceToDevice, s));
-        if (lp && lp->enabled && lp->restart_mode()) {
-            // learned coarse correction of the iterate from its true residual
-            o->apply_raw(x, rtmp, nullptr);
-            hipLaunchKernelGGL(k_bmax<T>, dim3(G), dim3(256), 0, s, b, rtmp, N);
-            lp->correct(rtmp, x);
-        }
+        if (lp && lp->active()) FEM_HIP(hipMemcpyAsync(xsave, x, vb, hipMemcpyDeviceToDevice, s));
         o->apply_raw(x, V, nullptr);
         hipLaunchKernelGGL(k_gm_start<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, V, b, N, st, part, cnt, first ? 1 : 0);
         hipLaunchKernelGGL(k_gm_scale<T>, dim3(G), dim3(256), 0, s, V, N, st);
@@ -563,11 +548,11 @@ static int run_krylov(Solver &S, int method, Operator *o, Precond *p, const T *b
         S.fetch_state();
         const double gres = std::fabs(S.hstate->g[S.hstate->jlast]);
         if (verbose) std::printf("Fem::%sGMRES it: %d : %g\n", flex ? "F" : "", S.hstate->it, gres);
-        if (lp && lp->enabled && !S.hstate->converged && S.hstate->jlast > 0 && !(gres < 0.95 * S.hstate->res)) {
+        if (lp && lp->active() && !S.hstate->converged && S.hstate->jlast > 0 && !(gres < 0.95 * S.hstate->res)) {
             // the learned correction stagnates or broke down: undo the cycle
             // and continue with block Jacobi only
             FEM_HIP(hipMemcpyAsync(x, xsave, vb, hipMemcpyDeviceToDevice, s));
-            lp->enabled = false;
+            lp->mode = FEM_LEARNED_OFF;
             S.fallback_used = 1;
             continue;
         }

@@ -28,6 +28,7 @@ F64, F32, BF16MFMA = 0, 1, 2
 OP_DG_SIPG_3D, OP_CG_Q2_2D = 1, 2
 KAPPA_CONST, KAPPA_QP, KAPPA_CELL = 0, 1, 2
 PRE_NONE, PRE_JACOBI, PRE_BLOCK_JACOBI, PRE_LEARNED = 0, 1, 2, 3
+LEARNED_OFF, LEARNED_NET, LEARNED_COARSE_ONLY = 0, 1, 2
 CG, GMRES, FGMRES = 0, 1, 2
 ABSOLUTE, RELATIVE, REDUCTION = 0, 1, 2
 
@@ -47,7 +48,7 @@ EXPORTS = [
     "fem_batch_autotune", "fem_batch_solve", "fem_batch_instance_results",
     "fem_cnn_create", "fem_cnn_destroy", "fem_cnn_nparams", "fem_cnn_set_weights", "fem_cnn_get_weights",
     "fem_cnn_forward", "fem_cnn_loss_grad", "fem_cnn_set_coefficient", "fem_cnn_collect_pairs",
-    "fem_cnn_train", "fem_precond_create_learned", "fem_precond_set_learned_enabled", "fem_cnn_coarse_apply",
+    "fem_cnn_train", "fem_precond_create_learned", "fem_precond_set_learned_mode", "fem_cnn_coarse_apply",
 ]
 
 
@@ -158,7 +159,7 @@ def load_library(path=LIBPATH):
         "fem_cnn_collect_pairs": ([_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.POINTER(C.c_int)], C.c_int),
         "fem_cnn_train": ([_vp, C.c_int, C.c_int, C.c_float, C.c_float, C.c_uint, _dp, _dp], C.c_int),
         "fem_precond_create_learned": ([_vp, _vp, C.c_int, C.POINTER(_vp)], C.c_int),
-        "fem_precond_set_learned_enabled": ([_vp, C.c_int], C.c_int),
+        "fem_precond_set_learned_mode": ([_vp, C.c_int], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -563,15 +564,19 @@ class CNN:
 
 
 class LearnedPreconditioner(Preconditioner):
-    """Two-level preconditioner: CNN coarse correction + exact block Jacobi."""
+    """Two-level preconditioner (FGMRES only): coarse solve on element means
+    started from the network's output and finished by ``coarse_steps``
+    Jacobi-CG steps, energy-optimal coarse step, exact block-Jacobi smoother.
+    ``mode``: LEARNED_NET, LEARNED_COARSE_ONLY (coarse CG from zero, i.e.
+    without the network) or LEARNED_OFF (block Jacobi)."""
 
-    def __init__(self, op, cnn, multiplicative=1):
-        """multiplicative: 0 additive, 1 multiplicative two-level, 2 correction at FGMRES restarts"""
+    def __init__(self, op, cnn, coarse_steps=5, mode=LEARNED_NET):
         self.op = op
         self.cnn = cnn
         h = _vp()
-        _check(_lib.fem_precond_create_learned(op.h, cnn.h, int(multiplicative), C.byref(h)))
+        _check(_lib.fem_precond_create_learned(op.h, cnn.h, int(coarse_steps), C.byref(h)))
         self.h = h
+        self.set_mode(mode)
 
-    def set_enabled(self, on):
-        _check(_lib.fem_precond_set_learned_enabled(self.h, 1 if on else 0))
+    def set_mode(self, mode):
+        _check(_lib.fem_precond_set_learned_mode(self.h, int(mode)))

@@ -203,12 +203,15 @@ int fem_solver_unbind(fem_solver *s);
 int fem_solver_solve(fem_solver *s, const fem_vec *b, fem_vec *x, fem_solver_info *info);
 
 /* ---- learned coarse-space preconditioner ----------------------------------------
- * A 4-layer 3D CNN (3x3x3 dilated convolutions, 32 channels, no biases, ReLU)
- * on the element grid maps the normalised element-mean residual (and residual
- * / kappa) to the element-mean error; BF16 weights and activations, FP32
- * accumulation, BF16 MFMA for the hidden layers.  Used in FGMRES as a
- * two-level preconditioner with the exact block-Jacobi smoother, falling back
- * to block Jacobi when a restart cycle stalls.  No reference counterpart. */
+ * Two-level preconditioner for SIPG DG operators with an element-mean coarse
+ * space (A_c = P^T A P, a 7-point stencil).  The coarse solve A_c c = R v is
+ * started from the output of a 4-layer 3D CNN (3x3x3 dilated convolutions,
+ * 32 channels, no biases, ReLU; BF16 weights and activations, FP32
+ * accumulation, BF16 MFMA for the hidden layers) and finished by a few
+ * Jacobi-preconditioned CG steps on the device; then z1 = alpha P c with the
+ * energy-optimal alpha and z = z1 + BJ(v - A z1) with the exact block-Jacobi
+ * smoother.  Nonlinear in v: FGMRES only.  FGMRES falls back to block Jacobi
+ * when a restart cycle stalls.  No reference counterpart. */
 typedef struct fem_cnn fem_cnn;
 
 int fem_cnn_create(fem_ctx *ctx, int nx, int ny, int nz, const int *dilations /* 4 */, fem_cnn **out);
@@ -224,7 +227,7 @@ int fem_cnn_loss_grad(fem_cnn *c, const float *x, const float *t, double *loss, 
 /* per-element kappa and the SIPG penalty factor: sets the second input
  * channel (1 / kappa) and the Galerkin coarse operator A_c = P^T A P of
  * element means (7-point stencil: only the penalty terms act on piecewise
- * constants), whose energy 1/2 c^T A_c c - c^T R r is the training objective */
+ * constants) */
 int fem_cnn_set_coefficient(fem_cnn *c, const double *kcell, double beta);
 /* y = A_c x on element means (host arrays [V]) */
 int fem_cnn_coarse_apply(fem_cnn *c, const float *x, float *y);
@@ -232,11 +235,20 @@ int fem_cnn_coarse_apply(fem_cnn *c, const float *x, float *y);
  * (residual, error) pairs of every `every`-th iterate (at most max_pairs) */
 int fem_cnn_collect_pairs(fem_cnn *c, fem_op *op64, fem_precond *bj64, const fem_vec *b, int max_pairs, int every,
                           int *npairs);
+/* Adam on the energy-norm distance between the coarse correction and the
+ * solver's error, ||e_k - P c||_A^2 (normalised per pair; 1 for c = 0).
+ * val_rel_energy: mean over the validation pairs of the relative coarse
+ * energy error (c - c*)^T A_c (c - c*) / (c*^T A_c c*), c* = A_c^-1 R r_k
+ * (-1 without validation pairs). */
 int fem_cnn_train(fem_cnn *c, int epochs, int batch, float lr, float val_frac, unsigned seed, double *train_loss,
-                  double *val_rel_mse);
-int fem_precond_create_learned(fem_op *op, fem_cnn *cnn, int multiplicative, fem_precond **out);
-/* 0: use block Jacobi only (ablation / manual fallback) */
-int fem_precond_set_learned_enabled(fem_precond *p, int enabled);
+                  double *val_rel_energy);
+
+#define FEM_LEARNED_OFF 0          /* block Jacobi only (fallback) */
+#define FEM_LEARNED_NET 1          /* coarse CG started from the network output */
+#define FEM_LEARNED_COARSE_ONLY 2  /* coarse CG started from zero: the method without the network */
+/* coarse_steps: Jacobi-CG steps on A_c per application (0..1000) */
+int fem_precond_create_learned(fem_op *op, fem_cnn *cnn, int coarse_steps, fem_precond **out);
+int fem_precond_set_learned_mode(fem_precond *p, int mode);
 
 /* ---- batched solves -----------------------------------------------------------
  * Many independent continuous-Q2 problems -div(kappa grad u) + c u = f on

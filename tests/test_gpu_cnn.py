@@ -98,33 +98,37 @@ def test_coarse_operator_is_galerkin_projection(ctx):
     assert np.abs(got - ref).max() <= 1e-5 * np.abs(ref).max()
 
 
+def fgmres_run(ctx, op, pre, F):
+    inv = fa.InverseOperator(ctx, method=fa.FGMRES, tolerance=1e-8, errormeasure=fa.RELATIVE,
+                             maxiterations=20000, restart=30)
+    inv.bind(op, pre)
+    X = op.vector()
+    inv(F, X)
+    return inv.info()
+
+
 def test_learned_preconditioner_fgmres_ablation_and_fallback(ctx):
+    # trained on pairs of the plain CG + block-Jacobi solves of 8 other
+    # right-hand sides; judged by FGMRES(30) iterations to a 1e-8 FP64
+    # residual on the workload's own right-hand side
     P, op64, bj64, net, tl, vr = learned_setup(ctx)
-    print("train loss", tl, "val rel mse", vr, flush=True)
+    print("train energy loss", tl, "val rel coarse energy error", vr, flush=True)
+    assert 0 <= vr < 0.6   # a zero correction scores 1
     F = op64.rhs(P["fq"])
-    res = {}
-    for name in ("bj", "learned_twolevel", "learned_restart"):
-        pre = (bj64 if name == "bj" else
-               fa.LearnedPreconditioner(op64, net, multiplicative=1 if name == "learned_twolevel" else 2))
-        inv = fa.InverseOperator(ctx, method=fa.FGMRES, tolerance=1e-8, errormeasure=fa.RELATIVE,
-                                 maxiterations=20000, restart=30)
-        inv.bind(op64, pre)
-        X = op64.vector()
-        inv(F, X)
-        res[name] = inv.info()
-        print(name, res[name], flush=True)
-        assert res[name]["converged"] and res[name]["true_rel_residual"] <= 1e-8
-    assert min(res["learned_twolevel"]["iterations"], res["learned_restart"]["iterations"]) < res["bj"]["iterations"]
-    # a network with destroyed weights must not break the solve: fallback to BJ
+    res = {"bj": fgmres_run(ctx, op64, bj64, F)}
+    for name, mode in (("coarse_only", fa.LEARNED_COARSE_ONLY), ("net", fa.LEARNED_NET)):
+        res[name] = fgmres_run(ctx, op64, fa.LearnedPreconditioner(op64, net, coarse_steps=5, mode=mode), F)
+    for name, info in res.items():
+        print(name, info, flush=True)
+        assert info["converged"] and info["true_rel_residual"] <= 1e-8 and not info["fallback"]
+    # the network's gain over the same two-level method without it, and over
+    # block Jacobi
+    assert res["net"]["iterations"] < 0.75 * res["coarse_only"]["iterations"]
+    assert res["coarse_only"]["iterations"] < res["bj"]["iterations"]
+    # a network with destroyed weights must not break the solve
     bad = fa.CNN(ctx, P["n"])
     bad.set_coefficient(P["kcell"], P["beta"])
     bad.set_weights(np.full(bad.nparams, 3.0, dtype=np.float32))
-    pre = fa.LearnedPreconditioner(op64, bad)
-    inv = fa.InverseOperator(ctx, method=fa.FGMRES, tolerance=1e-8, errormeasure=fa.RELATIVE,
-                             maxiterations=20000, restart=30)
-    inv.bind(op64, pre)
-    X = op64.vector()
-    inv(F, X)
-    info = inv.info()
+    info = fgmres_run(ctx, op64, fa.LearnedPreconditioner(op64, bad, coarse_steps=5), F)
     print("bad network", info, flush=True)
     assert info["converged"] and info["true_rel_residual"] <= 1e-8
