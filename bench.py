@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--combined", action="store_true",
+                    help="also run the 96^3 Q4 combined mixed + learned solve (several minutes)")
     ap.add_argument("--calib", action="store_true",
                     help="also run a plain FP64 axpy of known size (HBM counter calibration)")
     return ap.parse_args()
@@ -61,12 +63,103 @@ def time_applies(fa, ctx, op, U, W, steps, warmup):
     return wall, ev_ms / steps
 
 
-def solve_section(fa, wl, ctx):
-    """FGMRES(30) to 1e-8 FP64 relative residual on the high-contrast channel
-    problem; plain FP64 GPU path vs FP64 refinement around FP32 solves."""
-    P = wl.problem3(n=48)
-    out = {"workload": "3D SIPG DG Q2 diffusion, 48^3 hexes (2,985,984 DoFs), kappa=10^(4H), H binary "
-                       "channels 20%, f = 16 Fourier modes, FGMRES(30), rel tol 1e-8 (FP64 true residual)"}
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def _info(v):
+    return {"converged": v["converged"], "iterations": v["iterations"], "outer_iterations": v["outer_iterations"],
+            "time_s": round(v["timing"][0], 4), "true_rel_residual": v["true_rel_residual"],
+            "fallback": v.get("fallback", False)}
+
+
+def solve_section(fa, wl, ctx, n=48, pairs=2000, solves=8, epochs=3, lr=1e-4, coarse_steps=50):
+    """High-contrast channel problem: time to a 1e-8 FP64 relative residual.
+    Plain FP64 GPU paths (CG and FGMRES(30) with exact block Jacobi), FP64
+    refinement around FP32 inner solves, and the learned two-level
+    preconditioner in FGMRES(30): training pairs from plain CG solves of
+    other right-hand sides, training on the GPU, then FP64 and mixed solves."""
+    P = wl.problem3(n=n)
+    E = n ** 3
+    N = E * (P["p"] + 1) ** 3
+    out = {"workload": "3D SIPG DG Q2 diffusion, %d^3 hexes (%d DoFs), kappa=10^(4H), H binary channels 20%%, "
+                       "f = 16 Fourier modes; rel tol 1e-8 on the FP64 true residual" % (n, N), "dofs": N}
+
+    def dg(prec):
+        op = fa.GalerkinOperator(ctx, fa.OP_DG_SIPG_3D, P["n"], P["p"], P["beta"], (0, 0, 0), 0.0, fa.KAPPA_CELL,
+                                 precision=prec)
+        op.set_kappa_cell(P["kcell"])
+        return op
+
+    op64 = dg(fa.F64)
+    op32 = dg(fa.F32)
+    F = op64.rhs(P["fq"])
+    bj64 = fa.Preconditioner(op64, fa.PRE_BLOCK_JACOBI)
+    bj32 = fa.Preconditioner(op32, fa.PRE_BLOCK_JACOBI)
+    runs = {}
+
+    def run(name, method, pre, inner=None, maxit=20000, **kw):
+        X = op64.vector()
+        inv = fa.InverseOperator(ctx, method=method, tolerance=1e-8, errormeasure=fa.RELATIVE,
+                                 maxiterations=maxit, restart=30, residual_check=1, **kw)
+        if inner is None:
+            inv.bind(op64, pre)
+        else:
+            inv.bind(op64, None, inner, pre)
+        inv(F, X)
+        runs[name] = _info(inv.info())
+        log("solve", name, runs[name])
+
+    run("fp64_cg_bj", fa.CG, bj64)
+    run("fp64_fgmres_bj", fa.FGMRES, bj64, maxit=3000)
+    run("mixed_fp32_cg_bj_ir", fa.CG, bj32, op32, refinement=1, inner_tolerance=1e-3, inner_maxiterations=20000)
+
+    # learned preconditioner: pairs from plain FP64 CG + block-Jacobi solves
+    # of other seeded right-hand sides, `every` chosen so the pairs span each solve
+    net = fa.CNN(ctx, P["n"])
+    net.set_coefficient(P["kcell"], P["beta"])
+    every = max(1, abs(runs["fp64_cg_bj"]["iterations"]) // (pairs // solves))
+    rng = np.random.default_rng(7)
+    m = P["p"] + 1
+    c = wl.qp_coords(n, m)
+    t0 = time.perf_counter()
+    npairs = 0
+    for _ in range(solves):
+        modes = wl.fourier_modes(rng, 16, 3)
+        b = op64.rhs(wl.vol_layout3(wl.sine_sum_grid(modes, [c, c, c]), P["n"], m))
+        npairs += net.collect_pairs(op64, bj64, b, pairs // solves, every=every)
+    ctx.synchronize()
+    collect_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    tl, vr = net.train(epochs=epochs, batch=8, lr=lr)
+    train_s = time.perf_counter() - t0
+    log("trained on", npairs, "pairs in", round(train_s, 1), "s, val", vr)
+    out["learned_training"] = {"pairs": npairs, "every": every, "collect_s": round(collect_s, 2),
+                               "epochs": epochs, "lr": lr, "train_s": round(train_s, 2),
+                               "train_energy_loss": round(tl, 4), "val_rel_coarse_energy_error": round(vr, 4),
+                               "note": "loss 1 = zero correction, 0 = exact coarse solve"}
+    for name, mode in (("fp64_fgmres_twolevel_no_net", fa.LEARNED_COARSE_ONLY),
+                       ("fp64_fgmres_learned", fa.LEARNED_NET)):
+        run(name, fa.FGMRES, fa.LearnedPreconditioner(op64, net, coarse_steps=coarse_steps, mode=mode))
+    run("mixed_fp32_fgmres_learned_ir", fa.FGMRES, fa.LearnedPreconditioner(op32, net, coarse_steps=coarse_steps),
+        op32, refinement=1, inner_tolerance=1e-3, inner_maxiterations=20000)
+    out["coarse_cg_steps"] = coarse_steps
+    out["runs"] = runs
+    best = runs["fp64_cg_bj"]["time_s"]
+    out["speedup_vs_fp64_cg_bj"] = {k: round(best / v["time_s"], 3) for k, v in runs.items()}
+    return out
+
+
+def combined_section(fa, wl, ctx, n=96, pairs=256, solves=1, epochs=2, lr=1e-4, coarse_steps=50,
+                     learned_maxit=600):
+    """Q4 log-normal problem: plain FP64 CG + block Jacobi vs BF16-MFMA
+    applies in learned-preconditioned FGMRES(30) inside FP64 refinement."""
+    P = wl.problem4(n=n)
+    E = n ** 3
+    N = E * (P["p"] + 1) ** 3
+    out = {"workload": "3D SIPG DG Q4 diffusion, %d^3 hexes (%d DoFs), log-normal kappa (sigma 1.5, corr. "
+                       "length 0.1, per element), f = 32 Fourier modes; rel tol 1e-8 on the FP64 true residual"
+                       % (n, N), "dofs": N}
 
     def dg(prec):
         op = fa.GalerkinOperator(ctx, fa.OP_DG_SIPG_3D, P["n"], P["p"], P["beta"], (0, 0, 0), 0.0, fa.KAPPA_CELL,
@@ -76,32 +169,56 @@ def solve_section(fa, wl, ctx):
 
     op64 = dg(fa.F64)
     F = op64.rhs(P["fq"])
+    del P["fq"]
+    bj64 = fa.Preconditioner(op64, fa.PRE_BLOCK_JACOBI)
     runs = {}
-    pre64 = fa.Preconditioner(op64, fa.PRE_BLOCK_JACOBI)
-    for name, method in (("fp64_fgmres_bj", fa.FGMRES), ("fp64_cg_bj", fa.CG)):
+
+    def run(name, method, op, pre, inner=None, maxit=20000, **kw):
         X = op64.vector()
         inv = fa.InverseOperator(ctx, method=method, tolerance=1e-8, errormeasure=fa.RELATIVE,
-                                 maxiterations=100000, restart=30, residual_check=1)
-        inv.bind(op64, pre64)
+                                 maxiterations=maxit, restart=30, residual_check=1, **kw)
+        if inner is None:
+            inv.bind(op, pre)
+        else:
+            inv.bind(op, None, inner, pre)
         inv(F, X)
-        runs[name] = inv.info()
-    op32 = dg(fa.F32)
-    pre32 = fa.Preconditioner(op32, fa.PRE_BLOCK_JACOBI)
-    for name, method, itol in (("mixed_fp32_fgmres_bj_ir", fa.FGMRES, 1e-3), ("mixed_fp32_cg_bj_ir", fa.CG, 1e-3)):
-        X = op64.vector()
-        inv = fa.InverseOperator(ctx, method=method, tolerance=1e-8, errormeasure=fa.RELATIVE,
-                                 maxiterations=100000, restart=30, refinement=1, inner_tolerance=itol,
-                                 inner_maxiterations=20000, residual_check=1)
-        inv.bind(op64, None, op32, pre32)
-        inv(F, X)
-        runs[name] = inv.info()
-    for k, v in runs.items():
-        runs[k] = {"converged": v["converged"], "iterations": v["iterations"],
-                   "outer_iterations": v["outer_iterations"], "time_s": round(v["timing"][0], 4),
-                   "true_rel_residual": v["true_rel_residual"]}
+        runs[name] = _info(inv.info())
+        log("combined", name, runs[name])
+
+    run("fp64_cg_bj", fa.CG, op64, bj64)
+    opm = dg(fa.BF16MFMA)
+    bjm = fa.Preconditioner(opm, fa.PRE_BLOCK_JACOBI)
+    run("mixed_bf16mfma_cg_bj_ir", fa.CG, op64, bjm, opm, refinement=1, inner_tolerance=1e-3,
+        inner_maxiterations=20000)
+    net = fa.CNN(ctx, P["n"])
+    net.set_coefficient(P["kcell"], P["beta"])
+    every = max(1, abs(runs["fp64_cg_bj"]["iterations"]) // (pairs // solves))
+    rng = np.random.default_rng(7)
+    m = P["p"] + 1
+    c = wl.qp_coords(n, m)
+    t0 = time.perf_counter()
+    for _ in range(solves):
+        log("combined: collecting training pairs (plain FP64 CG solve of another right-hand side)")
+        modes = wl.fourier_modes(rng, 32, 3)
+        b = op64.rhs(wl.vol_layout3(wl.sine_sum_grid(modes, [c, c, c]), P["n"], m))
+        net.collect_pairs(op64, bj64, b, pairs // solves, every=every)
+        del b
+    ctx.synchronize()
+    collect_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    tl, vr = net.train(epochs=epochs, batch=8, lr=lr)
+    train_s = time.perf_counter() - t0
+    log("combined: trained in", round(train_s, 1), "s, val", vr)
+    out["learned_training"] = {"pairs": pairs, "collect_s": round(collect_s, 2), "epochs": epochs,
+                               "train_s": round(train_s, 2), "val_rel_coarse_energy_error": round(vr, 4)}
+    # capped: each FGMRES(30) step at this size costs two fine applies, the
+    # network and the coarse CG; the cycle residuals are printed as it runs
+    run("mixed_bf16mfma_fgmres_learned_ir", fa.FGMRES, op64,
+        fa.LearnedPreconditioner(opm, net, coarse_steps=coarse_steps), opm, refinement=1, inner_tolerance=1e-3,
+        inner_maxiterations=learned_maxit, verbose=1)
     out["runs"] = runs
-    base = runs["fp64_fgmres_bj"]["time_s"]
-    out["speedup_vs_fp64_fgmres"] = {k: round(base / v["time_s"], 3) for k, v in runs.items()}
+    best = runs["fp64_cg_bj"]["time_s"]
+    out["speedup_vs_fp64_cg_bj"] = {k: round(best / v["time_s"], 3) for k, v in runs.items()}
     return out
 
 
@@ -115,6 +232,7 @@ def batch_section(fa, wl, ctx, count=256):
     B = fa.Batch(ctx, sides, c=0.0)
     B.set_kappa_qp(np.concatenate([d["kvol"] for d in inst]))
     (tx, ty), tune = B.autotune()
+    log("batch: autotuned tile", tx, ty)
     F = B.rhs(np.concatenate([d["fq"] for d in inst]))
     X = B.vector()
     B.solve(F, X, tol=1e-8)          # warm-up (captures the graph)
@@ -140,6 +258,7 @@ def batch_section(fa, wl, ctx, count=256):
         loop_conv += int(r["converged"])
     ctx.synchronize()
     loop_s = time.perf_counter() - t1
+    log("batch: batched", round(info["time_s"], 3), "s, per-instance loop", round(loop_s, 3), "s")
     return {"workload": "%d x 2D continuous Q2, -div(kappa grad u) = f, sides U{32..256} (%d DoFs total), "
                         "log-normal kappa (sigma 1, corr. length 0.1) at quadrature points, 8-mode forcing; "
                         "tol 1e-8 FP64 true residual per instance" % (count, B.N),
@@ -248,6 +367,9 @@ def main():
 
     if not a.no_batch:
         result["batch"] = batch_section(fa, wl, ctx)
+
+    if a.combined:
+        result["combined"] = combined_section(fa, wl, ctx)
 
     # ---- CPU baseline: the oracle (reference algorithm) on the host cores ----------
     if not a.no_cpu:

@@ -53,6 +53,29 @@ def build(jobs=8, verbose=False):
     return LIB
 
 
+CPP_TESTS = ["inverseoperatortest"]
+CPP_TEST_DIR = os.path.join(ROOT, "tests", "cpp")
+
+
+def build_cpp_tests():
+    """C++ drivers of the host facade (include/dunefem_amd.hh): host code
+    only, linked against the library with an rpath relative to the binary."""
+    out_dir = os.path.join(CPP_TEST_DIR, "_bin")
+    os.makedirs(out_dir, exist_ok=True)
+    rel = os.path.relpath(OUT_DIR, out_dir)
+    outs = []
+    for name in CPP_TESTS:
+        exe = os.path.join(out_dir, name)
+        cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-Wextra",
+               "-I", os.path.join(ROOT, "include"), os.path.join(CPP_TEST_DIR, name + ".cc"),
+               "-L", OUT_DIR, "-ldunefem_amd", "-Wl,-rpath,$ORIGIN/" + rel, "-o", exe]
+        p = subprocess.run(cmd, capture_output=True, text=True)
+        if p.returncode != 0:
+            raise RuntimeError("C++ test build failed: %s\n%s\n%s" % (" ".join(cmd), p.stdout, p.stderr))
+        outs.append(exe)
+    return outs
+
+
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("-j", type=int, default=8)

@@ -479,7 +479,7 @@ static int run_krylov(Solver &S, int method, Operator *o, Precond *p, const T *b
             }
             FEM_HIP(hipGetLastError());
             S.fetch_state();
-            if (verbose) std::printf("Fem::CG it: %d : residual %g\n", S.hstate->it, std::sqrt(S.hstate->res));
+            if (verbose) { std::printf("Fem::CG it: %d : residual %g\n", S.hstate->it, std::sqrt(S.hstate->res)); std::fflush(stdout); }
             if (S.hstate->done) break;
             if (S.snapshot) S.snapshot(x, pp);
             else if (chunk < 64) chunk *= 2;
@@ -547,7 +547,7 @@ static int run_krylov(Solver &S, int method, Operator *o, Precond *p, const T *b
         FEM_HIP(hipGetLastError());
         S.fetch_state();
         const double gres = std::fabs(S.hstate->g[S.hstate->jlast]);
-        if (verbose) std::printf("Fem::%sGMRES it: %d : %g\n", flex ? "F" : "", S.hstate->it, gres);
+        if (verbose) { std::printf("Fem::%sGMRES it: %d : %g\n", flex ? "F" : "", S.hstate->it, gres); std::fflush(stdout); }
         if (lp && lp->active() && !S.hstate->converged && S.hstate->jlast > 0 && !(gres < 0.95 * S.hstate->res)) {
             // the learned correction stagnates or broke down: undo the cycle
             // and continue with block Jacobi only
@@ -617,7 +617,7 @@ void Solver::solve(const Vec &b, Vec &x, fem_solver_info &info)
             FEM_HIP(hipStreamSynchronize(s));
             const double rn = std::sqrt(rn2);
             const double rel = bn > 0 ? rn / bn : rn;
-            if (prm.verbose) std::printf("IR outer %d: rel residual %g (inner its %d)\n", outer, rel, total);
+            if (prm.verbose) { std::printf("IR outer %d: rel residual %g (inner its %d)\n", outer, rel, total); std::fflush(stdout); }
             const double target = prm.errormeasure == FEM_ERRMEASURE_RELATIVE ? prm.tolerance * bn : prm.tolerance;
             if (rn <= target || !(rn == rn)) { ok = rn == rn; break; }
             if (outer >= 100 || total >= prm.maxiterations) break;
