@@ -42,7 +42,7 @@ def parse():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--combined", action="store_true",
-                    help="also run the 96^3 Q4 combined mixed + learned solve (several minutes)")
+                    help="also run the 96^3 Q4 solves, FP64 vs mixed (about two minutes)")
     ap.add_argument("--calib", action="store_true",
                     help="also run a plain FP64 axpy of known size (HBM counter calibration)")
     return ap.parse_args()
@@ -150,10 +150,13 @@ def solve_section(fa, wl, ctx, n=48, pairs=2000, solves=8, epochs=3, lr=1e-4, co
     return out
 
 
-def combined_section(fa, wl, ctx, n=96, pairs=256, solves=1, epochs=2, lr=1e-4, coarse_steps=50,
-                     learned_maxit=600):
-    """Q4 log-normal problem: plain FP64 CG + block Jacobi vs BF16-MFMA
-    applies in learned-preconditioned FGMRES(30) inside FP64 refinement."""
+def combined_section(fa, wl, ctx, n=96):
+    """Q4 log-normal problem (110.6 M DoFs): plain FP64 CG + block Jacobi vs
+    FP64 refinement around FP32 CG + block Jacobi.  The BF16-MFMA operator is
+    not used as the inner operator here: its apply error (8e-3 relative on
+    rough vectors) is relative to |u|, so on the smooth modes that decide
+    convergence the inner operator is wrong by far more than the refinement
+    can absorb (measured: the inner CG diverges)."""
     P = wl.problem4(n=n)
     E = n ** 3
     N = E * (P["p"] + 1) ** 3
@@ -170,7 +173,6 @@ def combined_section(fa, wl, ctx, n=96, pairs=256, solves=1, epochs=2, lr=1e-4, 
     op64 = dg(fa.F64)
     F = op64.rhs(P["fq"])
     del P["fq"]
-    bj64 = fa.Preconditioner(op64, fa.PRE_BLOCK_JACOBI)
     runs = {}
 
     def run(name, method, op, pre, inner=None, maxit=20000, **kw):
@@ -185,37 +187,10 @@ def combined_section(fa, wl, ctx, n=96, pairs=256, solves=1, epochs=2, lr=1e-4, 
         runs[name] = _info(inv.info())
         log("combined", name, runs[name])
 
-    run("fp64_cg_bj", fa.CG, op64, bj64)
-    opm = dg(fa.BF16MFMA)
-    bjm = fa.Preconditioner(opm, fa.PRE_BLOCK_JACOBI)
-    run("mixed_bf16mfma_cg_bj_ir", fa.CG, op64, bjm, opm, refinement=1, inner_tolerance=1e-3,
-        inner_maxiterations=20000)
-    net = fa.CNN(ctx, P["n"])
-    net.set_coefficient(P["kcell"], P["beta"])
-    every = max(1, abs(runs["fp64_cg_bj"]["iterations"]) // (pairs // solves))
-    rng = np.random.default_rng(7)
-    m = P["p"] + 1
-    c = wl.qp_coords(n, m)
-    t0 = time.perf_counter()
-    for _ in range(solves):
-        log("combined: collecting training pairs (plain FP64 CG solve of another right-hand side)")
-        modes = wl.fourier_modes(rng, 32, 3)
-        b = op64.rhs(wl.vol_layout3(wl.sine_sum_grid(modes, [c, c, c]), P["n"], m))
-        net.collect_pairs(op64, bj64, b, pairs // solves, every=every)
-        del b
-    ctx.synchronize()
-    collect_s = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    tl, vr = net.train(epochs=epochs, batch=8, lr=lr)
-    train_s = time.perf_counter() - t0
-    log("combined: trained in", round(train_s, 1), "s, val", vr)
-    out["learned_training"] = {"pairs": pairs, "collect_s": round(collect_s, 2), "epochs": epochs,
-                               "train_s": round(train_s, 2), "val_rel_coarse_energy_error": round(vr, 4)}
-    # capped: each FGMRES(30) step at this size costs two fine applies, the
-    # network and the coarse CG; the cycle residuals are printed as it runs
-    run("mixed_bf16mfma_fgmres_learned_ir", fa.FGMRES, op64,
-        fa.LearnedPreconditioner(opm, net, coarse_steps=coarse_steps), opm, refinement=1, inner_tolerance=1e-3,
-        inner_maxiterations=learned_maxit, verbose=1)
+    run("fp64_cg_bj", fa.CG, op64, fa.Preconditioner(op64, fa.PRE_BLOCK_JACOBI))
+    op32 = dg(fa.F32)
+    run("mixed_fp32_cg_bj_ir", fa.CG, op64, fa.Preconditioner(op32, fa.PRE_BLOCK_JACOBI), op32, refinement=1,
+        inner_tolerance=1e-3, inner_maxiterations=20000, verbose=1)
     out["runs"] = runs
     best = runs["fp64_cg_bj"]["time_s"]
     out["speedup_vs_fp64_cg_bj"] = {k: round(best / v["time_s"], 3) for k, v in runs.items()}
