@@ -96,6 +96,45 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Cross-lane moves inside a 16-lane DPP row (one K = 4 element): CTRL is a
+// quad_perm pattern (0x00..0xFF) or row_ror:N (0x120 + N).  Every lane of the
+// wavefront must be active.  Doubles move as two 32-bit halves.
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float x)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double x)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xf, 0xf, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+// row_ror:4k moves lane (4 i' + j) of a row to lane (4 ((i' + k) mod 4) + j):
+// after it, lane (i, j) holds the value of lane (4 ((i - k) mod 4) + j)
+constexpr int ROR_SRC_SIGN = -1;
+
+// y(i, j) = sum_m B[j][m] x(i, m): x(i, m) sits in lane m of the lane's quad
+template <typename T>
+__device__ __forceinline__ T quad_contract(const T *Bj, T x)
+{
+    T y = Bj[0] * dpp_mov<0x00>(x);
+    y = fma(Bj[1], dpp_mov<0x55>(x), y);
+    y = fma(Bj[2], dpp_mov<0xAA>(x), y);
+    return fma(Bj[3], dpp_mov<0xFF>(x), y);
+}
+// z(i, j) = sum_m B[i][m] y(m, j): y(m, j) sits in lane 4 m + j of the row
+template <typename T>
+__device__ __forceinline__ T row_contract(const T *Bi, int i, T y)
+{
+    T z = Bi[i] * y;
+    z = fma(Bi[(i + ROR_SRC_SIGN * 1) & 3], dpp_mov<0x124>(y), z);
+    z = fma(Bi[(i + ROR_SRC_SIGN * 2) & 3], dpp_mov<0x128>(y), z);
+    return fma(Bi[(i + ROR_SRC_SIGN * 3) & 3], dpp_mov<0x12C>(y), z);
+}
+
 // Legendre-basis contractions use the parity of the basis on the symmetric
 // Gauss points, B[K-1-q][a] = (-1)^a B[q][a]: out[q] and out[K-1-q] share the
 // even and odd partial sums.
@@ -407,30 +446,55 @@ __global__ void __launch_bounds__(64)
     // ---- faces: neighbour traces, numerical fluxes, lifting into R ---------
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        wave_lds_sync();
-        if (wr) {
+        // neighbour trace value / normal derivative at face point (q1=i, q2=j):
+        // two tangential contractions of the lane's modal (m1=i, m2=j) values
+        T nvs[2], nds[2];
+        if constexpr (K == 4) {
+            // an element is one 16-lane DPP row: contract across lanes, no LDS
 #pragma unroll
             for (int s = 0; s < 2; ++s) {
-                Sx[(s * 2 + 0) * K2 + i * K + j] = nval[d][s];
-                Sx[(s * 2 + 1) * K2 + i * K + j] = nder[d][s];
+                nvs[s] = row_contract(tb.B[i], i, quad_contract(tb.B[j], nval[d][s]));
+                nds[s] = row_contract(tb.B[i], i, quad_contract(tb.B[j], nder[d][s]));
+            }
+            wave_lds_sync();
+        } else {
+            wave_lds_sync();
+            if (wr) {
+#pragma unroll
+                for (int s = 0; s < 2; ++s) {
+                    Sx[(s * 2 + 0) * K2 + i * K + j] = nval[d][s];
+                    Sx[(s * 2 + 1) * K2 + i * K + j] = nder[d][s];
+                }
+            }
+            wave_lds_sync();
+            // (2) second tangential index, in place: thread (i=m1, j=q2)
+            T t2v[4];
+#pragma unroll
+            for (int sq = 0; sq < 4; ++sq) {
+#pragma unroll
+                for (int m = 0; m < K; ++m) v[m] = Sx[sq * K2 + i * K + m];
+                t2v[sq] = dotk<K>(tb.B[j], v);
+            }
+            wave_lds_sync();
+            if (wr)
+#pragma unroll
+                for (int sq = 0; sq < 4; ++sq) Sx[sq * K2 + i * K + j] = t2v[sq];
+            wave_lds_sync();
+            // first tangential index: thread (i=q1, j=q2)
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                T nv = 0, nd = 0;
+#pragma unroll
+                for (int m = 0; m < K; ++m) {
+                    nv = fma(tb.B[i][m], Sx[(s * 2 + 0) * K2 + m * K + j], nv);
+                    nd = fma(tb.B[i][m], Sx[(s * 2 + 1) * K2 + m * K + j], nd);
+                }
+                nvs[s] = nv;
+                nds[s] = nd;
             }
         }
-        wave_lds_sync();
-        // (2) second tangential index, in place: thread (i=m1, j=q2)
-        T t2v[4];
-#pragma unroll
-        for (int sq = 0; sq < 4; ++sq) {
-#pragma unroll
-            for (int m = 0; m < K; ++m) v[m] = Sx[sq * K2 + i * K + m];
-            t2v[sq] = dotk<K>(tb.B[j], v);
-        }
-        wave_lds_sync();
-        if (wr)
-#pragma unroll
-            for (int sq = 0; sq < 4; ++sq) Sx[sq * K2 + i * K + j] = t2v[sq];
-        wave_lds_sync();
-        // (3) first tangential index, own traces from u_q, flux, lift:
-        //     thread (i=q1, j=q2) owns the R pencil along d through (q1, q2)
+        // own traces from u_q, flux, lift: thread (i=q1, j=q2) owns the R
+        // pencil along d through (q1, q2)
         const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
         const T Wf = wij * a.h[t1] * a.h[t2];
         const T ihd = T(1) / a.h[d];
@@ -463,12 +527,7 @@ __global__ void __launch_bounds__(64)
             const T pen = a.beta * ihd;
             T c0, c1;
             if (interior) {
-                T nv = 0, nd = 0;
-#pragma unroll
-                for (int m = 0; m < K; ++m) {
-                    nv = fma(tb.B[i][m], Sx[(s * 2 + 0) * K2 + m * K + j], nv);
-                    nd = fma(tb.B[i][m], Sx[(s * 2 + 1) * K2 + m * K + j], nd);
-                }
+                const T nv = nvs[s], nd = nds[s];
                 const T dout = nd * (ihd * nrm);
                 const T jump = uin - nv;
                 const T sigma = T(0.5) * fma(kin, din, kout * dout);
