@@ -63,6 +63,17 @@ def time_applies(fa, ctx, op, U, W, steps, warmup):
     return wall, ev_ms / steps
 
 
+def measured_traffic(p, n):
+    """HBM bytes per launch of the FP64 apply from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (calibrated against an axpy of known
+    size); only for the workload it was measured on, else None."""
+    f = os.path.join(ROOT, "profiles", "k_dg_apply_q3_64cubed_traffic.json")
+    if p != 3 or tuple(n) != (64, 64, 64) or not os.path.exists(f):
+        return None
+    with open(f) as fh:
+        return json.load(fh)["traffic_bytes_per_launch"]
+
+
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
@@ -300,8 +311,8 @@ def main():
                                "b=(1,0.5,0.25), upwind, beta=10p^2, u~N(0,1)" % (p, n[0], n[1], n[2], N),
                    "dofs": N, "order": p, "elements": E},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_dg_apply<4,double>", "kernel_ms": round(kern_ms, 5),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(p, n),
+                     "kernel": "k_dg_apply<%d,double>" % k, "kernel_ms": round(kern_ms, 5),
                      "alg_bytes_per_launch": b_alg},
     }
 

@@ -98,18 +98,20 @@ __device__ __forceinline__ void wave_lds_sync()
 
 // Cross-lane moves inside a 16-lane DPP row (one K = 4 element): CTRL is a
 // quad_perm pattern (0x00..0xFF) or row_ror:N (0x120 + N).  Every lane of the
-// wavefront must be active.  Doubles move as two 32-bit halves.
+// wavefront must be active (no lane reads an inactive or out-of-row source, so
+// the destination's previous value is never kept).  Doubles move as two
+// 32-bit halves.
 template <int CTRL>
 __device__ __forceinline__ float dpp_mov(float x)
 {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xf, 0xf, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xf, 0xf, false));
 }
 template <int CTRL>
 __device__ __forceinline__ double dpp_mov(double x)
 {
     const unsigned long long b = (unsigned long long)__double_as_longlong(x);
-    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xf, 0xf, false);
-    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, false);
+    const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)b, CTRL, 0xf, 0xf, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, false);
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 // row_ror:4k moves lane (4 i' + j) of a row to lane (4 ((i' + k) mod 4) + j):
@@ -287,7 +289,19 @@ __global__ void __launch_bounds__(64)
     const int e_raw = blk * EPB + el;
     const bool active = wr && e_raw < E;
     const int e = active ? e_raw : 0;
-    const int ex = e % nx, ey = (e / nx) % ny, ez = e / (nx * ny);
+    int ex, ey, ez;
+    if (nx % EPB == 0 && blk * EPB < E) {
+        // the workgroup's elements share one x-row: the divisions act on the
+        // wave-uniform block index only
+        const int rpb = nx / EPB, row = blk / rpb;
+        ex = (blk - row * rpb) * EPB + el;
+        ey = row % ny;
+        ez = row / ny;
+    } else {
+        ex = e % nx;
+        ey = (e / nx) % ny;
+        ez = e / (nx * ny);
+    }
     const int ec[3] = { ex, ey, ez };
     const int stride[3] = { 1, nx, nx * ny };
 
@@ -394,7 +408,7 @@ __global__ void __launch_bounds__(64)
     // x-direction volume flux (kept in registers, X layout)
     T Rx[K];
     {
-        const T sc = wij * detJ / (a.h[0] * a.h[0]), bh = a.b[0] * a.h[0];
+        const T sc = wij * detJ * (a.ih[0] * a.ih[0]), bh = a.b[0] * a.h[0];
         T g[K], r[K];
         C::D(tb, mr, uq, g);
 #pragma unroll
@@ -411,7 +425,7 @@ __global__ void __launch_bounds__(64)
     wave_lds_sync();
     // Y layout of u_q: thread (i=qx, j=qz) -> R (y part)
     {
-        const T sc = wij * detJ / (a.h[1] * a.h[1]), bh = a.b[1] * a.h[1];
+        const T sc = wij * detJ * (a.ih[1] * a.ih[1]), bh = a.b[1] * a.h[1];
         T g[K], r[K];
 #pragma unroll
         for (int q = 0; q < K; ++q) v[q] = A[lidx<K>(i, q, j)];
@@ -428,7 +442,7 @@ __global__ void __launch_bounds__(64)
     wave_lds_sync();
     // Z layout of u_q: thread (i=qx, j=qy) -> R += z part
     {
-        const T sc = wij * detJ / (a.h[2] * a.h[2]), bh = a.b[2] * a.h[2];
+        const T sc = wij * detJ * (a.ih[2] * a.ih[2]), bh = a.b[2] * a.h[2];
         T g[K], r[K];
 #pragma unroll
         for (int q = 0; q < K; ++q) v[q] = A[lidx<K>(i, j, q)];
@@ -497,7 +511,7 @@ __global__ void __launch_bounds__(64)
         // pencil along d through (q1, q2)
         const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
         const T Wf = wij * a.h[t1] * a.h[t2];
-        const T ihd = T(1) / a.h[d];
+        const T ihd = a.ih[d];
         const int idx = i + K * j;
         int poff[K];
 #pragma unroll

@@ -9,6 +9,7 @@ template <typename T>
 struct DGArgs {
     int n[3];
     T h[3];
+    T ih[3];    // 1 / h
     T beta;
     T b[3];
     T c;

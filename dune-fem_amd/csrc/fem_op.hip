@@ -65,6 +65,8 @@ void Operator::init(Context *c, const fem_op_desc &d)
             a64.n[i] = a32.n[i] = d.n[i];
             a64.h[i] = 1.0 / d.n[i];
             a32.h[i] = (float)(1.0 / d.n[i]);
+            a64.ih[i] = (double)d.n[i];
+            a32.ih[i] = (float)d.n[i];
             a64.b[i] = d.b[i];
             a32.b[i] = (float)d.b[i];
         }

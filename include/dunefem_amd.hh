@@ -21,6 +21,9 @@
  *                      (inverseoperatorinterface.hh:66-166); CgInverseOperator,
  *                      GmresInverseOperator (krylovinverseoperators.hh:280-295)
  *                      and FgmresInverseOperator
+ *   AffineGalerkinOperator, NewtonInverseOperator
+ *                      R(u) = A u - F and the Newton solve of R(w) = u
+ *                      (dune/fem/solver/newtoninverseoperator.hh:688-802)
  * Errors: every non-zero status of the C ABI becomes a dunefem_amd::Exception
  * (the reference throws Dune::Exception subclasses); non-convergence is not an
  * exception but a negative iteration count, as in the reference.
@@ -29,6 +32,7 @@
 #ifndef DUNEFEM_AMD_HH
 #define DUNEFEM_AMD_HH
 
+#include <cmath>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -346,6 +350,91 @@ class FgmresInverseOperator : public InverseOperator {
 public:
     explicit FgmresInverseOperator(Context &ctx, const SolverParameter &p = SolverParameter())
         : InverseOperator(ctx, p, FEM_SOLVER_FGMRES) {}
+};
+
+// R(u) = A u - F: a Galerkin operator with its source term, as the
+// reference's GalerkinOperator evaluates it (galerkin.hh:333-360); its
+// jacobian is the linear operator A (nonlinear() == false)
+class AffineGalerkinOperator {
+public:
+    using DomainFunctionType = DiscreteFunction;
+    using RangeFunctionType = DiscreteFunction;
+    using JacobianOperatorType = GalerkinOperator;
+
+    AffineGalerkinOperator(const GalerkinOperator &A, const DiscreteFunction &F) : A_(&A), F_(&F) {}
+    void operator()(const DiscreteFunction &u, DiscreteFunction &w) const
+    {
+        (*A_)(u, w);
+        w -= *F_;
+    }
+    bool nonlinear() const { return false; }
+    const JacobianOperatorType &jacobian(const DiscreteFunction &) const { return *A_; }
+
+private:
+    const GalerkinOperator *A_;
+    const DiscreteFunction *F_;
+};
+
+// NewtonInverseOperator<JacobianOperator, LInvOp>::operator()(u, w): solves
+// R(w) = u (dune/fem/solver/newtoninverseoperator.hh:688-802): residual
+// R(w) - u, dw = DR^-1 residual, w -= dw; for an operator with
+// nonlinear() == false one step, else until |residual| <= tolerance
+template <class LinearInverseOperator>
+class NewtonInverseOperator {
+public:
+    NewtonInverseOperator(Context &ctx, const SolverParameter &linear = SolverParameter(), double tolerance = 1e-6,
+                          int maxIterations = 100, int maxLinearIterations = 1 << 30)
+        : jInv_(ctx, linear), tolerance_(tolerance), maxIterations_(maxIterations),
+          maxLinearIterations_(maxLinearIterations) {}
+
+    void bind(const AffineGalerkinOperator &op) { op_ = &op; }
+    void unbind() { op_ = nullptr; }
+
+    void operator()(const DiscreteFunction &u, DiscreteFunction &w) const
+    {
+        if (!op_) throw Exception(FEM_ERR_STATE, "NewtonInverseOperator: bind an operator first");
+        DiscreteFunction residual(u), dw(w);
+        iterations_ = 0;
+        linearIterations_ = 0;
+        (*op_)(w, residual);
+        residual -= u;
+        delta_ = std::sqrt(residual.normSquaredDofs());
+        const bool nonlinear = op_->nonlinear();
+        nonlinear_ = nonlinear;
+        while (true) {
+            jInv_.bind(op_->jacobian(w));
+            const int left = maxLinearIterations_ - linearIterations_;
+            if (left <= 0) break;
+            jInv_.setMaxIterations(left);
+            dw.clear();
+            jInv_(residual, dw);
+            if (jInv_.iterations() < 0) {   // the linear solve did not converge
+                linearIterations_ = jInv_.iterations();
+                break;
+            }
+            linearIterations_ += jInv_.iterations();
+            w -= dw;
+            (*op_)(w, residual);
+            residual -= u;
+            delta_ = std::sqrt(residual.normSquaredDofs());
+            ++iterations_;
+            if (!nonlinear || delta_ <= tolerance_ || iterations_ >= maxIterations_) break;
+        }
+    }
+    int iterations() const { return iterations_; }
+    int linearIterations() const { return linearIterations_; }
+    double residualNorm() const { return delta_; }
+    // the linear solves converged and, for a nonlinear operator, |residual| <= tolerance
+    bool converged() const { return linearIterations_ >= 0 && iterations_ > 0 && (!nonlinear_ || delta_ <= tolerance_); }
+
+private:
+    const AffineGalerkinOperator *op_ = nullptr;
+    mutable LinearInverseOperator jInv_;
+    double tolerance_;
+    int maxIterations_, maxLinearIterations_;
+    mutable int iterations_ = 0, linearIterations_ = 0;
+    mutable double delta_ = 0;
+    mutable bool nonlinear_ = false;
 };
 
 // Gauss-Legendre rule with m points on [0,1] (the reference's GaussPts

@@ -111,6 +111,17 @@ static void algorithm(Context &ctx, const std::string &designation, const Solver
         dist = l2distance(n, u.download());
         report(dist < eps && inverseOperatorA.info().converged, designation + " inverseOp(NewSParam)", dist);
     }
+    // NewtonInverseOperator on the affine operator R(u) = M u - F with
+    // right-hand side 0 (inverseoperatortest.cc: newtonInvOp(rhs, u) after
+    // rhs.clear()); one Newton step for the linear operator
+    AffineGalerkinOperator affineMassOperator(massOperator, rhs);
+    DiscreteFunction zero("zero", ctx, massOperator.size());
+    NewtonInverseOperator<InverseOperatorType> newtonInvOp(ctx, param ? *param : SolverParameter());
+    newtonInvOp.bind(affineMassOperator);
+    u.clear();
+    newtonInvOp(zero, u);
+    const double dist = l2distance(n, u.download());
+    report(dist < eps && newtonInvOp.converged() && newtonInvOp.iterations() == 1, designation + " NewtonInvOp", dist);
 }
 
 static fem_op_desc dg_desc(int n, int p, int precision, bool advection = true)
