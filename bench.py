@@ -84,6 +84,37 @@ def _info(v):
             "fallback": v.get("fallback", False)}
 
 
+def plain_section(fa, wl, ctx, cpu=True):
+    """The reference's CPU-runnable case: 2D continuous Q2, 64x64 quads,
+    -lap u + u = f, strong u = 0, CG + Jacobi to 1e-10 relative.  GPU solve
+    (FP64) against the oracle's CG on one host core."""
+    P = wl.problem0(n=64)
+    op = fa.GalerkinOperator(ctx, fa.OP_CG_Q2_2D, P["n"], 2, c=1.0)
+    pre = fa.Preconditioner(op, fa.PRE_JACOBI)
+    F = op.rhs(P["fq"])
+    res = {}
+    for rep in range(2):   # the first call includes one-time set-up
+        X = op.vector()
+        inv = fa.InverseOperator(ctx, method=fa.CG, tolerance=1e-10, errormeasure=fa.RELATIVE, maxiterations=5000)
+        inv.bind(op, pre)
+        inv(F, X)
+        res = _info(inv.info())
+    out = {"workload": "2D continuous Q2, 64x64 quads (16,641 DoFs), -lap u + u = f (8 Fourier modes), strong "
+                       "u = 0, CG + Jacobi, rel tol 1e-10", "gpu_fp64_cg_jacobi": res}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as ora
+        oop = ora.CGOperator(P["n"], c=1.0)
+        Fh = oop.rhs(P["fq"])
+        t0 = time.perf_counter()
+        xo, ito = ora.cg(oop, Fh, pre=ora.jacobi(1.0 / oop.diag()), tol=1e-10, maxit=5000, criteria=1)
+        cpu_s = time.perf_counter() - t0
+        out["cpu_oracle_cg_jacobi"] = {"iterations": int(ito), "time_s": round(cpu_s, 4), "cores": 1}
+        out["iterations_match"] = int(ito) == res["iterations"]
+    log("plain", out)
+    return out
+
+
 def solve_section(fa, wl, ctx, n=48, pairs=2000, solves=8, epochs=3, lr=1e-4, coarse_steps=50):
     """High-contrast channel problem: time to a 1e-8 FP64 relative residual.
     Plain FP64 GPU paths (CG and FGMRES(30) with exact block Jacobi), FP64
@@ -346,6 +377,9 @@ def main():
                        "hbm_frac": round(b2 / (kms2 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
         del op, U2, W2
     result["mixed"] = mixed
+
+    if not a.no_solve:
+        result["plain"] = plain_section(fa, wl, ctx, cpu=not a.no_cpu)
 
     # ---- time to solve (problem 3: 48^3 SIPG Q2, kappa = 10^(4H) channels) --------
     if not a.no_solve:

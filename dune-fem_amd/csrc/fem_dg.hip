@@ -23,6 +23,7 @@
 #include <hip/hip_bf16.h>
 
 #include "fem_dg.hpp"
+#include "fem_dg_bj.hpp"
 
 namespace femamd {
 
@@ -874,7 +875,6 @@ void dg_block_setup(const DGArgs<double> &a, int k, double *Q, double *lam, hipS
     FEM_HIP(hipGetLastError());
 }
 
-// y_e = (Qx (x) Qy (x) Qz) diag(1/(lx+ly+lz)) (Qx (x) Qy (x) Qz)^T x_e
 template <int K, typename T>
 __global__ void __launch_bounds__(DGShape<K>::THREADS)
     k_dg_block_apply(int64_t E, const T *__restrict__ Q, const T *__restrict__ lam, const T *__restrict__ x,
@@ -888,81 +888,13 @@ __global__ void __launch_bounds__(DGShape<K>::THREADS)
     const int64_t e_raw = (int64_t)blockIdx.x * EPB + el;
     const bool active = e_raw < E;
     const int64_t e = active ? e_raw : 0;
-    T *A = smem + el * K3;
-    const T *Qe = Q + e * 3 * K2;
-    T v[K], tmp[K];
-    // Q_z^T along c: thread (i=a, j=b)
+    T v[K];
 #pragma unroll
     for (int c = 0; c < K; ++c) v[c] = x[e * K3 + i * K2 + j * K + c];
-#pragma unroll
-    for (int r = 0; r < K; ++r) {
-        T s = 0;
-#pragma unroll
-        for (int c = 0; c < K; ++c) s = fma(Qe[2 * K2 + c * K + r], v[c], s);
-        tmp[r] = s;
-    }
-#pragma unroll
-    for (int r = 0; r < K; ++r) A[i * K2 + j * K + r] = tmp[r];
-    __syncthreads();
-    // Q_y^T along b: thread (i=a, j=c')
-#pragma unroll
-    for (int b = 0; b < K; ++b) v[b] = A[i * K2 + b * K + j];
-#pragma unroll
-    for (int r = 0; r < K; ++r) {
-        T s = 0;
-#pragma unroll
-        for (int b = 0; b < K; ++b) s = fma(Qe[1 * K2 + b * K + r], v[b], s);
-        tmp[r] = s;
-    }
-#pragma unroll
-    for (int r = 0; r < K; ++r) A[i * K2 + r * K + j] = tmp[r];
-    __syncthreads();
-    // Q_x^T along a, scale, Q_x: thread (i=b', j=c')
-#pragma unroll
-    for (int b = 0; b < K; ++b) v[b] = A[b * K2 + i * K + j];
-#pragma unroll
-    for (int r = 0; r < K; ++r) {
-        T s = 0;
-#pragma unroll
-        for (int b = 0; b < K; ++b) s = fma(Qe[0 * K2 + b * K + r], v[b], s);
-        tmp[r] = s / (lam[(e * 3 + 0) * K + r] + lam[(e * 3 + 1) * K + i] + lam[(e * 3 + 2) * K + j]);
-    }
-#pragma unroll
-    for (int r = 0; r < K; ++r) {
-        T s = 0;
-#pragma unroll
-        for (int b = 0; b < K; ++b) s = fma(Qe[0 * K2 + r * K + b], tmp[b], s);
-        v[r] = s;
-    }
-#pragma unroll
-    for (int r = 0; r < K; ++r) A[r * K2 + i * K + j] = v[r];
-    __syncthreads();
-    // Q_y along b: thread (i=a, j=c')
-#pragma unroll
-    for (int b = 0; b < K; ++b) v[b] = A[i * K2 + b * K + j];
-#pragma unroll
-    for (int r = 0; r < K; ++r) {
-        T s = 0;
-#pragma unroll
-        for (int b = 0; b < K; ++b) s = fma(Qe[1 * K2 + r * K + b], v[b], s);
-        tmp[r] = s;
-    }
-#pragma unroll
-    for (int r = 0; r < K; ++r) A[i * K2 + r * K + j] = tmp[r];
-    __syncthreads();
-    // Q_z along c: thread (i=a, j=b)
-#pragma unroll
-    for (int c = 0; c < K; ++c) v[c] = A[i * K2 + j * K + c];
-#pragma unroll
-    for (int r = 0; r < K; ++r) {
-        T s = 0;
-#pragma unroll
-        for (int c = 0; c < K; ++c) s = fma(Qe[2 * K2 + r * K + c], v[c], s);
-        tmp[r] = s;
-    }
+    bj_element<K>(Q + e * 3 * K2, lam + e * 3 * K, smem + el * K3, i, j, v);
     if (active)
 #pragma unroll
-        for (int r = 0; r < K; ++r) y[e * K3 + i * K2 + j * K + r] = tmp[r];
+        for (int r = 0; r < K; ++r) y[e * K3 + i * K2 + j * K + r] = v[r];
 }
 
 template <typename T>
