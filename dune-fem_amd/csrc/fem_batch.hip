@@ -24,7 +24,8 @@
 namespace femamd {
 
 constexpr int SEG_THREADS = 256;
-constexpr int SEG_CHUNK = SEG_THREADS * 8;
+constexpr int SEG_ITEMS = 8;   // DoFs per thread of a segment
+constexpr int SEG_CHUNK = SEG_THREADS * SEG_ITEMS;
 
 struct SegTab {
     const int *inst;       // [nblocks] instance of each block
@@ -93,30 +94,71 @@ __global__ void __launch_bounds__(SEG_THREADS)
     const int64_t lo = S.lo[blockIdx.x], hi = S.hi[blockIdx.x];
     double a0 = 0, a1 = 0, a2 = 0;
     const double alpha = (MODE == SEG_UPD) ? B.alpha[inst] : 0.0;
-    for (int64_t i = lo + threadIdx.x; i < hi; i += SEG_THREADS) {
+    // a segment is at most SEG_CHUNK = SEG_ITEMS * SEG_THREADS DoFs: every
+    // thread issues all of its loads before it uses them (memory-level
+    // parallelism), then updates and accumulates
+    constexpr int U = SEG_ITEMS;
+    T va[U], vb[U], vc[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+        const int64_t i = lo + threadIdx.x + (int64_t)k * SEG_THREADS;
+        va[k] = vb[k] = vc[k] = T(0);
+        if (i < hi) {
+            if (MODE == SEG_INIT) {
+                va[k] = b[i];
+                vb[k] = dinv[i];
+            } else if (MODE == SEG_QH) {
+                va[k] = q[i];
+                vb[k] = h[i];
+            } else if (MODE == SEG_UPD) {
+                va[k] = p[i];
+                vb[k] = h[i];
+                vc[k] = dinv[i];
+            } else {
+                va[k] = b[i];
+                vb[k] = h[i];
+            }
+        }
+    }
+    T vx[U], vq[U];
+    if (MODE == SEG_UPD) {
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            const int64_t i = lo + threadIdx.x + (int64_t)k * SEG_THREADS;
+            vx[k] = vq[k] = T(0);
+            if (i < hi) {
+                vx[k] = x[i];
+                vq[k] = q[i];
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+        const int64_t i = lo + threadIdx.x + (int64_t)k * SEG_THREADS;
+        if (i >= hi) continue;
         if (MODE == SEG_INIT) {
-            const T bv = b[i];
-            const T sv = dinv[i] * bv;
+            const T bv = va[k];
+            const T sv = vb[k] * bv;
             p[i] = bv;
             q[i] = sv;
             s[i] = sv;
             a0 += (double)bv * (double)sv;
             a1 += (double)bv * (double)bv;
         } else if (MODE == SEG_QH) {
-            a0 += (double)q[i] * (double)h[i];
+            a0 += (double)va[k] * (double)vb[k];
         } else if (MODE == SEG_UPD) {
-            x[i] = (T)((double)x[i] + alpha * (double)q[i]);
-            const T pn = (T)((double)p[i] - alpha * (double)h[i]);
+            x[i] = (T)((double)vx[k] + alpha * (double)vq[k]);
+            const T pn = (T)((double)va[k] - alpha * (double)vb[k]);
             p[i] = pn;
-            const T sv = dinv[i] * pn;
+            const T sv = vc[k] * pn;
             s[i] = sv;
             a0 += (double)pn * (double)sv;
             a1 += (double)pn * (double)pn;
         } else {
-            const double r = (double)b[i] - (double)h[i];
+            const double r = (double)va[k] - (double)vb[k];
             p[i] = (T)r;
             a0 += r * r;
-            a2 += (double)b[i] * (double)b[i];
+            a2 += (double)va[k] * (double)va[k];
         }
     }
     double own[3];
@@ -159,8 +201,22 @@ __global__ void __launch_bounds__(SEG_THREADS)
     const int inst = S.inst[blockIdx.x];
     if (B.done[inst] || B.it[inst] == 0) return;
     const double beta = B.beta[inst];
-    for (int64_t i = S.lo[blockIdx.x] + threadIdx.x; i < S.hi[blockIdx.x]; i += SEG_THREADS)
-        q[i] = (T)(beta * (double)q[i] + (double)s[i]);
+    const int64_t lo = S.lo[blockIdx.x], hi = S.hi[blockIdx.x];
+    T vq[SEG_ITEMS], vs[SEG_ITEMS];
+#pragma unroll
+    for (int k = 0; k < SEG_ITEMS; ++k) {
+        const int64_t i = lo + threadIdx.x + (int64_t)k * SEG_THREADS;
+        vq[k] = vs[k] = T(0);
+        if (i < hi) {
+            vq[k] = q[i];
+            vs[k] = s[i];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < SEG_ITEMS; ++k) {
+        const int64_t i = lo + threadIdx.x + (int64_t)k * SEG_THREADS;
+        if (i < hi) q[i] = (T)(beta * (double)vq[k] + (double)vs[k]);
+    }
 }
 
 // r32 = r64 * scale_i, d = 0, inner done = outer converged

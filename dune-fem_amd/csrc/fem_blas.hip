@@ -33,7 +33,7 @@ __global__ void k_scal(T *__restrict__ x, int64_t n, double a)
         x[i] = (T)((double)x[i] * a);
 }
 
-// fixed grid of RED_BLOCKS x RED_THREADS threads; each thread accumulates a
+// fixed grid of RED_DOT_BLOCKS x RED_THREADS threads; each thread accumulates a
 // grid-stride slice in FP64 (FP32 inputs are widened before multiplying)
 template <typename TX, typename TY>
 __global__ void __launch_bounds__(RED_THREADS) k_dot(const TX *__restrict__ x, const TY *__restrict__ y,
@@ -114,7 +114,7 @@ template <typename TX, typename TY>
 static void dot_launch(Context &c, int64_t n, const TX *x, const TY *y, double *result, unsigned *counter,
                        double *partials)
 {
-    hipLaunchKernelGGL((k_dot<TX, TY>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, c.stream, x, y, n, partials,
+    hipLaunchKernelGGL((k_dot<TX, TY>), dim3(RED_DOT_BLOCKS), dim3(RED_THREADS), 0, c.stream, x, y, n, partials,
                        counter, result);
     FEM_HIP(hipGetLastError());
 }
@@ -137,7 +137,7 @@ void vec_dot_async(const Vec &x, const Vec &y, int slot)
     Context &c = *x.ctx;
     double *res = c.red_result + slot;
     unsigned *cnt = c.red_counter + slot;
-    double *part = c.red_partial + (size_t)slot * RED_BLOCKS;
+    double *part = c.red_partial + (size_t)slot * RED_DOT_BLOCKS;
     if (x.prec == FEM_F64 && y.prec == FEM_F64) dot_launch(c, x.n, (const double *)x.data, (const double *)y.data, res, cnt, part);
     else if (x.prec == FEM_F64) dot_launch(c, x.n, (const double *)x.data, (const float *)y.data, res, cnt, part);
     else if (y.prec == FEM_F64) dot_launch(c, x.n, (const float *)x.data, (const double *)y.data, res, cnt, part);

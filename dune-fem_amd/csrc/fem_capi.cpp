@@ -53,7 +53,7 @@ int fem_ctx_create(int device, fem_ctx **out)
         c.device = device;
         try {
             FEM_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
-            FEM_HIP(hipMalloc(&c.red_partial, sizeof(double) * RED_BLOCKS * RED_MAX_SLOTS));
+            FEM_HIP(hipMalloc(&c.red_partial, sizeof(double) * RED_DOT_BLOCKS * RED_MAX_SLOTS));
             FEM_HIP(hipMalloc(&c.red_counter, sizeof(unsigned) * RED_MAX_SLOTS));
             FEM_HIP(hipMemset(c.red_counter, 0, sizeof(unsigned) * RED_MAX_SLOTS));
             FEM_HIP(hipMalloc(&c.red_result, sizeof(double) * RED_MAX_SLOTS));

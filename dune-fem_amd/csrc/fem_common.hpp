@@ -71,7 +71,7 @@ struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
     // reduction workspace: partial sums + counters + result slots
-    double *red_partial = nullptr;     // [RED_MAX_BLOCKS * RED_MAX_SLOTS]
+    double *red_partial = nullptr;     // [RED_DOT_BLOCKS * RED_MAX_SLOTS]
     unsigned *red_counter = nullptr;   // [RED_MAX_SLOTS]
     double *red_result = nullptr;      // device scalar results [RED_MAX_SLOTS]
     double *host_result = nullptr;     // pinned mirror
@@ -79,6 +79,8 @@ struct Context {
 };
 
 constexpr int RED_BLOCKS = 512;      // fixed grid of the reduction kernels
+// fixed grid of the single and paired dot products (k_dot, the CG steps)
+constexpr int RED_DOT_BLOCKS = 512;
 constexpr int RED_THREADS = 256;
 constexpr int RED_MAX_SLOTS = 64;    // simultaneous dot products (multi-dot)
 
