@@ -28,6 +28,8 @@
 namespace femamd {
 
 constexpr int GRP = 8;   // vectors per register group in the multi-dot
+// the CG reductions use RED_DOT_BLOCKS partial slots per quantity (two)
+static_assert(2 * RED_DOT_BLOCKS <= RED_BLOCKS * (GM_MAX + 1), "solver partials buffer");
 
 __device__ __forceinline__ void fin_cg_alpha(KState *st, double qh)
 {
@@ -92,7 +94,29 @@ __global__ void __launch_bounds__(RED_THREADS)
     __shared__ double lds[RED_THREADS / 64];
     double acc = 0, acc2 = 0;
     const double alpha = st->alpha;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    int64_t i0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (MODE == 0 || MODE == 3) {
+        // pure reductions: four independent loads of each operand in flight
+        // per thread before they are used
+        const T *A = MODE == 0 ? q : p;
+        const T *Bv = MODE == 0 ? h : s;
+        constexpr int U = 4;
+        const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+        for (; i0 + (U - 1) * stride < n; i0 += U * stride) {
+            T va[U], vb[U];
+#pragma unroll
+            for (int k = 0; k < U; ++k) {
+                va[k] = A[i0 + k * stride];
+                vb[k] = Bv[i0 + k * stride];
+            }
+#pragma unroll
+            for (int k = 0; k < U; ++k) {
+                acc += (double)va[k] * (double)vb[k];
+                if (MODE == 3) acc2 += (double)va[k] * (double)va[k];
+            }
+        }
+    }
+    for (int64_t i = i0; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         if (MODE == 0) {          // dot(q, h) -> alpha
             acc += (double)q[i] * (double)h[i];
         } else if (MODE == 1) {   // pointwise Jacobi: x += a q, p -= a h, s = d p, <p, s>
@@ -466,15 +490,15 @@ static int run_krylov(Solver &S, int method, Operator *o, Precond *p, const T *b
             for (int c = 0; c < chunk; ++c) {
                 hipLaunchKernelGGL(k_cg_qupd<T>, dim3(G), dim3(256), 0, s, q, ss, pp, r, N, st, haspre ? 1 : 0);
                 o->apply_raw(q, h, &st->done);
-                hipLaunchKernelGGL((k_cg_step<T, 0>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, x, pp, q, h, ss, r, dinv, N, st, part, cnt);
+                hipLaunchKernelGGL((k_cg_step<T, 0>), dim3(RED_DOT_BLOCKS), dim3(RED_THREADS), 0, s, x, pp, q, h, ss, r, dinv, N, st, part, cnt);
                 if (!haspre) {
-                    hipLaunchKernelGGL((k_cg_step<T, 4>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, x, pp, q, h, ss, r, dinv, N, st, part, cnt);
+                    hipLaunchKernelGGL((k_cg_step<T, 4>), dim3(RED_DOT_BLOCKS), dim3(RED_THREADS), 0, s, x, pp, q, h, ss, r, dinv, N, st, part, cnt);
                 } else if (dinv) {
-                    hipLaunchKernelGGL((k_cg_step<T, 1>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, x, pp, q, h, ss, r, dinv, N, st, part, cnt);
+                    hipLaunchKernelGGL((k_cg_step<T, 1>), dim3(RED_DOT_BLOCKS), dim3(RED_THREADS), 0, s, x, pp, q, h, ss, r, dinv, N, st, part, cnt);
                 } else {
-                    hipLaunchKernelGGL((k_cg_step<T, 2>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, x, pp, q, h, ss, r, dinv, N, st, part, cnt);
+                    hipLaunchKernelGGL((k_cg_step<T, 2>), dim3(RED_DOT_BLOCKS), dim3(RED_THREADS), 0, s, x, pp, q, h, ss, r, dinv, N, st, part, cnt);
                     p->apply_raw(pp, ss, &st->done);
-                    hipLaunchKernelGGL((k_cg_step<T, 3>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, x, pp, q, h, ss, r, dinv, N, st, part, cnt);
+                    hipLaunchKernelGGL((k_cg_step<T, 3>), dim3(RED_DOT_BLOCKS), dim3(RED_THREADS), 0, s, x, pp, q, h, ss, r, dinv, N, st, part, cnt);
                 }
             }
             FEM_HIP(hipGetLastError());
