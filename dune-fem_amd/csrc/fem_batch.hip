@@ -55,24 +55,22 @@ __device__ __forceinline__ bool seg_finish(const SegTab &S, int inst, int nq, co
     __shared__ double lds2[SEG_THREADS / 64];
     const int nb = gridDim.x;
     if (threadIdx.x == 0) {
-        for (int q = 0; q < nq; ++q) S.partials[(size_t)q * nb + blockIdx.x] = own[q];
-        __threadfence();
+        for (int q = 0; q < nq; ++q) publish_partial(&S.partials[(size_t)q * nb + blockIdx.x], own[q]);
         const int nblk = S.first[inst + 1] - S.first[inst];
         amLast = atomicAdd(&S.counter[inst], 1u) == (unsigned)(nblk - 1);
     }
     __syncthreads();
     if (!amLast) return false;
-    __threadfence();
     const int b0 = S.first[inst], b1 = S.first[inst + 1];
     for (int q = 0; q < nq; ++q) {
         double s = 0;
         for (int b = b0 + threadIdx.x; b < b1; b += SEG_THREADS)
-            s += __builtin_nontemporal_load(&S.partials[(size_t)q * nb + b]);
+            s += read_partial(&S.partials[(size_t)q * nb + b]);
         const double t = block_sum(s, lds2);
         if (threadIdx.x == 0) tot[q] = t;
         __syncthreads();
     }
-    if (threadIdx.x == 0) S.counter[inst] = 0;
+    if (threadIdx.x == 0) reset_counter(&S.counter[inst]);
     return true;
 }
 

@@ -23,36 +23,59 @@ __device__ __forceinline__ T block_sum(T v, T *lds /* >= blockDim/64 */)
     return s;
 }
 
-// Deterministic finish of a fixed-grid reduction: every block stores its
-// partial, the last block to arrive sums the partials in block order and
+// Publication of block partials across the XCDs without a device-scope
+// fence: on this GPU a device-scope fence writes back and invalidates the
+// XCD's whole L2, which costs microseconds per block.  Instead a partial is
+// written with an atomic exchange (performed at the same coherence point as
+// the arrival counter's atomic add), and its returned value is consumed before
+// the counter is incremented, so the increment is issued only after the
+// partial is in place.  The finishing block reads partials with atomic
+// read-modify-writes that change nothing, from that same coherence point.
+__device__ __forceinline__ void publish_partial(double *dst, double v)
+{
+    const unsigned long long old =
+        atomicExch(reinterpret_cast<unsigned long long *>(dst), (unsigned long long)__double_as_longlong(v));
+    asm volatile("" ::"v"(old));   // wait for the exchange to return
+}
+
+__device__ __forceinline__ double read_partial(double *src)
+{
+    return __longlong_as_double((long long)atomicAdd(reinterpret_cast<unsigned long long *>(src), 0ull));
+}
+
+__device__ __forceinline__ void reset_counter(unsigned *c)
+{
+    (void)atomicExch(c, 0u);
+}
+
+// Deterministic finish of a fixed-grid reduction: every block publishes its
+// partials, the last block to arrive sums the partials in block order and
 // writes result[slot].  The counter is reset by that block.
-// Returns true in thread 0 of the last block (result written).
+// Returns true in all threads of the last block (result written).
 __device__ __forceinline__ bool finish_reduction(double partial, double *partials,
                                                  unsigned *counter, double *result, int nslots,
                                                  const double *own /* [nslots] in thread 0 */)
 {
     __shared__ bool amLast;
     if (threadIdx.x == 0) {
-        for (int s = 0; s < nslots; ++s) partials[(size_t)s * gridDim.x + blockIdx.x] = own[s];
-        __threadfence();
-        unsigned prev = atomicAdd(counter, 1u);
+        for (int s = 0; s < nslots; ++s) publish_partial(&partials[(size_t)s * gridDim.x + blockIdx.x], own[s]);
+        const unsigned prev = atomicAdd(counter, 1u);
         amLast = (prev == gridDim.x - 1);
     }
     __syncthreads();
     (void)partial;
     if (!amLast) return false;
-    __threadfence();
     __shared__ double red[RED_THREADS / 64];
     for (int s = 0; s < nslots; ++s) {
         double v = 0;
         // fixed order: thread t sums blocks t, t+256, ...
         for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x)
-            v += __builtin_nontemporal_load(&partials[(size_t)s * gridDim.x + b]);
+            v += read_partial(&partials[(size_t)s * gridDim.x + b]);
         double tot = block_sum(v, red);
         if (threadIdx.x == 0) result[s] = tot;
         __syncthreads();
     }
-    if (threadIdx.x == 0) *counter = 0;
+    if (threadIdx.x == 0) reset_counter(counter);
     return true;
 }
 

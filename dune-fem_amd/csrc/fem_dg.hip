@@ -875,8 +875,10 @@ void dg_block_setup(const DGArgs<double> &a, int k, double *Q, double *lam, hipS
     FEM_HIP(hipGetLastError());
 }
 
+// one wavefront per block; lanes beyond EPB*K2 shadow element 0 (identical
+// LDS writes, no global writes)
 template <int K, typename T>
-__global__ void __launch_bounds__(DGShape<K>::THREADS)
+__global__ void __launch_bounds__(64)
     k_dg_block_apply(int64_t E, const T *__restrict__ Q, const T *__restrict__ lam, const T *__restrict__ x,
                      T *__restrict__ y, const int *__restrict__ skip)
 {
@@ -884,10 +886,12 @@ __global__ void __launch_bounds__(DGShape<K>::THREADS)
     constexpr int K2 = S::K2, K3 = S::K3, EPB = S::EPB;
     if (skip && *skip) return;
     __shared__ T smem[EPB * K3];
-    const int tid = threadIdx.x, el = tid / K2, t = tid % K2, i = t / K, j = t % K;
+    const int tid = threadIdx.x, el0 = tid / K2, t = tid % K2, i = t / K, j = t % K;
+    const bool wr = el0 < EPB;
+    const int el = wr ? el0 : 0;
     const int64_t e_raw = (int64_t)blockIdx.x * EPB + el;
-    const bool active = e_raw < E;
-    const int64_t e = active ? e_raw : 0;
+    const bool active = wr && e_raw < E;
+    const int64_t e = e_raw < E ? e_raw : 0;
     T v[K];
 #pragma unroll
     for (int c = 0; c < K; ++c) v[c] = x[e * K3 + i * K2 + j * K + c];
@@ -903,7 +907,7 @@ void dg_block_apply(int k, int64_t E, const T *Q, const T *lam, const T *x, T *y
 #define BA_CASE(KK)                                                                                    \
     case KK: {                                                                                         \
         unsigned nb = (unsigned)((E + DGShape<KK>::EPB - 1) / DGShape<KK>::EPB);                       \
-        hipLaunchKernelGGL((k_dg_block_apply<KK, T>), dim3(nb), dim3(DGShape<KK>::THREADS), 0, s, E, Q, lam, x, y, skip); \
+        hipLaunchKernelGGL((k_dg_block_apply<KK, T>), dim3(nb), dim3(64), 0, s, E, Q, lam, x, y, skip); \
         break;                                                                                         \
     }
     switch (k) {

@@ -1,7 +1,7 @@
 // SIPG DG operator on structured hexahedral meshes: kernel arguments and launchers.
 #pragma once
 
-#include "fem_common.hpp"
+#include "fem_blas.hpp"
 
 namespace femamd {
 
@@ -40,5 +40,6 @@ void dg_block_setup(const DGArgs<double> &a, int k, double *Q /*[E][3][K][K]*/, 
                     hipStream_t s);
 template <typename T>
 void dg_block_apply(int k, int64_t E, const T *Q, const T *lam, const T *x, T *y, const int *skip, hipStream_t s);
+
 
 }  // namespace femamd

@@ -148,18 +148,16 @@ __global__ void __launch_bounds__(RED_THREADS)
     double b2 = block_sum(acc2, lds);
     __shared__ bool amLast;
     if (threadIdx.x == 0) {
-        partials[blockIdx.x] = b;
-        partials[gridDim.x + blockIdx.x] = b2;
-        __threadfence();
+        publish_partial(&partials[blockIdx.x], b);
+        publish_partial(&partials[gridDim.x + blockIdx.x], b2);
         amLast = atomicAdd(counter, 1u) == gridDim.x - 1;
     }
     __syncthreads();
     if (!amLast) return;
-    __threadfence();
     double v = 0, v2 = 0;
     for (unsigned k = threadIdx.x; k < gridDim.x; k += blockDim.x) {
-        v += __builtin_nontemporal_load(&partials[k]);
-        v2 += __builtin_nontemporal_load(&partials[gridDim.x + k]);
+        v += read_partial(&partials[k]);
+        v2 += read_partial(&partials[gridDim.x + k]);
     }
     double tot = block_sum(v, lds);
     __syncthreads();
@@ -167,7 +165,7 @@ __global__ void __launch_bounds__(RED_THREADS)
     if (threadIdx.x == 0) {
         if (MODE == 0) fin_cg_alpha(st, tot);
         else fin_cg_res(st, tot, tot2);
-        *counter = 0;
+        reset_counter(counter);
     }
 }
 
@@ -187,15 +185,13 @@ __global__ void __launch_bounds__(RED_THREADS)
     double bs = block_sum(acc, lds);
     __shared__ bool amLast;
     if (threadIdx.x == 0) {
-        partials[blockIdx.x] = bs;
-        __threadfence();
+        publish_partial(&partials[blockIdx.x], bs);
         amLast = atomicAdd(counter, 1u) == gridDim.x - 1;
     }
     __syncthreads();
     if (!amLast) return;
-    __threadfence();
     double v = 0;
-    for (unsigned k = threadIdx.x; k < gridDim.x; k += blockDim.x) v += __builtin_nontemporal_load(&partials[k]);
+    for (unsigned k = threadIdx.x; k < gridDim.x; k += blockDim.x) v += read_partial(&partials[k]);
     double tot = block_sum(v, lds);
     if (threadIdx.x == 0) {
         const double res = sqrt(tot);
@@ -212,7 +208,7 @@ __global__ void __launch_bounds__(RED_THREADS)
             for (int i = 1; i <= GM_MAX; ++i) st->g[i] = 0.0;
             st->hinv = 1.0 / res;
         }
-        *counter = 0;
+        reset_counter(counter);
     }
 }
 
@@ -250,28 +246,26 @@ __global__ void __launch_bounds__(RED_THREADS)
         for (int l = 0; l < GRP; ++l) {
             if (g0 + l < cnt) {
                 double b = block_sum(acc[l], lds);
-                if (threadIdx.x == 0) partials[(size_t)(g0 + l) * gridDim.x + blockIdx.x] = b;
+                if (threadIdx.x == 0) publish_partial(&partials[(size_t)(g0 + l) * gridDim.x + blockIdx.x], b);
                 __syncthreads();
             }
         }
     }
     __shared__ bool amLast;
     if (threadIdx.x == 0) {
-        __threadfence();
         amLast = atomicAdd(counter, 1u) == gridDim.x - 1;
     }
     __syncthreads();
     if (!amLast) return;
-    __threadfence();
     for (int l = 0; l < cnt; ++l) {
         double v = 0;
         for (unsigned k = threadIdx.x; k < gridDim.x; k += blockDim.x)
-            v += __builtin_nontemporal_load(&partials[(size_t)l * gridDim.x + k]);
+            v += read_partial(&partials[(size_t)l * gridDim.x + k]);
         double tot = block_sum(v, lds);
         if (threadIdx.x == 0) st->H[l * m + j] = tot;
         __syncthreads();
     }
-    if (threadIdx.x == 0) *counter = 0;
+    if (threadIdx.x == 0) reset_counter(counter);
 }
 
 __device__ __forceinline__ void rot(double &x, double &y, double c, double s)
@@ -304,15 +298,13 @@ __global__ void __launch_bounds__(RED_THREADS)
     double bs = block_sum(acc, lds);
     __shared__ bool amLast;
     if (threadIdx.x == 0) {
-        partials[blockIdx.x] = bs;
-        __threadfence();
+        publish_partial(&partials[blockIdx.x], bs);
         amLast = atomicAdd(counter, 1u) == gridDim.x - 1;
     }
     __syncthreads();
     if (!amLast) return;
-    __threadfence();
     double v = 0;
-    for (unsigned k = threadIdx.x; k < gridDim.x; k += blockDim.x) v += __builtin_nontemporal_load(&partials[k]);
+    for (unsigned k = threadIdx.x; k < gridDim.x; k += blockDim.x) v += read_partial(&partials[k]);
     double tot = block_sum(v, lds);
     if (threadIdx.x == 0) {
         double *H = st->H;
@@ -334,7 +326,7 @@ __global__ void __launch_bounds__(RED_THREADS)
         } else if (st->it >= st->maxit || j + 1 == m) {
             st->cycle_done = 1;
         }
-        *counter = 0;
+        reset_counter(counter);
     }
 }
 
