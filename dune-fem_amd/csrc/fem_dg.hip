@@ -289,7 +289,10 @@ __global__ void __launch_bounds__(64)
     const int blk = (int)xcd_remap(blockIdx.x, gridDim.x);
     const int e_raw = blk * EPB + el;
     const bool active = wr && e_raw < E;
-    const int e = active ? e_raw : 0;
+    // shadow lanes (el = 0 when !wr) take the block's first element, so e
+    // and its coordinates below always describe the same in-range element:
+    // every neighbour index formed from them stays inside the grid
+    const int e = e_raw < E ? e_raw : 0;
     int ex, ey, ez;
     if (nx % EPB == 0 && blk * EPB < E) {
         // the workgroup's elements share one x-row: the divisions act on the

@@ -51,6 +51,11 @@ CASES = [
     ((3, 4, 5), 3, (0, 0, 0), 0.0, "cell"),
     ((2, 2, 3), 4, (.3, .5, -1), 0.0, "qp"),
     ((3, 3, 3), 4, (0, 0, 0), 1.0, "cell"),
+    # Q4 with nx a multiple of the elements per workgroup and idle lanes in
+    # every workgroup: the lanes without an element of their own must form
+    # in-range neighbour indices (regression: they once addressed -nx*ny
+    # elements before the vector)
+    ((16, 16, 16), 4, (0, 0, 0), 0.0, "cell"),
 ]
 
 
