@@ -889,6 +889,7 @@ __global__ void __launch_bounds__(64)
     constexpr int K2 = S::K2, K3 = S::K3, EPB = S::EPB;
     if (skip && *skip) return;
     __shared__ T smem[EPB * K3];
+    __shared__ T qsm[EPB * (3 * K2 + 3 * K)];
     const int tid = threadIdx.x, el0 = tid / K2, t = tid % K2, i = t / K, j = t % K;
     const bool wr = el0 < EPB;
     const int el = wr ? el0 : 0;
@@ -898,7 +899,7 @@ __global__ void __launch_bounds__(64)
     T v[K];
 #pragma unroll
     for (int c = 0; c < K; ++c) v[c] = x[e * K3 + i * K2 + j * K + c];
-    bj_element<K>(Q + e * 3 * K2, lam + e * 3 * K, smem + el * K3, i, j, v);
+    bj_element<K>(Q + e * 3 * K2, lam + e * 3 * K, smem + el * K3, qsm + el * (3 * K2 + 3 * K), i, j, v);
     if (active)
 #pragma unroll
         for (int r = 0; r < K; ++r) y[e * K3 + i * K2 + j * K + r] = v[r];

@@ -2,7 +2,8 @@
 // kappa and b = 0: the element block is a Kronecker sum of 1D matrices, so
 //   y_e = (Qx (x) Qy (x) Qz) diag(1 / (lx + ly + lz)) (Qx (x) Qy (x) Qz)^T x_e
 // with the per-element eigenvectors Qe [3][K][K] (Q_d[r][c] = Qe[d*K2 + r*K + c],
-// columns are eigenvectors) and eigenvalues le [3][K].  One element is
+// columns are eigenvectors) and eigenvalues le [3][K], read from global
+// memory (Qg, lg) into the element's LDS slice Qs [3K^2 + 3K].  One element is
 // handled by K*K lanes (i, j); lane (i, j) passes its z pencil x[i][j][0..K)
 // in v and receives y[i][j][0..K).  A is the element's K^3 LDS scratch; each
 // phase reads and rewrites the pencils of one direction, which partition the
@@ -21,10 +22,18 @@ struct BJShape {
 };
 
 template <int K, typename T>
-__device__ __forceinline__ void bj_element(const T *__restrict__ Qe, const T *__restrict__ le, T *A, int i, int j,
-                                           T (&v)[K])
+__device__ __forceinline__ void bj_element(const T *__restrict__ Qg, const T *__restrict__ lg, T *A, T *Qs, int i,
+                                           int j, T (&v)[K])
 {
     constexpr int K2 = K * K;
+    // every lane of the element needs all of Qe and le: stage them in LDS once
+    // (the element's K*K lanes copy 3K^2 + 3K values) instead of each lane
+    // loading them from memory in every pass
+    const int t = i * K + j;
+    for (int k = t; k < 3 * K2; k += K2) Qs[k] = Qg[k];
+    for (int k = t; k < 3 * K; k += K2) Qs[3 * K2 + k] = lg[k];
+    __syncthreads();
+    const T *Qe = Qs, *le = Qs + 3 * K2;
     T tmp[K];
     // Q_z^T along c: lane (i=a, j=b)
 #pragma unroll
