@@ -41,6 +41,7 @@ def parse():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="solve section without the learned preconditioner")
     ap.add_argument("--combined", action="store_true",
                     help="also run the 96^3 Q4 solves, FP64 vs mixed (about two minutes)")
     ap.add_argument("--calib", action="store_true",
@@ -115,7 +116,7 @@ def plain_section(fa, wl, ctx, cpu=True):
     return out
 
 
-def solve_section(fa, wl, ctx, n=48, pairs=2000, solves=8, epochs=3, lr=1e-4, coarse_steps=50):
+def solve_section(fa, wl, ctx, n=48, pairs=2000, solves=8, epochs=3, lr=1e-4, coarse_steps=50, learned=True):
     """High-contrast channel problem: time to a 1e-8 FP64 relative residual.
     Plain FP64 GPU paths (CG and FGMRES(30) with exact block Jacobi), FP64
     refinement around FP32 inner solves, and the learned two-level
@@ -156,6 +157,9 @@ def solve_section(fa, wl, ctx, n=48, pairs=2000, solves=8, epochs=3, lr=1e-4, co
     run("fp64_fgmres_bj", fa.FGMRES, bj64, maxit=3000)
     run("mixed_fp32_cg_bj_ir", fa.CG, bj32, op32, refinement=1, inner_tolerance=1e-3, inner_maxiterations=20000)
 
+    out["runs"] = runs
+    if not learned:
+        return out
     # learned preconditioner: pairs from plain FP64 CG + block-Jacobi solves
     # of other seeded right-hand sides, `every` chosen so the pairs span each solve
     net = fa.CNN(ctx, P["n"])
@@ -383,7 +387,7 @@ def main():
 
     # ---- time to solve (problem 3: 48^3 SIPG Q2, kappa = 10^(4H) channels) --------
     if not a.no_solve:
-        result["solve"] = solve_section(fa, wl, ctx)
+        result["solve"] = solve_section(fa, wl, ctx, learned=not a.no_learned)
 
     if not a.no_batch:
         result["batch"] = batch_section(fa, wl, ctx)
