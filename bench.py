@@ -235,8 +235,11 @@ def combined_section(fa, wl, ctx, n=96):
 
     run("fp64_cg_bj", fa.CG, op64, fa.Preconditioner(op64, fa.PRE_BLOCK_JACOBI))
     op32 = dg(fa.F32)
+    # inner tolerance 0.03: the FP32 operator's error on the smooth modes caps
+    # the reduction of one refinement step at about 30x, so a tighter inner
+    # solve spends iterations the outer step cannot use
     run("mixed_fp32_cg_bj_ir", fa.CG, op64, fa.Preconditioner(op32, fa.PRE_BLOCK_JACOBI), op32, refinement=1,
-        inner_tolerance=1e-3, inner_maxiterations=20000, verbose=1)
+        inner_tolerance=0.03, inner_maxiterations=20000, verbose=1)
     out["runs"] = runs
     best = runs["fp64_cg_bj"]["time_s"]
     out["speedup_vs_fp64_cg_bj"] = {k: round(best / v["time_s"], 3) for k, v in runs.items()}
