@@ -22,6 +22,8 @@
 // tangential contractions.
 #include <hip/hip_bf16.h>
 
+#include <type_traits>
+
 #include "fem_dg.hpp"
 #include "fem_dg_bj.hpp"
 
@@ -136,6 +138,22 @@ __device__ __forceinline__ T row_contract(const T *Bi, int i, T y)
     z = fma(Bi[(i + ROR_SRC_SIGN * 1) & 3], dpp_mov<0x124>(y), z);
     z = fma(Bi[(i + ROR_SRC_SIGN * 2) & 3], dpp_mov<0x128>(y), z);
     return fma(Bi[(i + ROR_SRC_SIGN * 3) & 3], dpp_mov<0x12C>(y), z);
+}
+
+// ---- FP64 MFMA tangential contraction (K = 4) -----------------------------
+// v_mfma_f64_4x4x4f64 computes four independent products D_b = A_b B_b.  Lane
+// layout (measured on the device): A_b[r][k] and B_b[k][r] sit in lane
+// 16 k + 4 b + r, D_b[r][c] in lane 16 r + 4 b + c.  An element is block b
+// and its lane (i, j) is 16 i + 4 b + j.
+// With N(m1, m2) in lane (m1, m2) read as the A operand (A = N^T) and the
+// constant B operand Bop[k][c] = B[c][k] (lane (i, j) supplies B[j][i]):
+//   D1[r][c] = sum_m1 B[c][m1] N(m1, r)  ->  lane (m2, q1) holds Y(q1, m2)
+//   D2 = D1 (as A) x Bop: D2[r][c] = sum_m2 Y(r, m2) B[c][m2] = Z(q1=r, q2=c)
+// so lane (i, j) receives Z(q1 = i, q2 = j) = sum B[i][m1] B[j][m2] N(m1, m2).
+__device__ __forceinline__ double mf64_tangential(double nm, double bji)
+{
+    const double y = __builtin_amdgcn_mfma_f64_4x4x4f64(nm, bji, 0.0, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f64_4x4x4f64(y, bji, 0.0, 0, 0, 0);
 }
 
 // Legendre-basis contractions use the parity of the basis on the symmetric
@@ -260,7 +278,9 @@ struct Ctr {
     }
 };
 
-template <int K, typename T, bool MF>
+// KQ: kappa given at the quadrature points (FEM_KAPPA_QP); otherwise one
+// value per element (FEM_KAPPA_CELL) or one constant (FEM_KAPPA_CONST)
+template <int K, typename T, bool MF, bool KQ>
 __global__ void __launch_bounds__(64)
     k_dg_apply(const DGArgs<T> a, const T *__restrict__ u, T *__restrict__ w, const int *__restrict__ skip)
 {
@@ -280,10 +300,13 @@ __global__ void __launch_bounds__(64)
     }
 
     const int tid = threadIdx.x;
-    const int el0 = tid / K2;
+    // FP64 at K = 4: lane 16 i + 4 el + j, the block layout of the FP64 4x4x4
+    // MFMA (one block per element); otherwise lane el K^2 + i K + j
+    constexpr bool F64MF = (K == 4) && std::is_same<T, double>::value && !MF;
+    const int el0 = F64MF ? (tid >> 2) & 3 : tid / K2;
     const bool wr = el0 < EPB;   // lanes beyond EPB*K2 (K = 3, 5) only shadow element 0
     const int el = wr ? el0 : 0;
-    const int t = tid % K2, i = t / K, j = t % K;
+    const int i = F64MF ? tid >> 4 : (tid % K2) / K, j = F64MF ? tid & 3 : tid % K;
     const int nx = a.n[0], ny = a.n[1], nz = a.n[2];
     const int E = nx * ny * nz;
     const int blk = (int)xcd_remap(blockIdx.x, gridDim.x);
@@ -341,16 +364,16 @@ __global__ void __launch_bounds__(64)
                 nbr[d][s2][m] = un[off];
             }
         }
-    const bool kqp = a.kappa_mode == FEM_KAPPA_QP;
+    const bool kcell = a.kappa_mode == FEM_KAPPA_CELL;
     T kx[K], ky[K], kz[K], kf[3][2];
     {
-        const T kel0 = a.kappa_mode == FEM_KAPPA_CELL ? a.kcell[e] : a.kappa_const;
-        const T *kv = kqp ? a.kvol + (size_t)e * K3 : nullptr;
+        const T kel0 = KQ ? T(0) : kcell ? a.kcell[e] : a.kappa_const;
+        const T *kv = KQ ? a.kvol + (size_t)e * K3 : nullptr;
 #pragma unroll
         for (int q = 0; q < K; ++q) {
-            kx[q] = kqp ? kv[q + K * i + K2 * j] : kel0;
-            ky[q] = kqp ? kv[i + K * q + K2 * j] : kel0;
-            kz[q] = kqp ? kv[i + K * j + K2 * q] : kel0;
+            kx[q] = KQ ? kv[q + K * i + K2 * j] : kel0;
+            ky[q] = KQ ? kv[i + K * q + K2 * j] : kel0;
+            kz[q] = KQ ? kv[i + K * j + K2 * q] : kel0;
         }
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
@@ -359,15 +382,16 @@ __global__ void __launch_bounds__(64)
             const int fstep = d == 0 ? 1 : d == 1 ? nx : nx * ny;
 #pragma unroll
             for (int s2 = 0; s2 < 2; ++s2) {
-                if (kqp) kf[d][s2] = a.kface[d][(size_t)(fbase + (s2 ? fstep : 0)) * K2 + i + K * j];
-                else if (a.kappa_mode == FEM_KAPPA_CELL)
-                    kf[d][s2] = has[d][s2] ? a.kcell[e + (s2 ? stride[d] : -stride[d])] : kel0;
-                else kf[d][s2] = kel0;
+                if constexpr (KQ) kf[d][s2] = a.kface[d][(size_t)(fbase + (s2 ? fstep : 0)) * K2 + i + K * j];
+                else kf[d][s2] = (kcell && has[d][s2]) ? a.kcell[e + (s2 ? stride[d] : -stride[d])] : kel0;
             }
         }
     }
     // normal contraction of the neighbour pencils: thread (i=m1, j=m2)
     T nval[3][2], nder[3][2];
+    // B[j][i]: this lane's entry of the constant MFMA operand
+    T bji = T(0);
+    if constexpr (F64MF) bji = tb.B[j][i];
 
     // ---- forward: modal -> quadrature points ------------------------------
     // Z layout: thread (i=a, j=b) holds u[a][b][0..K)
@@ -467,7 +491,16 @@ __global__ void __launch_bounds__(64)
         // neighbour trace value / normal derivative at face point (q1=i, q2=j):
         // two tangential contractions of the lane's modal (m1=i, m2=j) values
         T nvs[2], nds[2];
-        if constexpr (K == 4) {
+        if constexpr (F64MF) {
+            // Z = B N B^T on the matrix cores: two chained 4x4x4 products per
+            // quantity, no LDS and no cross-lane moves
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                nvs[s] = mf64_tangential(nval[d][s], bji);
+                nds[s] = mf64_tangential(nder[d][s], bji);
+            }
+            wave_lds_sync();
+        } else if constexpr (K == 4) {
             // an element is one 16-lane DPP row: contract across lanes, no LDS
 #pragma unroll
             for (int s = 0; s < 2; ++s) {
@@ -535,7 +568,9 @@ __global__ void __launch_bounds__(64)
             const T bn = a.b[d] * nrm;
             const T bp = bn > T(0) ? bn : T(0), bm = bn < T(0) ? -bn : T(0);
             T kin, kout, kp;
-            if (a.kappa_mode == FEM_KAPPA_CELL) {
+            if constexpr (!KQ) {
+                // cellwise kappa: average of the two elements' values
+                // (the constant case has kout = kin)
                 kin = kx[0];
                 kout = kf[d][s];
                 kp = T(0.5) * (kin + kout);
@@ -603,7 +638,10 @@ static void launch_apply(const DGArgs<T> &a, const T *u, T *w, const int *skip, 
     const int64_t E = (int64_t)a.n[0] * a.n[1] * a.n[2];
     FEM_CHECK(E * S::K3 < (int64_t)1 << 31, FEM_ERR_UNSUPPORTED, "DG apply: more than 2^31 DoFs");
     const unsigned nb = (unsigned)((E + S::EPB - 1) / S::EPB);
-    hipLaunchKernelGGL((k_dg_apply<K, T, MF>), dim3(nb), dim3(64), 0, s, a, u, w, skip);
+    if (a.kappa_mode == FEM_KAPPA_QP)
+        hipLaunchKernelGGL((k_dg_apply<K, T, MF, true>), dim3(nb), dim3(64), 0, s, a, u, w, skip);
+    else
+        hipLaunchKernelGGL((k_dg_apply<K, T, MF, false>), dim3(nb), dim3(64), 0, s, a, u, w, skip);
     FEM_HIP(hipGetLastError());
 }
 
