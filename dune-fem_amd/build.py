@@ -26,10 +26,16 @@ FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-Wall", "-Wno-u
          "-I", os.path.join(ROOT, "include"), "-I", SRC]
 
 
+# per-source flags: the DG apply keeps the FP64 MFMA results in ordinary
+# VGPRs (the accumulation registers would add to the register total and cost
+# a wave per SIMD)
+SRC_FLAGS = {"fem_dg.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+
+
 def compile_one(src):
     obj = os.path.join(OBJ_DIR, src + ".o")
     lang = ["-x", "hip"] if src.endswith(".cpp") else []
-    cmd = [HIPCC] + FLAGS + lang + ["-c", os.path.join(SRC, src), "-o", obj]
+    cmd = [HIPCC] + FLAGS + SRC_FLAGS.get(src, []) + lang + ["-c", os.path.join(SRC, src), "-o", obj]
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:
         raise RuntimeError("compile failed: %s\n%s\n%s" % (" ".join(cmd), p.stdout, p.stderr))

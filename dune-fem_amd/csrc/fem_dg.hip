@@ -422,75 +422,10 @@ __global__ void __launch_bounds__(64)
             nval[d][s2] = has[d][s2] ? val : T(0);
             nder[d][s2] = has[d][s2] ? der : T(0);
         }
-    // X layout: thread (i=qy, j=qz)
-    T uq[K];
-#pragma unroll
-    for (int b = 0; b < K; ++b) v[b] = A[lidx<K>(b, i, j)];
-    C::B(tb, mr, v, uq);
-    wave_lds_sync();
-    if (wr)
-#pragma unroll
-        for (int q = 0; q < K; ++q) A[lidx<K>(q, i, j)] = uq[q];
-
-
-    // x-direction volume flux (kept in registers, X layout)
-    T Rx[K];
-    {
-        const T sc = wij * detJ * (a.ih[0] * a.ih[0]), bh = a.b[0] * a.h[0];
-        T g[K], r[K];
-        C::D(tb, mr, uq, g);
-#pragma unroll
-        for (int q = 0; q < K; ++q) {
-            r[q] = (tb.w[q] * sc) * fma(kx[q], g[q], -bh * uq[q]);
-        }
-        C::DT(tb, mr, r, Rx);
-        if (a.c != T(0)) {
-            const T cm = wij * detJ * a.c;
-#pragma unroll
-            for (int q = 0; q < K; ++q) Rx[q] = fma(tb.w[q] * cm, uq[q], Rx[q]);
-        }
-    }
-    wave_lds_sync();
-    // Y layout of u_q: thread (i=qx, j=qz) -> R (y part)
-    {
-        const T sc = wij * detJ * (a.ih[1] * a.ih[1]), bh = a.b[1] * a.h[1];
-        T g[K], r[K];
-#pragma unroll
-        for (int q = 0; q < K; ++q) v[q] = A[lidx<K>(i, q, j)];
-        C::D(tb, mr, v, g);
-#pragma unroll
-        for (int q = 0; q < K; ++q) {
-            r[q] = (tb.w[q] * sc) * fma(ky[q], g[q], -bh * v[q]);
-        }
-        C::DT(tb, mr, r, tmp);
-        if (wr)
-#pragma unroll
-            for (int q = 0; q < K; ++q) R[lidx<K>(i, q, j)] = tmp[q];
-    }
-    wave_lds_sync();
-    // Z layout of u_q: thread (i=qx, j=qy) -> R += z part
-    {
-        const T sc = wij * detJ * (a.ih[2] * a.ih[2]), bh = a.b[2] * a.h[2];
-        T g[K], r[K];
-#pragma unroll
-        for (int q = 0; q < K; ++q) v[q] = A[lidx<K>(i, j, q)];
-        C::D(tb, mr, v, g);
-#pragma unroll
-        for (int q = 0; q < K; ++q) {
-            r[q] = (tb.w[q] * sc) * fma(kz[q], g[q], -bh * v[q]);
-        }
-        C::DT(tb, mr, r, tmp);
-        if (wr)
-#pragma unroll
-            for (int q = 0; q < K; ++q) R[lidx<K>(i, j, q)] += tmp[q];
-    }
-
-    // ---- faces: neighbour traces, numerical fluxes, lifting into R ---------
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        // neighbour trace value / normal derivative at face point (q1=i, q2=j):
-        // two tangential contractions of the lane's modal (m1=i, m2=j) values
-        T nvs[2], nds[2];
+    // ---- faces: neighbour traces, numerical fluxes, lifting ----------------
+    // neighbour trace value / normal derivative at face point (q1=i, q2=j):
+    // two tangential contractions of the lane's modal (m1=i, m2=j) values
+    auto neighbour_traces = [&](int d, T (&nvs)[2], T (&nds)[2]) {
         if constexpr (F64MF) {
             // Z = B N B^T on the matrix cores: two chained 4x4x4 products per
             // quantity, no LDS and no cross-lane moves
@@ -499,7 +434,6 @@ __global__ void __launch_bounds__(64)
                 nvs[s] = mf64_tangential(nval[d][s], bji);
                 nds[s] = mf64_tangential(nder[d][s], bji);
             }
-            wave_lds_sync();
         } else if constexpr (K == 4) {
             // an element is one 16-lane DPP row: contract across lanes, no LDS
 #pragma unroll
@@ -507,7 +441,6 @@ __global__ void __launch_bounds__(64)
                 nvs[s] = row_contract(tb.B[i], i, quad_contract(tb.B[j], nval[d][s]));
                 nds[s] = row_contract(tb.B[i], i, quad_contract(tb.B[j], nder[d][s]));
             }
-            wave_lds_sync();
         } else {
             wave_lds_sync();
             if (wr) {
@@ -518,13 +451,13 @@ __global__ void __launch_bounds__(64)
                 }
             }
             wave_lds_sync();
-            // (2) second tangential index, in place: thread (i=m1, j=q2)
-            T t2v[4];
+            // second tangential index, in place: thread (i=m1, j=q2)
+            T t2v[4], w2[K];
 #pragma unroll
             for (int sq = 0; sq < 4; ++sq) {
 #pragma unroll
-                for (int m = 0; m < K; ++m) v[m] = Sx[sq * K2 + i * K + m];
-                t2v[sq] = dotk<K>(tb.B[j], v);
+                for (int m = 0; m < K; ++m) w2[m] = Sx[sq * K2 + i * K + m];
+                t2v[sq] = dotk<K>(tb.B[j], w2);
             }
             wave_lds_sync();
             if (wr)
@@ -544,21 +477,16 @@ __global__ void __launch_bounds__(64)
                 nds[s] = nd;
             }
         }
-        // own traces from u_q, flux, lift: thread (i=q1, j=q2) owns the R
-        // pencil along d through (q1, q2)
+    };
+    // the two faces normal to d: own traces from the u_q pencil along d (up),
+    // numerical fluxes, lifting into the residual pencil along d (rp); the
+    // lane (i=q1, j=q2) holds both pencils through face point (q1, q2)
+    auto face_terms = [&](int d, const T *up, T *rp) {
+        T nvs[2], nds[2];
+        neighbour_traces(d, nvs, nds);
         const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
         const T Wf = wij * a.h[t1] * a.h[t2];
         const T ihd = a.ih[d];
-        const int idx = i + K * j;
-        int poff[K];
-#pragma unroll
-        for (int m = 0; m < K; ++m) poff[m] = d == 0 ? lidx<K>(m, i, j) : d == 1 ? lidx<K>(i, m, j) : lidx<K>(i, j, m);
-        T up[K], rp[K];
-#pragma unroll
-        for (int m = 0; m < K; ++m) {
-            up[m] = A[poff[m]];
-            rp[m] = R[poff[m]];
-        }
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
             const T nrm = s ? T(1) : T(-1);
@@ -594,10 +522,74 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
             for (int m = 0; m < K; ++m) rp[m] = fma(tb.E[s][m], c0, fma(tb.dE[s][m], c1, rp[m]));
         }
-        wave_lds_sync();
+    };
+
+    // ---- volume fluxes, each direction with its two faces -------------------
+    // X layout: thread (i=qy, j=qz)
+    T uq[K];
+#pragma unroll
+    for (int b = 0; b < K; ++b) v[b] = A[lidx<K>(b, i, j)];
+    C::B(tb, mr, v, uq);
+    wave_lds_sync();
+    if (wr)
+#pragma unroll
+        for (int q = 0; q < K; ++q) A[lidx<K>(q, i, j)] = uq[q];
+
+    // x direction: residual pencil kept in registers (X layout)
+    T Rx[K];
+    {
+        const T sc = wij * detJ * (a.ih[0] * a.ih[0]), bh = a.b[0] * a.h[0];
+        T g[K], r[K];
+        C::D(tb, mr, uq, g);
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            r[q] = (tb.w[q] * sc) * fma(kx[q], g[q], -bh * uq[q]);
+        }
+        C::DT(tb, mr, r, Rx);
+        if (a.c != T(0)) {
+            const T cm = wij * detJ * a.c;
+#pragma unroll
+            for (int q = 0; q < K; ++q) Rx[q] = fma(tb.w[q] * cm, uq[q], Rx[q]);
+        }
+        face_terms(0, uq, Rx);
+    }
+    wave_lds_sync();
+    // Y layout of u_q: thread (i=qx, j=qz) -> R = y part
+    {
+        const T sc = wij * detJ * (a.ih[1] * a.ih[1]), bh = a.b[1] * a.h[1];
+        T g[K], r[K];
+#pragma unroll
+        for (int q = 0; q < K; ++q) v[q] = A[lidx<K>(i, q, j)];
+        C::D(tb, mr, v, g);
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            r[q] = (tb.w[q] * sc) * fma(ky[q], g[q], -bh * v[q]);
+        }
+        C::DT(tb, mr, r, tmp);
+        face_terms(1, v, tmp);
         if (wr)
 #pragma unroll
-            for (int m = 0; m < K; ++m) R[poff[m]] = rp[m];
+            for (int q = 0; q < K; ++q) R[lidx<K>(i, q, j)] = tmp[q];
+    }
+    wave_lds_sync();
+    // Z layout of u_q: thread (i=qx, j=qy) -> R += z part
+    {
+        const T sc = wij * detJ * (a.ih[2] * a.ih[2]), bh = a.b[2] * a.h[2];
+        T g[K], r[K];
+#pragma unroll
+        for (int q = 0; q < K; ++q) v[q] = A[lidx<K>(i, j, q)];
+        C::D(tb, mr, v, g);
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            r[q] = (tb.w[q] * sc) * fma(kz[q], g[q], -bh * v[q]);
+        }
+        C::DT(tb, mr, r, tmp);
+#pragma unroll
+        for (int q = 0; q < K; ++q) tmp[q] += R[lidx<K>(i, j, q)];
+        face_terms(2, v, tmp);
+        if (wr)
+#pragma unroll
+            for (int q = 0; q < K; ++q) R[lidx<K>(i, j, q)] = tmp[q];
     }
     wave_lds_sync();
 
