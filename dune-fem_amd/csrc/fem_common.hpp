@@ -44,6 +44,12 @@ constexpr int KMAX = 8;
 //   Bd[q][a]  derivative of Legendre basis a at Gauss point q
 //   S[a][b]   1D stiffness int_0^1 phi_a' phi_b'
 //   w[q], x[q] Gauss weights and points on [0,1]
+//   Dp, Dm    even/odd halves of Dt (skew-centrosymmetric on the symmetric
+//             points, Dt[m-1-q][m-1-r] = -Dt[q][r]), with h = m/2, c = m/2:
+//             Dp[q][r] = (Dt[q][r] + Dt[q][m-1-r]) / 2, Dp[q][h] = Dt[q][c]
+//             (odd m), Dm[q][r] = (Dt[q][r] - Dt[q][m-1-r]) / 2,
+//             Dm[h][r] = Dt[c][r] (odd m), for q, r < h
+//   DTp, DTm  the same for the transpose Dt^T
 template <typename T>
 struct Tab1D {
     T B[KMAX][KMAX];
@@ -56,6 +62,10 @@ struct Tab1D {
     T dP[2][KMAX];
     T w[KMAX];
     T x[KMAX];
+    T Dp[KMAX / 2][KMAX / 2 + 1];
+    T Dm[KMAX / 2 + 1][KMAX / 2];
+    T DTp[KMAX / 2][KMAX / 2 + 1];
+    T DTm[KMAX / 2 + 1][KMAX / 2];
 };
 
 // host-side table construction (fem_tables.cpp)

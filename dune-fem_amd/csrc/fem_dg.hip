@@ -198,6 +198,43 @@ __device__ __forceinline__ void mvBT(const T (&M)[KMAX][KMAX], const T *in, T *o
     }
 }
 
+// Collocation-derivative contractions by even/odd halves: on the symmetric
+// Gauss points Dt is skew-centrosymmetric, so with s_r = in[r] + in[K-1-r],
+// a_r = in[r] - in[K-1-r] (r < K/2):
+//   out[q] = E_q + O_q, out[K-1-q] = O_q - E_q,
+//   E_q = sum_r Dp[q][r] s_r (+ Dp[q][K/2] in[K/2]), O_q = sum_r Dm[q][r] a_r,
+// and for odd K the middle row out[K/2] = sum_r Dm[K/2][r] a_r.  Half the
+// coefficients of the full product, and fewer operations for odd K.
+template <int K, typename T>
+__device__ __forceinline__ void mvD(const T (&Pp)[KMAX / 2][KMAX / 2 + 1], const T (&Pm)[KMAX / 2 + 1][KMAX / 2],
+                                    const T *in, T *out)
+{
+    constexpr int H = K / 2;
+    T sv[H > 0 ? H : 1], av[H > 0 ? H : 1];
+#pragma unroll
+    for (int r = 0; r < H; ++r) {
+        sv[r] = in[r] + in[K - 1 - r];
+        av[r] = in[r] - in[K - 1 - r];
+    }
+#pragma unroll
+    for (int q = 0; q < H; ++q) {
+        T e = (K % 2) ? Pp[q][H] * in[H] : T(0), o = 0;
+#pragma unroll
+        for (int r = 0; r < H; ++r) {
+            e = fma(Pp[q][r], sv[r], e);
+            o = fma(Pm[q][r], av[r], o);
+        }
+        out[q] = e + o;
+        out[K - 1 - q] = o - e;
+    }
+    if (K % 2) {
+        T m = 0;
+#pragma unroll
+        for (int r = 0; r < H; ++r) m = fma(Pm[H][r], av[r], m);
+        out[H] = m;
+    }
+}
+
 // ---- BF16 MFMA contractions (v_mfma_f32_4x4x4bf16_1k, 16 blocks) --------
 // Operand layout of the 16-block 4x4x4 instruction: lane (i + 4b) supplies
 // row i of A_b and column i of B_b (4 values each) and receives column i of
@@ -269,12 +306,12 @@ struct Ctr {
     static __device__ __forceinline__ void D(const Tab1D<T> &tb, const MFRows &r, const T *in, T *out)
     {
         if constexpr (MF) mf_mv<K>(r.D, tb.Dt, false, in, out);
-        else mv<K>(tb.Dt, in, out);
+        else mvD<K>(tb.Dp, tb.Dm, in, out);
     }
     static __device__ __forceinline__ void DT(const Tab1D<T> &tb, const MFRows &r, const T *in, T *out)
     {
         if constexpr (MF) mf_mv<K>(r.DT, tb.Dt, true, in, out);
-        else mvT<K>(tb.Dt, in, out);
+        else mvD<K>(tb.DTp, tb.DTm, in, out);
     }
 };
 
@@ -413,11 +450,14 @@ __global__ void __launch_bounds__(64)
     for (int d = 0; d < 3; ++d)
 #pragma unroll
         for (int s2 = 0; s2 < 2; ++s2) {
+            // the neighbour below (s2 = 0) is evaluated at its face xi = 1,
+            // the one above at xi = 0: P(0) = (-1)^m P(1), P'(0) = -(-1)^m P'(1)
             T val = 0, der = 0;
 #pragma unroll
             for (int m = 0; m < K; ++m) {
-                val = fma(tb.P[1 - s2][m], nbr[d][s2][m], val);
-                der = fma(tb.dP[1 - s2][m], nbr[d][s2][m], der);
+                const T sg = (s2 == 1 && (m & 1)) ? T(-1) : T(1);
+                val = fma(sg * tb.P[1][m], nbr[d][s2][m], val);
+                der = fma(s2 == 1 ? -sg * tb.dP[1][m] : tb.dP[1][m], nbr[d][s2][m], der);
             }
             nval[d][s2] = has[d][s2] ? val : T(0);
             nder[d][s2] = has[d][s2] ? der : T(0);
@@ -487,12 +527,21 @@ __global__ void __launch_bounds__(64)
         const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
         const T Wf = wij * a.h[t1] * a.h[t2];
         const T ihd = a.ih[d];
+        // extrapolation to the faces from the symmetric Gauss points:
+        // E[1][q] = E[0][K-1-q], dE[1][q] = -dE[0][K-1-q]
+        auto Es = [&](int s, int m) { return s ? tb.E[0][K - 1 - m] : tb.E[0][m]; };
+        auto dEs = [&](int s, int m) { return s ? -tb.dE[0][K - 1 - m] : tb.dE[0][m]; };
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
             const T nrm = s ? T(1) : T(-1);
             const bool interior = has[d][s];
-            const T uin = dotk<K>(tb.E[s], up);
-            const T din = dotk<K>(tb.dE[s], up) * (ihd * nrm);
+            T uin = 0, din = 0;
+#pragma unroll
+            for (int m = 0; m < K; ++m) {
+                uin = fma(Es(s, m), up[m], uin);
+                din = fma(dEs(s, m), up[m], din);
+            }
+            din *= ihd * nrm;
             const T bn = a.b[d] * nrm;
             const T bp = bn > T(0) ? bn : T(0), bm = bn < T(0) ? -bn : T(0);
             T kin, kout, kp;
@@ -520,7 +569,7 @@ __global__ void __launch_bounds__(64)
             }
             c1 *= nrm * ihd;
 #pragma unroll
-            for (int m = 0; m < K; ++m) rp[m] = fma(tb.E[s][m], c0, fma(tb.dE[s][m], c1, rp[m]));
+            for (int m = 0; m < K; ++m) rp[m] = fma(Es(s, m), c0, fma(dEs(s, m), c1, rp[m]));
         }
     };
 

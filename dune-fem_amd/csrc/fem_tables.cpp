@@ -115,6 +115,31 @@ Tab1D<double> make_tab1d(int k)
             t.Dt[q][r] = (double)d;
         }
     }
+    // even/odd halves of Dt and Dt^T, from the long double values
+    {
+        long double D[KMAX][KMAX];
+        for (int q = 0; q < k; ++q)
+            for (int r = 0; r < k; ++r) {
+                long double v, d;
+                lagr_ld(k, xs, r, xs[q], v, d);
+                D[q][r] = d;
+            }
+        const int h = k / 2, c = k / 2;
+        for (int tr = 0; tr < 2; ++tr) {
+            auto at = [&](int q, int r) { return tr ? D[r][q] : D[q][r]; };
+            double(*Pp)[KMAX / 2 + 1] = tr ? t.DTp : t.Dp;
+            double(*Pm)[KMAX / 2] = tr ? t.DTm : t.Dm;
+            for (int q = 0; q < h; ++q) {
+                for (int r = 0; r < h; ++r) {
+                    Pp[q][r] = (double)(0.5L * (at(q, r) + at(q, k - 1 - r)));
+                    Pm[q][r] = (double)(0.5L * (at(q, r) - at(q, k - 1 - r)));
+                }
+                if (k % 2) Pp[q][h] = (double)at(q, c);
+            }
+            if (k % 2)
+                for (int r = 0; r < h; ++r) Pm[h][r] = (double)at(c, r);
+        }
+    }
     for (int a = 0; a < k; ++a)
         for (int b = 0; b < k; ++b) {
             long double s = 0;
