@@ -350,7 +350,7 @@ def main():
                    "dofs": N, "order": p, "elements": E},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(p, n),
-                     "kernel": "k_dg_apply<%d,double>" % k, "kernel_ms": round(kern_ms, 5),
+                     "kernel": "k_dg_apply<%d,double,false,true>" % k, "kernel_ms": round(kern_ms, 5),
                      "alg_bytes_per_launch": b_alg},
     }
 

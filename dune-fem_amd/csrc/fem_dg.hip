@@ -322,9 +322,20 @@ __global__ void __launch_bounds__(64)
     k_dg_apply(const DGArgs<T> a, const T *__restrict__ u, T *__restrict__ w, const int *__restrict__ skip)
 {
     using S = DGShape<K>;
-    constexpr int K2 = S::K2, K3 = S::K3, EPB = S::EPB, TS = S::TS;
+    constexpr int K2 = S::K2, K3 = S::K3, EPB = S::EPB;
     if (skip && *skip) return;
-    __shared__ T smem[EPB * S::SZ];
+    // FP64 MFMA path (K = 4, lane 16 i + 4 el + j): a 16-lane LDS access
+    // group holds one i of all four elements.  The layout (a, b, c) ->
+    // a * 20 + b * 5 + c with an element stride of 164 doubles puts every
+    // pencil access pattern of the kernel on 16 distinct 8-byte banks, with
+    // compile-time offsets only (searched exhaustively over additive layouts).
+    // Otherwise the layout of DGShape (one element per 16-lane group).
+    constexpr bool F64MF_ = (K == 4) && std::is_same<T, double>::value && !MF;
+    constexpr int S1 = F64MF_ ? 20 : S::S1;
+    constexpr int SB = F64MF_ ? 5 : K;
+    constexpr int SZ = F64MF_ ? 164 : S::SZ;
+    static_assert(!F64MF_ || SZ >= 2 * K * S1, "two tensors per element");
+    __shared__ T smem[EPB * SZ];
     const Tab1D<T> &tb = *a.tab;
     using C = Ctr<K, T, MF>;
     MFRows mr;
@@ -369,9 +380,13 @@ __global__ void __launch_bounds__(64)
     const int ec[3] = { ex, ey, ez };
     const int stride[3] = { 1, nx, nx * ny };
 
-    T *A = smem + el * S::SZ;
-    T *R = A + TS;
-    T *Sx = R + TS;   // [2 faces][2 quantities][K2]
+    T *A = smem + el * SZ;
+    T *R = A + K * S1;
+    auto L = [&](int a_, int b_, int c_) {
+        if constexpr (F64MF_) return a_ * S1 + b_ * SB + c_;
+        else return lidx<K>(a_, b_, c_);
+    };
+    T *Sx = R + K * S1;   // [2 faces][2 quantities][K2] (K != 4 only)
 
     // weights folded per thread: volume point (i, j) of a pencil, scaled by
     // |K| / h_d^2 for the diffusion part; advection scaled by h_d
@@ -435,16 +450,16 @@ __global__ void __launch_bounds__(64)
     C::B(tb, mr, v, tmp);
     if (wr)
 #pragma unroll
-        for (int q = 0; q < K; ++q) A[lidx<K>(i, j, q)] = tmp[q];
+        for (int q = 0; q < K; ++q) A[L(i, j, q)] = tmp[q];
     wave_lds_sync();
     // Y layout: thread (i=a, j=qz); each thread rewrites the pencil it read
 #pragma unroll
-    for (int b = 0; b < K; ++b) v[b] = A[lidx<K>(i, b, j)];
+    for (int b = 0; b < K; ++b) v[b] = A[L(i, b, j)];
     C::B(tb, mr, v, tmp);
     wave_lds_sync();
     if (wr)
 #pragma unroll
-        for (int q = 0; q < K; ++q) A[lidx<K>(i, q, j)] = tmp[q];
+        for (int q = 0; q < K; ++q) A[L(i, q, j)] = tmp[q];
     wave_lds_sync();
 #pragma unroll
     for (int d = 0; d < 3; ++d)
@@ -577,12 +592,12 @@ __global__ void __launch_bounds__(64)
     // X layout: thread (i=qy, j=qz)
     T uq[K];
 #pragma unroll
-    for (int b = 0; b < K; ++b) v[b] = A[lidx<K>(b, i, j)];
+    for (int b = 0; b < K; ++b) v[b] = A[L(b, i, j)];
     C::B(tb, mr, v, uq);
     wave_lds_sync();
     if (wr)
 #pragma unroll
-        for (int q = 0; q < K; ++q) A[lidx<K>(q, i, j)] = uq[q];
+        for (int q = 0; q < K; ++q) A[L(q, i, j)] = uq[q];
 
     // x direction: residual pencil kept in registers (X layout)
     T Rx[K];
@@ -608,7 +623,7 @@ __global__ void __launch_bounds__(64)
         const T sc = wij * detJ * (a.ih[1] * a.ih[1]), bh = a.b[1] * a.h[1];
         T g[K], r[K];
 #pragma unroll
-        for (int q = 0; q < K; ++q) v[q] = A[lidx<K>(i, q, j)];
+        for (int q = 0; q < K; ++q) v[q] = A[L(i, q, j)];
         C::D(tb, mr, v, g);
 #pragma unroll
         for (int q = 0; q < K; ++q) {
@@ -618,7 +633,7 @@ __global__ void __launch_bounds__(64)
         face_terms(1, v, tmp);
         if (wr)
 #pragma unroll
-            for (int q = 0; q < K; ++q) R[lidx<K>(i, q, j)] = tmp[q];
+            for (int q = 0; q < K; ++q) R[L(i, q, j)] = tmp[q];
     }
     wave_lds_sync();
     // Z layout of u_q: thread (i=qx, j=qy) -> R += z part
@@ -626,7 +641,7 @@ __global__ void __launch_bounds__(64)
         const T sc = wij * detJ * (a.ih[2] * a.ih[2]), bh = a.b[2] * a.h[2];
         T g[K], r[K];
 #pragma unroll
-        for (int q = 0; q < K; ++q) v[q] = A[lidx<K>(i, j, q)];
+        for (int q = 0; q < K; ++q) v[q] = A[L(i, j, q)];
         C::D(tb, mr, v, g);
 #pragma unroll
         for (int q = 0; q < K; ++q) {
@@ -634,11 +649,11 @@ __global__ void __launch_bounds__(64)
         }
         C::DT(tb, mr, r, tmp);
 #pragma unroll
-        for (int q = 0; q < K; ++q) tmp[q] += R[lidx<K>(i, j, q)];
+        for (int q = 0; q < K; ++q) tmp[q] += R[L(i, j, q)];
         face_terms(2, v, tmp);
         if (wr)
 #pragma unroll
-            for (int q = 0; q < K; ++q) R[lidx<K>(i, j, q)] = tmp[q];
+            for (int q = 0; q < K; ++q) R[L(i, j, q)] = tmp[q];
     }
     wave_lds_sync();
 
@@ -646,24 +661,24 @@ __global__ void __launch_bounds__(64)
     // X layout: thread (i=qy, j=qz)
     T Rt[K];
 #pragma unroll
-    for (int q = 0; q < K; ++q) Rt[q] = Rx[q] + R[lidx<K>(q, i, j)];
+    for (int q = 0; q < K; ++q) Rt[q] = Rx[q] + R[L(q, i, j)];
     C::BT(tb, mr, Rt, tmp);
     if (wr)
 #pragma unroll
-        for (int q = 0; q < K; ++q) A[lidx<K>(q, i, j)] = tmp[q];
+        for (int q = 0; q < K; ++q) A[L(q, i, j)] = tmp[q];
     wave_lds_sync();
     // Y layout: thread (i=a, j=qz)
 #pragma unroll
-    for (int q = 0; q < K; ++q) v[q] = A[lidx<K>(i, q, j)];
+    for (int q = 0; q < K; ++q) v[q] = A[L(i, q, j)];
     C::BT(tb, mr, v, tmp);
     wave_lds_sync();
     if (wr)
 #pragma unroll
-        for (int q = 0; q < K; ++q) A[lidx<K>(i, q, j)] = tmp[q];
+        for (int q = 0; q < K; ++q) A[L(i, q, j)] = tmp[q];
     wave_lds_sync();
     // Z layout: thread (i=a, j=b)
 #pragma unroll
-    for (int q = 0; q < K; ++q) v[q] = A[lidx<K>(i, j, q)];
+    for (int q = 0; q < K; ++q) v[q] = A[L(i, j, q)];
     C::BT(tb, mr, v, tmp);
     if (active) {
         T *we = w + (size_t)e * K3;
