@@ -56,6 +56,9 @@ CASES = [
     # in-range neighbour indices (regression: they once addressed -nx*ny
     # elements before the vector)
     ((16, 16, 16), 4, (0, 0, 0), 0.0, "cell"),
+    # Q3 traversal in slabs of 4 and 1 y-rows (ny = 12, ny = 3)
+    ((8, 12, 3), 3, (1, .5, .25), 0.0, "qp"),
+    ((4, 3, 5), 3, (0, 0, 0), 0.5, "cell"),
 ]
 
 
