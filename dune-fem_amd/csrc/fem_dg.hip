@@ -333,8 +333,11 @@ __global__ void __launch_bounds__(64)
     constexpr bool F64MF_ = (K == 4) && std::is_same<T, double>::value && !MF;
     constexpr int S1 = F64MF_ ? 20 : S::S1;
     constexpr int SB = F64MF_ ? 5 : K;
-    constexpr int SZ = F64MF_ ? 164 : S::SZ;
-    static_assert(!F64MF_ || SZ >= 2 * K * S1, "two tensors per element");
+    // with pointwise kappa a third tensor KA holds the element's volume kappa
+    // values: loaded once as x-pencils, the y- and z-pencils come from LDS
+    constexpr bool KLDS = F64MF_ && KQ;
+    constexpr int SZ = F64MF_ ? (KLDS ? 244 : 164) : S::SZ;
+    static_assert(!F64MF_ || SZ >= (KLDS ? 3 : 2) * K * S1, "tensors per element");
     __shared__ T smem[EPB * SZ];
     const Tab1D<T> &tb = *a.tab;
     using C = Ctr<K, T, MF>;
@@ -393,6 +396,7 @@ __global__ void __launch_bounds__(64)
 
     T *A = smem + el * SZ;
     T *R = A + K * S1;
+    T *KA = R + K * S1;   // pointwise kappa at the volume points (KLDS only)
     auto L = [&](int a_, int b_, int c_) {
         if constexpr (F64MF_) return a_ * S1 + b_ * SB + c_;
         else return lidx<K>(a_, b_, c_);
@@ -435,8 +439,10 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
         for (int q = 0; q < K; ++q) {
             kx[q] = KQ ? kv[q + K * i + K2 * j] : kel0;
-            ky[q] = KQ ? kv[i + K * q + K2 * j] : kel0;
-            kz[q] = KQ ? kv[i + K * j + K2 * q] : kel0;
+            if constexpr (!KLDS) {
+                ky[q] = KQ ? kv[i + K * q + K2 * j] : kel0;
+                kz[q] = KQ ? kv[i + K * j + K2 * q] : kel0;
+            }
         }
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
@@ -462,6 +468,9 @@ __global__ void __launch_bounds__(64)
     if (wr)
 #pragma unroll
         for (int q = 0; q < K; ++q) A[L(i, j, q)] = tmp[q];
+    if constexpr (KLDS)
+#pragma unroll
+        for (int q = 0; q < K; ++q) KA[L(q, i, j)] = kx[q];
     wave_lds_sync();
     // Y layout: thread (i=a, j=qz); each thread rewrites the pencil it read
 #pragma unroll
@@ -635,6 +644,9 @@ __global__ void __launch_bounds__(64)
         T g[K], r[K];
 #pragma unroll
         for (int q = 0; q < K; ++q) v[q] = A[L(i, q, j)];
+        if constexpr (KLDS)
+#pragma unroll
+            for (int q = 0; q < K; ++q) ky[q] = KA[L(i, q, j)];
         C::D(tb, mr, v, g);
 #pragma unroll
         for (int q = 0; q < K; ++q) {
@@ -653,6 +665,9 @@ __global__ void __launch_bounds__(64)
         T g[K], r[K];
 #pragma unroll
         for (int q = 0; q < K; ++q) v[q] = A[L(i, j, q)];
+        if constexpr (KLDS)
+#pragma unroll
+            for (int q = 0; q < K; ++q) kz[q] = KA[L(i, j, q)];
         C::D(tb, mr, v, g);
 #pragma unroll
         for (int q = 0; q < K; ++q) {
