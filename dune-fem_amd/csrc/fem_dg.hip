@@ -335,9 +335,12 @@ __global__ void __launch_bounds__(64)
     constexpr int SB = F64MF_ ? 5 : K;
     // with pointwise kappa a third tensor KA holds the element's volume kappa
     // values: loaded once as x-pencils, the y- and z-pencils come from LDS
-    constexpr bool KLDS = F64MF_ && KQ;
+    // (K = 4 only: the other orders keep the face-trace scratch there)
+    constexpr bool KLDS = (K == 4) && KQ;
     constexpr int SZ = F64MF_ ? (KLDS ? 244 : 164) : S::SZ;
     static_assert(!F64MF_ || SZ >= (KLDS ? 3 : 2) * K * S1, "tensors per element");
+    // DGShape layout at K = 4: the third tensor ends at 2 TS + lidx(3,3,3) + 1
+    static_assert(F64MF_ || !KLDS || SZ >= 2 * K * S1 + 3 * S1 + 3 * K + 4, "tensors per element");
     __shared__ T smem[EPB * SZ];
     const Tab1D<T> &tb = *a.tab;
     using C = Ctr<K, T, MF>;
@@ -396,7 +399,8 @@ __global__ void __launch_bounds__(64)
 
     T *A = smem + el * SZ;
     T *R = A + K * S1;
-    T *KA = R + K * S1;   // pointwise kappa at the volume points (KLDS only)
+    T *KA = R + K * S1;   // pointwise kappa at the volume points (KLDS only;
+                          // at K = 4 the face-trace scratch Sx is not used)
     auto L = [&](int a_, int b_, int c_) {
         if constexpr (F64MF_) return a_ * S1 + b_ * SB + c_;
         else return lidx<K>(a_, b_, c_);
