@@ -99,23 +99,15 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Cross-lane moves inside a 16-lane DPP row (one K = 4 element): CTRL is a
+// Cross-lane moves inside a 16-lane DPP row (one K = 4 element of the FP32
+// and BF16 kernels; the FP64 kernel uses the matrix cores instead): CTRL is a
 // quad_perm pattern (0x00..0xFF) or row_ror:N (0x120 + N).  Every lane of the
 // wavefront must be active (no lane reads an inactive or out-of-row source, so
-// the destination's previous value is never kept).  Doubles move as two
-// 32-bit halves.
+// the destination's previous value is never kept).
 template <int CTRL>
 __device__ __forceinline__ float dpp_mov(float x)
 {
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xf, 0xf, false));
-}
-template <int CTRL>
-__device__ __forceinline__ double dpp_mov(double x)
-{
-    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
-    const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)b, CTRL, 0xf, 0xf, false);
-    const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, false);
-    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 // row_ror:4k moves lane (4 i' + j) of a row to lane (4 ((i' + k) mod 4) + j):
 // after it, lane (i, j) holds the value of lane (4 ((i - k) mod 4) + j)
