@@ -311,7 +311,8 @@ struct Ctr {
 // value per element (FEM_KAPPA_CELL) or one constant (FEM_KAPPA_CONST)
 template <int K, typename T, bool MF, bool KQ>
 __global__ void __launch_bounds__(64)
-    k_dg_apply(const DGArgs<T> a, const T *__restrict__ u, T *__restrict__ w, const int *__restrict__ skip)
+    k_dg_apply(const DGArgs<T> a, const T *__restrict__ u, T *__restrict__ w, const int *__restrict__ skip,
+               double *__restrict__ dot)
 {
     using S = DGShape<K>;
     constexpr int K2 = S::K2, K3 = S::K3, EPB = S::EPB;
@@ -707,42 +708,61 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
         for (int c = 0; c < K; ++c) we[i * K2 + j * K + c] = tmp[c];
     }
+    if (dot) {
+        // this workgroup's <u, w> (fixed summation order)
+        double acc = 0;
+        if (active)
+#pragma unroll
+            for (int c = 0; c < K; ++c) acc += (double)ue[i * K2 + j * K + c] * (double)tmp[c];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+        if (tid == 0) dot[blockIdx.x] = acc;
+    }
 }
 
 template <int K, typename T, bool MF = false>
-static void launch_apply(const DGArgs<T> &a, const T *u, T *w, const int *skip, hipStream_t s)
+static void launch_apply(const DGArgs<T> &a, const T *u, T *w, const int *skip, hipStream_t s, double *dot = nullptr)
 {
     using S = DGShape<K>;
     const int64_t E = (int64_t)a.n[0] * a.n[1] * a.n[2];
     FEM_CHECK(E * S::K3 < (int64_t)1 << 31, FEM_ERR_UNSUPPORTED, "DG apply: more than 2^31 DoFs");
     const unsigned nb = (unsigned)((E + S::EPB - 1) / S::EPB);
     if (a.kappa_mode == FEM_KAPPA_QP)
-        hipLaunchKernelGGL((k_dg_apply<K, T, MF, true>), dim3(nb), dim3(64), 0, s, a, u, w, skip);
+        hipLaunchKernelGGL((k_dg_apply<K, T, MF, true>), dim3(nb), dim3(64), 0, s, a, u, w, skip, dot);
     else
-        hipLaunchKernelGGL((k_dg_apply<K, T, MF, false>), dim3(nb), dim3(64), 0, s, a, u, w, skip);
+        hipLaunchKernelGGL((k_dg_apply<K, T, MF, false>), dim3(nb), dim3(64), 0, s, a, u, w, skip, dot);
     FEM_HIP(hipGetLastError());
 }
 
 template <typename T>
-static void dispatch_apply(const DGArgs<T> &a, int k, const T *u, T *w, const int *skip, hipStream_t s)
+static void dispatch_apply(const DGArgs<T> &a, int k, const T *u, T *w, const int *skip, hipStream_t s, double *dot)
 {
     switch (k) {
-    case 2: launch_apply<2, T>(a, u, w, skip, s); break;
-    case 3: launch_apply<3, T>(a, u, w, skip, s); break;
-    case 4: launch_apply<4, T>(a, u, w, skip, s); break;
-    case 5: launch_apply<5, T>(a, u, w, skip, s); break;
+    case 2: launch_apply<2, T>(a, u, w, skip, s, dot); break;
+    case 3: launch_apply<3, T>(a, u, w, skip, s, dot); break;
+    case 4: launch_apply<4, T>(a, u, w, skip, s, dot); break;
+    case 5: launch_apply<5, T>(a, u, w, skip, s, dot); break;
     default: throw Error(FEM_ERR_UNSUPPORTED, "DG apply: order must be 1..4");
     }
 }
 
-void dg_apply_f64(const DGArgs<double> &a, int k, const double *u, double *w, const int *skip, hipStream_t s)
+void dg_apply_f64(const DGArgs<double> &a, int k, const double *u, double *w, const int *skip, hipStream_t s,
+                  double *dot)
 {
-    dispatch_apply<double>(a, k, u, w, skip, s);
+    dispatch_apply<double>(a, k, u, w, skip, s, dot);
 }
 
-void dg_apply_f32(const DGArgs<float> &a, int k, const float *u, float *w, const int *skip, hipStream_t s)
+void dg_apply_f32(const DGArgs<float> &a, int k, const float *u, float *w, const int *skip, hipStream_t s,
+                  double *dot)
 {
-    dispatch_apply<float>(a, k, u, w, skip, s);
+    dispatch_apply<float>(a, k, u, w, skip, s, dot);
+}
+
+int64_t dg_blocks(int k, int64_t E)
+{
+    const int epb = 64 / (k * k);
+    FEM_CHECK(k >= 2 && k <= 5, FEM_ERR_UNSUPPORTED, "DG: order must be 1..4");
+    return (E + epb - 1) / epb;
 }
 
 void dg_apply_bf16(const DGArgs<float> &a, int k, const float *u, float *w, const int *skip, hipStream_t s)
@@ -999,7 +1019,7 @@ void dg_block_setup(const DGArgs<double> &a, int k, double *Q, double *lam, hipS
 template <int K, typename T>
 __global__ void __launch_bounds__(64)
     k_dg_block_apply(int64_t E, const T *__restrict__ Q, const T *__restrict__ lam, const T *__restrict__ x,
-                     T *__restrict__ y, const int *__restrict__ skip)
+                     T *__restrict__ y, const int *__restrict__ skip, double *__restrict__ dot)
 {
     using S = DGShape<K>;
     constexpr int K2 = S::K2, K3 = S::K3, EPB = S::EPB;
@@ -1019,16 +1039,37 @@ __global__ void __launch_bounds__(64)
     if (active)
 #pragma unroll
         for (int r = 0; r < K; ++r) y[e * K3 + i * K2 + j * K + r] = v[r];
+    if (dot) {
+        // this workgroup's <x, y> and <x, x> (fixed summation order)
+        double a1 = 0, a2 = 0;
+        if (active)
+#pragma unroll
+            for (int r = 0; r < K; ++r) {
+                const double xv = (double)x[e * K3 + i * K2 + j * K + r];
+                a1 += xv * (double)v[r];
+                a2 += xv * xv;
+            }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            a1 += __shfl_xor(a1, off, 64);
+            a2 += __shfl_xor(a2, off, 64);
+        }
+        if (tid == 0) {
+            dot[blockIdx.x] = a1;
+            dot[gridDim.x + blockIdx.x] = a2;
+        }
+    }
 }
 
 template <typename T>
-void dg_block_apply(int k, int64_t E, const T *Q, const T *lam, const T *x, T *y, const int *skip, hipStream_t s)
+void dg_block_apply(int k, int64_t E, const T *Q, const T *lam, const T *x, T *y, const int *skip, hipStream_t s,
+                    double *dot)
 {
-#define BA_CASE(KK)                                                                                    \
-    case KK: {                                                                                         \
-        unsigned nb = (unsigned)((E + DGShape<KK>::EPB - 1) / DGShape<KK>::EPB);                       \
-        hipLaunchKernelGGL((k_dg_block_apply<KK, T>), dim3(nb), dim3(64), 0, s, E, Q, lam, x, y, skip); \
-        break;                                                                                         \
+#define BA_CASE(KK)                                                                                         \
+    case KK: {                                                                                              \
+        unsigned nb = (unsigned)((E + DGShape<KK>::EPB - 1) / DGShape<KK>::EPB);                            \
+        hipLaunchKernelGGL((k_dg_block_apply<KK, T>), dim3(nb), dim3(64), 0, s, E, Q, lam, x, y, skip, dot); \
+        break;                                                                                              \
     }
     switch (k) {
         BA_CASE(2) BA_CASE(3) BA_CASE(4) BA_CASE(5)
@@ -1038,7 +1079,9 @@ void dg_block_apply(int k, int64_t E, const T *Q, const T *lam, const T *x, T *y
     FEM_HIP(hipGetLastError());
 }
 
-template void dg_block_apply<double>(int, int64_t, const double *, const double *, const double *, double *, const int *, hipStream_t);
-template void dg_block_apply<float>(int, int64_t, const float *, const float *, const float *, float *, const int *, hipStream_t);
+template void dg_block_apply<double>(int, int64_t, const double *, const double *, const double *, double *, const int *,
+                                     hipStream_t, double *);
+template void dg_block_apply<float>(int, int64_t, const float *, const float *, const float *, float *, const int *,
+                                    hipStream_t, double *);
 
 }  // namespace femamd

@@ -22,11 +22,17 @@ struct DGArgs {
 };
 
 // w = A u.  `skip` (device flag, may be null): when *skip != 0 the launch
-// does nothing (device-driven solver loops).
-void dg_apply_f64(const DGArgs<double> &a, int k, const double *u, double *w, const int *skip, hipStream_t s);
-void dg_apply_f32(const DGArgs<float> &a, int k, const float *u, float *w, const int *skip, hipStream_t s);
+// does nothing (device-driven solver loops).  `dot` (may be null): each
+// workgroup also writes its FP64 partial of <u, w> to dot[workgroup]
+// (dg_blocks(k, E) entries), for the solver's <q, A q>.
+void dg_apply_f64(const DGArgs<double> &a, int k, const double *u, double *w, const int *skip, hipStream_t s,
+                  double *dot = nullptr);
+void dg_apply_f32(const DGArgs<float> &a, int k, const float *u, float *w, const int *skip, hipStream_t s,
+                  double *dot = nullptr);
 // BF16 MFMA contraction variant (FP32 data and accumulation)
 void dg_apply_bf16(const DGArgs<float> &a, int k, const float *u, float *w, const int *skip, hipStream_t s);
+// workgroups of the DG apply and block-Jacobi launches for order k-1
+int64_t dg_blocks(int k, int64_t E);
 // operator diagonal
 void dg_diagonal(const DGArgs<double> &a, int k, double *d, hipStream_t s);
 // load vector from f at volume quadrature points
@@ -38,8 +44,11 @@ void dg_rhs(const DGArgs<double> &a, int k, const double *fq, double *F, hipStre
 // and eigenvalues lam_d with A_KK = sum_d (I..A_d..I), A_d = Q_d diag(lam_d) Q_d^T.
 void dg_block_setup(const DGArgs<double> &a, int k, double *Q /*[E][3][K][K]*/, double *lam /*[E][3][K]*/,
                     hipStream_t s);
+// `dot` (may be null): each workgroup writes its FP64 partials of <x, y> to
+// dot[workgroup] and of <x, x> to dot[dg_blocks(k, E) + workgroup]
 template <typename T>
-void dg_block_apply(int k, int64_t E, const T *Q, const T *lam, const T *x, T *y, const int *skip, hipStream_t s);
+void dg_block_apply(int k, int64_t E, const T *Q, const T *lam, const T *x, T *y, const int *skip, hipStream_t s,
+                    double *dot = nullptr);
 
 
 }  // namespace femamd

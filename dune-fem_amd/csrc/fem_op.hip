@@ -156,15 +156,22 @@ void Operator::set_kappa_cell(const double *kc)
     a64.kappa_mode = a32.kappa_mode = FEM_KAPPA_CELL;
 }
 
-void Operator::apply_raw(const void *x, void *y, const int *skip) const
+int64_t Operator::dot_blocks() const
+{
+    return is_dg() ? dg_blocks(k, E) : 0;
+}
+
+void Operator::apply_raw(const void *x, void *y, const int *skip, double *dot) const
 {
     hipStream_t s = ctx->stream;
     if (is_dg()) {
         FEM_CHECK(a64.kappa_mode >= 0, FEM_ERR_STATE, "coefficient data not set");
-        if (desc.precision == FEM_F64) dg_apply_f64(a64, k, (const double *)x, (double *)y, skip, s);
-        else if (desc.precision == FEM_F32) dg_apply_f32(a32, k, (const float *)x, (float *)y, skip, s);
+        FEM_CHECK(!dot || desc.precision != FEM_BF16MFMA, FEM_ERR_UNSUPPORTED, "fused dot: FP64 or FP32 operator");
+        if (desc.precision == FEM_F64) dg_apply_f64(a64, k, (const double *)x, (double *)y, skip, s, dot);
+        else if (desc.precision == FEM_F32) dg_apply_f32(a32, k, (const float *)x, (float *)y, skip, s, dot);
         else dg_apply_bf16(a32, k, (const float *)x, (float *)y, skip, s);
     } else {
+        FEM_CHECK(!dot, FEM_ERR_UNSUPPORTED, "fused dot: DG operators only");
         FEM_CHECK(desc.kappa_mode == FEM_KAPPA_CONST || kvol64.p, FEM_ERR_STATE, "coefficient data not set");
         if (vec_prec == FEM_F64)
             cg_launch<double>(CG_MODE_APPLY, inst_dev.as<CGInst>(), 1, tiles, cgtab.as<CGTab>(),
@@ -280,10 +287,11 @@ void Precond::init(Operator *o, int k_)
 
 Precond::~Precond() { delete learned; }
 
-void Precond::apply_raw(const void *x, void *y, const int *skip) const
+void Precond::apply_raw(const void *x, void *y, const int *skip, double *dot) const
 {
     hipStream_t s = op->ctx->stream;
     const int64_t N = op->N;
+    FEM_CHECK(!dot || fused_dot(), FEM_ERR_UNSUPPORTED, "fused dot: block Jacobi only");
     if (kind == FEM_PRE_LEARNED) {
         learned->apply_raw(x, y, skip);
     } else if (kind == FEM_PRE_JACOBI) {
@@ -294,9 +302,10 @@ void Precond::apply_raw(const void *x, void *y, const int *skip) const
         FEM_HIP(hipGetLastError());
     } else if (kind == FEM_PRE_BLOCK_JACOBI) {
         if (prec == FEM_F64)
-            dg_block_apply<double>(op->k, op->E, Q.as<double>(), lam.as<double>(), (const double *)x, (double *)y, skip, s);
+            dg_block_apply<double>(op->k, op->E, Q.as<double>(), lam.as<double>(), (const double *)x, (double *)y, skip, s,
+                                   dot);
         else
-            dg_block_apply<float>(op->k, op->E, Q.as<float>(), lam.as<float>(), (const float *)x, (float *)y, skip, s);
+            dg_block_apply<float>(op->k, op->E, Q.as<float>(), lam.as<float>(), (const float *)x, (float *)y, skip, s, dot);
     } else {
         FEM_HIP(hipMemcpyAsync(y, x, N * (prec == FEM_F64 ? 8 : 4), hipMemcpyDeviceToDevice, s));
     }

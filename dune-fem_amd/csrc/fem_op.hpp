@@ -50,8 +50,11 @@ struct Operator {
     void init(Context *c, const fem_op_desc &d);
     void set_kappa_qp(const double *kvol, const double *kx, const double *ky, const double *kz);
     void set_kappa_cell(const double *kc);
-    // y = A x on raw device pointers of the operator's vector precision
-    void apply_raw(const void *x, void *y, const int *skip) const;
+    // y = A x on raw device pointers of the operator's vector precision;
+    // `dot` (DG only, may be null): per-workgroup FP64 partials of <x, y>,
+    // dot_blocks() entries
+    void apply_raw(const void *x, void *y, const int *skip, double *dot = nullptr) const;
+    int64_t dot_blocks() const;
     void apply(const Vec &x, Vec &y, const int *skip = nullptr) const;
     void rhs(const double *fq_host, Vec &F) const;
     void diagonal(Vec &d) const;  // FP64 or FP32 output
@@ -68,8 +71,11 @@ struct Precond {
     DevBuf Q, lam;     // block Jacobi (tensor-product eigen-decomposition)
     Learned *learned = nullptr;   // FEM_PRE_LEARNED (owned)
     void init(Operator *o, int kind);
-    void apply_raw(const void *x, void *y, const int *skip) const;
+    // `dot` (block Jacobi only, may be null): per-workgroup FP64 partials of
+    // <x, y> and then of <x, x>, op->dot_blocks() entries each
+    void apply_raw(const void *x, void *y, const int *skip, double *dot = nullptr) const;
     bool pointwise() const { return kind == FEM_PRE_JACOBI; }
+    bool fused_dot() const { return kind == FEM_PRE_BLOCK_JACOBI; }
     ~Precond();
 };
 

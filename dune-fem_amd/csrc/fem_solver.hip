@@ -169,6 +169,47 @@ __global__ void __launch_bounds__(RED_THREADS)
     }
 }
 
+// Finish of a dot whose per-workgroup partials were written by the DG apply
+// (MODE 0: <q, h> -> alpha) or by block Jacobi (MODE 3: <p, s> and <p, p> ->
+// beta, residual, convergence).  The partials are summed in a fixed order, so
+// the result is reproducible run to run.
+template <int MODE>
+__global__ void __launch_bounds__(RED_THREADS)
+    k_cg_fin_parts(const double *__restrict__ ext, int64_t nb, KState *st, double *partials, unsigned *counter)
+{
+    if (st->done) return;
+    __shared__ double lds[RED_THREADS / 64];
+    double acc = 0, acc2 = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nb; i += (int64_t)gridDim.x * blockDim.x) {
+        acc += ext[i];
+        if (MODE == 3) acc2 += ext[nb + i];
+    }
+    double b = block_sum(acc, lds);
+    __syncthreads();
+    double b2 = block_sum(acc2, lds);
+    __shared__ bool amLast;
+    if (threadIdx.x == 0) {
+        publish_partial(&partials[blockIdx.x], b);
+        publish_partial(&partials[gridDim.x + blockIdx.x], b2);
+        amLast = atomicAdd(counter, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!amLast) return;
+    double v = 0, v2 = 0;
+    for (unsigned k = threadIdx.x; k < gridDim.x; k += blockDim.x) {
+        v += read_partial(&partials[k]);
+        v2 += read_partial(&partials[gridDim.x + k]);
+    }
+    double tot = block_sum(v, lds);
+    __syncthreads();
+    double tot2 = block_sum(v2, lds);
+    if (threadIdx.x == 0) {
+        if (MODE == 0) fin_cg_alpha(st, tot);
+        else fin_cg_res(st, tot, tot2);
+        reset_counter(counter);
+    }
+}
+
 // ---- GMRES kernels --------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(RED_THREADS)
@@ -477,10 +518,25 @@ static int run_krylov(Solver &S, int method, Operator *o, Precond *p, const T *b
         hipLaunchKernelGGL(k_cg_init, dim3(1), dim3(1), 0, s, st, eps, criteria, maxit, S.prm.residual_check);
         FEM_HIP(hipGetLastError());
         const T *dinv = (haspre && p->pointwise()) ? (p->prec == FEM_F64 ? (const T *)p->dinv.p : (const T *)p->dinv.p) : nullptr;
+        // DG operator with block Jacobi: the apply and the preconditioner
+        // write per-workgroup partials of the two CG dots, which a small
+        // kernel finishes, instead of two more passes over the vectors
+        const bool fuse = o->is_dg() && haspre && p->fused_dot() && o->desc.precision != FEM_BF16MFMA;
+        const int64_t nbd = fuse ? o->dot_blocks() : 0;
+        double *dq = fuse ? (double *)S.workbuf(5, sizeof(double) * nbd) : nullptr;
+        double *dp = fuse ? (double *)S.workbuf(6, 2 * sizeof(double) * nbd) : nullptr;
         int chunk = S.snapshot ? S.snap_chunk : 4;
         while (true) {
             for (int c = 0; c < chunk; ++c) {
                 hipLaunchKernelGGL(k_cg_qupd<T>, dim3(G), dim3(256), 0, s, q, ss, pp, r, N, st, haspre ? 1 : 0);
+                if (fuse) {
+                    o->apply_raw(q, h, &st->done, dq);
+                    hipLaunchKernelGGL(k_cg_fin_parts<0>, dim3(RED_DOT_BLOCKS), dim3(RED_THREADS), 0, s, dq, nbd, st, part, cnt);
+                    hipLaunchKernelGGL((k_cg_step<T, 2>), dim3(RED_DOT_BLOCKS), dim3(RED_THREADS), 0, s, x, pp, q, h, ss, r, dinv, N, st, part, cnt);
+                    p->apply_raw(pp, ss, &st->done, dp);
+                    hipLaunchKernelGGL(k_cg_fin_parts<3>, dim3(RED_DOT_BLOCKS), dim3(RED_THREADS), 0, s, dp, nbd, st, part, cnt);
+                    continue;
+                }
                 o->apply_raw(q, h, &st->done);
                 hipLaunchKernelGGL((k_cg_step<T, 0>), dim3(RED_DOT_BLOCKS), dim3(RED_THREADS), 0, s, x, pp, q, h, ss, r, dinv, N, st, part, cnt);
                 if (!haspre) {
