@@ -556,8 +556,7 @@ __global__ void __launch_bounds__(64)
     auto face_terms = [&](int d, const T *up, T *rp) {
         T nvs[2], nds[2];
         neighbour_traces(d, nvs, nds);
-        const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
-        const T Wf = wij * a.h[t1] * a.h[t2];
+        const T Wf = wij * a.farea[d];
         const T ihd = a.ih[d];
         // extrapolation to the faces from the symmetric Gauss points:
         // E[1][q] = E[0][K-1-q], dE[1][q] = -dE[0][K-1-q]
@@ -574,8 +573,8 @@ __global__ void __launch_bounds__(64)
                 din = fma(dEs(s, m), up[m], din);
             }
             din *= ihd * nrm;
-            const T bn = a.b[d] * nrm;
-            const T bp = bn > T(0) ? bn : T(0), bm = bn < T(0) ? -bn : T(0);
+            // upwind split of b.n for the outward normal nrm e_d
+            const T bp = s ? a.bpos[d] : a.bneg[d], bm = s ? a.bneg[d] : a.bpos[d];
             T kin, kout, kp;
             if constexpr (!KQ) {
                 // cellwise kappa: average of the two elements' values
@@ -586,15 +585,22 @@ __global__ void __launch_bounds__(64)
             } else {
                 kin = kout = kp = kf[d][s];
             }
-            const T pen = a.beta * ihd;
+            const T pen = a.pen[d];
+            // one formula for interior and boundary faces: on a boundary face
+            // the neighbour trace is 0 (nval, nder are zeroed), kout = kp =
+            // kin, and the consistency and symmetry terms carry weight 1
+            // instead of 1/2 (Nitsche terms for u = 0)
+            // (the FP64 Q3 kernel takes this form for both; the other orders
+            // branch, which keeps their register use down)
             T c0, c1;
-            if (interior) {
+            if (F64MF || interior) {
+                const T wgt = interior ? T(0.5) : T(1);
                 const T nv = nvs[s], nd = nds[s];
                 const T dout = nd * (ihd * nrm);
                 const T jump = uin - nv;
-                const T sigma = T(0.5) * fma(kin, din, kout * dout);
+                const T sigma = wgt * fma(kin, din, kout * dout);
                 c0 = Wf * (fma(kp * pen, jump, -sigma) + fma(bp, uin, -bm * nv));
-                c1 = Wf * (T(-0.5) * kin * jump);
+                c1 = Wf * (-wgt * kin * jump);
             } else {
                 c0 = Wf * fma(fma(kin, pen, bp), uin, -kin * din);
                 c1 = Wf * (-kin * uin);
@@ -619,7 +625,7 @@ __global__ void __launch_bounds__(64)
     // x direction: residual pencil kept in registers (X layout)
     T Rx[K];
     {
-        const T sc = wij * detJ * (a.ih[0] * a.ih[0]), bh = a.b[0] * a.h[0];
+        const T sc = wij * a.vsc[0], bh = a.bh[0];
         T g[K], r[K];
         C::D(tb, mr, uq, g);
 #pragma unroll
@@ -637,7 +643,7 @@ __global__ void __launch_bounds__(64)
     wave_lds_sync();
     // Y layout of u_q: thread (i=qx, j=qz) -> R = y part
     {
-        const T sc = wij * detJ * (a.ih[1] * a.ih[1]), bh = a.b[1] * a.h[1];
+        const T sc = wij * a.vsc[1], bh = a.bh[1];
         T g[K], r[K];
 #pragma unroll
         for (int q = 0; q < K; ++q) v[q] = A[L(i, q, j)];
@@ -658,7 +664,7 @@ __global__ void __launch_bounds__(64)
     wave_lds_sync();
     // Z layout of u_q: thread (i=qx, j=qy) -> R += z part
     {
-        const T sc = wij * detJ * (a.ih[2] * a.ih[2]), bh = a.b[2] * a.h[2];
+        const T sc = wij * a.vsc[2], bh = a.bh[2];
         T g[K], r[K];
 #pragma unroll
         for (int q = 0; q < K; ++q) v[q] = A[L(i, j, q)];

@@ -19,7 +19,25 @@ struct DGArgs {
     const T *kface[3];
     const T *kcell;
     const Tab1D<T> *tab;   // device copy of the 1D tables
+    // per direction d, set on the host: beta / h_d, the face area of the
+    // faces normal to d, |K| / h_d^2, b_d h_d, max(b_d, 0), max(-b_d, 0)
+    T pen[3], farea[3], vsc[3], bh[3], bpos[3], bneg[3];
 };
+
+// fills the per-direction constants of DGArgs from n, h, ih, beta, b
+template <typename T>
+inline void dg_derived(DGArgs<T> &a)
+{
+    for (int d = 0; d < 3; ++d) {
+        const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+        a.pen[d] = a.beta * a.ih[d];
+        a.farea[d] = a.h[t1] * a.h[t2];
+        a.vsc[d] = a.h[0] * a.h[1] * a.h[2] * (a.ih[d] * a.ih[d]);
+        a.bh[d] = a.b[d] * a.h[d];
+        a.bpos[d] = a.b[d] > T(0) ? a.b[d] : T(0);
+        a.bneg[d] = a.b[d] < T(0) ? -a.b[d] : T(0);
+    }
+}
 
 // w = A u.  `skip` (device flag, may be null): when *skip != 0 the launch
 // does nothing (device-driven solver loops).  `dot` (may be null): each
