@@ -364,20 +364,18 @@ __global__ void __launch_bounds__(64)
     // every neighbour index formed from them stays inside the grid
     int e = e_raw < E ? e_raw : 0;
     int ex, ey, ez;
-    // slab height for the traversal below: the largest of 16, 8, 4, 2 that
-    // divides ny (wave-uniform)
-    const int ty = (ny % 16 == 0) ? 16 : (ny % 8 == 0) ? 8 : (ny % 4 == 0) ? 4 : (ny % 2 == 0) ? 2 : 1;
-    if (nx % EPB == 0 && blk * EPB < E) {
+    if (a.slabs && blk * EPB < E) {
         // the workgroup's elements share one x-row.  Rows are visited in
-        // slabs of ty y-rows, each slab through all z, so the rows below
-        // and above (z -+ 1) are read a few hundred workgroups apart on the
-        // same XCD and hit its L2: the launch then fetches about 1.1x the
-        // unique input instead of 1.8x.  The divisions act on the
-        // wave-uniform block index only.
-        const int rpb = nx / EPB, per_slab = rpb * ty * nz;
-        const int t = blk / per_slab, rem = blk - t * per_slab;
-        const int z = rem / (rpb * ty), rem2 = rem - z * (rpb * ty);
-        const int yl = rem2 / rpb, xu = rem2 - yl * rpb;
+        // slabs of ty (16, 8, 4, 2 or 1, dividing ny) y-rows, each slab
+        // through all z, so the rows below and above (z -+ 1) are read a few
+        // hundred workgroups apart on the same XCD and hit its L2: the launch
+        // then fetches about 1.1x the unique input instead of 1.8x.  The
+        // divisions act on the wave-uniform block index only, by host-made
+        // fixed-point reciprocals.
+        const int rpb = a.rpb, ty = a.ty;
+        const int t = (int)udiv((unsigned)blk, a.dv_slab), rem = blk - t * (rpb * ty * nz);
+        const int z = (int)udiv((unsigned)rem, a.dv_zrow), rem2 = rem - z * (rpb * ty);
+        const int yl = (int)udiv((unsigned)rem2, a.dv_rpb), xu = rem2 - yl * rpb;
         ex = xu * EPB + el;
         ey = t * ty + yl;
         ez = z;

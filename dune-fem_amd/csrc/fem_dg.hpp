@@ -5,6 +5,27 @@
 
 namespace femamd {
 
+// division by a launch constant d >= 1 for 0 <= x < 2^31: with
+// l = ceil(log2 d), m = ceil(2^(31+l) / d), x / d = (x m) >> (31 + l)
+// (Granlund and Montgomery, 1994); wave-uniform operands stay scalar
+struct UDiv {
+    unsigned long long m = 1ull << 31;
+    int sh = 31;
+};
+inline UDiv make_udiv(unsigned d)
+{
+    UDiv v;
+    int l = 0;
+    while ((1ull << l) < d) ++l;
+    v.sh = 31 + l;
+    v.m = ((1ull << v.sh) + d - 1) / d;
+    return v;
+}
+__host__ __device__ inline unsigned udiv(unsigned x, UDiv v)
+{
+    return (unsigned)(((unsigned long long)x * v.m) >> v.sh);
+}
+
 template <typename T>
 struct DGArgs {
     int n[3];
@@ -22,12 +43,28 @@ struct DGArgs {
     // per direction d, set on the host: beta / h_d, the face area of the
     // faces normal to d, |K| / h_d^2, b_d h_d, max(b_d, 0), max(-b_d, 0)
     T pen[3], farea[3], vsc[3], bh[3], bpos[3], bneg[3];
+    // element traversal of the apply (order k-1, epb = 64 / k^2 elements per
+    // workgroup): rows of epb elements in slabs of ty y-rows through all z;
+    // `slabs` is 0 when nx is not a multiple of epb (element order instead)
+    int slabs, rpb, ty;
+    UDiv dv_slab, dv_zrow, dv_rpb;   // by rpb ty nz, rpb ty, rpb
 };
 
-// fills the per-direction constants of DGArgs from n, h, ih, beta, b
+// fills the per-direction constants and the traversal of DGArgs from n, h,
+// ih, beta, b and the number of 1D points k
 template <typename T>
-inline void dg_derived(DGArgs<T> &a)
+inline void dg_derived(DGArgs<T> &a, int k)
 {
+    const int epb = 64 / (k * k);
+    a.slabs = a.n[0] % epb == 0;
+    a.rpb = a.n[0] / epb;
+    const int ny = a.n[1];
+    a.ty = (ny % 16 == 0) ? 16 : (ny % 8 == 0) ? 8 : (ny % 4 == 0) ? 4 : (ny % 2 == 0) ? 2 : 1;
+    if (a.slabs) {
+        a.dv_slab = make_udiv((unsigned)(a.rpb * a.ty * a.n[2]));
+        a.dv_zrow = make_udiv((unsigned)(a.rpb * a.ty));
+        a.dv_rpb = make_udiv((unsigned)a.rpb);
+    }
     for (int d = 0; d < 3; ++d) {
         const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
         a.pen[d] = a.beta * a.ih[d];

@@ -79,8 +79,8 @@ void Operator::init(Context *c, const fem_op_desc &d)
         a32.kappa_const = (float)d.kappa_const;
         a64.tab = tab64.as<Tab1D<double>>();
         a32.tab = tab32.as<Tab1D<float>>();
-        dg_derived(a64);
-        dg_derived(a32);
+        dg_derived(a64, k);
+        dg_derived(a32, k);
     } else if (d.kind == FEM_OP_CG_Q2_2D) {
         FEM_CHECK(d.order == 2, FEM_ERR_UNSUPPORTED, "CG operator supports order 2");
         FEM_CHECK(d.n[0] > 0 && d.n[1] > 0, FEM_ERR_ARG, "mesh size");
