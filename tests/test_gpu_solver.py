@@ -113,6 +113,38 @@ def test_block_jacobi_is_exact_element_inverse(ctx):
     assert err < 1e-11
 
 
+@pytest.mark.parametrize("n,p", [((12, 12, 12), 2), ((8, 6, 5), 3)])
+def test_cg_block_jacobi_matches_oracle(ctx, n, p):
+    # CG + block Jacobi on DG operators: the apply and the preconditioner
+    # hand the solver per-workgroup partials of its dots (Q2: the high-
+    # contrast channel problem at 12^3; Q3: random cellwise kappa)
+    if p == 2:
+        P = wl.problem3(n=n[0])
+        kcell, fq, beta = P["kcell"], P["fq"], P["beta"]
+    else:
+        rng = np.random.default_rng(31)
+        E = n[0] * n[1] * n[2]
+        kcell = rng.uniform(0.1, 10.0, size=E)
+        fq = rng.normal(size=(E, (p + 1) ** 3))
+        beta = 10.0 * p * p
+    gop = fa.GalerkinOperator(ctx, fa.OP_DG_SIPG_3D, n, p, beta, (0, 0, 0), 0.0, fa.KAPPA_CELL)
+    gop.set_kappa_cell(kcell)
+    pre = fa.Preconditioner(gop, fa.PRE_BLOCK_JACOBI)
+    F = gop.rhs(fq)
+    X = gop.vector()
+    inv = fa.InverseOperator(ctx, method=fa.CG, tolerance=1e-8, errormeasure=fa.RELATIVE, maxiterations=5000)
+    inv.bind(gop, pre)
+    inv(F, X)
+    info = inv.info()
+    oop = ora.DGOperator(n, p, beta, kappa=("cell", kcell))
+    inv_blocks = np.linalg.inv(oop.block_diag())
+    xo, ito = ora.cg(oop, F.download(), pre=ora.block_jacobi(inv_blocks), tol=1e-8, maxit=5000, criteria=1)
+    print("CG+BJ", n, p, "gpu its", info["iterations"], "oracle its", ito, "res", info["true_rel_residual"], flush=True)
+    assert info["converged"] and ito > 0
+    assert abs(info["iterations"] - ito) <= 2
+    assert np.abs(X.download() - xo).max() < 1e-6 * np.abs(xo).max()
+
+
 def test_fgmres_block_jacobi_problem3_small(ctx):
     # the high-contrast channel problem at 12^3: FGMRES(30) + block Jacobi to 1e-8
     P = wl.problem3(n=12)
