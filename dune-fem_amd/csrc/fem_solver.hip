@@ -523,6 +523,8 @@ static int run_krylov(Solver &S, int method, Operator *o, Precond *p, const T *b
         // kernel finishes, instead of two more passes over the vectors
         const bool fuse = o->is_dg() && haspre && p->fused_dot() && o->desc.precision != FEM_BF16MFMA;
         const int64_t nbd = fuse ? o->dot_blocks() : 0;
+        // about 4 partials per thread: a small launch finishes quickly
+        const unsigned GF = (unsigned)std::min<int64_t>(RED_DOT_BLOCKS, std::max<int64_t>(1, (nbd + 4 * RED_THREADS - 1) / (4 * RED_THREADS)));
         double *dq = fuse ? (double *)S.workbuf(5, sizeof(double) * nbd) : nullptr;
         double *dp = fuse ? (double *)S.workbuf(6, 2 * sizeof(double) * nbd) : nullptr;
         int chunk = S.snapshot ? S.snap_chunk : 4;
@@ -531,10 +533,10 @@ static int run_krylov(Solver &S, int method, Operator *o, Precond *p, const T *b
                 hipLaunchKernelGGL(k_cg_qupd<T>, dim3(G), dim3(256), 0, s, q, ss, pp, r, N, st, haspre ? 1 : 0);
                 if (fuse) {
                     o->apply_raw(q, h, &st->done, dq);
-                    hipLaunchKernelGGL(k_cg_fin_parts<0>, dim3(RED_DOT_BLOCKS), dim3(RED_THREADS), 0, s, dq, nbd, st, part, cnt);
+                    hipLaunchKernelGGL(k_cg_fin_parts<0>, dim3(GF), dim3(RED_THREADS), 0, s, dq, nbd, st, part, cnt);
                     hipLaunchKernelGGL((k_cg_step<T, 2>), dim3(RED_DOT_BLOCKS), dim3(RED_THREADS), 0, s, x, pp, q, h, ss, r, dinv, N, st, part, cnt);
                     p->apply_raw(pp, ss, &st->done, dp);
-                    hipLaunchKernelGGL(k_cg_fin_parts<3>, dim3(RED_DOT_BLOCKS), dim3(RED_THREADS), 0, s, dp, nbd, st, part, cnt);
+                    hipLaunchKernelGGL(k_cg_fin_parts<3>, dim3(GF), dim3(RED_THREADS), 0, s, dp, nbd, st, part, cnt);
                     continue;
                 }
                 o->apply_raw(q, h, &st->done);
