@@ -19,7 +19,10 @@
 // galerkin.hh:879,951-991).  Face traces of the own element come from the
 // quadrature-point values by 1D extrapolation; neighbour traces are a
 // normal-direction contraction of the neighbour's modal DoFs followed by two
-// tangential contractions.
+// tangential contractions (on the FP64 matrix cores at K = 4 in FP64, with
+// DPP cross-lane moves at K = 4 in FP32, through LDS otherwise).  The two
+// faces normal to a direction are integrated in the block that computes that
+// direction's volume flux, where both pencils along it are in registers.
 #include <hip/hip_bf16.h>
 
 #include <type_traits>
