@@ -238,12 +238,10 @@ __device__ __forceinline__ void mvD(const T (&Pp)[KMAX / 2][KMAX / 2 + 1], const
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
+// round to nearest even (v_cvt_pk_bf16_f32 on this GPU)
 __device__ __forceinline__ short to_bf16(float x)
 {
-    // round to nearest even
-    unsigned u = __float_as_uint(x);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (short)(u >> 16);
+    return __builtin_bit_cast(short, (__bf16)x);
 }
 
 struct MFRows {
