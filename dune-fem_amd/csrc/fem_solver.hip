@@ -277,7 +277,24 @@ __global__ void __launch_bounds__(RED_THREADS)
         double acc[GRP];
 #pragma unroll
         for (int l = 0; l < GRP; ++l) acc[l] = 0;
-        for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        // four elements per thread and step, so that 4 (GRP + 1) loads are in
+        // flight per thread (the grid has only two waves per SIMD)
+        int64_t i = lo + threadIdx.x;
+        for (; i + 3 * (int64_t)blockDim.x < hi; i += 4 * (int64_t)blockDim.x) {
+            double wi[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) wi[u] = (double)w[i + u * blockDim.x];
+#pragma unroll
+            for (int l = 0; l < GRP; ++l)
+                if (g0 + l < cnt) {
+                    const T *vl = V + (int64_t)(g0 + l) * n + i;
+                    double t = 0;
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) t = fma((double)vl[u * blockDim.x], wi[u], t);
+                    acc[l] += t;
+                }
+        }
+        for (; i < hi; i += blockDim.x) {
             const double wi = (double)w[i];
 #pragma unroll
             for (int l = 0; l < GRP; ++l)
@@ -298,14 +315,18 @@ __global__ void __launch_bounds__(RED_THREADS)
     }
     __syncthreads();
     if (!amLast) return;
-    for (int l = 0; l < cnt; ++l) {
+    // one wavefront per dot: lane k sums blocks k, k + 64, ... (independent
+    // reads in flight), then a butterfly over the wavefront; the dots are
+    // spread over the block's wavefronts, so they finish concurrently
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int l = wave; l < cnt; l += nw) {
         double v = 0;
-        for (unsigned k = threadIdx.x; k < gridDim.x; k += blockDim.x)
-            v += read_partial(&partials[(size_t)l * gridDim.x + k]);
-        double tot = block_sum(v, lds);
-        if (threadIdx.x == 0) st->H[l * m + j] = tot;
-        __syncthreads();
+        for (unsigned k = lane; k < gridDim.x; k += 64) v += read_partial(&partials[(size_t)l * gridDim.x + k]);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (lane == 0) st->H[l * m + j] = v;
     }
+    __syncthreads();
     if (threadIdx.x == 0) reset_counter(counter);
 }
 
@@ -331,6 +352,8 @@ __global__ void __launch_bounds__(RED_THREADS)
     const int64_t lo = n * blockIdx.x / gridDim.x, hi = n * (blockIdx.x + 1) / gridDim.x;
     for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
         double v = (double)w[i];
+        // unrolled so that several basis-vector loads are in flight
+#pragma unroll 8
         for (int l = 0; l <= j; ++l) v -= h[l] * (double)V[(int64_t)l * n + i];
         const T vt = (T)v;
         w[i] = vt;
