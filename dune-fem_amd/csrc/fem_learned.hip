@@ -23,7 +23,7 @@
 // so only R r_k enters; per pair it is normalised by the best achievable
 // reduction w = (R r_k)^T c* (loss 1 for c = 0, 0 for c = c*).
 //
-// No reference counterpart (new preconditioner); the solve is judged by its
+// No reference counterpart (new preconditioner); the solve is checked by its
 // FP64 true residual.
 #include <algorithm>
 #include <cmath>

@@ -109,7 +109,7 @@ def fgmres_run(ctx, op, pre, F):
 
 def test_learned_preconditioner_fgmres_ablation_and_fallback(ctx):
     # trained on pairs of the plain CG + block-Jacobi solves of 8 other
-    # right-hand sides; judged by FGMRES(30) iterations to a 1e-8 FP64
+    # right-hand sides; measured by FGMRES(30) iterations to a 1e-8 FP64
     # residual on the workload's own right-hand side
     P, op64, bj64, net, tl, vr = learned_setup(ctx)
     print("train energy loss", tl, "val rel coarse energy error", vr, flush=True)

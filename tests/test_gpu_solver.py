@@ -3,7 +3,7 @@
 CG follows LinearSolver::cg exactly (same recurrence and stopping rule), so the
 iteration count matches the oracle's up to rounding (+-1) and the solutions
 agree to the solver tolerance.  GMRES/FGMRES are checked against the oracle's
-restatement and a dense solve.  Every solve is also judged by its FP64 true
+restatement and a dense solve.  Every solve is also checked by its FP64 true
 relative residual.
 """
 import numpy as np
