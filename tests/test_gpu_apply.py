@@ -59,6 +59,10 @@ CASES = [
     # Q3 traversal in slabs of 4 and 1 y-rows (ny = 12, ny = 3)
     ((8, 12, 3), 3, (1, .5, .25), 0.0, "qp"),
     ((4, 3, 5), 3, (0, 0, 0), 0.5, "cell"),
+    # traversal divisors that are not powers of two (rows of 3 workgroups,
+    # slabs of 2 rows, 5 or 7 layers)
+    ((12, 6, 5), 3, (-1, .5, .25), 0.0, "qp"),
+    ((6, 10, 7), 4, (0, 0, 0), 0.0, "cell"),
 ]
 
 
