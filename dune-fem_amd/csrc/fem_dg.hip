@@ -33,18 +33,6 @@
 namespace femamd {
 
 template <int K, typename T>
-__device__ __forceinline__ void mv(const T (&M)[KMAX][KMAX], const T *in, T *out)
-{
-#pragma unroll
-    for (int q = 0; q < K; ++q) {
-        T s = 0;
-#pragma unroll
-        for (int a = 0; a < K; ++a) s = fma(M[q][a], in[a], s);
-        out[q] = s;
-    }
-}
-
-template <int K, typename T>
 __device__ __forceinline__ void mvT(const T (&M)[KMAX][KMAX], const T *in, T *out)
 {
 #pragma unroll
