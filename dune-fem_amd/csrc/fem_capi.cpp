@@ -135,6 +135,29 @@ int fem_vec_create(fem_ctx *ctx, int64_t n, int precision, fem_vec **out)
     });
 }
 
+int fem_vec_wrap(fem_ctx *ctx, int64_t n, int precision, void *device_ptr, fem_vec **out)
+{
+    return guarded([&] {
+        FEM_CHECK(ctx && out, FEM_ERR_ARG, "null argument");
+        FEM_CHECK(n >= 0, FEM_ERR_ARG, "negative size");
+        FEM_CHECK(precision == FEM_F64 || precision == FEM_F32, FEM_ERR_ARG, "vector precision must be FEM_F64 or FEM_F32");
+        FEM_CHECK(n == 0 || device_ptr, FEM_ERR_ARG, "null device pointer");
+        FEM_CHECK(((uintptr_t)device_ptr & 255) == 0, FEM_ERR_ARG, "device pointer must be 256-byte aligned");
+        hipPointerAttribute_t attr;
+        if (n > 0) {
+            FEM_CHECK(hipPointerGetAttributes(&attr, device_ptr) == hipSuccess && attr.type == hipMemoryTypeDevice,
+                      FEM_ERR_ARG, "not a device allocation");
+        }
+        fem_vec *h = new fem_vec();
+        h->v.ctx = &ctx->c;
+        h->v.n = n;
+        h->v.prec = precision;
+        h->v.data = n > 0 ? device_ptr : nullptr;
+        h->v.owner = false;
+        *out = h;
+    });
+}
+
 int fem_vec_destroy(fem_vec *v)
 {
     return guarded([&] {

@@ -35,7 +35,7 @@ ABSOLUTE, RELATIVE, REDUCTION = 0, 1, 2
 # every symbol declared in include/dunefem_amd.h
 EXPORTS = [
     "fem_last_error", "fem_version", "fem_ctx_create", "fem_ctx_destroy", "fem_ctx_synchronize",
-    "fem_ctx_mark", "fem_ctx_elapsed_ms", "fem_vec_create", "fem_vec_destroy", "fem_vec_size",
+    "fem_ctx_mark", "fem_ctx_elapsed_ms", "fem_vec_create", "fem_vec_wrap", "fem_vec_destroy", "fem_vec_size",
     "fem_vec_precision", "fem_vec_device_ptr", "fem_vec_upload", "fem_vec_download",
     "fem_vec_upload_f32", "fem_vec_fill", "fem_vec_copy", "fem_vec_axpy", "fem_vec_scal",
     "fem_vec_dot", "fem_vec_norm2sq", "fem_op_create", "fem_op_destroy", "fem_op_ndofs",
@@ -103,6 +103,7 @@ def load_library(path=LIBPATH):
         "fem_ctx_mark": ([_vp, C.c_int], C.c_int),
         "fem_ctx_elapsed_ms": ([_vp, C.POINTER(C.c_float)], C.c_int),
         "fem_vec_create": ([_vp, C.c_int64, C.c_int, C.POINTER(_vp)], C.c_int),
+        "fem_vec_wrap": ([_vp, C.c_int64, C.c_int, _vp, C.POINTER(_vp)], C.c_int),
         "fem_vec_destroy": ([_vp], C.c_int),
         "fem_vec_size": ([_vp], C.c_int64),
         "fem_vec_precision": ([_vp], C.c_int),
@@ -222,10 +223,17 @@ class Context:
 class DofVector:
     """Device DoF vector (the dofVector of a discrete function)."""
 
-    def __init__(self, ctx, n, precision=F64, data=None):
+    def __init__(self, ctx, n, precision=F64, data=None, device_ptr=None):
+        """A new device vector, or (device_ptr given) a view of n DoFs the
+        caller already holds in device memory: no copy, the memory is not
+        released with the view (AdaptiveDiscreteFunction over existing DoF
+        storage)."""
         self.ctx = ctx
         h = _vp()
-        _check(_lib.fem_vec_create(ctx.h, int(n), precision, C.byref(h)))
+        if device_ptr is None:
+            _check(_lib.fem_vec_create(ctx.h, int(n), precision, C.byref(h)))
+        else:
+            _check(_lib.fem_vec_wrap(ctx.h, int(n), precision, _vp(int(device_ptr)), C.byref(h)))
         self.h = h
         self.n = int(n)
         self.precision = precision

@@ -60,6 +60,12 @@ int fem_ctx_elapsed_ms(fem_ctx *ctx, float *ms);
  * (dune/fem/function/adaptivefunction/adaptivefunction.hh:45-61,
  *  dune/fem/function/blockvectors/defaultblockvectors.hh:444-505). */
 int fem_vec_create(fem_ctx *ctx, int64_t n, int precision, fem_vec **out);
+/* A view of n DoFs the caller already holds in device memory (`device_ptr`
+ * from hipMalloc, 256-byte aligned), as the AdaptiveDiscreteFunction
+ * constructor over existing DoF storage
+ * (dune/fem/function/adaptivefunction/adaptivefunction.hh:81-87): no copy;
+ * fem_vec_destroy releases the handle, not the memory, which must outlive it. */
+int fem_vec_wrap(fem_ctx *ctx, int64_t n, int precision, void *device_ptr, fem_vec **out);
 int fem_vec_destroy(fem_vec *v);
 int64_t fem_vec_size(const fem_vec *v);
 int fem_vec_precision(const fem_vec *v);

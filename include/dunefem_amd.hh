@@ -81,6 +81,15 @@ public:
         check(fem_vec_create(ctx.handle(), size, precision, &h_));
         clear();
     }
+    // over DoFs the caller already holds in device memory, without a copy
+    // (AdaptiveDiscreteFunction(name, space, const DofType *data),
+    // dune/fem/function/adaptivefunction/adaptivefunction.hh:81-87); the
+    // memory is not released with this object
+    DiscreteFunction(const std::string &name, Context &ctx, int64_t size, void *devicePtr, int precision = FEM_F64)
+        : name_(name), ctx_(&ctx)
+    {
+        check(fem_vec_wrap(ctx.handle(), size, precision, devicePtr, &h_));
+    }
     DiscreteFunction(const DiscreteFunction &other)
         : name_(other.name_), ctx_(other.ctx_)
     {
@@ -101,6 +110,8 @@ public:
 
     const std::string &name() const { return name_; }
     int64_t size() const { return fem_vec_size(h_); }
+    // the device DoF array (leakPointer(), adaptivefunction.hh:130)
+    void *leakPointer() const { return fem_vec_device_ptr(h_); }
     int precision() const { return fem_vec_precision(h_); }
 
     void clear() { check(fem_vec_fill(h_, 0.0)); }

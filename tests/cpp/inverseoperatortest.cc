@@ -163,6 +163,15 @@ static void dg_checks(Context &ctx)
     op(u, w2);
     w2 -= w;
     report(w2.normSquaredDofs() == 0.0, "DG apply overwrites w", std::sqrt(w2.normSquaredDofs()));
+    // a discrete function over existing device DoF storage is a view
+    {
+        DiscreteFunction uv("u view", ctx, N, u.leakPointer());
+        op(uv, w2);
+        w2 -= w;
+        report(w2.normSquaredDofs() == 0.0, "DG apply on a view of u", std::sqrt(w2.normSquaredDofs()));
+    }
+    report(std::fabs(u.normSquaredDofs() - u.scalarProductDofs(u)) == 0.0 && u.normSquaredDofs() > 0,
+           "u intact after its view is gone", u.normSquaredDofs());
     // linearity: A(2u + v) = 2 A u + A v
     t.assign(u);
     t *= 2.0;

@@ -218,3 +218,28 @@ def test_dg_apply_bf16_mfma_error_bound(ctx, n, p, kind):
     err = np.linalg.norm(W.download() - wref) / np.linalg.norm(wref)
     print("bf16 mfma", n, p, kind, "rel l2 err", err, flush=True)
     assert err < 1e-2
+
+
+def test_vec_wrap_is_a_view(ctx):
+    # fem_vec_wrap: a handle over DoFs the caller already holds on the device
+    # (here the memory of another vector); applies and BLAS act in place and
+    # the memory outlives the view
+    rng = np.random.default_rng(41)
+    n, p = (4, 3, 2), 3
+    kappa = ("cell", rng.uniform(0.5, 2.0, size=24))
+    gop = make_dg(ctx, n, p, 90.0, (1, .5, .25), 0.0, kappa)
+    u = rng.normal(size=gop.N)
+    A = gop.vector(u)
+    V = fa.DofVector(ctx, gop.N, fa.F64, device_ptr=A.device_ptr())
+    W1, W2 = gop.vector(), gop.vector()
+    gop(A, W1)
+    gop(V, W2)
+    assert np.array_equal(W1.download(), W2.download())
+    V.scale(2.0)
+    assert np.array_equal(A.download(), 2.0 * u)
+    del V
+    import gc
+    gc.collect()
+    assert np.array_equal(A.download(), 2.0 * u)
+    with pytest.raises(fa.FemError):
+        fa.DofVector(ctx, 8, fa.F64, device_ptr=A.device_ptr() + 8)   # misaligned
