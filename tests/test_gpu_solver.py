@@ -145,6 +145,30 @@ def test_cg_block_jacobi_matches_oracle(ctx, n, p):
     assert np.abs(X.download() - xo).max() < 1e-6 * np.abs(xo).max()
 
 
+def test_cg_block_jacobi_nonzero_initial_guess(ctx):
+    # x is the initial guess on entry (inverseoperatorinterface.hh operator()(u, w))
+    rng = np.random.default_rng(43)
+    n, p = (5, 4, 4), 3
+    E = n[0] * n[1] * n[2]
+    kcell = rng.uniform(0.2, 5.0, size=E)
+    gop = fa.GalerkinOperator(ctx, fa.OP_DG_SIPG_3D, n, p, 90.0, (0, 0, 0), 0.0, fa.KAPPA_CELL)
+    gop.set_kappa_cell(kcell)
+    pre = fa.Preconditioner(gop, fa.PRE_BLOCK_JACOBI)
+    F = gop.rhs(rng.normal(size=(E, (p + 1) ** 3)))
+    x0 = rng.normal(size=gop.N)
+    X = gop.vector(x0)
+    inv = fa.InverseOperator(ctx, method=fa.CG, tolerance=1e-9, errormeasure=fa.RELATIVE, maxiterations=5000)
+    inv.bind(gop, pre)
+    inv(F, X)
+    info = inv.info()
+    oop = ora.DGOperator(n, p, 90.0, kappa=("cell", kcell))
+    xo, ito = ora.cg(oop, F.download(), x0=x0, pre=ora.block_jacobi(np.linalg.inv(oop.block_diag())), tol=1e-9,
+                     maxit=5000, criteria=1)
+    print("CG+BJ x0", "gpu its", info["iterations"], "oracle its", ito, flush=True)
+    assert info["converged"] and abs(info["iterations"] - ito) <= 2
+    assert np.abs(X.download() - xo).max() < 1e-6 * np.abs(xo).max()
+
+
 def test_fgmres_block_jacobi_problem3_small(ctx):
     # the high-contrast channel problem at 12^3: FGMRES(30) + block Jacobi to 1e-8
     P = wl.problem3(n=12)
