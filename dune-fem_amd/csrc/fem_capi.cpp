@@ -567,6 +567,27 @@ int fem_cnn_coarse_apply(fem_cnn *c, const float *x, float *y)
     });
 }
 
+int fem_cnn_coarse_solve(fem_cnn *c, const float *b, int steps, float *x, double *rz)
+{
+    return guarded([&] {
+        FEM_CHECK(c && b && x, FEM_ERR_ARG, "null argument");
+        CNN &n = c->c;
+        hipStream_t s = n.ctx->stream;
+        DevBuf db, dx;
+        db.alloc(sizeof(float) * n.V);
+        dx.alloc(sizeof(float) * n.V);
+        FEM_HIP(hipMemcpyAsync(db.p, b, sizeof(float) * n.V, hipMemcpyHostToDevice, s));
+        double h[2];
+        cnn_coarse_solve(n, db.as<float>(), steps, dx.as<float>(), h);
+        FEM_HIP(hipMemcpyAsync(x, dx.p, sizeof(float) * n.V, hipMemcpyDeviceToHost, s));
+        FEM_HIP(hipStreamSynchronize(s));
+        if (rz) {
+            rz[0] = h[0];
+            rz[1] = h[1];
+        }
+    });
+}
+
 int fem_cnn_collect_pairs(fem_cnn *c, fem_op *op64, fem_precond *bj64, const fem_vec *b, int max_pairs, int every,
                           int *npairs)
 {

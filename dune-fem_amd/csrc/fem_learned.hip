@@ -67,19 +67,29 @@ __device__ __forceinline__ float offdiag(const CGrid &g, const float *w, int v, 
 }
 
 // ---- coarse CG (Jacobi-preconditioned, on the 7-point stencil) --------------------
-// sc[0] = (r,z) of the previous step, sc[1] = (p, A p), sc[2] = current (r,z),
-// sc[3] = (r,z) at the start
+// Chronopoulos-Gear form of PCG (one fused reduction of two inner products per
+// step, so one kernel per step).  With u = D^-1 r and w = A_c u:
+//   p_i = u_i + beta_i p_{i-1},  s_i = w_i + beta_i s_{i-1}   (s_i = A_c p_i)
+//   x_{i+1} = x_i + alpha_i p_i,  r_{i+1} = r_i - alpha_i s_i
+//   gamma_{i+1} = (r_{i+1}, u_{i+1}),  delta_{i+1} = (w_{i+1}, u_{i+1})
+//   beta_{i+1} = gamma_{i+1} / gamma_i
+//   alpha_{i+1} = gamma_{i+1} / (delta_{i+1} - beta_{i+1} gamma_{i+1} / alpha_i)
+// In exact arithmetic the iterates are those of the two-reduction PCG.  The
+// step kernel forms u_{i+1} of the six neighbours on the fly from the old r,
+// w, s, so r, w, s are double-buffered.
+// sc[0] = alpha_i, sc[1] = beta_i, sc[2] = gamma_i, sc[3] = gamma_0, sc[4] = delta_i
+
+// x0 = k0 y (or 0), r0 = b - A_c x0, p = s = 0
 template <typename T>
 __global__ void __launch_bounds__(256) k_cc_start(CGrid g, const float *__restrict__ st, const T *__restrict__ src,
                                                   int stride, const float *__restrict__ y,
                                                   const float *__restrict__ red, float gamma0,
                                                   float *__restrict__ x, float *__restrict__ r,
-                                                  float *__restrict__ z, float *__restrict__ pold, double *sc,
-                                                  double *partials, unsigned *counter, const int *__restrict__ skip)
+                                                  float *__restrict__ p, float *__restrict__ s,
+                                                  const int *__restrict__ skip)
 {
     if (skip && *skip) return;
     const float k0 = y ? red[0] * gamma0 : 0.f;
-    double acc = 0;
     for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < g.V; v += gridDim.x * blockDim.x) {
         const float *w = st + (size_t)v * 7;
         const float b = (float)src[(size_t)v * stride];
@@ -91,106 +101,125 @@ __global__ void __launch_bounds__(256) k_cc_start(CGrid g, const float *__restri
         } else {
             x[v] = 0.f;
         }
-        const float zv = rv / w[0];
         r[v] = rv;
-        z[v] = zv;
-        pold[v] = 0.f;
-        acc += (double)rv * zv;
-    }
-    __shared__ double lds[4];
-    const double tot = block_sum(acc, lds);
-    __shared__ double res[1];
-    if (finish_reduction(tot, partials, counter, res, 1, &tot)) {
-        sc[0] = 1.0;
-        sc[2] = res[0];
-        sc[3] = res[0];
+        p[v] = 0.f;
+        s[v] = 0.f;
     }
 }
 
-// p = z + beta pold, Ap = A_c p (p of the neighbours formed on the fly)
-__global__ void __launch_bounds__(256) k_cc_ap(CGrid g, const float *__restrict__ st, const float *__restrict__ z,
-                                               const float *__restrict__ pold, float *__restrict__ pnew,
-                                               float *__restrict__ Ap, double *sc, double *partials,
-                                               unsigned *counter, const int *__restrict__ skip)
+// w0 = A_c u0, gamma_0, delta_0, alpha_0 = gamma_0 / delta_0, beta_0 = 0
+__global__ void __launch_bounds__(256) k_cc_init(CGrid g, const float *__restrict__ st, const float *__restrict__ r,
+                                                 float *__restrict__ wo, double *sc, double *partials,
+                                                 unsigned *counter, const int *__restrict__ skip)
 {
     if (skip && *skip) return;
-    const float beta = sc[0] != 0.0 ? (float)(sc[2] / sc[0]) : 0.f;
-    double acc = 0;
+    double ag = 0, ad = 0;
     for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < g.V; v += gridDim.x * blockDim.x) {
         const float *w = st + (size_t)v * 7;
-        const float pv = z[v] + beta * pold[v];
-        const float av = w[0] * pv - offdiag(g, w, v, [&](int q) { return z[q] + beta * pold[q]; });
-        pnew[v] = pv;
-        Ap[v] = av;
-        acc += (double)pv * av;
+        const float uv = r[v] / w[0];
+        const float wv = w[0] * uv - offdiag(g, w, v, [&](int q) { return r[q] / st[(size_t)q * 7]; });
+        wo[v] = wv;
+        ag += (double)r[v] * uv;
+        ad += (double)wv * uv;
     }
     __shared__ double lds[4];
-    const double tot = block_sum(acc, lds);
-    __shared__ double res[1];
-    // every block has read sc[0] and sc[2] before the last one arrives
-    if (finish_reduction(tot, partials, counter, res, 1, &tot)) {
-        sc[1] = res[0];
-        sc[0] = sc[2];
+    double own[2];
+    own[0] = block_sum(ag, lds);
+    __syncthreads();
+    own[1] = block_sum(ad, lds);
+    __shared__ double res[2];
+    if (finish_reduction(0.0, partials, counter, res, 2, own) && threadIdx.x == 0) {
+        const double ga = res[0], de = res[1];
+        sc[0] = (ga > 0.0 && de > 0.0) ? ga / de : 0.0;
+        sc[1] = 0.0;
+        sc[2] = ga;
+        sc[3] = ga;
+        sc[4] = de;
     }
 }
 
-__global__ void __launch_bounds__(256) k_cc_upd(int V, const float *__restrict__ st, float *__restrict__ x,
-                                                float *__restrict__ r, float *__restrict__ z,
-                                                const float *__restrict__ p, const float *__restrict__ Ap, double *sc,
-                                                double *partials, unsigned *counter, const int *__restrict__ skip)
+// one CG step: (x, p in place; r, w, s from the `o` buffers into the `n` buffers)
+__global__ void __launch_bounds__(256) k_cc_step(CGrid g, const float *__restrict__ st, float *__restrict__ x,
+                                                 float *__restrict__ p, const float *__restrict__ ro,
+                                                 const float *__restrict__ wo, const float *__restrict__ so,
+                                                 float *__restrict__ rn, float *__restrict__ wn,
+                                                 float *__restrict__ sn, double *sc, double *partials,
+                                                 unsigned *counter, const int *__restrict__ skip)
 {
     if (skip && *skip) return;
-    const float a = (sc[1] > 0.0 && sc[0] > 0.0) ? (float)(sc[0] / sc[1]) : 0.f;
-    double acc = 0;
-    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) {
-        x[v] += a * p[v];
-        const float rv = r[v] - a * Ap[v];
-        const float zv = rv / st[(size_t)v * 7];
-        r[v] = rv;
-        z[v] = zv;
-        acc += (double)rv * zv;
+    const float al = (float)sc[0], be = (float)sc[1];
+    // r_{i+1} at cell q (the same expression for the cell itself and its neighbours)
+    auto rnext = [&](int q) { return ro[q] - al * (wo[q] + be * so[q]); };
+    double ag = 0, ad = 0;
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < g.V; v += gridDim.x * blockDim.x) {
+        const float *w = st + (size_t)v * 7;
+        const float d = w[0];
+        const float pv = ro[v] / d + be * p[v];
+        const float sv = wo[v] + be * so[v];
+        x[v] += al * pv;
+        const float rv = rnext(v);
+        const float uv = rv / d;
+        const float wv = d * uv - offdiag(g, w, v, [&](int q) { return rnext(q) / st[(size_t)q * 7]; });
+        p[v] = pv;
+        sn[v] = sv;
+        rn[v] = rv;
+        wn[v] = wv;
+        ag += (double)rv * uv;
+        ad += (double)wv * uv;
     }
     __shared__ double lds[4];
-    const double tot = block_sum(acc, lds);
-    __shared__ double res[1];
-    if (finish_reduction(tot, partials, counter, res, 1, &tot)) sc[2] = res[0];
+    double own[2];
+    own[0] = block_sum(ag, lds);
+    __syncthreads();
+    own[1] = block_sum(ad, lds);
+    __shared__ double res[2];
+    // every block has read sc before the last one arrives
+    if (finish_reduction(0.0, partials, counter, res, 2, own) && threadIdx.x == 0) {
+        const double ga = res[0], de = res[1], gold = sc[2], aold = sc[0];
+        const double beta = (gold > 0.0 && aold > 0.0) ? ga / gold : 0.0;
+        const double den = de - (beta != 0.0 ? beta * ga / aold : 0.0);
+        sc[0] = (ga > 0.0 && den > 0.0) ? ga / den : 0.0;
+        sc[1] = beta;
+        sc[2] = ga;
+        sc[4] = de;
+    }
 }
 
 void CoarseCG::init(const CNN *c)
 {
     cnn = c;
     V = (int)c->V;
-    for (DevBuf *b : { &x, &r, &z, &p[0], &p[1], &Ap }) b->alloc(sizeof(float) * V);
-    sc.alloc(sizeof(double) * 4);
-    partials.alloc(sizeof(double) * LR_BLOCKS);
+    for (DevBuf *b : { &x, &p, &r[0], &r[1], &w[0], &w[1], &s[0], &s[1] }) b->alloc(sizeof(float) * V);
+    sc.alloc(sizeof(double) * 8);
+    partials.alloc(sizeof(double) * 2 * LR_BLOCKS);
     counter.alloc(sizeof(unsigned));
     FEM_HIP(hipMemset(counter.p, 0, sizeof(unsigned)));
     FEM_HIP(hipMemset(sc.p, 0, sc.bytes));
 }
 
 template <typename T>
-void CoarseCG::start(const T *src, int stride, const float *y, const float *red, float gamma0, hipStream_t s,
+void CoarseCG::start(const T *src, int stride, const float *y, const float *red, float gamma0, hipStream_t st,
                      const int *skip)
 {
     CGrid g{ cnn->n[0], cnn->n[1], cnn->n[2], V };
     cur = 0;
-    hipLaunchKernelGGL(k_cc_start<T>, dim3(LR_BLOCKS), dim3(256), 0, s, g, cnn->stencil.as<float>(), src, stride, y, red,
-                       gamma0, x.as<float>(), r.as<float>(), z.as<float>(), p[1].as<float>(), sc.as<double>(),
-                       partials.as<double>(), counter.as<unsigned>(), skip);
+    hipLaunchKernelGGL(k_cc_start<T>, dim3(LR_BLOCKS), dim3(256), 0, st, g, cnn->stencil.as<float>(), src, stride, y,
+                       red, gamma0, x.as<float>(), r[0].as<float>(), p.as<float>(), s[0].as<float>(), skip);
+    hipLaunchKernelGGL(k_cc_init, dim3(LR_BLOCKS), dim3(256), 0, st, g, cnn->stencil.as<float>(), r[0].as<float>(),
+                       w[0].as<float>(), sc.as<double>(), partials.as<double>(), counter.as<unsigned>(), skip);
     FEM_HIP(hipGetLastError());
 }
 
-void CoarseCG::step(hipStream_t s, const int *skip)
+void CoarseCG::step(hipStream_t st, const int *skip)
 {
     CGrid g{ cnn->n[0], cnn->n[1], cnn->n[2], V };
-    float *pold = p[cur ^ 1].as<float>(), *pnew = p[cur].as<float>();
-    hipLaunchKernelGGL(k_cc_ap, dim3(LR_BLOCKS), dim3(256), 0, s, g, cnn->stencil.as<float>(), z.as<float>(), pold, pnew,
-                       Ap.as<float>(), sc.as<double>(), partials.as<double>(), counter.as<unsigned>(), skip);
-    hipLaunchKernelGGL(k_cc_upd, dim3(LR_BLOCKS), dim3(256), 0, s, V, cnn->stencil.as<float>(), x.as<float>(),
-                       r.as<float>(), z.as<float>(), pnew, Ap.as<float>(), sc.as<double>(), partials.as<double>(),
+    const int o = cur, n = cur ^ 1;
+    hipLaunchKernelGGL(k_cc_step, dim3(LR_BLOCKS), dim3(256), 0, st, g, cnn->stencil.as<float>(), x.as<float>(),
+                       p.as<float>(), r[o].as<float>(), w[o].as<float>(), s[o].as<float>(), r[n].as<float>(),
+                       w[n].as<float>(), s[n].as<float>(), sc.as<double>(), partials.as<double>(),
                        counter.as<unsigned>(), skip);
     FEM_HIP(hipGetLastError());
-    cur ^= 1;
+    cur = n;
 }
 
 double CoarseCG::rz(hipStream_t s) const
@@ -385,6 +414,23 @@ __global__ void k_err_from_snap(const double *__restrict__ xstar, int nb, int E,
 void cnn_set_coefficient(CNN &c, const double *kcell, double beta)
 {
     c.set_problem(kcell, beta);
+}
+
+void cnn_coarse_solve(const CNN &c, const float *b, int steps, float *x, double rz[2])
+{
+    FEM_CHECK(steps >= 0 && steps <= 100000, FEM_ERR_ARG, "coarse solve: steps in [0, 100000]");
+    FEM_CHECK(c.stencil.p, FEM_ERR_STATE, "coarse solve: set the network's coefficient first");
+    hipStream_t s = c.ctx->stream;
+    CoarseCG cg;
+    cg.init(&c);
+    cg.start<float>(b, 1, nullptr, nullptr, 0.f, s, nullptr);
+    for (int k = 0; k < steps; ++k) cg.step(s, nullptr);
+    double h[4];
+    FEM_HIP(hipMemcpyAsync(x, cg.x.p, sizeof(float) * c.V, hipMemcpyDeviceToDevice, s));
+    FEM_HIP(hipMemcpyAsync(h, cg.sc.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    FEM_HIP(hipStreamSynchronize(s));
+    rz[0] = h[2];
+    rz[1] = h[3];
 }
 
 int cnn_collect_pairs(CNN &c, Operator *op64, Precond *bj64, const Vec &b, int max_pairs, int every)

@@ -10,11 +10,12 @@ struct CNN;
 // Jacobi-preconditioned CG on the 7-point Galerkin coarse operator of a CNN
 // grid, entirely on the device (scalars stay in device memory, no host sync
 // per step); FP32 vectors, FP64 reductions with a deterministic finish.
+// Chronopoulos-Gear recurrences: one kernel and one fused reduction per step.
 struct CoarseCG {
     const CNN *cnn = nullptr;
     int V = 0;
-    DevBuf x, r, z, p[2], Ap, sc, partials, counter;
-    int cur = 0;   // index of the current search direction buffer
+    DevBuf x, p, r[2], w[2], s[2], sc, partials, counter;
+    int cur = 0;   // index of the current r, w, s buffers
     void init(const CNN *c);
     // x = x0, r = b - A_c x0 with b = src[v * stride]; x0 = s0 gamma0 y if
     // y is given (s0 = red[0]), else 0
@@ -22,7 +23,7 @@ struct CoarseCG {
     void start(const T *src, int stride, const float *y, const float *red, float gamma0, hipStream_t s,
                const int *skip);
     void step(hipStream_t s, const int *skip);
-    // host: current (r, z) product, its start value
+    // host: current (r, D^-1 r) product, its start value
     double rz(hipStream_t s) const;
 };
 
@@ -40,6 +41,9 @@ struct Learned {
 };
 
 void cnn_set_coefficient(CNN &c, const double *kcell, double beta);
+// `steps` coarse CG steps from x = 0 on device arrays [V]; rz[0] = final
+// (r, D^-1 r), rz[1] = its start value
+void cnn_coarse_solve(const CNN &c, const float *b, int steps, float *x, double rz[2]);
 // runs the plain FP64 CG + block-Jacobi solve of A x = b from x = 0 and stores
 // coarse (residual, error) pairs of every `every`-th iterate; returns the count
 int cnn_collect_pairs(CNN &c, Operator *op64, Precond *bj64, const Vec &b, int max_pairs, int every);

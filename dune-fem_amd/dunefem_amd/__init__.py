@@ -49,6 +49,7 @@ EXPORTS = [
     "fem_cnn_create", "fem_cnn_destroy", "fem_cnn_nparams", "fem_cnn_set_weights", "fem_cnn_get_weights",
     "fem_cnn_forward", "fem_cnn_loss_grad", "fem_cnn_set_coefficient", "fem_cnn_collect_pairs",
     "fem_cnn_train", "fem_precond_create_learned", "fem_precond_set_learned_mode", "fem_cnn_coarse_apply",
+    "fem_cnn_coarse_solve",
 ]
 
 
@@ -157,6 +158,7 @@ def load_library(path=LIBPATH):
         "fem_cnn_loss_grad": ([_vp, C.POINTER(C.c_float), C.POINTER(C.c_float), _dp, C.POINTER(C.c_float)], C.c_int),
         "fem_cnn_set_coefficient": ([_vp, _dp, C.c_double], C.c_int),
         "fem_cnn_coarse_apply": ([_vp, C.POINTER(C.c_float), C.POINTER(C.c_float)], C.c_int),
+        "fem_cnn_coarse_solve": ([_vp, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_float), _dp], C.c_int),
         "fem_cnn_collect_pairs": ([_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.POINTER(C.c_int)], C.c_int),
         "fem_cnn_train": ([_vp, C.c_int, C.c_int, C.c_float, C.c_float, C.c_uint, _dp, _dp], C.c_int),
         "fem_precond_create_learned": ([_vp, _vp, C.c_int, C.POINTER(_vp)], C.c_int),
@@ -551,6 +553,15 @@ class CNN:
         y = np.zeros(self.V, dtype=np.float32)
         _check(_lib.fem_cnn_coarse_apply(self.h, _fptr(x), _fptr(y)))
         return y
+
+    def coarse_solve(self, b, steps):
+        """`steps` Jacobi-CG steps on A_c x = b from x = 0; returns x and the
+        final and initial (r, D^-1 r)."""
+        b = np.ascontiguousarray(b, dtype=np.float32).ravel()
+        x = np.zeros(self.V, dtype=np.float32)
+        rz = np.zeros(2)
+        _check(_lib.fem_cnn_coarse_solve(self.h, _fptr(b), int(steps), _fptr(x), rz.ctypes.data_as(_dp)))
+        return x, rz[0], rz[1]
 
     def collect_pairs(self, op64, bj64, b, max_pairs, every=10):
         n = C.c_int()

@@ -237,6 +237,10 @@ int fem_cnn_loss_grad(fem_cnn *c, const float *x, const float *t, double *loss, 
 int fem_cnn_set_coefficient(fem_cnn *c, const double *kcell, double beta);
 /* y = A_c x on element means (host arrays [V]) */
 int fem_cnn_coarse_apply(fem_cnn *c, const float *x, float *y);
+/* `steps` Jacobi-preconditioned CG steps on A_c x = b from x = 0, the coarse
+ * solve of the learned preconditioner (host arrays [V]); rz (may be null):
+ * rz[0] = final (r, D^-1 r), rz[1] = its start value */
+int fem_cnn_coarse_solve(fem_cnn *c, const float *b, int steps, float *x, double *rz);
 /* plain FP64 CG + block-Jacobi solve of op64 x = b from x = 0; stores coarse
  * (residual, error) pairs of every `every`-th iterate (at most max_pairs) */
 int fem_cnn_collect_pairs(fem_cnn *c, fem_op *op64, fem_precond *bj64, const fem_vec *b, int max_pairs, int every,

@@ -98,6 +98,53 @@ def test_coarse_operator_is_galerkin_projection(ctx):
     assert np.abs(got - ref).max() <= 1e-5 * np.abs(ref).max()
 
 
+def test_coarse_cg_matches_textbook_pcg(ctx):
+    # the device coarse solve (Chronopoulos-Gear recurrences, FP32 vectors,
+    # FP64 inner products) against k steps of the textbook two-reduction
+    # Jacobi-PCG in float64 on A_c = P^T A P taken from the oracle's DG
+    # operator; many steps must solve A_c x = b
+    import pyoracle as ora
+    P = wl.problem3(n=8)
+    oop = ora.DGOperator(P["n"], P["p"], P["beta"], kappa=("cell", P["kcell"]))
+    nb = (P["p"] + 1) ** 3
+    V = int(np.prod(P["n"]))
+    Ac = np.zeros((V, V))
+    for j in range(V):
+        u = np.zeros(oop.N)
+        u[j * nb] = 1.0
+        Ac[:, j] = oop.apply(u)[::nb]
+    d = np.diag(Ac).copy()
+    net = fa.CNN(ctx, P["n"])
+    net.set_coefficient(P["kcell"], P["beta"])
+    b = np.random.default_rng(5).normal(size=V).astype(np.float32).astype(np.float64)
+
+    def pcg(k):
+        x = np.zeros(V); r = b.copy(); z = r / d; p = z.copy(); rz = r @ z
+        for _ in range(k):
+            q = Ac @ p
+            a = rz / (p @ q)
+            x += a * p; r -= a * q; z = r / d
+            rz, rz_old = r @ z, rz
+            p = z + (rz / rz_old) * p
+        return x, rz
+
+    for k in (1, 2, 5, 10, 20, 50):
+        x, g, g0 = net.coarse_solve(b, k)
+        xr, gr = pcg(k)
+        ex = relf(x, xr)
+        print("coarse CG steps", k, "x rel err", ex, "gamma", g, gr, flush=True)
+        assert ex < 1e-5
+        assert abs(g0 - b @ (b / d)) <= 1e-6 * (b @ (b / d))
+        if k <= 20:
+            assert abs(g - gr) <= 1e-4 * gr
+    x, g, g0 = net.coarse_solve(b, 200)
+    res = np.linalg.norm(b - Ac @ x) / np.linalg.norm(b)
+    print("coarse CG 200 steps: true rel residual", res, flush=True)
+    assert res < 1e-5 and g <= 1e-12 * g0
+    x, g, g0 = net.coarse_solve(np.zeros(V), 3)
+    assert not np.any(x) and g == 0 and g0 == 0
+
+
 def fgmres_run(ctx, op, pre, F):
     inv = fa.InverseOperator(ctx, method=fa.FGMRES, tolerance=1e-8, errormeasure=fa.RELATIVE,
                              maxiterations=20000, restart=30)
