@@ -67,8 +67,8 @@ __device__ __forceinline__ float offdiag(const CGrid &g, const float *w, int v, 
 }
 
 // ---- coarse CG (Jacobi-preconditioned, on the 7-point stencil) --------------------
-// Chronopoulos-Gear form of PCG (one fused reduction of two inner products per
-// step, so one kernel per step).  With u = D^-1 r and w = A_c u:
+// Chronopoulos-Gear form of PCG (the two inner products of a step are reduced
+// together, so one kernel per step).  With u = D^-1 r and w = A_c u:
 //   p_i = u_i + beta_i p_{i-1},  s_i = w_i + beta_i s_{i-1}   (s_i = A_c p_i)
 //   x_{i+1} = x_i + alpha_i p_i,  r_{i+1} = r_i - alpha_i s_i
 //   gamma_{i+1} = (r_{i+1}, u_{i+1}),  delta_{i+1} = (w_{i+1}, u_{i+1})
@@ -77,18 +77,26 @@ __device__ __forceinline__ float offdiag(const CGrid &g, const float *w, int v, 
 // In exact arithmetic the iterates are those of the two-reduction PCG.  The
 // step kernel forms u_{i+1} of the six neighbours on the fly from the old r,
 // w, s, so r, w, s are double-buffered.
-// sc[0] = alpha_i, sc[1] = beta_i, sc[2] = gamma_i, sc[3] = gamma_0, sc[4] = delta_i
+// No atomics: a kernel leaves one (gamma, delta) partial per block, and every
+// block of the next step sums all of them in the same fixed order, so all
+// blocks derive bitwise the same alpha and beta.  Block 0 records
+// (gamma_i, alpha_i) for the step after (ping-pong scalar slots).
+// sc[0..1], sc[2..3]: (gamma, alpha) of the previous step; sc[4] = gamma_0
 
-// x0 = k0 y (or 0), r0 = b - A_c x0, p = s = 0
+// x0 = k0 y (or 0), r0 = b - A_c x0, p = s = 0; no previous step
 template <typename T>
 __global__ void __launch_bounds__(256) k_cc_start(CGrid g, const float *__restrict__ st, const T *__restrict__ src,
                                                   int stride, const float *__restrict__ y,
                                                   const float *__restrict__ red, float gamma0,
                                                   float *__restrict__ x, float *__restrict__ r,
-                                                  float *__restrict__ p, float *__restrict__ s,
+                                                  float *__restrict__ p, float *__restrict__ s, double *sc0,
                                                   const int *__restrict__ skip)
 {
     if (skip && *skip) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        sc0[0] = 0.0;
+        sc0[1] = 0.0;
+    }
     const float k0 = y ? red[0] * gamma0 : 0.f;
     for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < g.V; v += gridDim.x * blockDim.x) {
         const float *w = st + (size_t)v * 7;
@@ -107,10 +115,22 @@ __global__ void __launch_bounds__(256) k_cc_start(CGrid g, const float *__restri
     }
 }
 
-// w0 = A_c u0, gamma_0, delta_0, alpha_0 = gamma_0 / delta_0, beta_0 = 0
+// block partials of (gamma, delta): part[b], part[gridDim + b]
+__device__ __forceinline__ void cc_partials(double ag, double ad, double *lds, double *part)
+{
+    const double tg = block_sum(ag, lds);
+    __syncthreads();
+    const double td = block_sum(ad, lds);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = tg;
+        part[gridDim.x + blockIdx.x] = td;
+    }
+}
+
+// w0 = A_c u0 and the partials of gamma_0, delta_0
 __global__ void __launch_bounds__(256) k_cc_init(CGrid g, const float *__restrict__ st, const float *__restrict__ r,
-                                                 float *__restrict__ wo, double *sc, double *partials,
-                                                 unsigned *counter, const int *__restrict__ skip)
+                                                 float *__restrict__ wo, double *__restrict__ pout,
+                                                 const int *__restrict__ skip)
 {
     if (skip && *skip) return;
     double ag = 0, ad = 0;
@@ -123,31 +143,46 @@ __global__ void __launch_bounds__(256) k_cc_init(CGrid g, const float *__restric
         ad += (double)wv * uv;
     }
     __shared__ double lds[4];
-    double own[2];
-    own[0] = block_sum(ag, lds);
-    __syncthreads();
-    own[1] = block_sum(ad, lds);
-    __shared__ double res[2];
-    if (finish_reduction(0.0, partials, counter, res, 2, own) && threadIdx.x == 0) {
-        const double ga = res[0], de = res[1];
-        sc[0] = (ga > 0.0 && de > 0.0) ? ga / de : 0.0;
-        sc[1] = 0.0;
-        sc[2] = ga;
-        sc[3] = ga;
-        sc[4] = de;
-    }
+    cc_partials(ag, ad, lds, pout);
 }
 
-// one CG step: (x, p in place; r, w, s from the `o` buffers into the `n` buffers)
+// one CG step: (x, p in place; r, w, s from the `o` buffers into the `n`
+// buffers); pin holds the partials of gamma_i, delta_i (LR_BLOCKS each)
 __global__ void __launch_bounds__(256) k_cc_step(CGrid g, const float *__restrict__ st, float *__restrict__ x,
                                                  float *__restrict__ p, const float *__restrict__ ro,
                                                  const float *__restrict__ wo, const float *__restrict__ so,
                                                  float *__restrict__ rn, float *__restrict__ wn,
-                                                 float *__restrict__ sn, double *sc, double *partials,
-                                                 unsigned *counter, const int *__restrict__ skip)
+                                                 float *__restrict__ sn, const double *__restrict__ pin,
+                                                 double *__restrict__ pout, const double *__restrict__ scin,
+                                                 double *__restrict__ scout, double *__restrict__ g0, int first,
+                                                 const int *__restrict__ skip)
 {
     if (skip && *skip) return;
-    const float al = (float)sc[0], be = (float)sc[1];
+    __shared__ double lds[4];
+    __shared__ float scal[2];
+    double pg = 0, pd = 0;
+    for (int b = threadIdx.x; b < LR_BLOCKS; b += blockDim.x) {
+        pg += pin[b];
+        pd += pin[LR_BLOCKS + b];
+    }
+    const double ga = block_sum(pg, lds);
+    __syncthreads();
+    const double de = block_sum(pd, lds);
+    if (threadIdx.x == 0) {
+        const double gold = scin[0], aold = scin[1];
+        const double beta = (gold > 0.0 && aold > 0.0) ? ga / gold : 0.0;
+        const double den = de - (beta != 0.0 ? beta * ga / aold : 0.0);
+        const double alpha = (ga > 0.0 && den > 0.0) ? ga / den : 0.0;
+        scal[0] = (float)alpha;
+        scal[1] = (float)beta;
+        if (blockIdx.x == 0) {
+            scout[0] = ga;
+            scout[1] = alpha;
+            if (first) *g0 = ga;
+        }
+    }
+    __syncthreads();
+    const float al = scal[0], be = scal[1];
     // r_{i+1} at cell q (the same expression for the cell itself and its neighbours)
     auto rnext = [&](int q) { return ro[q] - al * (wo[q] + be * so[q]); };
     double ag = 0, ad = 0;
@@ -167,22 +202,8 @@ __global__ void __launch_bounds__(256) k_cc_step(CGrid g, const float *__restric
         ag += (double)rv * uv;
         ad += (double)wv * uv;
     }
-    __shared__ double lds[4];
-    double own[2];
-    own[0] = block_sum(ag, lds);
     __syncthreads();
-    own[1] = block_sum(ad, lds);
-    __shared__ double res[2];
-    // every block has read sc before the last one arrives
-    if (finish_reduction(0.0, partials, counter, res, 2, own) && threadIdx.x == 0) {
-        const double ga = res[0], de = res[1], gold = sc[2], aold = sc[0];
-        const double beta = (gold > 0.0 && aold > 0.0) ? ga / gold : 0.0;
-        const double den = de - (beta != 0.0 ? beta * ga / aold : 0.0);
-        sc[0] = (ga > 0.0 && den > 0.0) ? ga / den : 0.0;
-        sc[1] = beta;
-        sc[2] = ga;
-        sc[4] = de;
-    }
+    cc_partials(ag, ad, lds, pout);
 }
 
 void CoarseCG::init(const CNN *c)
@@ -191,9 +212,7 @@ void CoarseCG::init(const CNN *c)
     V = (int)c->V;
     for (DevBuf *b : { &x, &p, &r[0], &r[1], &w[0], &w[1], &s[0], &s[1] }) b->alloc(sizeof(float) * V);
     sc.alloc(sizeof(double) * 8);
-    partials.alloc(sizeof(double) * 2 * LR_BLOCKS);
-    counter.alloc(sizeof(unsigned));
-    FEM_HIP(hipMemset(counter.p, 0, sizeof(unsigned)));
+    for (DevBuf *b : { &part[0], &part[1] }) b->alloc(sizeof(double) * 2 * LR_BLOCKS);
     FEM_HIP(hipMemset(sc.p, 0, sc.bytes));
 }
 
@@ -203,10 +222,12 @@ void CoarseCG::start(const T *src, int stride, const float *y, const float *red,
 {
     CGrid g{ cnn->n[0], cnn->n[1], cnn->n[2], V };
     cur = 0;
+    nstep = 0;
     hipLaunchKernelGGL(k_cc_start<T>, dim3(LR_BLOCKS), dim3(256), 0, st, g, cnn->stencil.as<float>(), src, stride, y,
-                       red, gamma0, x.as<float>(), r[0].as<float>(), p.as<float>(), s[0].as<float>(), skip);
+                       red, gamma0, x.as<float>(), r[0].as<float>(), p.as<float>(), s[0].as<float>(), sc.as<double>(),
+                       skip);
     hipLaunchKernelGGL(k_cc_init, dim3(LR_BLOCKS), dim3(256), 0, st, g, cnn->stencil.as<float>(), r[0].as<float>(),
-                       w[0].as<float>(), sc.as<double>(), partials.as<double>(), counter.as<unsigned>(), skip);
+                       w[0].as<float>(), part[0].as<double>(), skip);
     FEM_HIP(hipGetLastError());
 }
 
@@ -216,18 +237,23 @@ void CoarseCG::step(hipStream_t st, const int *skip)
     const int o = cur, n = cur ^ 1;
     hipLaunchKernelGGL(k_cc_step, dim3(LR_BLOCKS), dim3(256), 0, st, g, cnn->stencil.as<float>(), x.as<float>(),
                        p.as<float>(), r[o].as<float>(), w[o].as<float>(), s[o].as<float>(), r[n].as<float>(),
-                       w[n].as<float>(), s[n].as<float>(), sc.as<double>(), partials.as<double>(),
-                       counter.as<unsigned>(), skip);
+                       w[n].as<float>(), s[n].as<float>(), part[o].as<double>(), part[n].as<double>(),
+                       sc.as<double>() + 2 * o, sc.as<double>() + 2 * n, sc.as<double>() + 4, nstep == 0 ? 1 : 0,
+                       skip);
     FEM_HIP(hipGetLastError());
     cur = n;
+    ++nstep;
 }
 
-double CoarseCG::rz(hipStream_t s) const
+void CoarseCG::gammas(hipStream_t st, double &now, double &first) const
 {
-    double h[4];
-    FEM_HIP(hipMemcpyAsync(h, sc.p, sizeof(h), hipMemcpyDeviceToHost, s));
-    FEM_HIP(hipStreamSynchronize(s));
-    return h[2];
+    double h[2 * LR_BLOCKS + 1];
+    FEM_HIP(hipMemcpyAsync(h, part[cur].p, sizeof(double) * 2 * LR_BLOCKS, hipMemcpyDeviceToHost, st));
+    FEM_HIP(hipMemcpyAsync(h + 2 * LR_BLOCKS, sc.as<double>() + 4, sizeof(double), hipMemcpyDeviceToHost, st));
+    FEM_HIP(hipStreamSynchronize(st));
+    now = 0;
+    for (int b = 0; b < LR_BLOCKS; ++b) now += h[b];
+    first = nstep ? h[2 * LR_BLOCKS] : now;
 }
 
 // ---- network input ---------------------------------------------------------------------
@@ -425,12 +451,8 @@ void cnn_coarse_solve(const CNN &c, const float *b, int steps, float *x, double 
     cg.init(&c);
     cg.start<float>(b, 1, nullptr, nullptr, 0.f, s, nullptr);
     for (int k = 0; k < steps; ++k) cg.step(s, nullptr);
-    double h[4];
     FEM_HIP(hipMemcpyAsync(x, cg.x.p, sizeof(float) * c.V, hipMemcpyDeviceToDevice, s));
-    FEM_HIP(hipMemcpyAsync(h, cg.sc.p, sizeof(h), hipMemcpyDeviceToHost, s));
-    FEM_HIP(hipStreamSynchronize(s));
-    rz[0] = h[2];
-    rz[1] = h[3];
+    cg.gammas(s, rz[0], rz[1]);
 }
 
 int cnn_collect_pairs(CNN &c, Operator *op64, Precond *bj64, const Vec &b, int max_pairs, int every)
@@ -586,10 +608,9 @@ void cnn_train(CNN &c, int epochs, int batch, float lr, float val_frac, unsigned
         cg.start<float>(c.ds_r.as<float>() + (size_t)p * E, 1, nullptr, nullptr, 0.f, s, nullptr);
         for (int it = 0; it < 100000; it += 32) {
             for (int k = 0; k < 32; ++k) cg.step(s, nullptr);
-            double h[4];
-            FEM_HIP(hipMemcpyAsync(h, cg.sc.p, sizeof(h), hipMemcpyDeviceToHost, s));
-            FEM_HIP(hipStreamSynchronize(s));
-            if (!(h[2] > 1e-12 * h[3])) break;
+            double gnow, gfirst;
+            cg.gammas(s, gnow, gfirst);
+            if (!(gnow > 1e-12 * gfirst)) break;
         }
         FEM_HIP(hipMemcpyAsync(tg.as<float>() + (size_t)p * E, cg.x.p, sizeof(float) * E, hipMemcpyDeviceToDevice, s));
     }

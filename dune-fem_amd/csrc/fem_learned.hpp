@@ -10,12 +10,14 @@ struct CNN;
 // Jacobi-preconditioned CG on the 7-point Galerkin coarse operator of a CNN
 // grid, entirely on the device (scalars stay in device memory, no host sync
 // per step); FP32 vectors, FP64 reductions with a deterministic finish.
-// Chronopoulos-Gear recurrences: one kernel and one fused reduction per step.
+// Chronopoulos-Gear recurrences: one kernel per step; each step reduces the
+// previous step's block partials itself, so no atomics.
 struct CoarseCG {
     const CNN *cnn = nullptr;
     int V = 0;
-    DevBuf x, p, r[2], w[2], s[2], sc, partials, counter;
-    int cur = 0;   // index of the current r, w, s buffers
+    DevBuf x, p, r[2], w[2], s[2], part[2], sc;
+    int cur = 0;     // index of the current r, w, s and partial buffers
+    int nstep = 0;   // steps since start
     void init(const CNN *c);
     // x = x0, r = b - A_c x0 with b = src[v * stride]; x0 = s0 gamma0 y if
     // y is given (s0 = red[0]), else 0
@@ -23,8 +25,8 @@ struct CoarseCG {
     void start(const T *src, int stride, const float *y, const float *red, float gamma0, hipStream_t s,
                const int *skip);
     void step(hipStream_t s, const int *skip);
-    // host: current (r, D^-1 r) product, its start value
-    double rz(hipStream_t s) const;
+    // host: current (r, D^-1 r) product and its start value
+    void gammas(hipStream_t s, double &now, double &first) const;
 };
 
 struct Learned {
