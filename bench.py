@@ -155,7 +155,11 @@ def solve_section(fa, wl, ctx, n=48, pairs=2000, solves=8, epochs=3, lr=1e-4, co
 
     run("fp64_cg_bj", fa.CG, bj64)
     run("fp64_fgmres_bj", fa.FGMRES, bj64, maxit=3000)
-    run("mixed_fp32_cg_bj_ir", fa.CG, bj32, op32, refinement=1, inner_tolerance=1e-3, inner_maxiterations=20000)
+    # inner reduction 0.03: FP32 CG on this high-contrast problem loses
+    # convergence speed over long runs, and frequent restarts from the FP64
+    # residual beat longer inner solves (3111 inner iterations at 0.03
+    # against 3888 at 1e-3)
+    run("mixed_fp32_cg_bj_ir", fa.CG, bj32, op32, refinement=1, inner_tolerance=0.03, inner_maxiterations=20000)
 
     out["runs"] = runs
     if not learned:
