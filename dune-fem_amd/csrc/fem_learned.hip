@@ -146,8 +146,50 @@ __global__ void __launch_bounds__(256) k_cc_init(CGrid g, const float *__restric
     cc_partials(ag, ad, lds, pout);
 }
 
+// The cell data one step reads: the cell's old r, w, s, p and its seven
+// stencil weights, and the old r, w, s and diagonal of the six neighbours
+// (zero weight outside the grid), in the stencil's neighbour order.
+struct CCCell {
+    float r, w, s, p, wt[7];
+    float nr[6], nw[6], ns[6], nd[6];
+};
+
+__device__ __forceinline__ void cc_load(const CGrid &g, int v, const float *__restrict__ st,
+                                        const float *__restrict__ ro, const float *__restrict__ wo,
+                                        const float *__restrict__ so, const float *__restrict__ p, CCCell &c)
+{
+    c.r = ro[v];
+    c.w = wo[v];
+    c.s = so[v];
+    c.p = p[v];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) c.wt[k] = st[(size_t)v * 7 + k];
+    const int x = v % g.nx, y = (v / g.nx) % g.ny, z = v / (g.nx * g.ny);
+    const int sy = g.nx, sz = g.nx * g.ny;
+    const bool in[6] = { x > 0, x < g.nx - 1, y > 0, y < g.ny - 1, z > 0, z < g.nz - 1 };
+    const int off[6] = { -1, 1, -sy, sy, -sz, sz };
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const int q = in[k] ? v + off[k] : v;
+        c.nr[k] = ro[q];
+        c.nw[k] = wo[q];
+        c.ns[k] = so[q];
+        c.nd[k] = st[(size_t)q * 7];
+        if (!in[k]) c.wt[k + 1] = 0.f;
+    }
+}
+
+// r_{i+1} of a cell from its old r, w, s (the same expression for a cell and
+// for the neighbours that form it on the fly)
+__device__ __forceinline__ float cc_rnext(float r, float w, float s, float al, float be)
+{
+    return r - al * (w + be * s);
+}
+
 // one CG step: (x, p in place; r, w, s from the `o` buffers into the `n`
-// buffers); pin holds the partials of gamma_i, delta_i (LR_BLOCKS each)
+// buffers); pin holds the gridDim partials of gamma_i and of delta_i.  The
+// first cell of each thread is loaded before the scalars are formed, so the
+// loads overlap the partial sums.
 __global__ void __launch_bounds__(256) k_cc_step(CGrid g, const float *__restrict__ st, float *__restrict__ x,
                                                  float *__restrict__ p, const float *__restrict__ ro,
                                                  const float *__restrict__ wo, const float *__restrict__ so,
@@ -160,10 +202,13 @@ __global__ void __launch_bounds__(256) k_cc_step(CGrid g, const float *__restric
     if (skip && *skip) return;
     __shared__ double lds[4];
     __shared__ float scal[2];
+    const int v0 = blockIdx.x * blockDim.x + threadIdx.x;
+    CCCell c;
+    if (v0 < g.V) cc_load(g, v0, st, ro, wo, so, p, c);
     double pg = 0, pd = 0;
-    for (int b = threadIdx.x; b < LR_BLOCKS; b += blockDim.x) {
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
         pg += pin[b];
-        pd += pin[LR_BLOCKS + b];
+        pd += pin[gridDim.x + b];
     }
     const double ga = block_sum(pg, lds);
     __syncthreads();
@@ -183,18 +228,19 @@ __global__ void __launch_bounds__(256) k_cc_step(CGrid g, const float *__restric
     }
     __syncthreads();
     const float al = scal[0], be = scal[1];
-    // r_{i+1} at cell q (the same expression for the cell itself and its neighbours)
-    auto rnext = [&](int q) { return ro[q] - al * (wo[q] + be * so[q]); };
     double ag = 0, ad = 0;
-    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < g.V; v += gridDim.x * blockDim.x) {
-        const float *w = st + (size_t)v * 7;
-        const float d = w[0];
-        const float pv = ro[v] / d + be * p[v];
-        const float sv = wo[v] + be * so[v];
+    for (int v = v0; v < g.V; v += gridDim.x * blockDim.x) {
+        if (v != v0) cc_load(g, v, st, ro, wo, so, p, c);
+        const float d = c.wt[0];
+        const float pv = c.r / d + be * c.p;
+        const float sv = c.w + be * c.s;
         x[v] += al * pv;
-        const float rv = rnext(v);
+        const float rv = cc_rnext(c.r, c.w, c.s, al, be);
         const float uv = rv / d;
-        const float wv = d * uv - offdiag(g, w, v, [&](int q) { return rnext(q) / st[(size_t)q * 7]; });
+        float off = 0.f;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) off += c.wt[k + 1] * (cc_rnext(c.nr[k], c.nw[k], c.ns[k], al, be) / c.nd[k]);
+        const float wv = d * uv - off;
         p[v] = pv;
         sn[v] = sv;
         rn[v] = rv;
@@ -206,13 +252,20 @@ __global__ void __launch_bounds__(256) k_cc_step(CGrid g, const float *__restric
     cc_partials(ag, ad, lds, pout);
 }
 
+// coarse CG launches: one cell per thread up to CC_MAX_BLOCKS workgroups
+constexpr int CC_MAX_BLOCKS = 1024;
+static unsigned cc_blocks(int V)
+{
+    return (unsigned)std::max(1, std::min(CC_MAX_BLOCKS, (V + 255) / 256));
+}
+
 void CoarseCG::init(const CNN *c)
 {
     cnn = c;
     V = (int)c->V;
     for (DevBuf *b : { &x, &p, &r[0], &r[1], &w[0], &w[1], &s[0], &s[1] }) b->alloc(sizeof(float) * V);
     sc.alloc(sizeof(double) * 8);
-    for (DevBuf *b : { &part[0], &part[1] }) b->alloc(sizeof(double) * 2 * LR_BLOCKS);
+    for (DevBuf *b : { &part[0], &part[1] }) b->alloc(sizeof(double) * 2 * CC_MAX_BLOCKS);
     FEM_HIP(hipMemset(sc.p, 0, sc.bytes));
 }
 
@@ -226,7 +279,7 @@ void CoarseCG::start(const T *src, int stride, const float *y, const float *red,
     hipLaunchKernelGGL(k_cc_start<T>, dim3(LR_BLOCKS), dim3(256), 0, st, g, cnn->stencil.as<float>(), src, stride, y,
                        red, gamma0, x.as<float>(), r[0].as<float>(), p.as<float>(), s[0].as<float>(), sc.as<double>(),
                        skip);
-    hipLaunchKernelGGL(k_cc_init, dim3(LR_BLOCKS), dim3(256), 0, st, g, cnn->stencil.as<float>(), r[0].as<float>(),
+    hipLaunchKernelGGL(k_cc_init, dim3(cc_blocks(V)), dim3(256), 0, st, g, cnn->stencil.as<float>(), r[0].as<float>(),
                        w[0].as<float>(), part[0].as<double>(), skip);
     FEM_HIP(hipGetLastError());
 }
@@ -235,7 +288,7 @@ void CoarseCG::step(hipStream_t st, const int *skip)
 {
     CGrid g{ cnn->n[0], cnn->n[1], cnn->n[2], V };
     const int o = cur, n = cur ^ 1;
-    hipLaunchKernelGGL(k_cc_step, dim3(LR_BLOCKS), dim3(256), 0, st, g, cnn->stencil.as<float>(), x.as<float>(),
+    hipLaunchKernelGGL(k_cc_step, dim3(cc_blocks(V)), dim3(256), 0, st, g, cnn->stencil.as<float>(), x.as<float>(),
                        p.as<float>(), r[o].as<float>(), w[o].as<float>(), s[o].as<float>(), r[n].as<float>(),
                        w[n].as<float>(), s[n].as<float>(), part[o].as<double>(), part[n].as<double>(),
                        sc.as<double>() + 2 * o, sc.as<double>() + 2 * n, sc.as<double>() + 4, nstep == 0 ? 1 : 0,
@@ -247,13 +300,14 @@ void CoarseCG::step(hipStream_t st, const int *skip)
 
 void CoarseCG::gammas(hipStream_t st, double &now, double &first) const
 {
-    double h[2 * LR_BLOCKS + 1];
-    FEM_HIP(hipMemcpyAsync(h, part[cur].p, sizeof(double) * 2 * LR_BLOCKS, hipMemcpyDeviceToHost, st));
-    FEM_HIP(hipMemcpyAsync(h + 2 * LR_BLOCKS, sc.as<double>() + 4, sizeof(double), hipMemcpyDeviceToHost, st));
+    const int nb = (int)cc_blocks(V);
+    double h[CC_MAX_BLOCKS + 1];
+    FEM_HIP(hipMemcpyAsync(h, part[cur].p, sizeof(double) * nb, hipMemcpyDeviceToHost, st));
+    FEM_HIP(hipMemcpyAsync(h + nb, sc.as<double>() + 4, sizeof(double), hipMemcpyDeviceToHost, st));
     FEM_HIP(hipStreamSynchronize(st));
     now = 0;
-    for (int b = 0; b < LR_BLOCKS; ++b) now += h[b];
-    first = nstep ? h[2 * LR_BLOCKS] : now;
+    for (int b = 0; b < nb; ++b) now += h[b];
+    first = nstep ? h[nb] : now;
 }
 
 // ---- network input ---------------------------------------------------------------------
