@@ -313,14 +313,22 @@ __global__ void __launch_bounds__(64)
     // compile-time offsets only (searched exhaustively over additive layouts).
     // Otherwise the layout of DGShape (one element per 16-lane group).
     constexpr bool F64MF_ = (K == 4) && std::is_same<T, double>::value && !MF;
-    constexpr int S1 = F64MF_ ? 20 : S::S1;
+    // FP64 at K = 3 and 5 (lane el K^2 + i K + j): unpadded tensors
+    // (a, b, c) -> a K^2 + b K + c, face-trace scratch rows of 4 at K = 3, and
+    // an odd element stride (107 and 361 doubles); from the same exhaustive
+    // bank model this cuts the modelled conflicts 3.1x (K = 3) and 2.2x (K = 5)
+    constexpr bool D35 = (K == 3 || K == 5) && std::is_same<T, double>::value && !MF;
+    constexpr int S1 = F64MF_ ? 20 : D35 ? K2 : S::S1;
     constexpr int SB = F64MF_ ? 5 : K;
+    constexpr int SXI = (D35 && K == 3) ? 4 : K;   // face-trace scratch: row stride
+    constexpr int SXQ = K * SXI;                   // and quantity stride
     // with pointwise kappa a third tensor KA holds the element's volume kappa
     // values: loaded once as x-pencils, the y- and z-pencils come from LDS
     // (K = 4 only: the other orders keep the face-trace scratch there)
     constexpr bool KLDS = (K == 4) && KQ;
-    constexpr int SZ = F64MF_ ? (KLDS ? 244 : 164) : S::SZ;
+    constexpr int SZ = F64MF_ ? (KLDS ? 244 : 164) : D35 ? (K == 3 ? 107 : 361) : S::SZ;
     static_assert(!F64MF_ || SZ >= (KLDS ? 3 : 2) * K * S1, "tensors per element");
+    static_assert(F64MF_ || SZ >= 2 * K * S1 + 4 * SXQ, "tensors and face-trace scratch per element");
     // DGShape layout at K = 4: the third tensor ends at 2 TS + lidx(3,3,3) + 1
     static_assert(F64MF_ || !KLDS || SZ >= 2 * K * S1 + 3 * S1 + 3 * K + 4, "tensors per element");
     __shared__ T smem[EPB * SZ];
@@ -382,7 +390,7 @@ __global__ void __launch_bounds__(64)
     T *KA = R + K * S1;   // pointwise kappa at the volume points (KLDS only;
                           // at K = 4 the face-trace scratch Sx is not used)
     auto L = [&](int a_, int b_, int c_) {
-        if constexpr (F64MF_) return a_ * S1 + b_ * SB + c_;
+        if constexpr (F64MF_ || D35) return a_ * S1 + b_ * SB + c_;
         else return lidx<K>(a_, b_, c_);
     };
     T *Sx = R + K * S1;   // [2 faces][2 quantities][K2] (K != 4 only)
@@ -505,8 +513,8 @@ __global__ void __launch_bounds__(64)
             if (wr) {
 #pragma unroll
                 for (int s = 0; s < 2; ++s) {
-                    Sx[(s * 2 + 0) * K2 + i * K + j] = nval[d][s];
-                    Sx[(s * 2 + 1) * K2 + i * K + j] = nder[d][s];
+                    Sx[(s * 2 + 0) * SXQ + i * SXI + j] = nval[d][s];
+                    Sx[(s * 2 + 1) * SXQ + i * SXI + j] = nder[d][s];
                 }
             }
             wave_lds_sync();
@@ -515,13 +523,13 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
             for (int sq = 0; sq < 4; ++sq) {
 #pragma unroll
-                for (int m = 0; m < K; ++m) w2[m] = Sx[sq * K2 + i * K + m];
+                for (int m = 0; m < K; ++m) w2[m] = Sx[sq * SXQ + i * SXI + m];
                 t2v[sq] = dotk<K>(tb.B[j], w2);
             }
             wave_lds_sync();
             if (wr)
 #pragma unroll
-                for (int sq = 0; sq < 4; ++sq) Sx[sq * K2 + i * K + j] = t2v[sq];
+                for (int sq = 0; sq < 4; ++sq) Sx[sq * SXQ + i * SXI + j] = t2v[sq];
             wave_lds_sync();
             // first tangential index: thread (i=q1, j=q2)
 #pragma unroll
@@ -529,8 +537,8 @@ __global__ void __launch_bounds__(64)
                 T nv = 0, nd = 0;
 #pragma unroll
                 for (int m = 0; m < K; ++m) {
-                    nv = fma(tb.B[i][m], Sx[(s * 2 + 0) * K2 + m * K + j], nv);
-                    nd = fma(tb.B[i][m], Sx[(s * 2 + 1) * K2 + m * K + j], nd);
+                    nv = fma(tb.B[i][m], Sx[(s * 2 + 0) * SXQ + m * SXI + j], nv);
+                    nd = fma(tb.B[i][m], Sx[(s * 2 + 1) * SXQ + m * SXI + j], nd);
                 }
                 nvs[s] = nv;
                 nds[s] = nd;
