@@ -313,11 +313,12 @@ __global__ void __launch_bounds__(64)
     // compile-time offsets only (searched exhaustively over additive layouts).
     // Otherwise the layout of DGShape (one element per 16-lane group).
     constexpr bool F64MF_ = (K == 4) && std::is_same<T, double>::value && !MF;
-    // FP64 at K = 3 and 5 (lane el K^2 + i K + j): unpadded tensors
-    // (a, b, c) -> a K^2 + b K + c, face-trace scratch rows of 4 at K = 3, and
-    // an odd element stride (107 and 361 doubles); from the same exhaustive
-    // bank model this cuts the modelled conflicts 3.1x (K = 3) and 2.2x (K = 5)
-    constexpr bool D35 = (K == 3 || K == 5) && std::is_same<T, double>::value && !MF;
+    // K = 3 and 5 without BF16 MFMA (lane el K^2 + i K + j): unpadded
+    // tensors (a, b, c) -> a K^2 + b K + c, face-trace scratch rows of 4 at
+    // K = 3, and an odd element stride (107 and 361 words); from the same
+    // exhaustive bank model this cuts the modelled conflicts of the FP64
+    // accesses 3.1x (K = 3) and 2.2x (K = 5)
+    constexpr bool D35 = (K == 3 || K == 5) && !MF;
     constexpr int S1 = F64MF_ ? 20 : D35 ? K2 : S::S1;
     constexpr int SB = F64MF_ ? 5 : K;
     constexpr int SXI = (D35 && K == 3) ? 4 : K;   // face-trace scratch: row stride
