@@ -407,6 +407,53 @@ __global__ void k_scaled_residual(const T *__restrict__ v, const T *__restrict__
     }
 }
 
+// The same three steps when the apply hands over per-workgroup partials of
+// <z1, A z1>: z1 = P c is zero outside the phi_0 slots, so <v, z1> is a sum
+// over the elements, t = v - alpha w1 needs no pass over z1, and y += alpha z1
+// touches the phi_0 slots only.
+template <typename T>
+__global__ void __launch_bounds__(256) k_alpha_coarse(const T *__restrict__ v, int nb, const float *__restrict__ c,
+                                                      int E, const double *__restrict__ parts, int64_t np,
+                                                      double *partials, unsigned *counter, float *red,
+                                                      const int *__restrict__ skip)
+{
+    if (skip && *skip) return;
+    __shared__ double lds[4];
+    double a = 0, b = 0;
+    const int64_t g0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, gs = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t e = g0; e < E; e += gs) a += (double)v[e * nb] * (double)(T)c[e];
+    for (int64_t k = g0; k < np; k += gs) b += parts[k];
+    double own[2];
+    own[0] = block_sum(a, lds);
+    __syncthreads();
+    own[1] = block_sum(b, lds);
+    __shared__ double res[2];
+    if (finish_reduction(0.0, partials, counter, res, 2, own))
+        red[2] = (res[1] > 0 && res[0] == res[0]) ? (float)(res[0] / res[1]) : 0.f;
+}
+
+// t = v - alpha w
+template <typename T>
+__global__ void k_residual_t(const T *__restrict__ v, const T *__restrict__ w, T *__restrict__ t, int64_t N,
+                             const float *__restrict__ red, const int *__restrict__ skip)
+{
+    if (skip && *skip) return;
+    const T al = (T)red[2];
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x)
+        t[i] = v[i] - al * w[i];
+}
+
+// y[e nb] += alpha c_e (the same rounding as scaling z1 and adding it)
+template <typename T>
+__global__ void k_add_coarse(T *__restrict__ y, const float *__restrict__ c, int nb, int E,
+                             const float *__restrict__ red, const int *__restrict__ skip)
+{
+    if (skip && *skip) return;
+    const T al = (T)red[2];
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < E) y[(int64_t)e * nb] += (T)c[e] * al;
+}
+
 // ---------------------------------------------------------------------------------
 void Learned::init(Operator *o, CNN *c, int steps)
 {
@@ -432,6 +479,8 @@ void Learned::init(Operator *o, CNN *c, int steps)
     partials.alloc(sizeof(double) * 2 * LR_BLOCKS);
     counter.alloc(sizeof(unsigned));
     FEM_HIP(hipMemset(counter.p, 0, sizeof(unsigned)));
+    // the FP64 and FP32 DG applies can write per-workgroup <z1, A z1> partials
+    if (o->desc.precision != FEM_BF16MFMA) dparts.alloc(sizeof(double) * o->dot_blocks());
 }
 
 template <typename T>
@@ -460,6 +509,18 @@ static void learned_apply_t(Learned &L, const T *x, T *y, const int *skip)
     // z1 = alpha P c; y = z1 + BJ(x - A z1)
     T *z1 = L.z1.as<T>(), *t1 = L.t1.as<T>(), *w1 = L.w1.as<T>();
     hipLaunchKernelGGL(k_prolong<T>, dim3(G), dim3(256), 0, s, L.ccg.x.as<float>(), nb, N, z1, skip);
+    if (L.dparts.p) {
+        o.apply_raw(z1, w1, skip, L.dparts.as<double>());
+        hipLaunchKernelGGL(k_alpha_coarse<T>, dim3(LR_BLOCKS), dim3(256), 0, s, x, nb, L.ccg.x.as<float>(), E,
+                           L.dparts.as<double>(), o.dot_blocks(), L.partials.as<double>(), L.counter.as<unsigned>(),
+                           L.red.as<float>(), skip);
+        hipLaunchKernelGGL(k_residual_t<T>, dim3(G), dim3(256), 0, s, x, w1, t1, N, L.red.as<float>(), skip);
+        L.bj.apply_raw(t1, y, skip);
+        hipLaunchKernelGGL(k_add_coarse<T>, dim3((E + 255) / 256), dim3(256), 0, s, y, L.ccg.x.as<float>(), nb, E,
+                           L.red.as<float>(), skip);
+        FEM_HIP(hipGetLastError());
+        return;
+    }
     o.apply_raw(z1, w1, skip);
     hipLaunchKernelGGL(k_alpha<T>, dim3(LR_BLOCKS), dim3(256), 0, s, x, z1, w1, N, L.partials.as<double>(),
                        L.counter.as<unsigned>(), L.red.as<float>(), skip);

@@ -37,6 +37,7 @@ struct Learned {
     int coarse_steps = 5;
     int mode = FEM_LEARNED_NET;
     DevBuf z1, t1, w1, x0, yc, red, partials, counter;
+    DevBuf dparts;   // per-workgroup <z1, A z1> partials of the apply (not for BF16 applies)
     void init(Operator *o, CNN *c, int coarse_steps);
     void apply_raw(const void *x, void *y, const int *skip);
     bool active() const { return mode != FEM_LEARNED_OFF; }
