@@ -52,6 +52,34 @@ __device__ __forceinline__ int nbr(const Grid &g, int v, int t, int dil, int sig
     return X + g.nx * (Y + g.ny * Z);
 }
 
+// a voxel with its coordinates, stepped along the voxel order without
+// divisions (the weight-gradient kernels walk runs of consecutive voxels)
+struct Vox {
+    int v, x, y, z;
+};
+__device__ __forceinline__ Vox vox(const Grid &g, int v)
+{
+    return Vox{ v, v % g.nx, (v / g.nx) % g.ny, v / (g.nx * g.ny) };
+}
+__device__ __forceinline__ void vox_next(const Grid &g, Vox &p)
+{
+    ++p.v;
+    if (++p.x == g.nx) {
+        p.x = 0;
+        if (++p.y == g.ny) {
+            p.y = 0;
+            ++p.z;
+        }
+    }
+}
+__device__ __forceinline__ int nbr(const Grid &g, const Vox &p, int t, int dil, int sign)
+{
+    const int sx = sign * dil * (t % 3 - 1), sy = sign * dil * ((t / 3) % 3 - 1), sz = sign * dil * (t / 9 - 1);
+    const int X = p.x + sx, Y = p.y + sy, Z = p.z + sz;
+    if (X < 0 || Y < 0 || Z < 0 || X >= g.nx || Y >= g.ny || Z >= g.nz) return -1;
+    return p.v + sx + g.nx * (sy + g.ny * sz);
+}
+
 // ---- input layer: 2 -> 32, ReLU --------------------------------------------------
 __global__ void __launch_bounds__(256) k_in_fwd(Grid g, int dil, const float *__restrict__ x0,
                                                 const float *__restrict__ wq, u16 *__restrict__ a1,
@@ -240,12 +268,13 @@ __global__ void __launch_bounds__(256) k_hid_wgrad(Grid g, int dil, const u16 *_
     f32x4_t acc = { 0.f, 0.f, 0.f, 0.f };
     for (int k0 = c0; k0 < c0 + WG_CHUNK; k0 += 32) {
         bf16x8_t A, B;
+        Vox pv = vox(g, k0 + 8 * kb);
 #pragma unroll
-        for (int r = 0; r < 8; ++r) {
-            const int v = k0 + 8 * kb + r;
+        for (int r = 0; r < 8; ++r, vox_next(g, pv)) {
+            const int v = pv.v;
             float av = 0.f, gv = 0.f;
             if (v < g.V) {
-                const int w = nbr(g, v, t, dil, 1);
+                const int w = nbr(g, pv, t, dil, 1);
                 if (w >= 0) av = bf2f(ain[(size_t)w * CH + 16 * tci + i]);
                 gv = gout[(size_t)v * CH + 16 * tco + i];
             }
@@ -271,8 +300,9 @@ __global__ void k_thin_wgrad(Grid g, int dil, const void *__restrict__ ain_, int
     if (ci >= ci_n) return;
     const int c0 = chunk * WG_CHUNK;
     float acc = 0.f;
-    for (int v = c0; v < c0 + WG_CHUNK && v < g.V; ++v) {
-        const int w = nbr(g, v, t, dil, 1);
+    Vox pv = vox(g, c0);
+    for (int v = c0; v < c0 + WG_CHUNK && v < g.V; ++v, vox_next(g, pv)) {
+        const int w = nbr(g, pv, t, dil, 1);
         if (w < 0) continue;
         const float av = BF ? bf2f(((const u16 *)ain_)[(size_t)w * ci_n + ci]) : ((const float *)ain_)[(size_t)w * ci_n + ci];
         acc = fmaf(av, bf2f(f2bf(gout[(size_t)v * co_n + co])), acc);
