@@ -352,6 +352,43 @@ def test_gmres_matches_dense_solve(kind):
         assert np.abs(x - xref).max() < 1e-8 * np.abs(xref).max()
 
 
+@pytest.mark.parametrize("kind", ["gmres", "fgmres"])
+def test_gmres_minimal_residual_property(kind):
+    # the right-preconditioned (F)GMRES iterate after k < restart steps from
+    # x0 = 0 minimises ||b - A x|| over x in M K_k(A M, b); with a fixed linear
+    # M both recurrences must reach the brute-force least-squares minimum over
+    # an explicit Krylov basis (nonsymmetric operator: advection on)
+    rng = np.random.default_rng(11)
+    n, p = (2, 2, 2), 1
+    op = ora.DGOperator(n, p, beta=10, b=(1, .5, .25), kappa=random_qp_kappa(rng, n, p))
+    A = assemble(op)
+    b = rng.normal(size=op.N)
+    inv = np.linalg.inv(op.block_diag())
+    E, nb, _ = inv.shape
+    M = np.zeros((op.N, op.N))
+    for e in range(E):
+        M[e * nb:(e + 1) * nb, e * nb:(e + 1) * nb] = inv[e]
+    f = ora.gmres if kind == "gmres" else ora.fgmres
+    for pre, Mm in ((None, np.eye(op.N)), (ora.block_jacobi(inv), M)):
+        AM = A @ Mm
+        for k in range(1, 9):
+            x, it = f(op, b, pre=pre, tol=1e-300, maxit=k, criteria=1, restart=20)
+            assert it == -k
+            K = np.empty((op.N, k))
+            v = b / np.linalg.norm(b)
+            for j in range(k):
+                K[:, j] = v
+                v = AM @ v
+                v -= K[:, :j + 1] @ (K[:, :j + 1].T @ v)
+                v -= K[:, :j + 1] @ (K[:, :j + 1].T @ v)
+                v /= np.linalg.norm(v)
+            c, *_ = np.linalg.lstsq(AM @ K, b, rcond=None)
+            rmin = np.linalg.norm(b - AM @ (K @ c))
+            rk = np.linalg.norm(b - A @ x)
+            assert abs(rk - rmin) <= 1e-10 * np.linalg.norm(b), (kind, k, rk, rmin)
+            assert np.linalg.norm(x - Mm @ (K @ c)) <= 1e-8 * np.linalg.norm(x)
+
+
 def test_block_diag_matches_assembled_blocks():
     rng = np.random.default_rng(9)
     n, p = (2, 3, 2), 1
