@@ -352,6 +352,43 @@ def test_gmres_matches_dense_solve(kind):
         assert np.abs(x - xref).max() < 1e-8 * np.abs(xref).max()
 
 
+def test_cg_energy_minimisation_property():
+    # the preconditioned CG iterate after k steps from x0 = 0 minimises the
+    # energy error ||x - x*||_A over K_k(M A, M b) = M K_k(A M, b); checked
+    # against the brute-force Galerkin solution on an explicit Krylov basis
+    # (SPD SIPG operator, with and without block Jacobi)
+    rng = np.random.default_rng(12)
+    n, p = (2, 2, 2), 1
+    op = ora.DGOperator(n, p, beta=10, kappa=random_qp_kappa(rng, n, p))
+    A = assemble(op)
+    b = rng.normal(size=op.N)
+    xs = np.linalg.solve(A, b)
+    inv = np.linalg.inv(op.block_diag())
+    E, nb, _ = inv.shape
+    M = np.zeros((op.N, op.N))
+    for e in range(E):
+        M[e * nb:(e + 1) * nb, e * nb:(e + 1) * nb] = inv[e]
+    for pre, Mm in ((None, np.eye(op.N)), (ora.block_jacobi(inv), M)):
+        AM = A @ Mm
+        for k in range(1, 9):
+            x, it = ora.cg(op, b, pre=pre, tol=1e-300, maxit=k, criteria=1)
+            assert it == -k
+            K = np.empty((op.N, k))
+            v = b / np.linalg.norm(b)
+            for j in range(k):
+                K[:, j] = v
+                v = AM @ v
+                v -= K[:, :j + 1] @ (K[:, :j + 1].T @ v)
+                v -= K[:, :j + 1] @ (K[:, :j + 1].T @ v)
+                v /= np.linalg.norm(v)
+            Z = Mm @ K
+            c = np.linalg.solve(Z.T @ A @ Z, Z.T @ b)
+            xg = Z @ c
+            ek, eg = x - xs, xg - xs
+            assert abs(np.sqrt(ek @ A @ ek) - np.sqrt(eg @ A @ eg)) <= 1e-9 * np.sqrt(xs @ A @ xs), (k,)
+            assert np.linalg.norm(x - xg) <= 1e-7 * np.linalg.norm(xg)
+
+
 @pytest.mark.parametrize("kind", ["gmres", "fgmres"])
 def test_gmres_minimal_residual_property(kind):
     # the right-preconditioned (F)GMRES iterate after k < restart steps from
