@@ -335,18 +335,33 @@ void Batch::init(Context *c, int n, const int *nx_, const int *ny_, double c_)
     FEM_HIP(hipHostMalloc((void **)&hactive, sizeof(int), hipHostMallocDefault));
 }
 
-Batch::~Batch()
+void Batch::drop_graph()
 {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+}
+
+Batch::~Batch()
+{
+    drop_graph();
     if (hactive) (void)hipHostFree(hactive);
 }
 
 void Batch::set_kappa_qp(const double *kv)
 {
     FEM_CHECK(kv, FEM_ERR_ARG, "null kappa");
-    kvol64.alloc(sizeof(double) * E * 9);
-    kvol32.alloc(sizeof(float) * E * 9);
+    // the coefficient and Jacobi buffers keep their addresses after the first
+    // call (the captured graph reads kvol32 and dinv32); a first allocation
+    // drops any graph
+    if (!kvol64.p) {
+        drop_graph();
+        kvol64.alloc(sizeof(double) * E * 9);
+        kvol32.alloc(sizeof(float) * E * 9);
+        dinv64.alloc(sizeof(double) * N);
+        dinv32.alloc(sizeof(float) * N);
+    }
     FEM_HIP(hipMemcpy(kvol64.p, kv, sizeof(double) * E * 9, hipMemcpyHostToDevice));
     std::vector<float> f(E * 9);
     for (int64_t i = 0; i < E * 9; ++i) f[i] = (float)kv[i];
@@ -356,8 +371,6 @@ void Batch::set_kappa_qp(const double *kv)
     d.alloc(sizeof(double) * N);
     cg_launch_shape<double>(0, CG_MODE_DIAG, inst_dev[0].as<CGInst>(), count, tiles[0], tab.as<CGTab>(),
                             kvol64.as<double>(), nullptr, d.as<double>(), nullptr, nullptr, ctx->stream);
-    dinv64.alloc(sizeof(double) * N);
-    dinv32.alloc(sizeof(float) * N);
     hipLaunchKernelGGL(k_jacobi_inv_b<double>, dim3(grid_for(N, 256, 8192)), dim3(256), 0, ctx->stream, d.as<double>(), dinv64.as<double>(), N);
     hipLaunchKernelGGL(k_jacobi_inv_b<float>, dim3(grid_for(N, 256, 8192)), dim3(256), 0, ctx->stream, d.as<double>(), dinv32.as<float>(), N);
     FEM_HIP(hipGetLastError());
@@ -410,8 +423,7 @@ int Batch::autotune(int reps)
         if (ms < best) { best = ms; bestsh = sh; }
     }
     shape = bestsh;
-    if (graph_exec) { (void)hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
-    if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
+    drop_graph();
     return bestsh;
 }
 
@@ -424,10 +436,14 @@ void Batch::solve(const Vec &b, Vec &x, double tol, double inner_tol, int inner_
     hipStream_t st = ctx->stream;
     std::memset(&info, 0, sizeof(info));
     auto t0 = std::chrono::steady_clock::now();
-    // work vectors
-    h64.alloc(sizeof(double) * N);
-    r64.alloc(sizeof(double) * N);
-    for (DevBuf *v : { &r32, &d32, &p32, &q32, &s32, &h32 }) v->alloc(sizeof(float) * N);
+    // work vectors: allocated on the first solve only (N is fixed for a
+    // batch), so the pointers captured in the HIP graph stay valid
+    if (!h64.p) {
+        drop_graph();
+        h64.alloc(sizeof(double) * N);
+        r64.alloc(sizeof(double) * N);
+        for (DevBuf *v : { &r32, &d32, &p32, &q32, &s32, &h32 }) v->alloc(sizeof(float) * N);
+    }
     const int n = count;
     double *D = sdbl.as<double>();
     int *I = sint.as<int>();

@@ -22,8 +22,12 @@ struct Batch {
     DevBuf sdbl, sint, active;
     int *hactive = nullptr;
     DevBuf h64, r64, r32, d32, p32, q32, s32, h32;
+    // one captured inner CG step; it holds the addresses of the work,
+    // coefficient and Jacobi buffers, so it is dropped whenever one of them
+    // is (re)allocated or the tile shape changes
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    void drop_graph();
     std::vector<int> last_iters;
     std::vector<double> last_relres;
 

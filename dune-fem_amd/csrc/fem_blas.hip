@@ -33,21 +33,6 @@ __global__ void k_scal(T *__restrict__ x, int64_t n, double a)
         x[i] = (T)((double)x[i] * a);
 }
 
-// fixed grid of RED_DOT_BLOCKS x RED_THREADS threads; each thread accumulates a
-// grid-stride slice in FP64 (FP32 inputs are widened before multiplying)
-template <typename TX, typename TY>
-__global__ void __launch_bounds__(RED_THREADS) k_dot(const TX *__restrict__ x, const TY *__restrict__ y,
-                                                     int64_t n, double *partials, unsigned *counter,
-                                                     double *result)
-{
-    __shared__ double lds[RED_THREADS / 64];
-    double s = 0;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        s += (double)x[i] * (double)y[i];
-    double b = block_sum(s, lds);
-    finish_reduction(0.0, partials, counter, result, 1, &b);
-}
-
 static void launch_geom(int64_t n, unsigned &g, unsigned &t)
 {
     t = 256;

@@ -79,6 +79,21 @@ __device__ __forceinline__ bool finish_reduction(double partial, double *partial
     return true;
 }
 
+// fixed grid of RED_DOT_BLOCKS x RED_THREADS threads; each thread accumulates a
+// grid-stride slice in FP64 (FP32 inputs are widened before multiplying)
+template <typename TX, typename TY>
+__global__ void __launch_bounds__(RED_THREADS) k_dot(const TX *__restrict__ x, const TY *__restrict__ y,
+                                                     int64_t n, double *partials, unsigned *counter,
+                                                     double *result)
+{
+    __shared__ double lds[RED_THREADS / 64];
+    double s = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        s += (double)x[i] * (double)y[i];
+    double b = block_sum(s, lds);
+    finish_reduction(0.0, partials, counter, result, 1, &b);
+}
+
 // host-side launchers (fem_blas.hip)
 void vec_fill(Vec &v, double a);
 void vec_copy(Vec &dst, const Vec &src);
@@ -87,6 +102,9 @@ void vec_scal(Vec &x, double a);
 // device result in ctx.red_result[slot]
 void vec_dot_async(const Vec &x, const Vec &y, int slot);
 double vec_dot(const Vec &x, const Vec &y);
+// the same scalar product with its reduction scratch in a file-scope device
+// array instead of the context's workspace (fem_noalloc.hip): no allocation
+double vec_dot_noalloc(const Vec &x, const Vec &y);
 // raw-pointer variants used by the solvers
 void dot_async(Context &c, int64_t n, const double *x, const double *y, double *result, unsigned *counter, double *partials);
 void dot_async(Context &c, int64_t n, const float *x, const float *y, double *result, unsigned *counter, double *partials);

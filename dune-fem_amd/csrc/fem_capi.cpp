@@ -263,6 +263,14 @@ int fem_vec_dot(const fem_vec *x, const fem_vec *y, double *out)
     });
 }
 
+int fem_vec_dot_noalloc(const fem_vec *x, const fem_vec *y, double *out)
+{
+    return guarded([&] {
+        FEM_CHECK(x && y && out, FEM_ERR_ARG, "null argument");
+        *out = vec_dot_noalloc(x->v, y->v);
+    });
+}
+
 int fem_vec_norm2sq(const fem_vec *x, double *out)
 {
     return guarded([&] {
@@ -339,6 +347,14 @@ int fem_op_diagonal(fem_op *op, fem_vec *d)
     });
 }
 
+int fem_op_block_diagonal(fem_op *op, fem_vec *blocks)
+{
+    return guarded([&] {
+        FEM_CHECK(op && blocks, FEM_ERR_ARG, "null argument");
+        op->o.block_diagonal(blocks->v);
+    });
+}
+
 // ---- preconditioners -----------------------------------------------------------
 int fem_precond_create(fem_op *op, int kind, fem_precond **out)
 {
@@ -384,6 +400,35 @@ void fem_solver_default_params(fem_solver_params *p)
     p->refinement = 0;
     p->inner_tolerance = 1e-3;
     p->inner_maxiterations = 1000;
+    p->preconditioning = FEM_PRE_NONE;   // parameter.hh:206-217 (default: the first method, none)
+    p->relaxation = 1.1;                 // parameter.hh:219-222
+    p->precond_iterations = 1;           // parameter.hh:224-233
+}
+
+// "true"/"false" as ParameterReader::getValue<bool> reads them (also 1/0, yes/no)
+static int parse_bool(const std::string &k, const std::string &v)
+{
+    if (v == "true" || v == "1" || v == "yes") return 1;
+    if (v == "false" || v == "0" || v == "no") return 0;
+    throw Error(FEM_ERR_ARG, k + ": expected true or false, got " + v);
+}
+
+static double parse_double(const std::string &k, const std::string &v)
+{
+    size_t pos = 0;
+    double d = 0;
+    try { d = std::stod(v, &pos); } catch (...) { pos = 0; }
+    FEM_CHECK(pos == v.size() && pos > 0, FEM_ERR_ARG, k + ": not a number: " + v);
+    return d;
+}
+
+static int parse_int(const std::string &k, const std::string &v)
+{
+    size_t pos = 0;
+    long d = 0;
+    try { d = std::stol(v, &pos); } catch (...) { pos = 0; }
+    FEM_CHECK(pos == v.size() && pos > 0, FEM_ERR_ARG, k + ": not an integer: " + v);
+    return (int)d;
 }
 
 int fem_solver_params_set(fem_solver_params *p, const char *key, const char *value)
@@ -391,29 +436,62 @@ int fem_solver_params_set(fem_solver_params *p, const char *key, const char *val
     return guarded([&] {
         FEM_CHECK(p && key && value, FEM_ERR_ARG, "null argument");
         std::string k(key), v(value);
-        for (const char *pre : { "fem.solver.", "linear.", "fem.", "" }) {
-            if (k.rfind(pre, 0) == 0 && std::strlen(pre)) { k = k.substr(std::strlen(pre)); break; }
-        }
+        // SolverParameter's key prefix (parameter.hh:74-78), then the scheme
+        // prefixes of the linear solver (python/dune/fem/scheme/_schemes.py:40-60)
+        auto strip = [&](const char *pre) {
+            const size_t n = std::strlen(pre);
+            if (k.compare(0, n, pre) == 0) { k = k.substr(n); return true; }
+            return false;
+        };
+        strip("fem.solver.");
+        strip("newton.linear.") || strip("nonlinear.linear.") || strip("linear.");
+        FEM_CHECK(k.compare(0, 10, "nonlinear.") != 0 && k.compare(0, 7, "newton.") != 0, FEM_ERR_ARG,
+                  std::string("a Newton parameter, not a linear-solver one: ") + key);
         if (k == "method") {
             if (v == "cg") p->method = FEM_SOLVER_CG;
             else if (v == "gmres") p->method = FEM_SOLVER_GMRES;
             else if (v == "fgmres") p->method = FEM_SOLVER_FGMRES;
-            else throw Error(FEM_ERR_UNSUPPORTED, "method: " + v);
-        } else if (k == "tolerance") {
-            p->tolerance = std::stod(v);
+            else throw Error(FEM_ERR_UNSUPPORTED, "method: " + v + " (supported: cg, gmres, fgmres)");
+        } else if (k == "tolerance" || k == "absolutetol" || k == "reductiontol") {
+            p->tolerance = parse_double(k, v);
+            FEM_CHECK(p->tolerance >= 0, FEM_ERR_ARG, k + " must be >= 0");
         } else if (k == "errormeasure") {
             if (v == "absolute") p->errormeasure = FEM_ERRMEASURE_ABSOLUTE;
             else if (v == "relative") p->errormeasure = FEM_ERRMEASURE_RELATIVE;
             else if (v == "residualreduction") p->errormeasure = FEM_ERRMEASURE_REDUCTION;
             else throw Error(FEM_ERR_ARG, "errormeasure: " + v);
-        } else if (k == "maxiterations") {
-            p->maxiterations = std::stoi(v);
+        } else if (k == "maxiterations" || k == "maxlineariterations") {
+            p->maxiterations = parse_int(k, v);
+            FEM_CHECK(p->maxiterations >= 0, FEM_ERR_ARG, k + " must be >= 0");
         } else if (k == "gmres.restart") {
-            p->restart = std::stoi(v);
+            p->restart = parse_int(k, v);
         } else if (k == "verbose") {
-            p->verbose = (v == "true" || v == "1") ? 1 : 0;
+            p->verbose = parse_bool(k, v);
         } else if (k == "residualcheck") {
-            p->residual_check = (v == "true" || v == "1") ? 1 : 0;
+            p->residual_check = parse_bool(k, v);
+        } else if (k == "preconditioning.method") {
+            // the reference's names (parameter.hh:48-59; underscores read as dashes)
+            if (v == "none") p->preconditioning = FEM_PRE_NONE;
+            else if (v == "jacobi") p->preconditioning = FEM_PRE_JACOBI;
+            else if (v == "blockjacobi" || v == "block-jacobi") p->preconditioning = FEM_PRE_BLOCK_JACOBI;
+            else if (v == "multigrid" || v == "pmg") p->preconditioning = FEM_PRE_MULTIGRID;
+            else if (v == "sor" || v == "ssor" || v == "gauss-seidel" || v == "ilu" || v == "amg-ilu" ||
+                     v == "amg-jacobi" || v == "ildl" || v == "oas" || v == "icc" || v == "kspoptions")
+                throw Error(FEM_ERR_UNSUPPORTED, "preconditioning.method " + v +
+                                                     ": needs an assembled matrix; the matrix-free path offers none, "
+                                                     "jacobi, blockjacobi and multigrid");
+            else throw Error(FEM_ERR_ARG, "preconditioning.method: unknown value " + v);
+        } else if (k == "preconditioning.relaxation") {
+            p->relaxation = parse_double(k, v);
+        } else if (k == "preconditioning.iterations" || k == "preconditioning.iteration") {
+            p->precond_iterations = parse_int(k, v);
+            FEM_CHECK(p->precond_iterations >= 1, FEM_ERR_ARG, k + " must be >= 1");
+        } else if (k == "preconditioning.level") {
+            FEM_CHECK(parse_int(k, v) == 0, FEM_ERR_UNSUPPORTED, "preconditioning.level: only 0 (no ILU levels)");
+        } else if (k == "threading") {
+            (void)parse_bool(k, v);   // one device stream per context: nothing to switch
+        } else if (k == "knolltrick") {
+            FEM_CHECK(parse_bool(k, v) == 0, FEM_ERR_UNSUPPORTED, "knolltrick: only false");
         } else {
             throw Error(FEM_ERR_ARG, std::string("unknown solver parameter: ") + key);
         }

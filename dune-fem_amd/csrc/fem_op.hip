@@ -237,6 +237,70 @@ void Operator::diagonal(Vec &d) const
     FEM_HIP(hipStreamSynchronize(s));
 }
 
+// ---- element blocks by coloured probing -------------------------------------------
+// The SIPG operator couples an element only with its six face neighbours.
+// With elements coloured by the parity of their grid coordinates (8
+// colours), no two elements of one colour are face neighbours, so applying
+// the operator to the probe "phi_j on every element of colour c" gives, on
+// each element e of colour c, column j of its diagonal block A_ee and
+// nothing else.  8 nb applies of the production kernel yield every block,
+// for every coefficient mode and with advection.
+template <typename T>
+__global__ void k_probe_fill(T *__restrict__ p, int nx, int ny, int64_t E, int nb, int j, int col, int ncol)
+{
+    const int64_t n = E * nb;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t e = i / nb;
+        const int l = (int)(i - e * nb);
+        const int ex = (int)(e % nx), ey = (int)((e / nx) % ny), ez = (int)(e / ((int64_t)nx * ny));
+        const int c = (ex % ncol) + ncol * ((ey % ncol) + ncol * (ez % ncol));
+        p[i] = (l == j && c == col) ? T(1) : T(0);
+    }
+}
+
+template <typename T>
+__global__ void k_probe_diag_extract(const T *__restrict__ w, double *__restrict__ blocks, int nx, int ny, int64_t E,
+                                     int nb, int j, int col)
+{
+    const int64_t n = E * nb;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t e = i / nb;
+        const int r = (int)(i - e * nb);
+        const int ex = (int)(e % nx), ey = (int)((e / nx) % ny), ez = (int)(e / ((int64_t)nx * ny));
+        const int c = (ex & 1) + 2 * ((ey & 1) + 2 * (ez & 1));
+        if (c == col) blocks[(e * nb + r) * nb + j] = (double)w[i];
+    }
+}
+
+void Operator::block_diagonal(Vec &blocks) const
+{
+    FEM_CHECK(is_dg(), FEM_ERR_UNSUPPORTED, "block diagonal: DG operators only");
+    FEM_CHECK(a64.kappa_mode >= 0, FEM_ERR_STATE, "coefficient data not set");
+    const int nb = k * k * k;
+    FEM_CHECK(blocks.prec == FEM_F64 && blocks.n == E * nb * nb, FEM_ERR_ARG,
+              "block diagonal: an FP64 vector of E nb^2 entries");
+    hipStream_t s = ctx->stream;
+    DevBuf pb, wb;
+    const size_t es = vec_prec == FEM_F64 ? 8 : 4;
+    pb.alloc(N * es);
+    wb.alloc(N * es);
+    const unsigned G = grid_for(N, 256, 8192);
+    for (int col = 0; col < 8; ++col)
+        for (int j = 0; j < nb; ++j) {
+            if (vec_prec == FEM_F64) {
+                hipLaunchKernelGGL(k_probe_fill<double>, dim3(G), dim3(256), 0, s, pb.as<double>(), desc.n[0], desc.n[1], E, nb, j, col, 2);
+                apply_raw(pb.p, wb.p, nullptr);
+                hipLaunchKernelGGL(k_probe_diag_extract<double>, dim3(G), dim3(256), 0, s, wb.as<double>(), (double *)blocks.data, desc.n[0], desc.n[1], E, nb, j, col);
+            } else {
+                hipLaunchKernelGGL(k_probe_fill<float>, dim3(G), dim3(256), 0, s, pb.as<float>(), desc.n[0], desc.n[1], E, nb, j, col, 2);
+                apply_raw(pb.p, wb.p, nullptr);
+                hipLaunchKernelGGL(k_probe_diag_extract<float>, dim3(G), dim3(256), 0, s, wb.as<float>(), (double *)blocks.data, desc.n[0], desc.n[1], E, nb, j, col);
+            }
+            FEM_HIP(hipGetLastError());
+        }
+    FEM_HIP(hipStreamSynchronize(s));
+}
+
 void Precond::init(Operator *o, int k_)
 {
     op = o;

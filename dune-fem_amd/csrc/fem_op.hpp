@@ -58,6 +58,9 @@ struct Operator {
     void apply(const Vec &x, Vec &y, const int *skip = nullptr) const;
     void rhs(const double *fq_host, Vec &F) const;
     void diagonal(Vec &d) const;  // FP64 or FP32 output
+    // DG: the element blocks A_ee, [E][nb][nb] row-major in the local DoF
+    // order, FP64 (by coloured probing with the apply kernel)
+    void block_diagonal(Vec &blocks) const;
     bool is_dg() const { return desc.kind == FEM_OP_DG_SIPG_3D; }
 };
 

@@ -479,24 +479,56 @@ Solver::~Solver()
 
 void Solver::bind(Operator *o, Precond *p, Operator *io, Precond *ip)
 {
-    op = o;
-    pre = p;
-    iop = io;
-    ipre = ip;
+    op = iop = nullptr;
+    pre = ipre = nullptr;
+    FEM_CHECK(o, FEM_ERR_ARG, "bind: null operator");
     FEM_CHECK(!p || p->op == o || p->op->N == o->N, FEM_ERR_ARG, "preconditioner does not match the operator");
     FEM_CHECK(!p || p->prec == o->vec_prec, FEM_ERR_ARG, "preconditioner precision does not match the operator");
+    // the learned preconditioner is nonlinear in its argument (the coarse
+    // scaling depends on it): only the flexible GMRES, which stores
+    // z_j = M(v_j), stays a valid Krylov method with it
+    auto flexible_only = [&](const Precond *q) {
+        FEM_CHECK(!q || q->kind != FEM_PRE_LEARNED || prm.method == FEM_SOLVER_FGMRES, FEM_ERR_UNSUPPORTED,
+                  "the learned preconditioner is nonlinear: it needs fem.solver.method = fgmres");
+    };
+    flexible_only(p);
     if (prm.refinement) {
         FEM_CHECK(io, FEM_ERR_ARG, "refinement needs an inner operator");
         FEM_CHECK(io->N == o->N, FEM_ERR_ARG, "inner operator size");
         FEM_CHECK(o->vec_prec == FEM_F64, FEM_ERR_ARG, "refinement: the outer operator must be FP64");
+        // the inner Krylov solve runs in FP32 on FP32 work vectors
+        FEM_CHECK(io->vec_prec == FEM_F32, FEM_ERR_ARG, "refinement: the inner operator must be FP32 or BF16-MFMA");
         FEM_CHECK(!ip || ip->prec == io->vec_prec, FEM_ERR_ARG, "inner preconditioner precision");
+        FEM_CHECK(!ip || ip->op->N == io->N, FEM_ERR_ARG, "inner preconditioner size");
+        flexible_only(ip);
     }
+    // fem.solver.preconditioning.method: built here when no preconditioner
+    // object is passed (KrylovInverseOperator::bind creates its own the same
+    // way, krylovinverseoperators.hh:86-97)
+    own_pre.reset();
+    own_ipre.reset();
+    if (!p && prm.preconditioning != FEM_PRE_NONE && !prm.refinement) {
+        own_pre.reset(new Precond());
+        own_pre->init(o, prm.preconditioning);
+        p = own_pre.get();
+    }
+    if (prm.refinement && !ip && prm.preconditioning != FEM_PRE_NONE) {
+        own_ipre.reset(new Precond());
+        own_ipre->init(io, prm.preconditioning);
+        ip = own_ipre.get();
+    }
+    op = o;
+    pre = p;
+    iop = io;
+    ipre = ip;
 }
 
 void Solver::unbind()
 {
     op = iop = nullptr;
     pre = ipre = nullptr;
+    own_pre.reset();
+    own_ipre.reset();
     for (DevBuf *b : work) delete b;
     work.clear();
 }
@@ -685,6 +717,15 @@ void Solver::solve(const Vec &b, Vec &x, fem_solver_info &info)
     FEM_CHECK(b.prec == op->vec_prec && x.prec == op->vec_prec, FEM_ERR_ARG, "solve: vector precision");
     std::memset(&info, 0, sizeof(info));
     fallback_used = 0;
+    // a stall fallback switches the learned preconditioner off for the rest
+    // of this solve only: its mode is restored on return, so the next solve
+    // runs the network again (and reports its own fallback)
+    struct ModeGuard {
+        Learned *l;
+        int mode;
+        ~ModeGuard() { if (l) l->mode = mode; }
+    } g_outer{ pre && pre->learned ? pre->learned : nullptr, pre && pre->learned ? pre->learned->mode : 0 },
+        g_inner{ ipre && ipre->learned ? ipre->learned : nullptr, ipre && ipre->learned ? ipre->learned->mode : 0 };
     const int64_t N = op->N;
     FEM_HIP(hipStreamSynchronize(ctx->stream));
     auto t0 = std::chrono::steady_clock::now();

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <functional>
+#include <memory>
 
 #include "fem_op.hpp"
 
@@ -30,6 +31,9 @@ struct Solver {
     fem_solver_params prm{};
     Operator *op = nullptr, *iop = nullptr;
     Precond *pre = nullptr, *ipre = nullptr;
+    // preconditioners built at bind from fem.solver.preconditioning.method
+    // when the caller passes none
+    std::unique_ptr<Precond> own_pre, own_ipre;
     DevBuf state, partials, counters;
     KState *hstate = nullptr;   // pinned mirror
     std::vector<DevBuf *> work;

@@ -27,7 +27,7 @@ LIBPATH = os.path.join(_HERE, "libdunefem_amd.so")
 F64, F32, BF16MFMA = 0, 1, 2
 OP_DG_SIPG_3D, OP_CG_Q2_2D = 1, 2
 KAPPA_CONST, KAPPA_QP, KAPPA_CELL = 0, 1, 2
-PRE_NONE, PRE_JACOBI, PRE_BLOCK_JACOBI, PRE_LEARNED = 0, 1, 2, 3
+PRE_NONE, PRE_JACOBI, PRE_BLOCK_JACOBI, PRE_LEARNED, PRE_MULTIGRID = 0, 1, 2, 3, 4
 LEARNED_OFF, LEARNED_NET, LEARNED_COARSE_ONLY = 0, 1, 2
 CG, GMRES, FGMRES = 0, 1, 2
 ABSOLUTE, RELATIVE, REDUCTION = 0, 1, 2
@@ -38,9 +38,9 @@ EXPORTS = [
     "fem_ctx_mark", "fem_ctx_elapsed_ms", "fem_vec_create", "fem_vec_wrap", "fem_vec_destroy", "fem_vec_size",
     "fem_vec_precision", "fem_vec_device_ptr", "fem_vec_upload", "fem_vec_download",
     "fem_vec_upload_f32", "fem_vec_fill", "fem_vec_copy", "fem_vec_axpy", "fem_vec_scal",
-    "fem_vec_dot", "fem_vec_norm2sq", "fem_op_create", "fem_op_destroy", "fem_op_ndofs",
+    "fem_vec_dot", "fem_vec_dot_noalloc", "fem_vec_norm2sq", "fem_op_create", "fem_op_destroy", "fem_op_ndofs",
     "fem_op_set_kappa_qp", "fem_op_set_kappa_cell", "fem_op_apply", "fem_op_assemble_rhs",
-    "fem_op_diagonal", "fem_precond_create", "fem_precond_destroy", "fem_precond_apply",
+    "fem_op_diagonal", "fem_op_block_diagonal", "fem_precond_create", "fem_precond_destroy", "fem_precond_apply",
     "fem_solver_default_params", "fem_solver_params_set", "fem_solver_create",
     "fem_solver_destroy", "fem_solver_bind", "fem_solver_unbind", "fem_solver_solve",
     "fem_gauss_points", "fem_batch_create", "fem_batch_destroy", "fem_batch_ndofs",
@@ -67,7 +67,8 @@ class SolverParams(C.Structure):
     _fields_ = [("method", C.c_int), ("tolerance", C.c_double), ("errormeasure", C.c_int),
                 ("maxiterations", C.c_int), ("restart", C.c_int), ("verbose", C.c_int),
                 ("refinement", C.c_int), ("inner_tolerance", C.c_double),
-                ("inner_maxiterations", C.c_int), ("residual_check", C.c_int)]
+                ("inner_maxiterations", C.c_int), ("residual_check", C.c_int),
+                ("preconditioning", C.c_int), ("relaxation", C.c_double), ("precond_iterations", C.c_int)]
 
 
 class BatchInfo(C.Structure):
@@ -117,6 +118,7 @@ def load_library(path=LIBPATH):
         "fem_vec_axpy": ([_vp, C.c_double, _vp], C.c_int),
         "fem_vec_scal": ([_vp, C.c_double], C.c_int),
         "fem_vec_dot": ([_vp, _vp, _dp], C.c_int),
+        "fem_vec_dot_noalloc": ([_vp, _vp, _dp], C.c_int),
         "fem_vec_norm2sq": ([_vp, _dp], C.c_int),
         "fem_op_create": ([_vp, C.POINTER(OpDesc), C.POINTER(_vp)], C.c_int),
         "fem_op_destroy": ([_vp], C.c_int),
@@ -126,6 +128,7 @@ def load_library(path=LIBPATH):
         "fem_op_apply": ([_vp, _vp, _vp], C.c_int),
         "fem_op_assemble_rhs": ([_vp, _dp, _vp], C.c_int),
         "fem_op_diagonal": ([_vp, _vp], C.c_int),
+        "fem_op_block_diagonal": ([_vp, _vp], C.c_int),
         "fem_precond_create": ([_vp, C.c_int, C.POINTER(_vp)], C.c_int),
         "fem_precond_destroy": ([_vp], C.c_int),
         "fem_precond_apply": ([_vp, _vp, _vp], C.c_int),
@@ -346,6 +349,14 @@ class GalerkinOperator:
         _check(_lib.fem_op_diagonal(self.h, d.h))
         return d
 
+    def block_diagonal(self):
+        """Element blocks [E][nb][nb] (FP64, host array) of a DG operator."""
+        nb = (self.desc.order + 1) ** 3
+        E = self.N // nb
+        B = DofVector(self.ctx, E * nb * nb, F64)
+        _check(_lib.fem_op_block_diagonal(self.h, B.h))
+        return B.download().reshape(E, nb, nb)
+
     def nonlinear(self):
         return False
 
@@ -382,9 +393,9 @@ def solver_parameters(**kw):
     load_library().fem_solver_default_params(C.byref(p))
     for k, v in kw.items():
         if k in ("method", "errormeasure", "maxiterations", "restart", "verbose", "refinement",
-                 "inner_maxiterations", "residual_check"):
+                 "inner_maxiterations", "residual_check", "preconditioning", "precond_iterations"):
             setattr(p, k, int(v))
-        elif k in ("tolerance", "inner_tolerance"):
+        elif k in ("tolerance", "inner_tolerance", "relaxation"):
             setattr(p, k, float(v))
         else:
             _check(_lib.fem_solver_params_set(C.byref(p), k.encode(), str(v).encode()))

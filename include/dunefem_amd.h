@@ -83,6 +83,13 @@ int fem_vec_axpy(fem_vec *y, double a, const fem_vec *x); /* y += a x         */
 int fem_vec_scal(fem_vec *x, double a);                   /* x *= a           */
 int fem_vec_dot(const fem_vec *x, const fem_vec *y, double *out);
 int fem_vec_norm2sq(const fem_vec *x, double *out);
+/* fem_vec_dot with nothing allocated: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The reduction
+ * scratch is a fixed 4,352-byte device array per GPU, independent of the
+ * vector length (every length is accepted), held until the process ends.
+ * All calls on one GPU share it: calls that overlap on one GPU race. */
+int fem_vec_dot_noalloc(const fem_vec *x, const fem_vec *y, double *out);
 
 /* ---- operators ---------------------------------------------------------------
  * Matrix-free Galerkin operator on a structured Cartesian mesh of the unit
@@ -135,12 +142,21 @@ int fem_op_assemble_rhs(fem_op *op, const double *fq, fem_vec *F);
  * and SparseRowMatrix::extractDiagonal, dune/fem/operator/matrix/spmatrix.hh:846,
  * used by FemPreconditioning, dune/fem/solver/fempreconditioning.hh:85-107) */
 int fem_op_diagonal(fem_op *op, fem_vec *d);
+/* DG operators: the element (diagonal) blocks of the linear operator, the
+ * local matrices that DifferentiableGalerkinOperator::jacobian assembles on
+ * the diagonal (dune/fem/schemes/galerkin.hh:1548-1551, 1195-1314) and that
+ * block-Jacobi preconditioning inverts.  `blocks` is an FP64 vector of
+ * E * nb * nb entries: block of element e at e * nb^2, row-major, rows and
+ * columns in the local DoF order.  Every coefficient mode and advection are
+ * supported (computed by probing the apply). */
+int fem_op_block_diagonal(fem_op *op, fem_vec *blocks);
 
 /* ---- preconditioners -------------------------------------------------------- */
 #define FEM_PRE_NONE 0
 #define FEM_PRE_JACOBI 1        /* d^-1 with |d| < 16 eps -> 1 (fempreconditioning.hh:99-106) */
 #define FEM_PRE_BLOCK_JACOBI 2  /* DG element blocks A_KK^-1                        */
 #define FEM_PRE_LEARNED 3       /* CNN coarse correction + block Jacobi (FGMRES only) */
+#define FEM_PRE_MULTIGRID 4     /* DG: p-multigrid to the element means, then h-multigrid (V-cycle) */
 
 int fem_precond_create(fem_op *op, int kind, fem_precond **out);
 int fem_precond_destroy(fem_precond *p);
@@ -178,6 +194,18 @@ typedef struct {
      * (dune/fem/solver/linear/cg.hh:62-70); 1 = on the unpreconditioned
      * residual ||r||_2^2 <= eps^2 ref (GMRES always stops on ||r||_2) */
     int residual_check;
+    /* fem.solver.preconditioning.method (dune/fem/solver/parameter.hh:206-217):
+     * the preconditioner the solver builds for itself at bind when no
+     * preconditioner object is passed (and for the inner operator of a
+     * refinement solve when no inner one is passed): FEM_PRE_NONE,
+     * FEM_PRE_JACOBI, FEM_PRE_BLOCK_JACOBI or FEM_PRE_MULTIGRID */
+    int preconditioning;
+    /* fem.solver.preconditioning.relaxation (parameter.hh:219-222, default
+     * 1.1) and .iterations (:224-233, default 1); as in the reference's
+     * FemPreconditioning<jacobi> (fempreconditioning.hh:178-180) the Jacobi
+     * preconditioner applies d^-1 once and does not use them */
+    double relaxation;
+    int precond_iterations;
 } fem_solver_params;
 
 typedef struct {
@@ -192,11 +220,27 @@ typedef struct {
 
 void fem_solver_default_params(fem_solver_params *p);
 /* key/value setter mirroring the fem.solver.* keys of SolverParameter
- * (dune/fem/solver/parameter.hh:71-250): "fem.solver.method" (cg|gmres|fgmres),
- * "fem.solver.tolerance", "fem.solver.errormeasure" (absolute|relative|
- * residualreduction), "fem.solver.maxiterations", "fem.solver.gmres.restart",
- * "fem.solver.verbose", "fem.solver.residualcheck" (true|false); the "linear." prefix
- * is accepted as well */
+ * (dune/fem/solver/parameter.hh:71-250).  Accepted (prefix "fem.solver.",
+ * optionally followed by "linear.", "newton.linear." or "nonlinear.linear.",
+ * or one of those prefixes alone, as scheme parameters are written):
+ *   method                 cg | gmres | fgmres              (parameter.hh:170-195)
+ *   tolerance, absolutetol, reductiontol   (:118-146; the last two deprecated)
+ *   errormeasure           absolute | relative | residualreduction (:108-116)
+ *   maxiterations, maxlineariterations (deprecated)           (:148-162)
+ *   gmres.restart                                            (:197-201)
+ *   verbose                true | false                      (:90-97)
+ *   preconditioning.method none | jacobi | blockjacobi | multigrid (:206-217;
+ *                          the reference's sor, ssor, gauss-seidel, ilu, amg-*,
+ *                          ildl, oas and icc need an assembled matrix and are
+ *                          rejected with FEM_ERR_UNSUPPORTED)
+ *   preconditioning.relaxation, preconditioning.iterations,
+ *   preconditioning.iteration (deprecated)                   (:219-233)
+ *   preconditioning.level  0 only                            (:235-238)
+ *   threading              true | false (accepted; one stream per context)
+ *   knolltrick             false only                        (:245-248)
+ *   residualcheck          true | false (this library: CG stops on ||r||)
+ * Newton keys (nonlinear.*, newton.* without linear.) are not linear-solver
+ * keys: FEM_ERR_ARG. */
 int fem_solver_params_set(fem_solver_params *p, const char *key, const char *value);
 int fem_solver_create(fem_ctx *ctx, const fem_solver_params *params, fem_solver **out);
 int fem_solver_destroy(fem_solver *s);

@@ -428,7 +428,7 @@ void ora_dg_block_diag(const ora_dg_t *op, double *blocks)
                 int c0 = (int)(e % nx), c1 = (int)((e / nx) % ny), c2 = (int)(e / (nx * ny));
                 if ((c0 & 1) == cx && (c1 & 1) == cy && (c2 & 1) == cz) u[e * nb + j] = 1.0;
             }
-            ora_dg_apply(op, u, w, 1);
+            ora_dg_apply(op, u, w, omp_get_max_threads());
             for (int64_t e = 0; e < E; ++e) {
                 int c0 = (int)(e % nx), c1 = (int)((e / nx) % ny), c2 = (int)(e / (nx * ny));
                 if ((c0 & 1) == cx && (c1 & 1) == cy && (c2 & 1) == cz)

@@ -155,8 +155,8 @@ class DGOperator:
         lib().ora_dg_block_diag(C.byref(self.s), _ptr(B))
         return B
 
-    def callback(self):
-        f = _APPLY(lambda ctx, x, y: lib().ora_dg_apply(C.byref(self.s), x, y, 1))
+    def callback(self, nthreads=1):
+        f = _APPLY(lambda ctx, x, y: lib().ora_dg_apply(C.byref(self.s), x, y, nthreads))
         self._keep.append(f)
         return f
 
@@ -237,10 +237,10 @@ def block_jacobi(inv_blocks):
     return p
 
 
-def _solve(kind, op, b, x0=None, pre=None, tol=1e-10, maxit=1000, criteria=1, restart=20):
+def _solve(kind, op, b, x0=None, pre=None, tol=1e-10, maxit=1000, criteria=1, restart=20, nthreads=1):
     b = np.ascontiguousarray(b, dtype=np.float64)
     x = np.zeros_like(b) if x0 is None else np.array(x0, dtype=np.float64)
-    cb = op.callback()
+    cb = op.callback(nthreads) if isinstance(op, DGOperator) else op.callback()
     pcb = pre.cb if pre is not None else None
     L = lib()
     if kind == "cg":

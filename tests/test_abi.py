@@ -69,3 +69,41 @@ def test_workload_layouts_match_direct_evaluation():
                 for q in range(m * m):
                     qx, qz = q % m, q // m
                     assert abs(F[1][f, q] - g((ix + xg[qx]) / 2, fy / 3, (iz + xg[qz]) / 2)) < 1e-14
+
+
+def test_every_reference_solver_key_is_accepted():
+    # the linear-solver keys of the reference's parameter file
+    # (dune/fem/solver/test/parameter) and Python scheme parameters
+    # (pydemo/testthreading.py:70-71, pydemo/advectiondiffusion.py:69-79)
+    keys = {
+        "fem.solver.verbose": "false", "fem.solver.method": "gmres", "fem.solver.tolerance": "1e-15",
+        "fem.solver.errormeasure": "absolute", "fem.solver.gmres.restart": "15",
+        "fem.solver.maxiterations": "100", "fem.solver.linear.verbose": "false",
+        "fem.solver.linear.method": "gmres", "fem.solver.linear.tolerance": "1e-12",
+        "fem.solver.linear.gmres.restart": "15", "fem.solver.linear.maxiterations": "100",
+        "newton.linear.preconditioning.method": "jacobi", "newton.linear.tolerance": "1e-13",
+        "linear.verbose": "False".lower(), "linear.tolerance": "1e-5",
+        "fem.solver.preconditioning.relaxation": "1.1", "fem.solver.preconditioning.iterations": "1",
+        "fem.solver.preconditioning.level": "0", "fem.solver.threading": "true",
+        "fem.solver.knolltrick": "false", "fem.solver.maxlineariterations": "1000",
+        "fem.solver.absolutetol": "1e-11", "fem.solver.reductiontol": "1e-11",
+    }
+    p = fa.solver_parameters(**keys)
+    assert p.method == fa.GMRES and p.restart == 15 and p.maxiterations == 1000
+    assert p.preconditioning == fa.PRE_JACOBI and p.tolerance == 1e-11 and p.verbose == 0
+    for v, kind in (("none", fa.PRE_NONE), ("jacobi", fa.PRE_JACOBI), ("blockjacobi", fa.PRE_BLOCK_JACOBI),
+                    ("multigrid", fa.PRE_MULTIGRID)):
+        assert fa.solver_parameters(**{"fem.solver.preconditioning.method": v}).preconditioning == kind
+    # the reference's assembled-matrix preconditioners are refused by name,
+    # Newton keys and malformed values are argument errors
+    import pytest
+    for v in ("ilu", "sor", "ssor", "gauss-seidel", "amg-jacobi", "ildl", "oas", "icc"):
+        with pytest.raises(fa.FemError, match="assembled matrix"):
+            fa.solver_parameters(**{"fem.solver.preconditioning.method": v})
+    for k, v in (("fem.solver.nonlinear.tolerance", "1e-8"), ("nonlinear.verbose", "true"),
+                 ("fem.solver.tolerance", "abc"), ("fem.solver.maxiterations", "1.5"),
+                 ("fem.solver.verbose", "maybe"), ("fem.solver.preconditioning.method", "bogus")):
+        with pytest.raises(fa.FemError):
+            fa.solver_parameters(**{k: v})
+    d = fa.solver_parameters()
+    assert d.preconditioning == fa.PRE_NONE and d.relaxation == 1.1 and d.precond_iterations == 1

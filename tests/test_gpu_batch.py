@@ -72,3 +72,57 @@ def test_problem2_generator_batch_subset(ctx):
     X = B.vector()
     info = B.solve(F, X, tol=1e-8)
     assert info["converged"] == 8 and info["max_true_rel_residual"] <= 1e-8
+
+
+def test_batch_graph_replays_after_resolve_and_new_coefficient(ctx):
+    # the captured inner step holds buffer addresses: a second graph solve and
+    # a solve after new coefficient data must both run on live, current data
+    sides, kv, fq = small_batch(seed=5, count=6)
+    B = fa.Batch(ctx, sides, c=0.0)
+    F_host = np.concatenate(fq)
+
+    def check(kvs, X, F):
+        x, Fh = X.download(), F.download()
+        for i, s in enumerate(sides):
+            lo, hi = B.offsets[i], B.offsets[i + 1]
+            op = ora.CGOperator(s, c=0.0, kappa=kvs[i])
+            r = Fh[lo:hi] - op.apply(x[lo:hi])
+            assert np.linalg.norm(r) <= 1.01e-8 * np.linalg.norm(Fh[lo:hi]), i
+
+    B.set_kappa_qp(np.concatenate(kv))
+    F = B.rhs(F_host)
+    for rep in range(2):
+        X = B.vector()
+        info = B.solve(F, X, tol=1e-8, inner_tol=1e-3, use_graph=True)
+        assert info["converged"] == len(sides), rep
+        check(kv, X, F)
+    rng = np.random.default_rng(11)
+    kv2 = [np.exp(rng.normal(0, 1.5, size=k.shape)) for k in kv]
+    B.set_kappa_qp(np.concatenate(kv2))
+    X = B.vector()
+    info = B.solve(F, X, tol=1e-8, inner_tol=1e-3, use_graph=True)
+    assert info["converged"] == len(sides)
+    check(kv2, X, F)
+
+
+def test_problem2_full_config_256_instances(ctx):
+    # the batched workload at its configured size: 256 instances, sides
+    # U{32..256}, log-normal kappa (sigma 1, correlation length 0.1); the
+    # FP64 true residual of a sample of instances is recomputed on the host
+    # by the oracle's operator
+    inst = wl.problem2_instances(count=256)
+    B = fa.Batch(ctx, [d["n"] for d in inst], c=0.0)
+    B.set_kappa_qp(np.concatenate([d["kvol"] for d in inst]))
+    B.autotune()
+    F = B.rhs(np.concatenate([d["fq"] for d in inst]))
+    X = B.vector()
+    info = B.solve(F, X, tol=1e-8)
+    print("C2 full batch:", {k: v for k, v in info.items() if k not in ("iterations", "rel_residual")}, flush=True)
+    assert info["converged"] == 256 and info["max_true_rel_residual"] <= 1e-8
+    x, Fh = X.download(), F.download()
+    sample = list(range(0, 256, 37)) + [int(np.argmax([d["n"][0] * d["n"][1] for d in inst]))]
+    for i in sample:
+        lo, hi = B.offsets[i], B.offsets[i + 1]
+        op = ora.CGOperator(inst[i]["n"], c=0.0, kappa=inst[i]["kvol"])
+        r = Fh[lo:hi] - op.apply(x[lo:hi])
+        assert np.linalg.norm(r) <= 1.01e-8 * np.linalg.norm(Fh[lo:hi]), i

@@ -33,14 +33,32 @@ def test_cnn_forward_and_gradients_match_oracle(ctx, n, dil):
     loss_o, grads_o, yo = cnn.loss_and_grads(x.astype(np.float64), t, Ws, list(dil), bf=True)
     ef = relf(y, yo.ravel())
     print("forward rel err", n, ef, flush=True)
-    assert ef < 2e-2
+    # the oracle rounds to BF16 where the kernels do; what remains is FP32
+    # accumulation order (measured about 1e-4 forward, 5e-4 gradients)
+    assert ef < 1e-3
     loss, g = net.loss_grad(x.reshape(-1, 2), t.ravel())
-    assert abs(loss - loss_o) < 2e-2 * abs(loss_o)
+    assert abs(loss - loss_o) < 1e-3 * abs(loss_o)
     gs = net.split(g)
     for l in range(4):
         eg = relf(gs[l], grads_o[l])
         print("layer", l, "grad rel err", eg, flush=True)
-        assert eg < 5e-2
+        assert eg < 2e-3
+
+
+def test_cnn_forward_matches_oracle_on_the_48cubed_grid(ctx):
+    # the element grid of the learned-preconditioner workload (48^3 Q2)
+    n, dil = (48, 48, 48), (1, 2, 4, 1)
+    rng = np.random.default_rng(2)
+    net = fa.CNN(ctx, n, dil)
+    w = (rng.normal(size=net.nparams) * 0.2).astype(np.float32)
+    net.set_weights(w)
+    Ws = [x.astype(np.float64) for x in net.split(w)]
+    x = grid_input(rng, n)
+    y = net.forward(x.reshape(-1, 2))
+    yo = cnn.forward(x.astype(np.float64), Ws, list(dil), bf=True)[-1]
+    ef = relf(y, yo.ravel())
+    print("48^3 forward rel err", ef, flush=True)
+    assert ef < 1e-3
 
 
 def test_cnn_training_reduces_loss_on_a_linear_target(ctx):

@@ -434,4 +434,4 @@ def test_block_diag_matches_assembled_blocks():
     B = op.block_diag()
     nb = (p + 1) ** 3
     for e in range(12):
-        assert np.allclose(B[e], A[e * nb:(e + 1) * nb, e * nb:(e + 1) * nb], rtol=0, atol=1e-13)
+        assert np.allclose(B[e], A[e * nb:(e + 1) * nb, e * nb:(e + 1) * nb], rtol=0, atol=1e-14 * np.abs(A).max())
