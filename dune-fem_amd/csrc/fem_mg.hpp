@@ -1,0 +1,47 @@
+// Multigrid preconditioner for SIPG DG operators with cellwise or constant
+// kappa and no advection: p-levels, a conforming trilinear vertex level and
+// geometric vertex h-levels (one symmetric V-cycle per application).
+#pragma once
+
+#include <memory>
+
+#include "fem_op.hpp"
+
+namespace femamd {
+
+// one DG level: Q_p on the fine mesh, its exact element-block inverse
+struct MGDGLevel {
+    std::unique_ptr<Operator> own;   // coarser DG levels own their operator
+    Operator *op = nullptr;          // level 0: the preconditioner's operator
+    Precond bj;
+    int k = 0;                       // points/modes per direction, p + 1
+    DevBuf x, r, t;                  // correction, residual, A x
+};
+
+// one conforming trilinear level on a vertex lattice of nv[0] x nv[1] x nv[2]
+// vertices: 27-point stencil [V][27], offset s = (dx+1) + 3 (dy+1) + 9 (dz+1)
+struct MGVertexLevel {
+    int nv[3] = { 0, 0, 0 };
+    int64_t V = 0;
+    DevBuf A, dinv, x, r, t;
+};
+
+struct Multigrid {
+    Operator *op = nullptr;
+    int prec = FEM_F64;
+    std::vector<std::unique_ptr<MGDGLevel>> dg;
+    std::vector<std::unique_ptr<MGVertexLevel>> vx;
+    DevBuf coarse_inv;               // dense inverse of the coarsest vertex level
+    int nu_dg = 2, nu_vertex = 1;    // smoothing sweeps before and after the coarse correction
+    double omega_dg = 0.8, omega_vertex = 0.7;
+
+    void init(Operator *fine);
+    // z = V-cycle(r); `skip` (device flag, may be null) skips all work
+    void apply_raw(const void *r, void *z, const int *skip);
+
+private:
+    template <typename T> void cycle_dg(size_t l, const T *r, T *x, const int *skip);
+    template <typename T> void cycle_vertex(size_t l, const T *r, T *x, const int *skip);
+};
+
+}  // namespace femamd

@@ -4,6 +4,7 @@
 
 #include "fem_blas.hpp"
 #include "fem_learned.hpp"
+#include "fem_mg.hpp"
 #include "fem_op.hpp"
 
 namespace femamd {
@@ -345,13 +346,20 @@ void Precond::init(Operator *o, int k_)
             vec_copy(a, b);
         }
         FEM_HIP(hipStreamSynchronize(s));
+    } else if (kind == FEM_PRE_MULTIGRID) {
+        mg = new Multigrid();
+        mg->init(o);
     } else if (kind == FEM_PRE_NONE) {
     } else {
         throw Error(FEM_ERR_UNSUPPORTED, "preconditioner kind (the learned one is made by fem_precond_create_learned)");
     }
 }
 
-Precond::~Precond() { delete learned; }
+Precond::~Precond()
+{
+    delete learned;
+    delete mg;
+}
 
 void Precond::apply_raw(const void *x, void *y, const int *skip, double *dot) const
 {
@@ -360,6 +368,8 @@ void Precond::apply_raw(const void *x, void *y, const int *skip, double *dot) co
     FEM_CHECK(!dot || fused_dot(), FEM_ERR_UNSUPPORTED, "fused dot: block Jacobi only");
     if (kind == FEM_PRE_LEARNED) {
         learned->apply_raw(x, y, skip);
+    } else if (kind == FEM_PRE_MULTIGRID) {
+        mg->apply_raw(x, y, skip);
     } else if (kind == FEM_PRE_JACOBI) {
         if (prec == FEM_F64)
             hipLaunchKernelGGL(k_pointwise_mul<double>, dim3(grid_for(N, 256, 8192)), dim3(256), 0, s, dinv.as<double>(), (const double *)x, (double *)y, N, skip);

@@ -65,6 +65,7 @@ struct Operator {
 };
 
 struct Learned;
+struct Multigrid;
 
 struct Precond {
     Operator *op = nullptr;
@@ -73,6 +74,7 @@ struct Precond {
     DevBuf dinv;       // Jacobi
     DevBuf Q, lam;     // block Jacobi (tensor-product eigen-decomposition)
     Learned *learned = nullptr;   // FEM_PRE_LEARNED (owned)
+    Multigrid *mg = nullptr;      // FEM_PRE_MULTIGRID (owned)
     void init(Operator *o, int kind);
     // `dot` (block Jacobi only, may be null): per-workgroup FP64 partials of
     // <x, y> and then of <x, x>, op->dot_blocks() entries each

@@ -224,6 +224,16 @@ def jacobi(dinv):
     return p
 
 
+def function_pre(fn, n):
+    """Preconditioner y = fn(x) of any function on numpy arrays of n entries
+    (as a callback)."""
+    def cb(x, y):
+        xa = np.ctypeslib.as_array(x, shape=(n,))
+        ya = np.ctypeslib.as_array(y, shape=(n,))
+        ya[:] = fn(xa.copy())
+    return _Pre(cb)
+
+
 def block_jacobi(inv_blocks):
     inv = np.ascontiguousarray(inv_blocks, dtype=np.float64)
     E, nb, _ = inv.shape

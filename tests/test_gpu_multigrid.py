@@ -1,0 +1,118 @@
+"""The multigrid preconditioner (FEM_PRE_MULTIGRID) against its CPU
+restatement (oracle/mg_oracle.py): one V-cycle M r on the same residual, in
+FP64 to rounding and in FP32 to single precision; symmetry; and PCG with it
+on the channel and log-normal workloads against the oracle's PCG."""
+import numpy as np
+import pytest
+
+import dunefem_amd as fa
+import mg_oracle as mgo
+import pyoracle as ora
+from dunefem_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def dg(ctx, n, p, beta, kappa, prec=fa.F64):
+    if isinstance(kappa, tuple):
+        op = fa.GalerkinOperator(ctx, fa.OP_DG_SIPG_3D, n, p, beta, (0, 0, 0), 0.0, fa.KAPPA_CELL, precision=prec)
+        op.set_kappa_cell(kappa[1])
+    else:
+        op = fa.GalerkinOperator(ctx, fa.OP_DG_SIPG_3D, n, p, beta, (0, 0, 0), 0.0, fa.KAPPA_CONST, kappa,
+                                 precision=prec)
+    return op
+
+
+CASES = [
+    # n, p, kappa kind: a single vertex level (dense), one and two h-levels,
+    # odd vertex counts (non-uniform last coarse cell), every DG order
+    ((3, 2, 4), 1, "cell"),
+    ((4, 3, 5), 2, "cell"),
+    ((5, 4, 3), 3, "const"),
+    ((4, 4, 3), 4, "cell"),
+    ((9, 7, 8), 2, "cell"),
+    ((16, 12, 10), 1, "cell"),
+    ((20, 20, 20), 1, "cell"),
+]
+
+
+@pytest.mark.parametrize("n,p,kind", CASES)
+def test_multigrid_vcycle_matches_oracle(ctx, n, p, kind):
+    rng = np.random.default_rng(hash((n, p)) % 2 ** 32)
+    E = n[0] * n[1] * n[2]
+    kappa = ("cell", 10.0 ** (2 * rng.integers(0, 2, size=E) + rng.uniform(-0.3, 0.3, size=E))) if kind == "cell" \
+        else 1.7
+    beta = 10.0 * p * p
+    M = mgo.MGOracle(n, p, beta, kappa)
+    op = dg(ctx, n, p, beta, kappa)
+    pre = fa.Preconditioner(op, fa.PRE_MULTIGRID)
+    r = rng.normal(size=op.N)
+    zref = M.apply(r)
+    R, Z = op.vector(r), op.vector()
+    Z.fill(np.nan)
+    pre(R, Z)
+    z = Z.download()
+    err = np.abs(z - zref).max() / np.abs(zref).max()
+    print("multigrid", n, p, kind, "vertex levels", [L["A"].shape[0] for L in M.vx], "rel err", err, flush=True)
+    assert err < 1e-12
+    # FP32 preconditioner on an FP32 operator
+    op32 = dg(ctx, n, p, beta, kappa, fa.F32)
+    pre32 = fa.Preconditioner(op32, fa.PRE_MULTIGRID)
+    R32, Z32 = op32.vector(r), op32.vector()
+    pre32(R32, Z32)
+    e32 = np.linalg.norm(Z32.download() - zref) / np.linalg.norm(zref)
+    print("  fp32 rel l2 err", e32, flush=True)
+    assert e32 < 1e-4
+
+
+def test_multigrid_is_symmetric_positive(ctx):
+    P = wl.problem3(n=12)
+    op = dg(ctx, P["n"], P["p"], P["beta"], ("cell", P["kcell"]))
+    pre = fa.Preconditioner(op, fa.PRE_MULTIGRID)
+    rng = np.random.default_rng(2)
+    R1, R2, Z1, Z2 = op.vector(rng.normal(size=op.N)), op.vector(rng.normal(size=op.N)), op.vector(), op.vector()
+    pre(R1, Z1)
+    pre(R2, Z2)
+    a, b = Z1.scalarProductDofs(R2), R1.scalarProductDofs(Z2)
+    assert abs(a - b) <= 1e-12 * abs(a)
+    assert Z1.scalarProductDofs(R1) > 0
+
+
+def test_multigrid_pcg_channel_problem_matches_oracle(ctx):
+    # the channel workload (kappa = 10^(4H)) at 12^3: GPU CG + multigrid against
+    # the oracle's CG with the CPU V-cycle, same stopping rule
+    P = wl.problem3(n=12)
+    kappa = ("cell", P["kcell"])
+    oop = ora.DGOperator(P["n"], P["p"], P["beta"], kappa=kappa)
+    M = mgo.MGOracle(P["n"], P["p"], P["beta"], kappa)
+    F = oop.rhs(P["fq"])
+    xo, ito = ora.cg(oop, F, pre=ora.function_pre(M.apply, oop.N), tol=1e-8, maxit=500, criteria=1, nthreads=8)
+    op = dg(ctx, P["n"], P["p"], P["beta"], kappa)
+    inv = fa.InverseOperator(ctx, method=fa.CG, tolerance=1e-8, errormeasure=fa.RELATIVE, maxiterations=500)
+    inv.bind(op, fa.Preconditioner(op, fa.PRE_MULTIGRID))
+    X = op.vector()
+    inv(op.vector(F), X)
+    info = inv.info()
+    print("12^3 channel CG+MG: gpu its", info["iterations"], "oracle its", ito, "true rel res",
+          info["true_rel_residual"], flush=True)
+    assert info["converged"] and abs(info["iterations"] - ito) <= 1
+    assert np.abs(X.download() - xo).max() <= 1e-6 * np.abs(xo).max()
+
+
+@pytest.mark.parametrize("prob", ["problem3", "problem4"])
+def test_multigrid_solves_full_size_workloads(ctx, prob):
+    # the configured sizes: 48^3 Q2 channels (2.99 M DoFs) and 96^3 Q4
+    # log-normal (110.6 M DoFs); plain FP64 CG with the multigrid
+    # preconditioner to a 1e-8 FP64 true residual
+    P = getattr(wl, prob)()
+    op = dg(ctx, P["n"], P["p"], P["beta"], ("cell", P["kcell"]))
+    F = op.rhs(P["fq"])
+    del P["fq"]
+    inv = fa.InverseOperator(ctx, method=fa.CG, tolerance=1e-8, errormeasure=fa.RELATIVE, maxiterations=1000,
+                             residual_check=1)
+    inv.bind(op, fa.Preconditioner(op, fa.PRE_MULTIGRID))
+    X = op.vector()
+    inv(F, X)
+    info = inv.info()
+    print(prob, "full size CG+MG:", info, flush=True)
+    assert info["converged"] and info["true_rel_residual"] < 1.01e-8

@@ -435,3 +435,64 @@ def test_block_diag_matches_assembled_blocks():
     nb = (p + 1) ** 3
     for e in range(12):
         assert np.allclose(B[e], A[e * nb:(e + 1) * nb, e * nb:(e + 1) * nb], rtol=0, atol=1e-14 * np.abs(A).max())
+
+
+# ---- multigrid oracle (oracle/mg_oracle.py) -------------------------------------------
+def test_mg_dg_levels_are_galerkin_products():
+    # the embedded low Legendre modes: P^T A_p P equals the Q_p' operator with
+    # the same penalty factor (exact quadrature of the Q_p' products)
+    import mg_oracle as mgo
+    rng = np.random.default_rng(30)
+    n = (3, 2, 3)
+    kc = rng.uniform(0.1, 10.0, size=18)
+    for pf, pc in ((2, 1), (4, 2), (3, 1)):
+        beta = 10.0 * pf * pf
+        Af = mgo.assemble_dg(ora.DGOperator(n, pf, beta, kappa=("cell", kc)), n, pf)
+        Ac = mgo.assemble_dg(ora.DGOperator(n, pc, beta, kappa=("cell", kc)), n, pc)
+        P = mgo.embed_modes(18, pf + 1, pc + 1)
+        G = (P.T @ Af @ P).toarray()
+        assert np.abs(G - Ac.toarray()).max() <= 1e-13 * np.abs(G).max()
+
+
+def test_mg_conforming_stencil_is_the_trilinear_stiffness():
+    # kappa = 1: at an interior vertex the Galerkin operator of the conforming
+    # trilinear space is the Q1 stiffness stencil h (8/3 centre, 0 faces,
+    # -1/6 edges, -1/12 corners); the SIPG face terms vanish on continuous functions
+    import mg_oracle as mgo
+    n = (6, 6, 6)
+    A1 = mgo.assemble_dg(ora.DGOperator(n, 1, 10.0, kappa=1.0), n, 1)
+    Pc = mgo.conforming_P(n)
+    A = (Pc.T @ A1 @ Pc).toarray()
+    h = 1.0 / 6
+    v = 3 + 7 * (3 + 7 * 3)
+    for dz in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                w = v + dx + 7 * (dy + 7 * dz)
+                nz = abs(dx) + abs(dy) + abs(dz)
+                ref = h * {0: 8.0 / 3.0, 1: 0.0, 2: -1.0 / 6.0, 3: -1.0 / 12.0}[nz]
+                assert abs(A[v, w] - ref) < 1e-13, (dx, dy, dz, A[v, w], ref)
+    assert abs(A[v].sum()) < 1e-13   # no entries beyond distance 1
+    assert np.allclose(A, A.T, atol=1e-13)
+
+
+def test_mg_vcycle_is_symmetric_and_preconditions_cg():
+    import mg_oracle as mgo
+    from dunefem_amd import workloads as wl
+    P = wl.problem3(n=8)
+    kappa = ("cell", P["kcell"])
+    M = mgo.MGOracle(P["n"], P["p"], P["beta"], kappa)
+    assert len(M.vx) == 2 and M.vx[1]["A"].shape[0] == 125   # 9^3 vertices, then 5^3
+    rng = np.random.default_rng(31)
+    op = ora.DGOperator(P["n"], P["p"], P["beta"], kappa=kappa)
+    r1, r2 = rng.normal(size=op.N), rng.normal(size=op.N)
+    a, b = M.apply(r1) @ r2, r1 @ M.apply(r2)
+    assert abs(a - b) <= 1e-12 * abs(a)
+    assert M.apply(r1) @ r1 > 0
+    F = op.rhs(P["fq"])
+    pre = ora.function_pre(M.apply, op.N)
+    _, it_mg = ora.cg(op, F, pre=pre, tol=1e-8, maxit=500, criteria=1)
+    _, it_bj = ora.cg(op, F, pre=ora.block_jacobi(np.linalg.inv(op.block_diag())), tol=1e-8, maxit=5000,
+                      criteria=1)
+    print("8^3 channel problem: PCG iterations multigrid", it_mg, "block Jacobi", it_bj)
+    assert 0 < it_mg < 40 and it_bj > 5 * it_mg
