@@ -42,8 +42,11 @@ def parse():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="solve section without the learned preconditioner")
-    ap.add_argument("--combined", action="store_true",
-                    help="also run the 96^3 Q4 solves, FP64 vs mixed (about two minutes)")
+    ap.add_argument("--no-combined", action="store_true", help="skip the 96^3 Q4 solves")
+    ap.add_argument("--apply-only", choices=["c1", "c3", "c4"], default=None,
+                    help="time only the operator apply of one workload (for profiling): c1 = 64^3 Q3 pointwise "
+                         "kappa with advection, c3 = 48^3 Q2 channels, c4 = 96^3 Q4 log-normal")
+    ap.add_argument("--prec", choices=["f64", "f32", "bf16"], default="f64", help="precision of --apply-only")
     ap.add_argument("--calib", action="store_true",
                     help="also run a plain FP64 axpy of known size (HBM counter calibration)")
     return ap.parse_args()
@@ -103,7 +106,6 @@ def plain_section(fa, wl, ctx, cpu=True):
     out = {"workload": "2D continuous Q2, 64x64 quads (16,641 DoFs), -lap u + u = f (8 Fourier modes), strong "
                        "u = 0, CG + Jacobi, rel tol 1e-10", "gpu_fp64_cg_jacobi": res}
     if cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle as ora
         oop = ora.CGOperator(P["n"], c=1.0)
         Fh = oop.rhs(P["fq"])
@@ -116,31 +118,14 @@ def plain_section(fa, wl, ctx, cpu=True):
     return out
 
 
-def solve_section(fa, wl, ctx, n=48, pairs=2000, solves=8, epochs=3, lr=1e-4, coarse_steps=50, learned=True):
-    """High-contrast channel problem: time to a 1e-8 FP64 relative residual.
-    Plain FP64 GPU paths (CG and FGMRES(30) with exact block Jacobi), FP64
-    refinement around FP32 inner solves, and the learned two-level
-    preconditioner in FGMRES(30): training pairs from plain CG solves of
-    other right-hand sides, training on the GPU, then FP64 and mixed solves."""
-    P = wl.problem3(n=n)
-    E = n ** 3
-    N = E * (P["p"] + 1) ** 3
-    out = {"workload": "3D SIPG DG Q2 diffusion, %d^3 hexes (%d DoFs), kappa=10^(4H), H binary channels 20%%, "
-                       "f = 16 Fourier modes; rel tol 1e-8 on the FP64 true residual" % (n, N), "dofs": N}
+def _dg_cell(fa, ctx, P, prec):
+    op = fa.GalerkinOperator(ctx, fa.OP_DG_SIPG_3D, P["n"], P["p"], P["beta"], (0, 0, 0), 0.0, fa.KAPPA_CELL,
+                             precision=prec)
+    op.set_kappa_cell(P["kcell"])
+    return op
 
-    def dg(prec):
-        op = fa.GalerkinOperator(ctx, fa.OP_DG_SIPG_3D, P["n"], P["p"], P["beta"], (0, 0, 0), 0.0, fa.KAPPA_CELL,
-                                 precision=prec)
-        op.set_kappa_cell(P["kcell"])
-        return op
 
-    op64 = dg(fa.F64)
-    op32 = dg(fa.F32)
-    F = op64.rhs(P["fq"])
-    bj64 = fa.Preconditioner(op64, fa.PRE_BLOCK_JACOBI)
-    bj32 = fa.Preconditioner(op32, fa.PRE_BLOCK_JACOBI)
-    runs = {}
-
+def _runner(fa, ctx, op64, F, runs, tag):
     def run(name, method, pre, inner=None, maxit=20000, **kw):
         X = op64.vector()
         inv = fa.InverseOperator(ctx, method=method, tolerance=1e-8, errormeasure=fa.RELATIVE,
@@ -151,102 +136,133 @@ def solve_section(fa, wl, ctx, n=48, pairs=2000, solves=8, epochs=3, lr=1e-4, co
             inv.bind(op64, None, inner, pre)
         inv(F, X)
         runs[name] = _info(inv.info())
-        log("solve", name, runs[name])
+        log(tag, name, runs[name])
+    return run
 
-    run("fp64_cg_bj", fa.CG, bj64)
-    run("fp64_fgmres_bj", fa.FGMRES, bj64, maxit=3000)
-    # inner reduction 0.03: FP32 CG on this high-contrast problem loses
-    # convergence speed over long runs, and frequent restarts from the FP64
-    # residual beat longer inner solves (3111 inner iterations at 0.03
-    # against 3888 at 1e-3)
-    run("mixed_fp32_cg_bj_ir", fa.CG, bj32, op32, refinement=1, inner_tolerance=0.03, inner_maxiterations=20000)
 
-    out["runs"] = runs
-    if not learned:
-        return out
-    # learned preconditioner: pairs from plain FP64 CG + block-Jacobi solves
-    # of other seeded right-hand sides, `every` chosen so the pairs span each solve
+def _train_learned(fa, wl, ctx, P, op64, mg64, pairs, epochs, lr, out, key):
+    """Pairs (post-V-cycle residual, error) from plain FP64 CG + multigrid
+    solves of other seeded right-hand sides, every iterate; training on the GPU."""
+    n = P["n"][0]
     net = fa.CNN(ctx, P["n"])
     net.set_coefficient(P["kcell"], P["beta"])
-    every = max(1, abs(runs["fp64_cg_bj"]["iterations"]) // (pairs // solves))
     rng = np.random.default_rng(7)
     m = P["p"] + 1
     c = wl.qp_coords(n, m)
     t0 = time.perf_counter()
-    npairs = 0
-    for _ in range(solves):
+    npairs, solves = 0, 0
+    while npairs < pairs and solves < 400:
         modes = wl.fourier_modes(rng, 16, 3)
         b = op64.rhs(wl.vol_layout3(wl.sine_sum_grid(modes, [c, c, c]), P["n"], m))
-        npairs += net.collect_pairs(op64, bj64, b, pairs // solves, every=every)
+        npairs += net.collect_pairs(op64, mg64, b, pairs - npairs, every=1)
+        solves += 1
     ctx.synchronize()
     collect_s = time.perf_counter() - t0
     t0 = time.perf_counter()
     tl, vr = net.train(epochs=epochs, batch=8, lr=lr)
     train_s = time.perf_counter() - t0
-    log("trained on", npairs, "pairs in", round(train_s, 1), "s, val", vr)
-    out["learned_training"] = {"pairs": npairs, "every": every, "collect_s": round(collect_s, 2),
-                               "epochs": epochs, "lr": lr, "train_s": round(train_s, 2),
-                               "train_energy_loss": round(tl, 4), "val_rel_coarse_energy_error": round(vr, 4),
-                               "note": "loss 1 = zero correction, 0 = exact coarse solve"}
-    for name, mode in (("fp64_fgmres_twolevel_no_net", fa.LEARNED_COARSE_ONLY),
-                       ("fp64_fgmres_learned", fa.LEARNED_NET)):
-        run(name, fa.FGMRES, fa.LearnedPreconditioner(op64, net, coarse_steps=coarse_steps, mode=mode))
-    run("mixed_fp32_fgmres_learned_ir", fa.FGMRES, fa.LearnedPreconditioner(op32, net, coarse_steps=coarse_steps),
-        op32, refinement=1, inner_tolerance=1e-3, inner_maxiterations=20000)
-    out["coarse_cg_steps"] = coarse_steps
+    log(key, "trained on", npairs, "pairs in", round(train_s, 1), "s, val", vr)
+    out[key] = {"pairs": npairs, "solves": solves, "collect_s": round(collect_s, 2), "epochs": epochs, "lr": lr,
+                "train_s": round(train_s, 2), "train_energy_loss": round(tl, 4),
+                "val_rel_coarse_energy_error": round(vr, 4),
+                "note": "pairs: element means of the residual one V-cycle leaves and of the error, from plain "
+                        "FP64 CG + multigrid solves; loss 1 = zero correction, 0 = exact coarse solve"}
+    return net
+
+
+def solve_section(fa, wl, ctx, n=48, pairs=2000, epochs=3, lr=1e-4, learned=True):
+    """High-contrast channel problem: time to a 1e-8 FP64 relative residual.
+    The plain FP64 GPU path (CG + exact block Jacobi, the reference's Krylov
+    recurrence with element-block preconditioning), block-Jacobi FGMRES(30)
+    (the configuration's comparison), CG with the multigrid preconditioner
+    in FP64, FP64 refinement around FP32 inner solves with an FP32 multigrid
+    and with a multigrid on the BF16-MFMA operator, and the learned
+    correction after the V-cycle in FGMRES(30), with and without the network
+    and inside FP64 refinement."""
+    P = wl.problem3(n=n)
+    E = n ** 3
+    N = E * (P["p"] + 1) ** 3
+    out = {"workload": "3D SIPG DG Q2 diffusion, %d^3 hexes (%d DoFs), kappa=10^(4H), H binary channels 20%%, "
+                       "f = 16 Fourier modes; rel tol 1e-8 on the FP64 true residual" % (n, N), "dofs": N}
+    op64 = _dg_cell(fa, ctx, P, fa.F64)
+    op32 = _dg_cell(fa, ctx, P, fa.F32)
+    opbf = _dg_cell(fa, ctx, P, fa.BF16MFMA)
+    F = op64.rhs(P["fq"])
+    runs = {}
+    run = _runner(fa, ctx, op64, F, runs, "solve")
+    bj64 = fa.Preconditioner(op64, fa.PRE_BLOCK_JACOBI)
+    t0 = time.perf_counter()
+    mg64 = fa.Preconditioner(op64, fa.PRE_MULTIGRID)
+    mg32 = fa.Preconditioner(op32, fa.PRE_MULTIGRID)
+    mgbf = fa.Preconditioner(opbf, fa.PRE_MULTIGRID)
+    ctx.synchronize()
+    out["multigrid_setup_s_three_precisions"] = round(time.perf_counter() - t0, 3)
+    run("fp64_cg_bj", fa.CG, bj64)
+    run("fp64_fgmres_bj", fa.FGMRES, bj64, maxit=3000)
+    run("fp64_cg_mg", fa.CG, mg64)
+    run("mixed_fp32_cg_mg_ir", fa.CG, mg32, op32, refinement=1, inner_tolerance=1e-3, inner_maxiterations=2000)
+    # the V-cycle on the BF16-MFMA operator: the inner Krylov solve keeps the
+    # FP32 operator; FGMRES, since BF16 rounding makes the V-cycle slightly
+    # nonlinear
+    run("mixed_bf16mg_fgmres_ir", fa.FGMRES, mgbf, op32, refinement=1, inner_tolerance=1e-3,
+        inner_maxiterations=2000)
+    if learned:
+        net = _train_learned(fa, wl, ctx, P, op64, mg64, pairs, epochs, lr, out, "learned_training")
+        run("fp64_fgmres_mg", fa.FGMRES,
+            fa.LearnedPreconditioner(op64, net, coarse_steps=0, mode=fa.LEARNED_OFF, smoother=fa.PRE_MULTIGRID))
+        run("fp64_fgmres_mg_learned", fa.FGMRES,
+            fa.LearnedPreconditioner(op64, net, coarse_steps=0, mode=fa.LEARNED_NET, smoother=fa.PRE_MULTIGRID))
+        run("mixed_fp32_fgmres_mg_learned_ir", fa.FGMRES,
+            fa.LearnedPreconditioner(op32, net, coarse_steps=0, mode=fa.LEARNED_NET, smoother=fa.PRE_MULTIGRID),
+            op32, refinement=1, inner_tolerance=1e-3, inner_maxiterations=2000)
     out["runs"] = runs
-    best = runs["fp64_cg_bj"]["time_s"]
-    out["speedup_vs_fp64_cg_bj"] = {k: round(best / v["time_s"], 3) for k, v in runs.items()}
+    base = runs["fp64_cg_bj"]["time_s"]
+    out["speedup_vs_fp64_cg_bj"] = {k: round(base / v["time_s"], 2) for k, v in runs.items()}
+    mgb = runs["fp64_cg_mg"]["time_s"]
+    out["speedup_vs_fp64_cg_mg"] = {k: round(mgb / v["time_s"], 2) for k, v in runs.items()}
     return out
 
 
-def combined_section(fa, wl, ctx, n=96):
-    """Q4 log-normal problem (110.6 M DoFs): plain FP64 CG + block Jacobi vs
-    FP64 refinement around FP32 CG + block Jacobi.  The BF16-MFMA operator is
-    not used as the inner operator here: its apply error (8e-3 relative on
-    rough vectors) is relative to |u|, so on the smooth modes that decide
-    convergence the inner operator is wrong by far more than the refinement
-    can absorb (measured: the inner CG diverges)."""
+def combined_section(fa, wl, ctx, n=96, learned=True, pairs=400, epochs=2, lr=1e-4):
+    """Q4 log-normal problem (110.6 M DoFs): the plain FP64 GPU path (CG +
+    block Jacobi) against CG + multigrid in FP64, FP64 refinement around FP32
+    inner solves with FP32 and BF16-MFMA multigrid, and the learned correction
+    after the V-cycle inside FP64 refinement (trained on a reduced budget of
+    pairs at this size)."""
     P = wl.problem4(n=n)
     E = n ** 3
     N = E * (P["p"] + 1) ** 3
     out = {"workload": "3D SIPG DG Q4 diffusion, %d^3 hexes (%d DoFs), log-normal kappa (sigma 1.5, corr. "
                        "length 0.1, per element), f = 32 Fourier modes; rel tol 1e-8 on the FP64 true residual"
                        % (n, N), "dofs": N}
-
-    def dg(prec):
-        op = fa.GalerkinOperator(ctx, fa.OP_DG_SIPG_3D, P["n"], P["p"], P["beta"], (0, 0, 0), 0.0, fa.KAPPA_CELL,
-                                 precision=prec)
-        op.set_kappa_cell(P["kcell"])
-        return op
-
-    op64 = dg(fa.F64)
+    op64 = _dg_cell(fa, ctx, P, fa.F64)
     F = op64.rhs(P["fq"])
     del P["fq"]
     runs = {}
-
-    def run(name, method, op, pre, inner=None, maxit=20000, **kw):
-        X = op64.vector()
-        inv = fa.InverseOperator(ctx, method=method, tolerance=1e-8, errormeasure=fa.RELATIVE,
-                                 maxiterations=maxit, restart=30, residual_check=1, **kw)
-        if inner is None:
-            inv.bind(op, pre)
-        else:
-            inv.bind(op, None, inner, pre)
-        inv(F, X)
-        runs[name] = _info(inv.info())
-        log("combined", name, runs[name])
-
-    run("fp64_cg_bj", fa.CG, op64, fa.Preconditioner(op64, fa.PRE_BLOCK_JACOBI))
-    op32 = dg(fa.F32)
-    # inner tolerance 0.03: the FP32 operator's error on the smooth modes caps
-    # the reduction of one refinement step at about 30x, so a tighter inner
-    # solve spends iterations the outer step cannot use
-    run("mixed_fp32_cg_bj_ir", fa.CG, op64, fa.Preconditioner(op32, fa.PRE_BLOCK_JACOBI), op32, refinement=1,
-        inner_tolerance=0.03, inner_maxiterations=20000, verbose=1)
+    run = _runner(fa, ctx, op64, F, runs, "combined")
+    run("fp64_cg_bj", fa.CG, fa.Preconditioner(op64, fa.PRE_BLOCK_JACOBI))
+    t0 = time.perf_counter()
+    mg64 = fa.Preconditioner(op64, fa.PRE_MULTIGRID)
+    ctx.synchronize()
+    out["multigrid_setup_s_fp64"] = round(time.perf_counter() - t0, 3)
+    run("fp64_cg_mg", fa.CG, mg64)
+    op32 = _dg_cell(fa, ctx, P, fa.F32)
+    run("mixed_fp32_cg_mg_ir", fa.CG, fa.Preconditioner(op32, fa.PRE_MULTIGRID), op32, refinement=1,
+        inner_tolerance=1e-3, inner_maxiterations=2000)
+    opbf = _dg_cell(fa, ctx, P, fa.BF16MFMA)
+    run("mixed_bf16mg_fgmres_ir", fa.FGMRES, fa.Preconditioner(opbf, fa.PRE_MULTIGRID), op32, refinement=1,
+        inner_tolerance=1e-3, inner_maxiterations=2000)
+    del opbf
+    if learned:
+        net = _train_learned(fa, wl, ctx, P, op64, mg64, pairs, epochs, lr, out, "learned_training")
+        run("mixed_fp32_fgmres_mg_learned_ir", fa.FGMRES,
+            fa.LearnedPreconditioner(op32, net, coarse_steps=0, mode=fa.LEARNED_NET, smoother=fa.PRE_MULTIGRID),
+            op32, refinement=1, inner_tolerance=1e-3, inner_maxiterations=2000)
     out["runs"] = runs
-    best = runs["fp64_cg_bj"]["time_s"]
-    out["speedup_vs_fp64_cg_bj"] = {k: round(best / v["time_s"], 3) for k, v in runs.items()}
+    base = runs["fp64_cg_bj"]["time_s"]
+    out["speedup_vs_fp64_cg_bj"] = {k: round(base / v["time_s"], 2) for k, v in runs.items()}
+    mgb = runs["fp64_cg_mg"]["time_s"]
+    out["speedup_vs_fp64_cg_mg"] = {k: round(mgb / v["time_s"], 2) for k, v in runs.items()}
     return out
 
 
@@ -302,6 +318,69 @@ def batch_section(fa, wl, ctx, count=256):
             "problems_per_s_batched": round(count / info["time_s"], 1)}
 
 
+def cpu_threads():
+    """Threads of the CPU baseline: OMP_NUM_THREADS when set (the host-core
+    share a one-GPU job is given), else every core the process may run on."""
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    if env > 0:
+        return env
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def host_info(ora):
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = None
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cores": affinity,
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS"), "oracle_library": ora._LIBNAME,
+            "oracle_flags": "-O3 -fopenmp " + ("-march=native" if "native" in ora._LIBNAME else "-march=x86-64-v3")}
+
+
+def apply_only(a, fa, wl):
+    """One workload's operator apply, timed alone (the profiling driver)."""
+    ctx = fa.Context(0)
+    prec = {"f64": fa.F64, "f32": fa.F32, "bf16": fa.BF16MFMA}[a.prec]
+    if a.apply_only == "c1":
+        P = wl.problem1(n=a.n)
+        op = fa.GalerkinOperator(ctx, fa.OP_DG_SIPG_3D, P["n"], P["p"], P["beta"], P["b"], 0.0, fa.KAPPA_QP,
+                                 precision=prec)
+        op.set_kappa_qp(P["kvol"], P["kfaces"])
+        n, k = P["n"], P["p"] + 1
+        qc = n[0] * n[1] * n[2] * k ** 3 + sum((n[0] + (d == 0)) * (n[1] + (d == 1)) * (n[2] + (d == 2))
+                                               for d in range(3)) * k * k
+        u = P["u"]
+    else:
+        P = wl.problem3() if a.apply_only == "c3" else wl.problem4()
+        del P["fq"]
+        op = fa.GalerkinOperator(ctx, fa.OP_DG_SIPG_3D, P["n"], P["p"], P["beta"], (0, 0, 0), 0.0, fa.KAPPA_CELL,
+                                 precision=prec)
+        op.set_kappa_cell(P["kcell"])
+        qc = int(np.prod(P["n"]))
+        u = np.random.default_rng(1).normal(size=op.N)
+    U, W = op.vector(u), op.vector()
+    del u
+    wall, kern_ms = time_applies(fa, ctx, op, U, W, a.steps, a.warmup)
+    sv = 8 if prec == fa.F64 else 4
+    b_alg = sv * (2 * op.N + qc)
+    ach = b_alg / (kern_ms * 1e-3) / 1e9
+    print(json.dumps({"apply_only": a.apply_only, "precision": a.prec, "dofs": int(op.N), "kernel_ms": round(kern_ms, 5),
+                      "gdofs": round(op.N / (kern_ms * 1e-3) / 1e9, 3), "alg_bytes_per_launch": int(b_alg),
+                      "achieved_gbs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4)}), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -310,6 +389,14 @@ def main():
         sys.exit(2)
     import dunefem_amd as fa
     from dunefem_amd import workloads as wl
+    if a.apply_only:
+        apply_only(a, fa, wl)
+        return
+    if not a.no_cpu:
+        # the CPU baseline runs the oracle built for this host (-march=native)
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as ora
+        ora.select_native()
 
     P = wl.problem1(n=a.n)
     n, p = P["n"], P["p"]
@@ -399,14 +486,13 @@ def main():
     if not a.no_batch:
         result["batch"] = batch_section(fa, wl, ctx)
 
-    if a.combined:
-        result["combined"] = combined_section(fa, wl, ctx)
+    if not a.no_combined and not a.no_solve:
+        result["combined"] = combined_section(fa, wl, ctx, learned=not a.no_learned)
 
     # ---- CPU baseline: the oracle (reference algorithm) on the host cores ----------
     if not a.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle as ora
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+        threads = cpu_threads()
         oop = ora.DGOperator(n, p, P["beta"], P["b"], 0.0, kappa=("qp", P["kvol"], tuple(P["kfaces"])))
         wc = oop.apply(P["u"], nthreads=threads)   # warm-up and parity spot check
         times = []
@@ -421,7 +507,8 @@ def main():
                                             "oracle = reference algorithm restated (dense per-element tables, "
                                             "skeleton once per face pair, OpenMP element chunks)" % (n[0], p, a.cpu_reps),
                                   "seconds_per_apply": round(tc, 4),
-                                  "gpu_fp64_vs_oracle_rel_err": float(np.abs(w64 - wc).max() / np.abs(wc).max())}
+                                  "gpu_fp64_vs_oracle_rel_err": float(np.abs(w64 - wc).max() / np.abs(wc).max()),
+                                  "host": host_info(ora)}
     print(json.dumps(result), flush=True)
 
 

@@ -347,6 +347,22 @@ int fem_op_diagonal(fem_op *op, fem_vec *d)
     });
 }
 
+int fem_op_set_boundary_data(fem_op *op, const double *g_x, const double *g_y, const double *g_z)
+{
+    return guarded([&] {
+        FEM_CHECK(op, FEM_ERR_ARG, "null operator");
+        op->o.set_boundary_data(g_x, g_y, g_z);
+    });
+}
+
+int fem_op_constrain(fem_op *op, fem_vec *x, const fem_vec *add)
+{
+    return guarded([&] {
+        FEM_CHECK(op && x, FEM_ERR_ARG, "null argument");
+        op->o.constrain(x->v, add ? &add->v : nullptr);
+    });
+}
+
 int fem_op_block_diagonal(fem_op *op, fem_vec *blocks)
 {
     return guarded([&] {
@@ -688,7 +704,7 @@ int fem_cnn_train(fem_cnn *c, int epochs, int batch, float lr, float val_frac, u
     });
 }
 
-int fem_precond_create_learned(fem_op *op, fem_cnn *cnn, int coarse_steps, fem_precond **out)
+int fem_precond_create_learned_ex(fem_op *op, fem_cnn *cnn, int coarse_steps, int smoother, fem_precond **out)
 {
     return guarded([&] {
         FEM_CHECK(op && cnn && out, FEM_ERR_ARG, "null argument");
@@ -698,13 +714,18 @@ int fem_precond_create_learned(fem_op *op, fem_cnn *cnn, int coarse_steps, fem_p
             h->p.kind = FEM_PRE_LEARNED;
             h->p.prec = op->o.vec_prec;
             h->p.learned = new Learned();
-            h->p.learned->init(&op->o, &cnn->c, coarse_steps);
+            h->p.learned->init(&op->o, &cnn->c, coarse_steps, smoother);
         } catch (...) {
             delete h;
             throw;
         }
         *out = h;
     });
+}
+
+int fem_precond_create_learned(fem_op *op, fem_cnn *cnn, int coarse_steps, fem_precond **out)
+{
+    return fem_precond_create_learned_ex(op, cnn, coarse_steps, FEM_PRE_BLOCK_JACOBI, out);
 }
 
 int fem_precond_set_learned_mode(fem_precond *p, int mode)

@@ -835,6 +835,65 @@ void dg_rhs(const DGArgs<double> &a, int k, const double *fq, double *F, hipStre
 }
 
 // ---------------------------------------------------------------------------
+// boundary data of the load vector: F_i += sum over the element's boundary
+// faces of int (kappa beta/h + max(-b.n, 0)) g phi_i - kappa g dphi_i/dn,
+// the g-dependent part of the Nitsche and inflow boundary terms
+// (dune/fempy/test/parallelTest.py:127-129, pydemo/advectiondiffusion.py:43-49);
+// g at the face quadrature points [faces_d][m^2].  One thread per DoF of a
+// boundary element (the grid covers all elements; interior ones return).
+// ---------------------------------------------------------------------------
+__global__ void k_dg_rhs_bnd(const DGArgs<double> a, int K, const double *__restrict__ g0,
+                             const double *__restrict__ g1, const double *__restrict__ g2, double *__restrict__ F)
+{
+    const int K2 = K * K, K3 = K2 * K;
+    const int64_t E = (int64_t)a.n[0] * a.n[1] * a.n[2];
+    const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (gid >= E * K3) return;
+    const int64_t e = gid / K3;
+    const int li = (int)(gid % K3);
+    const int ec[3] = { (int)(e % a.n[0]), (int)((e / a.n[0]) % a.n[1]), (int)(e / ((int64_t)a.n[0] * a.n[1])) };
+    bool bnd = false;
+    for (int d = 0; d < 3; ++d) bnd = bnd || ec[d] == 0 || ec[d] == a.n[d] - 1;
+    if (!bnd) return;
+    const Tab1D<double> &tb = *a.tab;
+    const int ai[3] = { li / K2, (li / K) % K, li % K };
+    const double *gf[3] = { g0, g1, g2 };
+    double acc = 0;
+    for (int d = 0; d < 3; ++d) {
+        const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+        for (int s = 0; s < 2; ++s) {
+            if (ec[d] + (s ? 1 : -1) >= 0 && ec[d] + (s ? 1 : -1) < a.n[d]) continue;
+            const double nrm = s ? 1.0 : -1.0;
+            const double bn = a.b[d] * nrm, bneg = bn < 0 ? -bn : 0.0;
+            int cc[3] = { ec[0], ec[1], ec[2] };
+            cc[d] += s;
+            const int64_t N0 = a.n[0] + (d == 0), N1 = a.n[1] + (d == 1);
+            const int64_t fidx = cc[0] + N0 * (cc[1] + N1 * (int64_t)cc[2]);
+            for (int q2 = 0; q2 < K; ++q2)
+                for (int q1 = 0; q1 < K; ++q1) {
+                    const int qf = q1 + K * q2;
+                    const double tang = tb.B[q1][ai[t1]] * tb.B[q2][ai[t2]];
+                    const double phi = tb.P[s][ai[d]] * tang;
+                    const double dn = nrm * tb.dP[s][ai[d]] * tang / a.h[d];
+                    const double W = tb.w[q1] * tb.w[q2] * a.h[t1] * a.h[t2];
+                    const double kap = a.kappa_mode == FEM_KAPPA_QP ? a.kface[d][fidx * K2 + qf]
+                                     : a.kappa_mode == FEM_KAPPA_CELL ? a.kcell[e] : a.kappa_const;
+                    const double g = gf[d][fidx * K2 + qf];
+                    acc += W * g * ((kap * a.beta / a.h[d] + bneg) * phi - kap * dn);
+                }
+        }
+    }
+    F[gid] += acc;
+}
+
+void dg_rhs_bnd(const DGArgs<double> &a, int k, const double *const g[3], double *F, hipStream_t s)
+{
+    const int64_t N = (int64_t)a.n[0] * a.n[1] * a.n[2] * k * k * k;
+    hipLaunchKernelGGL(k_dg_rhs_bnd, dim3(grid_for(N, 256)), dim3(256), 0, s, a, k, g[0], g[1], g[2], F);
+    FEM_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
 // operator diagonal: one thread per DoF, direct sums over quadrature points
 // ---------------------------------------------------------------------------
 template <typename T>

@@ -92,6 +92,9 @@ int64_t dg_blocks(int k, int64_t E);
 void dg_diagonal(const DGArgs<double> &a, int k, double *d, hipStream_t s);
 // load vector from f at volume quadrature points
 void dg_rhs(const DGArgs<double> &a, int k, const double *fq, double *F, hipStream_t s);
+// F += boundary-data part of the load vector (u = g on the boundary, g at
+// the face quadrature points [faces_d][m^2] per normal direction)
+void dg_rhs_bnd(const DGArgs<double> &a, int k, const double *const g[3], double *F, hipStream_t s);
 
 // Exact element-block inverse for cellwise kappa (Kronecker-sum structure of
 // the element block of the SIPG diffusion operator in the orthonormal

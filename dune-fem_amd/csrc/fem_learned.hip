@@ -455,9 +455,45 @@ __global__ void k_add_coarse(T *__restrict__ y, const float *__restrict__ c, int
 }
 
 // ---------------------------------------------------------------------------------
-void Learned::init(Operator *o, CNN *c, int steps)
+// t = v - w
+template <typename T>
+__global__ void k_sub(const T *__restrict__ v, const T *__restrict__ w, T *__restrict__ t, int64_t N,
+                      const int *__restrict__ skip)
+{
+    if (skip && *skip) return;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x)
+        t[i] = v[i] - w[i];
+}
+
+// energy-optimal step of the coarse correction P c of the residual t:
+// alpha = <t, P c> / (c^T A_c c) with A_c c given (acc), into red[2]
+template <typename T>
+__global__ void __launch_bounds__(256) k_alpha_galerkin(const T *__restrict__ t, int nb, const float *__restrict__ c,
+                                                        const float *__restrict__ acc, int E, double *partials,
+                                                        unsigned *counter, float *red, const int *__restrict__ skip)
+{
+    if (skip && *skip) return;
+    __shared__ double lds[4];
+    double a = 0, b = 0;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < E; e += (int64_t)gridDim.x * blockDim.x) {
+        a += (double)t[e * nb] * (double)c[e];
+        b += (double)c[e] * (double)acc[e];
+    }
+    double own[2];
+    own[0] = block_sum(a, lds);
+    __syncthreads();
+    own[1] = block_sum(b, lds);
+    __shared__ double res[2];
+    if (finish_reduction(0.0, partials, counter, res, 2, own))
+        red[2] = (res[1] > 0 && res[0] == res[0]) ? (float)(res[0] / res[1]) : 0.f;
+}
+
+void Learned::init(Operator *o, CNN *c, int steps, int smoother_kind)
 {
     op = o;
+    FEM_CHECK(smoother_kind == FEM_PRE_BLOCK_JACOBI || smoother_kind == FEM_PRE_MULTIGRID, FEM_ERR_ARG,
+              "learned preconditioner: smoother is FEM_PRE_BLOCK_JACOBI or FEM_PRE_MULTIGRID");
+    smoother = smoother_kind;
     cnn = c;
     FEM_CHECK(steps >= 0 && steps <= 1000, FEM_ERR_ARG, "learned preconditioner: coarse_steps in [0, 1000]");
     coarse_steps = steps;
@@ -467,8 +503,9 @@ void Learned::init(Operator *o, CNN *c, int steps)
               "learned preconditioner: network grid must match the element grid");
     FEM_CHECK(c->kinv.p && c->stencil.p, FEM_ERR_STATE,
               "learned preconditioner: set the network's coefficient first");
-    bj.init(o, FEM_PRE_BLOCK_JACOBI);
+    bj.init(o, smoother);
     ccg.init(c);
+    acc.alloc(sizeof(float) * o->E);
     const size_t vb = (size_t)o->N * (o->vec_prec == FEM_F64 ? 8 : 4);
     z1.alloc(vb);
     t1.alloc(vb);
@@ -497,6 +534,33 @@ static void learned_apply_t(Learned &L, const T *x, T *y, const int *skip)
         return;
     }
     const bool net = L.mode == FEM_LEARNED_NET;
+    if (L.smoother == FEM_PRE_MULTIGRID) {
+        // y = MG(x); t = x - A y; c from the network on R t (and coarse CG
+        // steps); y += alpha P c with the energy-optimal alpha
+        T *t1 = L.t1.as<T>(), *w1 = L.w1.as<T>();
+        L.bj.apply_raw(x, y, skip);
+        if (!net && L.coarse_steps == 0) return;   // the method without the network: the V-cycle alone
+        o.apply_raw(y, w1, skip);
+        hipLaunchKernelGGL(k_sub<T>, dim3(G), dim3(256), 0, s, x, (const T *)w1, t1, N, skip);
+        if (net) {
+            hipLaunchKernelGGL(k_coarse_rms<T>, dim3(LR_BLOCKS), dim3(256), 0, s, (const T *)t1, nb,
+                               L.cnn->kinv.as<float>(), E, L.partials.as<double>(), L.counter.as<unsigned>(),
+                               L.red.as<float>(), skip);
+            hipLaunchKernelGGL(k_coarse_feat<T>, dim3((E + 255) / 256), dim3(256), 0, s, (const T *)t1, nb,
+                               L.cnn->kinv.as<float>(), E, L.red.as<float>(), L.x0.as<float>(), skip);
+            L.cnn->forward(L.x0.as<float>(), L.yc.as<float>(), s, skip);
+        }
+        L.ccg.start<T>(t1, nb, net ? L.yc.as<float>() : nullptr, L.red.as<float>(), L.cnn->gamma0, s, skip);
+        for (int k = 0; k < L.coarse_steps; ++k) L.ccg.step(s, skip);
+        L.cnn->coarse_apply(L.ccg.x.as<float>(), L.acc.as<float>(), s);
+        hipLaunchKernelGGL(k_alpha_galerkin<T>, dim3(LR_BLOCKS), dim3(256), 0, s, (const T *)t1, nb,
+                           (const float *)L.ccg.x.as<float>(), (const float *)L.acc.as<float>(), E,
+                           L.partials.as<double>(), L.counter.as<unsigned>(), L.red.as<float>(), skip);
+        hipLaunchKernelGGL(k_add_coarse<T>, dim3((E + 255) / 256), dim3(256), 0, s, y, L.ccg.x.as<float>(), nb, E,
+                           L.red.as<float>(), skip);
+        FEM_HIP(hipGetLastError());
+        return;
+    }
     if (net) {
         hipLaunchKernelGGL(k_coarse_rms<T>, dim3(LR_BLOCKS), dim3(256), 0, s, x, nb, L.cnn->kinv.as<float>(), E,
                            L.partials.as<double>(), L.counter.as<unsigned>(), L.red.as<float>(), skip);
@@ -572,6 +636,10 @@ void cnn_coarse_solve(const CNN &c, const float *b, int steps, float *x, double 
 
 int cnn_collect_pairs(CNN &c, Operator *op64, Precond *bj64, const Vec &b, int max_pairs, int every)
 {
+    FEM_CHECK(bj64 && (bj64->kind == FEM_PRE_BLOCK_JACOBI || bj64->kind == FEM_PRE_MULTIGRID) && bj64->op->N == op64->N &&
+                  bj64->prec == FEM_F64,
+              FEM_ERR_ARG, "collect: an FP64 block-Jacobi or multigrid preconditioner of the operator");
+    const bool post_mg = bj64->kind == FEM_PRE_MULTIGRID;
     FEM_CHECK(op64->vec_prec == FEM_F64 && op64->is_dg() && op64->E == c.V, FEM_ERR_ARG,
               "collect: FP64 DG operator on the network grid");
     FEM_CHECK(max_pairs > 0 && every > 0, FEM_ERR_ARG, "collect: max_pairs, every");
@@ -605,12 +673,28 @@ int cnn_collect_pairs(CNN &c, Operator *op64, Precond *bj64, const Vec &b, int m
     const int first = c.ds_n;
     int taken = 0;
     S.snap_chunk = every;
+    DevBuf mz, mw, mt;
+    if (post_mg) {
+        mz.alloc(b.bytes());
+        mw.alloc(b.bytes());
+        mt.alloc(b.bytes());
+    }
     S.snapshot = [&](const void *x, const void *r) {
         if (taken >= max_pairs) return;
         const int k = c.ds_n + taken;
-        hipLaunchKernelGGL(k_gather_coarse, dim3((E + 255) / 256), dim3(256), 0, op64->ctx->stream, (const double *)x, nb, E,
+        hipStream_t st = op64->ctx->stream;
+        hipLaunchKernelGGL(k_gather_coarse, dim3((E + 255) / 256), dim3(256), 0, st, (const double *)x, nb, E,
                            c.ds_e.as<float>() + (size_t)k * E);
-        hipLaunchKernelGGL(k_gather_coarse, dim3((E + 255) / 256), dim3(256), 0, op64->ctx->stream, (const double *)r, nb, E,
+        const double *rr = (const double *)r;
+        if (post_mg) {
+            // what one V-cycle leaves of the residual: r - A MG(r)
+            bj64->apply_raw(r, mz.p, nullptr);
+            op64->apply_raw(mz.p, mw.p, nullptr);
+            hipLaunchKernelGGL(k_sub<double>, dim3(grid_for(b.n, 256, 8192)), dim3(256), 0, st, rr,
+                               (const double *)mw.p, mt.as<double>(), b.n, (const int *)nullptr);
+            rr = mt.as<double>();
+        }
+        hipLaunchKernelGGL(k_gather_coarse, dim3((E + 255) / 256), dim3(256), 0, st, rr, nb, E,
                            c.ds_r.as<float>() + (size_t)k * E);
         ++taken;
     };

@@ -32,13 +32,18 @@ struct CoarseCG {
 struct Learned {
     Operator *op = nullptr;
     CNN *cnn = nullptr;
-    Precond bj;                 // exact element-block inverse
+    // smoother kind: FEM_PRE_BLOCK_JACOBI (coarse correction first, then the
+    // block-Jacobi smoother on the remaining residual) or FEM_PRE_MULTIGRID
+    // (a multigrid V-cycle first, then the coarse correction of what it
+    // leaves); FEM_LEARNED_OFF applies the smoother alone
+    int smoother = FEM_PRE_BLOCK_JACOBI;
+    Precond bj;                 // exact element-block inverse, or the multigrid
     CoarseCG ccg;
     int coarse_steps = 5;
     int mode = FEM_LEARNED_NET;
-    DevBuf z1, t1, w1, x0, yc, red, partials, counter;
+    DevBuf z1, t1, w1, x0, yc, acc, red, partials, counter;
     DevBuf dparts;   // per-workgroup <z1, A z1> partials of the apply (not for BF16 applies)
-    void init(Operator *o, CNN *c, int coarse_steps);
+    void init(Operator *o, CNN *c, int coarse_steps, int smoother = FEM_PRE_BLOCK_JACOBI);
     void apply_raw(const void *x, void *y, const int *skip);
     bool active() const { return mode != FEM_LEARNED_OFF; }
 };
@@ -47,9 +52,12 @@ void cnn_set_coefficient(CNN &c, const double *kcell, double beta);
 // `steps` coarse CG steps from x = 0 on device arrays [V]; rz[0] = final
 // (r, D^-1 r), rz[1] = its start value
 void cnn_coarse_solve(const CNN &c, const float *b, int steps, float *x, double rz[2]);
-// runs the plain FP64 CG + block-Jacobi solve of A x = b from x = 0 and stores
-// coarse (residual, error) pairs of every `every`-th iterate; returns the count
-int cnn_collect_pairs(CNN &c, Operator *op64, Precond *bj64, const Vec &b, int max_pairs, int every);
+// runs the plain FP64 CG solve of A x = b from x = 0 with `pre64` (block
+// Jacobi or multigrid) and stores coarse (residual, error) pairs of every
+// `every`-th iterate; with the multigrid the stored residual is what one
+// V-cycle leaves of it, r - A MG(r) (the input of the learned correction that
+// follows the V-cycle); returns the count
+int cnn_collect_pairs(CNN &c, Operator *op64, Precond *pre64, const Vec &b, int max_pairs, int every);
 void cnn_train(CNN &c, int epochs, int batch, float lr, float val_frac, unsigned seed, double &train_loss,
                double &val_rel);
 

@@ -159,6 +159,64 @@ void Operator::set_kappa_cell(const double *kc)
     a64.kappa_mode = a32.kappa_mode = FEM_KAPPA_CELL;
 }
 
+// Dirichlet nodes of the (2 nx + 1) x (2 ny + 1) Q2 node lattice
+__device__ __forceinline__ bool q2_boundary(int64_t i, int nx, int ny)
+{
+    const int64_t W = 2 * (int64_t)nx + 1;
+    const int64_t I = i % W, J = i / W;
+    return I == 0 || J == 0 || I == 2 * nx || J == 2 * ny;
+}
+
+// x_D = (add ? x_D : 0) + g_D on the Dirichlet nodes, interior untouched
+template <typename T>
+__global__ void k_q2_set_bnd(T *__restrict__ x, const double *__restrict__ g, int64_t n, int nx, int ny, int add)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        if (q2_boundary(i, nx, ny)) x[i] = (T)((add ? (double)x[i] : 0.0) + g[i]);
+}
+
+// x_D = g_D + r_D (r may be null)
+template <typename T>
+__global__ void k_q2_constrain(T *__restrict__ x, const double *__restrict__ g, const T *__restrict__ r, int64_t n,
+                               int nx, int ny)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        if (q2_boundary(i, nx, ny)) x[i] = (T)((g ? g[i] : 0.0) + (r ? (double)r[i] : 0.0));
+}
+
+void Operator::set_boundary_data(const double *g0, const double *g1, const double *g2)
+{
+    hipStream_t s = ctx->stream;
+    if (is_dg()) {
+        FEM_CHECK(g0 && g1 && g2, FEM_ERR_ARG, "boundary data: three face arrays");
+        const double *g[3] = { g0, g1, g2 };
+        for (int d = 0; d < 3; ++d) {
+            const size_t nf = (size_t)(desc.n[0] + (d == 0)) * (desc.n[1] + (d == 1)) * (desc.n[2] + (d == 2)) * k * k;
+            upload_as<double>(gbnd64[d], g[d], nf, s);
+        }
+    } else {
+        FEM_CHECK(g0, FEM_ERR_ARG, "boundary data: nodal array");
+        upload_as<double>(gbnd64[0], g0, (size_t)N, s);
+    }
+    has_bnd = true;
+}
+
+void Operator::constrain(Vec &x, const Vec *add) const
+{
+    FEM_CHECK(x.n == N && (!add || add->n == N), FEM_ERR_ARG, "constrain: vector size");
+    FEM_CHECK(!add || add->prec == x.prec, FEM_ERR_ARG, "constrain: vector precisions differ");
+    if (is_dg()) return;   // weak boundary conditions: nothing to set
+    hipStream_t s = ctx->stream;
+    const double *g = has_bnd ? gbnd64[0].as<double>() : nullptr;
+    if (x.prec == FEM_F64)
+        hipLaunchKernelGGL(k_q2_constrain<double>, dim3(grid_for(N, 256, 8192)), dim3(256), 0, s, (double *)x.data, g,
+                           add ? (const double *)add->data : nullptr, N, desc.n[0], desc.n[1]);
+    else
+        hipLaunchKernelGGL(k_q2_constrain<float>, dim3(grid_for(N, 256, 8192)), dim3(256), 0, s, (float *)x.data, g,
+                           add ? (const float *)add->data : nullptr, N, desc.n[0], desc.n[1]);
+    FEM_HIP(hipGetLastError());
+}
+
 int64_t Operator::dot_blocks() const
 {
     return is_dg() ? dg_blocks(k, E) : 0;
@@ -205,8 +263,21 @@ void Operator::rhs(const double *fq_host, Vec &F) const
     double *dst;
     if (F.prec == FEM_F64) dst = (double *)F.data;
     else { out.alloc(N * sizeof(double)); dst = out.as<double>(); }
-    if (is_dg()) dg_rhs(a64, k, fq.as<double>(), dst, s);
-    else cg_launch<double>(CG_MODE_RHS, inst_dev.as<CGInst>(), 1, tiles, cgtab.as<CGTab>(), fq.as<double>(), nullptr, dst, nullptr, s);
+    if (is_dg()) {
+        dg_rhs(a64, k, fq.as<double>(), dst, s);
+        if (has_bnd) {
+            const double *g[3] = { gbnd64[0].as<double>(), gbnd64[1].as<double>(), gbnd64[2].as<double>() };
+            dg_rhs_bnd(a64, k, g, dst, s);
+        }
+    } else {
+        cg_launch<double>(CG_MODE_RHS, inst_dev.as<CGInst>(), 1, tiles, cgtab.as<CGTab>(), fq.as<double>(), nullptr, dst, nullptr, s);
+        // R(u) = A u - F has Dirichlet rows u_D - g_D, as the reference's
+        // DirichletWrapperOperator (dirichletwrapper.hh:101-105 with
+        // subConstraints, dirichletconstraints.hh:494-554): F_D = g_D
+        if (has_bnd)
+            hipLaunchKernelGGL(k_q2_set_bnd<double>, dim3(grid_for(N, 256, 8192)), dim3(256), 0, s, dst,
+                               gbnd64[0].as<double>(), N, desc.n[0], desc.n[1], 1);
+    }
     if (F.prec != FEM_F64) {
         Vec tmp;
         tmp.ctx = ctx; tmp.n = N; tmp.prec = FEM_F64; tmp.data = dst; tmp.owner = false;

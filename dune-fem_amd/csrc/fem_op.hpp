@@ -50,6 +50,15 @@ struct Operator {
     void init(Context *c, const fem_op_desc &d);
     void set_kappa_qp(const double *kvol, const double *kx, const double *ky, const double *kz);
     void set_kappa_cell(const double *kc);
+    // boundary data u = g: DG (weak, Nitsche and inflow terms): g at the face
+    // quadrature points of each normal direction [faces_d][m^2]; CG Q2
+    // (strong): nodal values [N], only the Dirichlet nodes are read
+    void set_boundary_data(const double *g0, const double *g1, const double *g2);
+    // CG Q2: x_D = g_D (+ add_D) (the reference's setConstraints and
+    // addConstraints); DG: nothing
+    void constrain(Vec &x, const Vec *add = nullptr) const;
+    DevBuf gbnd64[3];   // boundary data as set (DG: per direction; CG: gbnd64[0])
+    bool has_bnd = false;
     // y = A x on raw device pointers of the operator's vector precision;
     // `dot` (DG only, may be null): per-workgroup FP64 partials of <x, y>,
     // dot_blocks() entries

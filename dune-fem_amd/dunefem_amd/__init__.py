@@ -40,7 +40,7 @@ EXPORTS = [
     "fem_vec_upload_f32", "fem_vec_fill", "fem_vec_copy", "fem_vec_axpy", "fem_vec_scal",
     "fem_vec_dot", "fem_vec_dot_noalloc", "fem_vec_norm2sq", "fem_op_create", "fem_op_destroy", "fem_op_ndofs",
     "fem_op_set_kappa_qp", "fem_op_set_kappa_cell", "fem_op_apply", "fem_op_assemble_rhs",
-    "fem_op_diagonal", "fem_op_block_diagonal", "fem_precond_create", "fem_precond_destroy", "fem_precond_apply",
+    "fem_op_diagonal", "fem_op_block_diagonal", "fem_op_set_boundary_data", "fem_op_constrain", "fem_precond_create", "fem_precond_destroy", "fem_precond_apply",
     "fem_solver_default_params", "fem_solver_params_set", "fem_solver_create",
     "fem_solver_destroy", "fem_solver_bind", "fem_solver_unbind", "fem_solver_solve",
     "fem_gauss_points", "fem_batch_create", "fem_batch_destroy", "fem_batch_ndofs",
@@ -48,7 +48,7 @@ EXPORTS = [
     "fem_batch_autotune", "fem_batch_solve", "fem_batch_instance_results",
     "fem_cnn_create", "fem_cnn_destroy", "fem_cnn_nparams", "fem_cnn_set_weights", "fem_cnn_get_weights",
     "fem_cnn_forward", "fem_cnn_loss_grad", "fem_cnn_set_coefficient", "fem_cnn_collect_pairs",
-    "fem_cnn_train", "fem_precond_create_learned", "fem_precond_set_learned_mode", "fem_cnn_coarse_apply",
+    "fem_cnn_train", "fem_precond_create_learned", "fem_precond_create_learned_ex", "fem_precond_set_learned_mode", "fem_cnn_coarse_apply",
     "fem_cnn_coarse_solve",
 ]
 
@@ -129,6 +129,8 @@ def load_library(path=LIBPATH):
         "fem_op_assemble_rhs": ([_vp, _dp, _vp], C.c_int),
         "fem_op_diagonal": ([_vp, _vp], C.c_int),
         "fem_op_block_diagonal": ([_vp, _vp], C.c_int),
+        "fem_op_set_boundary_data": ([_vp, _dp, _dp, _dp], C.c_int),
+        "fem_op_constrain": ([_vp, _vp, _vp], C.c_int),
         "fem_precond_create": ([_vp, C.c_int, C.POINTER(_vp)], C.c_int),
         "fem_precond_destroy": ([_vp], C.c_int),
         "fem_precond_apply": ([_vp, _vp, _vp], C.c_int),
@@ -165,6 +167,7 @@ def load_library(path=LIBPATH):
         "fem_cnn_collect_pairs": ([_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.POINTER(C.c_int)], C.c_int),
         "fem_cnn_train": ([_vp, C.c_int, C.c_int, C.c_float, C.c_float, C.c_uint, _dp, _dp], C.c_int),
         "fem_precond_create_learned": ([_vp, _vp, C.c_int, C.POINTER(_vp)], C.c_int),
+        "fem_precond_create_learned_ex": ([_vp, _vp, C.c_int, C.c_int, C.POINTER(_vp)], C.c_int),
         "fem_precond_set_learned_mode": ([_vp, C.c_int], C.c_int),
     }
     for name, (args, res) in sig.items():
@@ -348,6 +351,20 @@ class GalerkinOperator:
         d = d if d is not None else self.vector()
         _check(_lib.fem_op_diagonal(self.h, d.h))
         return d
+
+    def set_boundary_data(self, g):
+        """u = g on the boundary.  DG: three arrays, g at the face quadrature
+        points [faces_d][m^2] (kface layout); CG Q2: the nodal values [N]."""
+        if isinstance(g, (list, tuple)):
+            gs = [np.ascontiguousarray(x, dtype=np.float64) for x in g]
+            _check(_lib.fem_op_set_boundary_data(self.h, _dptr(gs[0]), _dptr(gs[1]), _dptr(gs[2])))
+        else:
+            g = np.ascontiguousarray(g, dtype=np.float64)
+            _check(_lib.fem_op_set_boundary_data(self.h, _dptr(g), None, None))
+
+    def constrain(self, x, add=None):
+        """x_D = g_D (+ add_D) on the Dirichlet DoFs (CG Q2; no effect for DG)."""
+        _check(_lib.fem_op_constrain(self.h, x.h, add.h if add is not None else None))
 
     def block_diagonal(self):
         """Element blocks [E][nb][nb] (FP64, host array) of a DG operator."""
@@ -596,17 +613,62 @@ class CNN:
 class LearnedPreconditioner(Preconditioner):
     """Two-level preconditioner (FGMRES only): coarse solve on element means
     started from the network's output and finished by ``coarse_steps``
-    Jacobi-CG steps, energy-optimal coarse step, exact block-Jacobi smoother.
+    Jacobi-CG steps, energy-optimal coarse step, and a smoother: the exact
+    block Jacobi on what the coarse step leaves (``smoother=PRE_BLOCK_JACOBI``)
+    or a multigrid V-cycle before the coarse step (``PRE_MULTIGRID``).
     ``mode``: LEARNED_NET, LEARNED_COARSE_ONLY (coarse CG from zero, i.e.
-    without the network) or LEARNED_OFF (block Jacobi)."""
+    without the network) or LEARNED_OFF (the smoother alone)."""
 
-    def __init__(self, op, cnn, coarse_steps=5, mode=LEARNED_NET):
+    def __init__(self, op, cnn, coarse_steps=5, mode=LEARNED_NET, smoother=PRE_BLOCK_JACOBI):
         self.op = op
         self.cnn = cnn
         h = _vp()
-        _check(_lib.fem_precond_create_learned(op.h, cnn.h, int(coarse_steps), C.byref(h)))
+        _check(_lib.fem_precond_create_learned_ex(op.h, cnn.h, int(coarse_steps), int(smoother), C.byref(h)))
         self.h = h
         self.set_mode(mode)
 
     def set_mode(self, mode):
         _check(_lib.fem_precond_set_learned_mode(self.h, int(mode)))
+
+
+class Scheme:
+    """The scheme-level solve of the reference, FemScheme::solve/_solve
+    (dune/fem/schemes/femscheme.hh:195-253), for the affine operators of
+    this library: R(u) = A u - F, the boundary data inside F (and, for the
+    continuous Q2 operator, the Dirichlet rows u_D - g_D).
+
+    ``solve(target, rhs=None)``: setConstraints (u_D = g_D) and
+    addConstraints (u_D += rhs_D) on the target, then the Newton step of
+    NewtonInverseOperator::operator() (dune/fem/solver/newtoninverseoperator.hh:688-802;
+    one step, the operator is linear): residual = R(u) - rhs, the linear
+    solver on J du = residual from du = 0, u -= du.  Returns the info dict
+    of scheme.solve (dune/fempy/py/scheme.hh:156-187): converged,
+    iterations (Newton steps), linear_iterations, timing."""
+
+    def __init__(self, op, F, preconditioner=None, **solver_parameters):
+        self.op, self.F = op, F
+        self.pre = preconditioner
+        self.inv = InverseOperator(op.ctx, **solver_parameters)
+
+    def __call__(self, u, w):
+        """w = R(u) = A u - F"""
+        self.op(u, w)
+        w.axpy(-1.0, self.F)
+
+    def solve(self, target, rhs=None):
+        import time as _time
+        t0 = _time.perf_counter()
+        op = self.op
+        op.constrain(target, rhs)                 # u_D = g_D + rhs_D
+        res, du = op.vector(), op.vector()
+        self(target, res)                         # residual R(u)
+        if rhs is not None:
+            res.axpy(-1.0, rhs)                   #          - rhs
+        self.inv.bind(op, self.pre) if self.pre is not None else self.inv.bind(op)
+        du.clear()
+        self.inv(res, du)
+        lin = self.inv.info()
+        target.axpy(-1.0, du)
+        self.inv.unbind()
+        return {"converged": bool(lin["converged"]), "iterations": 1 if lin["converged"] else 0,
+                "linear_iterations": lin["iterations"], "timing": [_time.perf_counter() - t0]}

@@ -142,6 +142,23 @@ int fem_op_assemble_rhs(fem_op *op, const double *fq, fem_vec *F);
  * and SparseRowMatrix::extractDiagonal, dune/fem/operator/matrix/spmatrix.hh:846,
  * used by FemPreconditioning, dune/fem/solver/fempreconditioning.hh:85-107) */
 int fem_op_diagonal(fem_op *op, fem_vec *d);
+/* Boundary data u = g (the reference's boundary integrands with the exact
+ * data, pydemo/advectiondiffusion.py:43-49 and dune/fempy/test/parallelTest.py:127-129
+ * for DG; DirichletConstraints with a boundary function,
+ * dune/fem/schemes/dirichletconstraints.hh:237-262, for CG).  Host arrays, copied.
+ *  DG (weak): g at the face quadrature points in the face layout of
+ *     fem_op_set_kappa_qp; only boundary faces are read.  fem_op_assemble_rhs
+ *     then adds the g-dependent Nitsche and inflow terms
+ *     int (kappa beta/h + max(-b.n, 0)) g v - kappa g dv/dn.
+ *  CG Q2 (strong): nodal values [N] in g_x (g_y, g_z unused); only the
+ *     Dirichlet nodes are read.  fem_op_assemble_rhs then puts F_D = g_D, so
+ *     that R(u) = A u - F has the Dirichlet rows u_D - g_D of the reference's
+ *     DirichletWrapperOperator (dirichletwrapper.hh:101-105). */
+int fem_op_set_boundary_data(fem_op *op, const double *g_x, const double *g_y, const double *g_z);
+/* CG Q2: x_D = g_D + add_D (FemScheme::_solve's setConstraints and
+ * addConstraints, dune/fem/schemes/femscheme.hh:247-253; g = 0 when no data
+ * was set, add may be NULL); DG: no effect (weak boundary conditions). */
+int fem_op_constrain(fem_op *op, fem_vec *x, const fem_vec *add);
 /* DG operators: the element (diagonal) blocks of the linear operator, the
  * local matrices that DifferentiableGalerkinOperator::jacobian assembles on
  * the diagonal (dune/fem/schemes/galerkin.hh:1548-1551, 1195-1314) and that
@@ -285,9 +302,12 @@ int fem_cnn_coarse_apply(fem_cnn *c, const float *x, float *y);
  * solve of the learned preconditioner (host arrays [V]); rz (may be null):
  * rz[0] = final (r, D^-1 r), rz[1] = its start value */
 int fem_cnn_coarse_solve(fem_cnn *c, const float *b, int steps, float *x, double *rz);
-/* plain FP64 CG + block-Jacobi solve of op64 x = b from x = 0; stores coarse
- * (residual, error) pairs of every `every`-th iterate (at most max_pairs) */
-int fem_cnn_collect_pairs(fem_cnn *c, fem_op *op64, fem_precond *bj64, const fem_vec *b, int max_pairs, int every,
+/* plain FP64 CG solve of op64 x = b from x = 0 with `pre64` (FP64 block
+ * Jacobi or multigrid); stores coarse (residual, error) pairs of every
+ * `every`-th iterate (at most max_pairs).  With the multigrid the stored
+ * residual is r - A MG(r), what one V-cycle leaves (the input of the
+ * multigrid form of the learned preconditioner). */
+int fem_cnn_collect_pairs(fem_cnn *c, fem_op *op64, fem_precond *pre64, const fem_vec *b, int max_pairs, int every,
                           int *npairs);
 /* Adam on the energy-norm distance between the coarse correction and the
  * solver's error, ||e_k - P c||_A^2 (normalised per pair; 1 for c = 0).
@@ -302,6 +322,12 @@ int fem_cnn_train(fem_cnn *c, int epochs, int batch, float lr, float val_frac, u
 #define FEM_LEARNED_COARSE_ONLY 2  /* coarse CG started from zero: the method without the network */
 /* coarse_steps: Jacobi-CG steps on A_c per application (0..1000) */
 int fem_precond_create_learned(fem_op *op, fem_cnn *cnn, int coarse_steps, fem_precond **out);
+/* the same with a choice of smoother: FEM_PRE_BLOCK_JACOBI (the form above:
+ * coarse correction, then block Jacobi on what remains) or FEM_PRE_MULTIGRID
+ * (a multigrid V-cycle, then the learned coarse correction of the residual
+ * it leaves; FEM_LEARNED_OFF is the V-cycle alone).  Train such a network on
+ * pairs collected with a multigrid preconditioner (fem_cnn_collect_pairs). */
+int fem_precond_create_learned_ex(fem_op *op, fem_cnn *cnn, int coarse_steps, int smoother, fem_precond **out);
 int fem_precond_set_learned_mode(fem_precond *p, int mode);
 
 /* ---- batched solves -----------------------------------------------------------

@@ -408,6 +408,49 @@ void ora_dg_rhs(const ora_dg_t *op, const double *fq, double *F)
     dg_tables_free(&T);
 }
 
+/* F += boundary-data part of the load vector for u = g on the boundary:
+ * the Nitsche terms kappa beta/h g v - kappa g grad v . n and the inflow
+ * term max(-b.n, 0) g v of dune/fempy/test/parallelTest.py:127-129 and
+ * pydemo/advectiondiffusion.py:43-49 (the g-dependent part of the
+ * boundary integrand, moved to the right-hand side); g at the face
+ * quadrature points [faces_d][m^2] in the kface layout */
+void ora_dg_rhs_bnd(const ora_dg_t *op, const double *const gface[3], double *F)
+{
+    dg_tables T;
+    dg_tables_init(&T, op->p);
+    int nb = T.nb, nqf = T.nqf;
+    double h[3] = { 1.0 / op->n[0], 1.0 / op->n[1], 1.0 / op->n[2] };
+    int64_t E = (int64_t)op->n[0] * op->n[1] * op->n[2];
+    int64_t nx = op->n[0], ny = op->n[1];
+    double c0[256], c1[768];
+    for (int64_t e = 0; e < E; ++e) {
+        int c[3] = { (int)(e % nx), (int)((e / nx) % ny), (int)(e / (nx * ny)) };
+        for (int d = 0; d < 3; ++d) {
+            int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+            double area = h[t1] * h[t2];
+            for (int s = 0; s < 2; ++s) {
+                int nbc = c[d] + (s ? 1 : -1);
+                if (nbc >= 0 && nbc < op->n[d]) continue;
+                int lf = 2 * d + s;
+                double nrm = s ? 1.0 : -1.0;
+                int64_t fidx = dg_face_index(op, d, c, s);
+                double bn = op->b[d] * nrm;
+                for (int q = 0; q < nqf; ++q) {
+                    double kap = op->kappa_mode == ORA_KAPPA_QP ? op->kface[d][fidx * nqf + q]
+                               : op->kappa_mode == ORA_KAPPA_CELL ? op->kcell[e] : op->kappa_const;
+                    double W = T.wface[q] * area;
+                    double g = gface[d][fidx * nqf + q];
+                    c0[q] = W * (kap * op->beta / h[d] + (bn < 0 ? -bn : 0.0)) * g;
+                    for (int dd = 0; dd < 3; ++dd) c1[dd * nqf + q] = 0.0;
+                    c1[d * nqf + q] = -W * kap * g * nrm;
+                }
+                dg_face_axpy(&T, lf, c0, c1, h, F + e * nb);
+            }
+        }
+    }
+    dg_tables_free(&T);
+}
+
 /* element self block: contributions to element e from u supported on e only */
 void ora_dg_block_diag(const ora_dg_t *op, double *blocks)
 {

@@ -88,6 +88,9 @@ int64_t ora_dg_ndofs(const ora_dg_t *op);
 void ora_dg_apply(const ora_dg_t *op, const double *u, double *w, int nthreads);
 /* F_i = int f phi_i with f given at the volume quadrature points [E][m^3] */
 void ora_dg_rhs(const ora_dg_t *op, const double *fq, double *F);
+/* F += the boundary-data part of the load vector (Nitsche and inflow terms
+ * for u = g), g at the boundary face quadrature points [faces_d][m^2] */
+void ora_dg_rhs_bnd(const ora_dg_t *op, const double *const gface[3], double *F);
 /* element blocks A_KK, [E][nb][nb] row-major (row = test function) */
 void ora_dg_block_diag(const ora_dg_t *op, double *blocks);
 

@@ -38,10 +38,29 @@ class Block(C.Structure):
     _fields_ = [("nblocks", C.c_int64), ("nb", C.c_int), ("inv", _dp)]
 
 
+_LIBNAME = "liboracle.so"
+
+
+def select_native():
+    """Build (make native) and use the -march=native oracle of this host
+    instead of the portable one; returns the library name in use.  Must run
+    before the first oracle call."""
+    global _LIBNAME
+    import subprocess
+    if _LIB is not None:
+        return _LIBNAME
+    try:
+        subprocess.run(["make", "-s", "-C", _HERE, "native"], check=True, capture_output=True, timeout=120)
+        _LIBNAME = "liboracle_native.so"
+    except Exception:
+        pass
+    return _LIBNAME
+
+
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+        path = os.path.join(_HERE, _LIBNAME)
         if not os.path.exists(path):
             raise RuntimeError("oracle not built: run `make -C oracle`")
         L = C.CDLL(path)
@@ -55,6 +74,7 @@ def lib():
         L.ora_dg_apply.argtypes = [C.POINTER(DG), _dp, _dp, C.c_int]
         L.ora_dg_rhs.argtypes = [C.POINTER(DG), _dp, _dp]
         L.ora_dg_block_diag.argtypes = [C.POINTER(DG), _dp]
+        L.ora_dg_rhs_bnd.argtypes = [C.POINTER(DG), C.POINTER(_dp), _dp]
         L.ora_cg_ndofs.argtypes = [C.POINTER(CG)]
         L.ora_cg_ndofs.restype = C.c_int64
         L.ora_cg_apply.argtypes = [C.POINTER(CG), _dp, _dp]
@@ -145,6 +165,15 @@ class DGOperator:
         fq = np.ascontiguousarray(fq, dtype=np.float64)
         F = np.empty(self.N)
         lib().ora_dg_rhs(C.byref(self.s), _ptr(fq), _ptr(F))
+        return F
+
+    def rhs_bnd(self, gfaces, F=None):
+        """F (+)= boundary-data part of the load vector for u = g on the
+        boundary; gfaces: three arrays [faces_d][m^2] (kface layout)."""
+        F = np.zeros(self.N) if F is None else np.array(F, dtype=np.float64)
+        gs = [np.ascontiguousarray(g, dtype=np.float64) for g in gfaces]
+        arr = (_dp * 3)(*[_ptr(g) for g in gs])
+        lib().ora_dg_rhs_bnd(C.byref(self.s), arr, _ptr(F))
         return F
 
     def block_diag(self):

@@ -116,3 +116,62 @@ def test_solver_builds_preconditioner_from_parameter_key(ctx):
     X = gop.vector()
     inv(gop.vector(F), X)
     assert abs(inv.iterations() - ito) <= 1 and inv.info()["true_rel_residual"] < 1e-9
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
+def test_sipg_boundary_data_consistency_on_gpu(ctx, p):
+    # the closed-form identity A u*_h = F(f) + F_bnd(g) for a polynomial u*,
+    # pointwise kappa = 1 + x + 2y + 3z, advection and reaction (through the
+    # C ABI; the oracle satisfies the same identity), and the GPU load vector
+    # with boundary data against the oracle's
+    from manufactured import sipg_problem
+    n, b, c = (4, 3, 5), (0.7, -0.4, 0.3), 0.5
+    M = sipg_problem(n, p, b, c, seed=p)
+    op = fa.GalerkinOperator(ctx, fa.OP_DG_SIPG_3D, n, p, 10.0 * p * p, b, c, fa.KAPPA_QP)
+    op.set_kappa_qp(M["kvol"], M["kfaces"])
+    op.set_boundary_data(M["gfaces"])
+    U, W = op.vector(M["u"]), op.vector()
+    op(U, W)
+    F = op.rhs(M["fq"]).download()
+    lhs = W.download()
+    err = np.abs(lhs - F).max() / np.abs(F).max()
+    print("p", p, "A u* - F rel", err, flush=True)
+    assert err <= 1e-11
+    oop = ora.DGOperator(n, p, 10.0 * p * p, b, c, kappa=("qp", M["kvol"], tuple(M["kfaces"])))
+    Fo = oop.rhs_bnd(M["gfaces"], oop.rhs(M["fq"]))
+    assert np.abs(F - Fo).max() <= 1e-12 * np.abs(Fo).max()
+
+
+def test_q2_scheme_solve_with_dirichlet_data_reproduces_polynomial(ctx):
+    # continuous Q2, -div(kappa grad u) + u = f, kappa = 1 + x + 2y, u = u* on
+    # the boundary with u* in Q2: the scheme solve (constraints, one Newton
+    # step with CG + Jacobi) returns u* at every node
+    from manufactured import q2_problem
+    n = (12, 9)
+    M = q2_problem(n, c=1.0, seed=3)
+    op = fa.GalerkinOperator(ctx, fa.OP_CG_Q2_2D, n, 2, c=1.0, kappa_mode=fa.KAPPA_QP)
+    op.set_kappa_qp(M["kvol"])
+    op.set_boundary_data(M["nodes"])
+    F = op.rhs(M["fq"])
+    scheme = fa.Scheme(op, F, fa.Preconditioner(op, fa.PRE_JACOBI), method=fa.CG, tolerance=1e-13,
+                       errormeasure=fa.RELATIVE, maxiterations=5000)
+    U = op.vector()
+    info = scheme.solve(U)
+    print("Q2 scheme solve:", info, flush=True)
+    assert set(info) == {"converged", "iterations", "linear_iterations", "timing"}
+    assert info["converged"] and info["iterations"] == 1
+    u = U.download()
+    assert np.abs(u - M["nodes"]).max() <= 1e-9 * np.abs(M["nodes"]).max()
+    # R(u*) = A u* - F vanishes, Dirichlet rows included (u_D - g_D)
+    R = op.vector()
+    scheme(op.vector(M["nodes"]), R)
+    assert np.abs(R.download()).max() <= 1e-10 * np.abs(F.download()).max()
+    # a right-hand side: u_D = g_D + rhs_D and R(u) = rhs
+    rng = np.random.default_rng(4)
+    rhs = op.vector(rng.normal(size=op.N))
+    U2 = op.vector()
+    info2 = scheme.solve(U2, rhs)
+    assert info2["converged"]
+    scheme(U2, R)
+    R.axpy(-1.0, rhs)
+    assert np.sqrt(R.normSquaredDofs()) <= 1e-9 * np.sqrt(rhs.normSquaredDofs() + F.normSquaredDofs())

@@ -496,3 +496,21 @@ def test_mg_vcycle_is_symmetric_and_preconditions_cg():
                       criteria=1)
     print("8^3 channel problem: PCG iterations multigrid", it_mg, "block Jacobi", it_bj)
     assert 0 < it_mg < 40 and it_bj > 5 * it_mg
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
+def test_sipg_consistency_with_pointwise_kappa_and_boundary_data(p):
+    # u* a polynomial of degree <= p per direction, kappa = 1 + x + 2y + 3z at
+    # the volume and face quadrature points, advection and reaction, u = u*
+    # on the boundary (Nitsche + inflow data): A u*_h = F(f) + F_bnd(g) to
+    # rounding.  A transposed face-point index of kappa or g, a sign of the
+    # Nitsche data terms or a wrong inflow side breaks the identity.
+    from manufactured import sipg_problem
+    n, b, c = (3, 2, 4), (0.7, -0.4, 0.3), 0.5
+    M = sipg_problem(n, p, b, c, seed=p)
+    op = ora.DGOperator(n, p, 10.0 * p * p, b, c, kappa=("qp", M["kvol"], tuple(M["kfaces"])))
+    lhs = op.apply(M["u"])
+    rhs = op.rhs_bnd(M["gfaces"], op.rhs(M["fq"]))
+    assert np.abs(lhs - rhs).max() <= 1e-11 * np.abs(rhs).max()
+    # the identity needs the boundary data: without it, it fails
+    assert np.abs(lhs - op.rhs(M["fq"])).max() > 1e-3 * np.abs(rhs).max()
