@@ -9,6 +9,7 @@
 #include "fem_batch.hpp"
 #include "fem_blas.hpp"
 #include "fem_cnn.hpp"
+#include "fem_csr.hpp"
 #include "fem_learned.hpp"
 #include "fem_op.hpp"
 #include "fem_solver.hpp"
@@ -368,6 +369,57 @@ int fem_op_block_diagonal(fem_op *op, fem_vec *blocks)
     return guarded([&] {
         FEM_CHECK(op && blocks, FEM_ERR_ARG, "null argument");
         op->o.block_diagonal(blocks->v);
+    });
+}
+
+// ---- assembled Jacobian ----------------------------------------------------------
+int fem_op_assemble_csr(fem_op *op, fem_csr **out)
+{
+    return guarded([&] {
+        FEM_CHECK(op && out, FEM_ERR_ARG, "null argument");
+        FEM_CHECK(op->o.is_dg() ? op->o.a64.kappa_mode >= 0
+                                : (op->o.desc.kappa_mode == FEM_KAPPA_CONST || op->o.kvol64.p),
+                  FEM_ERR_STATE, "coefficient data not set");
+        fem_csr *h = new fem_csr();
+        try {
+            h->m.assemble(&op->o);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int fem_csr_destroy(fem_csr *A)
+{
+    return guarded([&] { delete A; });
+}
+
+int64_t fem_csr_rows(const fem_csr *A) { return A ? A->m.N : -1; }
+int64_t fem_csr_nnz(const fem_csr *A) { return A ? A->m.nnz : -1; }
+
+int fem_csr_apply(const fem_csr *A, const fem_vec *x, fem_vec *y)
+{
+    return guarded([&] {
+        FEM_CHECK(A && x && y, FEM_ERR_ARG, "null argument");
+        A->m.apply(x->v, y->v);
+    });
+}
+
+int fem_csr_diagonal(const fem_csr *A, fem_vec *d)
+{
+    return guarded([&] {
+        FEM_CHECK(A && d, FEM_ERR_ARG, "null argument");
+        A->m.diagonal(d->v);
+    });
+}
+
+int fem_csr_download(const fem_csr *A, int64_t *rowptr, int *col, double *val)
+{
+    return guarded([&] {
+        FEM_CHECK(A, FEM_ERR_ARG, "null argument");
+        A->m.download(rowptr, col, val);
     });
 }
 

@@ -49,7 +49,8 @@ EXPORTS = [
     "fem_cnn_create", "fem_cnn_destroy", "fem_cnn_nparams", "fem_cnn_set_weights", "fem_cnn_get_weights",
     "fem_cnn_forward", "fem_cnn_loss_grad", "fem_cnn_set_coefficient", "fem_cnn_collect_pairs",
     "fem_cnn_train", "fem_precond_create_learned", "fem_precond_create_learned_ex", "fem_precond_set_learned_mode", "fem_cnn_coarse_apply",
-    "fem_cnn_coarse_solve",
+    "fem_cnn_coarse_solve", "fem_op_assemble_csr", "fem_csr_destroy", "fem_csr_rows", "fem_csr_nnz",
+    "fem_csr_apply", "fem_csr_diagonal", "fem_csr_download",
 ]
 
 
@@ -131,6 +132,13 @@ def load_library(path=LIBPATH):
         "fem_op_block_diagonal": ([_vp, _vp], C.c_int),
         "fem_op_set_boundary_data": ([_vp, _dp, _dp, _dp], C.c_int),
         "fem_op_constrain": ([_vp, _vp, _vp], C.c_int),
+        "fem_op_assemble_csr": ([_vp, C.POINTER(_vp)], C.c_int),
+        "fem_csr_destroy": ([_vp], C.c_int),
+        "fem_csr_rows": ([_vp], C.c_int64),
+        "fem_csr_nnz": ([_vp], C.c_int64),
+        "fem_csr_apply": ([_vp, _vp, _vp], C.c_int),
+        "fem_csr_diagonal": ([_vp, _vp], C.c_int),
+        "fem_csr_download": ([_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int), _dp], C.c_int),
         "fem_precond_create": ([_vp, C.c_int, C.POINTER(_vp)], C.c_int),
         "fem_precond_destroy": ([_vp], C.c_int),
         "fem_precond_apply": ([_vp, _vp, _vp], C.c_int),
@@ -381,6 +389,46 @@ class GalerkinOperator:
         try:
             if self.h:
                 _lib.fem_op_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class SparseRowMatrix:
+    """The operator's Jacobian assembled on the device in CSR form
+    (DifferentiableGalerkinOperator::jacobian into SparseRowMatrix):
+    ``A(x, y)`` is y = A x, ``diagonal()`` extractDiagonal, ``to_scipy()`` a
+    host copy."""
+
+    def __init__(self, op):
+        self.op = op
+        h = _vp()
+        _check(_lib.fem_op_assemble_csr(op.h, C.byref(h)))
+        self.h = h
+        self.rows = _lib.fem_csr_rows(h)
+        self.nnz = _lib.fem_csr_nnz(h)
+
+    def __call__(self, x, y):
+        _check(_lib.fem_csr_apply(self.h, x.h, y.h))
+
+    def diagonal(self):
+        d = DofVector(self.op.ctx, self.rows, F64)
+        _check(_lib.fem_csr_diagonal(self.h, d.h))
+        return d
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        rp = np.zeros(self.rows + 1, dtype=np.int64)
+        col = np.zeros(self.nnz, dtype=np.int32)
+        val = np.zeros(self.nnz)
+        _check(_lib.fem_csr_download(self.h, rp.ctypes.data_as(C.POINTER(C.c_int64)),
+                                     col.ctypes.data_as(C.POINTER(C.c_int)), _dptr(val)))
+        return sp.csr_matrix((val, col, rp), shape=(self.rows, self.rows))
+
+    def __del__(self):
+        try:
+            if self.h:
+                _lib.fem_csr_destroy(self.h)
                 self.h = None
         except Exception:
             pass

@@ -168,6 +168,27 @@ int fem_op_constrain(fem_op *op, fem_vec *x, const fem_vec *add);
  * supported (computed by probing the apply). */
 int fem_op_block_diagonal(fem_op *op, fem_vec *blocks);
 
+/* ---- assembled Jacobian -------------------------------------------------------
+ * DifferentiableGalerkinOperator::jacobian / assemble into a SparseRowMatrix
+ * (dune/fem/schemes/galerkin.hh:1195-1314, 1548-1643;
+ * dune/fem/operator/matrix/spmatrix.hh:39-860), on the device in CSR form:
+ * int64 row offsets, int32 column indices increasing within a row, FP64
+ * values.  Rows of a DG element hold the blocks of the element and its face
+ * neighbours in element order; Q2 rows the nodes of the elements around the
+ * node, Dirichlet rows unit rows.  Assembled from the operator's own apply
+ * (the operators are linear), so A x equals fem_op_apply to rounding. */
+typedef struct fem_csr fem_csr;
+int fem_op_assemble_csr(fem_op *op, fem_csr **out);
+int fem_csr_destroy(fem_csr *A);
+int64_t fem_csr_rows(const fem_csr *A);
+int64_t fem_csr_nnz(const fem_csr *A);
+/* y = A x (SparseRowMatrix::apply, spmatrix.hh:148-178); FP64 or FP32 vectors */
+int fem_csr_apply(const fem_csr *A, const fem_vec *x, fem_vec *y);
+/* d = diag(A) (SparseRowMatrix::extractDiagonal, spmatrix.hh:846), FP64 */
+int fem_csr_diagonal(const fem_csr *A, fem_vec *d);
+/* host copies: rowptr [rows + 1], col [nnz], val [nnz] (any may be NULL) */
+int fem_csr_download(const fem_csr *A, int64_t *rowptr, int *col, double *val);
+
 /* ---- preconditioners -------------------------------------------------------- */
 #define FEM_PRE_NONE 0
 #define FEM_PRE_JACOBI 1        /* d^-1 with |d| < 16 eps -> 1 (fempreconditioning.hh:99-106) */

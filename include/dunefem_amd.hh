@@ -186,6 +186,21 @@ public:
         check(fem_op_assemble_rhs(h_, fq.data(), F.handle()));
     }
     void diagonal(DiscreteFunction &d) const { check(fem_op_diagonal(h_, d.handle())); }
+    // DG: the element blocks [E][nb][nb] (an FP64 function of E nb^2 entries)
+    void blockDiagonal(DiscreteFunction &blocks) const { check(fem_op_block_diagonal(h_, blocks.handle())); }
+    // u = g on the boundary.  DG: g at the face quadrature points per normal
+    // direction (gx, gy, gz in the kface layout); Q2: nodal values in gx
+    void setBoundaryData(const std::vector<double> &gx, const std::vector<double> &gy = {},
+                         const std::vector<double> &gz = {})
+    {
+        check(fem_op_set_boundary_data(h_, gx.data(), gy.empty() ? nullptr : gy.data(),
+                                       gz.empty() ? nullptr : gz.data()));
+    }
+    // Q2: x_D = g_D (+ add_D), the reference's setConstraints / addConstraints
+    void constrain(DiscreteFunction &x, const DiscreteFunction *add = nullptr) const
+    {
+        check(fem_op_constrain(h_, x.handle(), add ? add->handle() : nullptr));
+    }
     int64_t size() const { return fem_op_ndofs(h_); }
     int precision() const { return desc_.precision; }
     // a dof vector matching this operator (FP64 for FEM_F64, FP32 otherwise)
@@ -446,6 +461,47 @@ private:
     mutable int iterations_ = 0, linearIterations_ = 0;
     mutable double delta_ = 0;
     mutable bool nonlinear_ = false;
+};
+
+// FemScheme::solve / _solve (dune/fem/schemes/femscheme.hh:195-253) for the
+// affine operator R(u) = A u - F (boundary data inside F and, for Q2, in the
+// Dirichlet rows): setConstraints and addConstraints on the solution, then
+// the Newton step of NewtonInverseOperator over R with the linear inverse
+// operator (one step: R is affine)
+template <class LinearInverseOperator>
+class FemScheme {
+public:
+    FemScheme(const GalerkinOperator &A, const DiscreteFunction &F, LinearInverseOperator &inv)
+        : A_(&A), R_(A, F), inv_(&inv) {}
+    SolverInfo solve(const DiscreteFunction &rhs, DiscreteFunction &solution) const
+    {
+        A_->constrain(solution, &rhs);          // sol = g + rhs on the boundary
+        return newton(&rhs, solution);
+    }
+    SolverInfo solve(DiscreteFunction &solution) const
+    {
+        A_->constrain(solution);                // sol = g on the boundary
+        return newton(nullptr, solution);
+    }
+
+private:
+    SolverInfo newton(const DiscreteFunction *rhs, DiscreteFunction &w) const
+    {
+        DiscreteFunction residual(w), dw(w);
+        R_(w, residual);
+        if (rhs) residual -= *rhs;
+        inv_->bind(*A_);
+        dw.clear();
+        (*inv_)(residual, dw);
+        SolverInfo info = inv_->info();
+        w -= dw;
+        inv_->unbind();
+        info.nonlinearIterations = info.converged ? 1 : 0;
+        return info;
+    }
+    const GalerkinOperator *A_;
+    AffineGalerkinOperator R_;
+    LinearInverseOperator *inv_;
 };
 
 // Gauss-Legendre rule with m points on [0,1] (the reference's GaussPts

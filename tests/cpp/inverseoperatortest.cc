@@ -242,6 +242,55 @@ static void dg_checks(Context &ctx)
     report(thrown && code == FEM_ERR_ARG, "apply on a wrong-size vector throws FEM_ERR_ARG", (double)code);
 }
 
+// FemScheme::solve with inhomogeneous Dirichlet data (femscheme.hh:247-253):
+// -lap u + u = f with u* = 1 + x - 2 y + x y + x^2 y^2 in Q2, f = -lap u* + u*
+// and u = u* on the boundary; the scheme returns u* at every node, and the
+// preconditioner comes from fem.solver.preconditioning.method
+static void scheme_check(Context &ctx)
+{
+    const int n = 10, NX = 2 * n + 1;
+    auto us = [](double x, double y) { return 1 + x - 2 * y + x * y + x * x * y * y; };
+    auto fs = [&](double x, double y) { return -(2 * y * y + 2 * x * x) + us(x, y); };
+    fem_op_desc d{};
+    d.kind = FEM_OP_CG_Q2_2D;
+    d.n[0] = d.n[1] = n;
+    d.n[2] = 1;
+    d.order = 2;
+    d.c = 1.0;
+    d.kappa_mode = FEM_KAPPA_CONST;
+    d.kappa_const = 1.0;
+    d.precision = FEM_F64;
+    GalerkinOperator A(ctx, d);
+    std::vector<double> gx, gw;
+    gaussPoints(3, gx, gw);
+    std::vector<double> fq((size_t)n * n * 9), g((size_t)NX * NX);
+    for (int ey = 0; ey < n; ++ey)
+        for (int ex = 0; ex < n; ++ex)
+            for (int qy = 0; qy < 3; ++qy)
+                for (int qx = 0; qx < 3; ++qx)
+                    fq[((size_t)(ey * n + ex) * 3 + qy) * 3 + qx] = fs((ex + gx[qx]) / n, (ey + gx[qy]) / n);
+    for (int J = 0; J < NX; ++J)
+        for (int I = 0; I < NX; ++I) g[(size_t)J * NX + I] = us(I / (2.0 * n), J / (2.0 * n));
+    A.setBoundaryData(g);
+    DiscreteFunction F = A.function("F"), u = A.function("u");
+    A.assembleRHS(fq, F);
+    SolverParameter p;
+    p.set("fem.solver.tolerance", "1e-13").set("fem.solver.errormeasure", "relative")
+        .set("fem.solver.preconditioning.method", "jacobi");
+    CgInverseOperator inv(ctx, p);
+    FemScheme<CgInverseOperator> scheme(A, F, inv);
+    SolverInfo info = scheme.solve(u);
+    const std::vector<double> uh = u.download();
+    double err = 0, mx = 0;
+    for (size_t i = 0; i < uh.size(); ++i) {
+        err = std::max(err, std::fabs(uh[i] - g[i]));
+        mx = std::max(mx, std::fabs(g[i]));
+    }
+    report(info.converged && info.nonlinearIterations == 1, "FemScheme::solve converged in one Newton step",
+           (double)info.linearIterations);
+    report(err <= 1e-9 * mx, "FemScheme::solve reproduces u* in Q2 with Dirichlet data", err / mx);
+}
+
 int main()
 {
     try {
@@ -254,6 +303,7 @@ int main()
         algorithm<CgInverseOperator>(ctx, " === CgInverseOperator + SolverParameter === ", &p);
         algorithm<GmresInverseOperator>(ctx, " === GmresInverseOperator + SolverParameter === ", &p);
         dg_checks(ctx);
+        scheme_check(ctx);
     } catch (const Exception &e) {
         std::printf("FAIL unexpected exception %d: %s\n", e.code(), e.what());
         ++failures;
