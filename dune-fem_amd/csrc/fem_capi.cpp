@@ -519,7 +519,8 @@ int fem_solver_params_set(fem_solver_params *p, const char *key, const char *val
             if (v == "cg") p->method = FEM_SOLVER_CG;
             else if (v == "gmres") p->method = FEM_SOLVER_GMRES;
             else if (v == "fgmres") p->method = FEM_SOLVER_FGMRES;
-            else throw Error(FEM_ERR_UNSUPPORTED, "method: " + v + " (supported: cg, gmres, fgmres)");
+            else if (v == "bicgstab") p->method = FEM_SOLVER_BICGSTAB;
+            else throw Error(FEM_ERR_UNSUPPORTED, "method: " + v + " (supported: cg, gmres, fgmres, bicgstab)");
         } else if (k == "tolerance" || k == "absolutetol" || k == "reductiontol") {
             p->tolerance = parse_double(k, v);
             FEM_CHECK(p->tolerance >= 0, FEM_ERR_ARG, k + " must be >= 0");

@@ -428,6 +428,120 @@ __global__ void k_add(T *__restrict__ u, const T *__restrict__ z, int64_t n)
         u[i] = u[i] + z[i];
 }
 
+// ---- BiCGStab kernels --------------------------------------------------------------
+// LinearSolver::bicgstab (dune/fem/solver/linear/bicgstab.hh:56-222), right
+// preconditioned; the scalars stay on the device.
+// nu = <r, r*>, tolerance by criterion (relative: eps ||b||, reduction:
+// eps sqrt(nu), absolute: eps)
+__global__ void k_bicg_init(KState *st, double eps, int criteria, int maxit)
+{
+    const double nu = st->nu;
+    st->tol = criteria == FEM_ERRMEASURE_RELATIVE ? eps * sqrt(st->bnorm2)
+            : criteria == FEM_ERRMEASURE_REDUCTION ? eps * sqrt(nu) : eps;
+    st->it = 0;
+    st->maxit = maxit;
+    st->done = maxit <= 0;
+    st->fin = 0;
+}
+
+// alpha = nu / <tmp, r*>
+template <typename T>
+__global__ void __launch_bounds__(RED_THREADS)
+    k_bicg_alpha(const T *__restrict__ tmp, const T *__restrict__ rs, int64_t n, KState *st, double *partials,
+                 unsigned *counter)
+{
+    if (st->done) return;
+    __shared__ double lds[RED_THREADS / 64];
+    double acc = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        acc += (double)tmp[i] * (double)rs[i];
+    double own = block_sum(acc, lds);
+    __shared__ double res[1];
+    if (finish_reduction(0.0, partials, counter, res, 1, &own) && threadIdx.x == 0) st->alpha = st->nu / res[0];
+}
+
+// s = r - alpha tmp
+template <typename T>
+__global__ void k_bicg_s(const T *__restrict__ r, const T *__restrict__ tmp, T *__restrict__ s, int64_t n,
+                         const KState *__restrict__ st)
+{
+    if (st->done) return;
+    const double a = st->alpha;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        s[i] = (T)((double)r[i] - a * (double)tmp[i]);
+}
+
+// the five inner products of scalarProductVecs (bicgstab.hh:19-54) and the
+// scalars of the step: omega, the residual estimate, beta, nu; fin marks the
+// last iteration (res < tol or it >= maxit)
+template <typename T>
+__global__ void __launch_bounds__(RED_THREADS)
+    k_bicg_dots(const T *__restrict__ r, const T *__restrict__ s, const T *__restrict__ rs, int64_t n, KState *st,
+                double *partials, unsigned *counter)
+{
+    if (st->done) return;
+    __shared__ double lds[RED_THREADS / 64];
+    double a[5] = { 0, 0, 0, 0, 0 };
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double ri = (double)r[i], si = (double)s[i], ti = (double)rs[i];
+        a[0] += ri * si;
+        a[1] += ri * ri;
+        a[2] += si * si;
+        a[3] += si * ti;
+        a[4] += ri * ti;
+    }
+    double own[5];
+    for (int k = 0; k < 5; ++k) {
+        own[k] = block_sum(a[k], lds);
+        __syncthreads();
+    }
+    __shared__ double g[5];
+    if (finish_reduction(0.0, partials, counter, g, 5, own) && threadIdx.x == 0) {
+        const double omega = g[0] / g[1];
+        const double res = sqrt(g[2] - omega * (2.0 * g[0] - omega * g[1]));
+        const double beta = (g[3] - omega * g[4]) * st->alpha / (omega * st->nu);
+        st->nu = g[3] - omega * g[4];
+        st->omega = omega;
+        st->beta = beta;
+        st->bres = res;
+        st->it += 1;
+        st->fin = !(res >= st->tol) || st->it >= st->maxit;
+    }
+}
+
+// x += alpha p + omega s; unless this was the last iteration:
+// r = s - omega r, p = beta (p - omega tmp) + r
+template <typename T>
+__global__ void k_bicg_update(T *__restrict__ x, T *__restrict__ p, T *__restrict__ r, const T *__restrict__ s,
+                              const T *__restrict__ tmp, int64_t n, const KState *__restrict__ st)
+{
+    if (st->done) return;
+    const double a = st->alpha, w = st->omega, b = st->beta;
+    const bool last = st->fin;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double pi = (double)p[i], si = (double)s[i];
+        x[i] = (T)((double)x[i] + a * pi + w * si);
+        if (!last) {
+            const double rn = si - w * (double)r[i];
+            r[i] = (T)rn;
+            p[i] = (T)(b * (pi - w * (double)tmp[i]) + rn);
+        }
+    }
+}
+
+// r = b - r
+template <typename T>
+__global__ void k_bicg_res(const T *__restrict__ b, T *__restrict__ r, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        r[i] = (T)((double)b[i] - (double)r[i]);
+}
+
+__global__ void k_bicg_commit(KState *st)
+{
+    if (!st->done && st->fin) st->done = 1;
+}
+
 // ---- refinement helpers -----------------------------------------------------------
 // r64 = b - h (h = A x in FP64), partial ||r||^2 -> out
 __global__ void __launch_bounds__(RED_THREADS)
@@ -462,7 +576,7 @@ void Solver::init(Context *c, const fem_solver_params &p)
 {
     ctx = c;
     prm = p;
-    FEM_CHECK(p.method >= FEM_SOLVER_CG && p.method <= FEM_SOLVER_FGMRES, FEM_ERR_ARG, "solver method");
+    FEM_CHECK(p.method >= FEM_SOLVER_CG && p.method <= FEM_SOLVER_BICGSTAB, FEM_ERR_ARG, "solver method");
     FEM_CHECK(p.restart >= 1 && p.restart <= GM_MAX, FEM_ERR_ARG, "gmres restart must be in 1..64");
     state.alloc(sizeof(KState));
     partials.alloc(sizeof(double) * RED_BLOCKS * (GM_MAX + 2));
@@ -612,6 +726,61 @@ static int run_krylov(Solver &S, int method, Operator *o, Precond *p, const T *b
             if (S.hstate->done) break;
             if (S.snapshot) S.snapshot(x, pp);
             else if (chunk < 64) chunk *= 2;
+        }
+        const int it = S.hstate->it;
+        return it < maxit ? it : -it;
+    }
+    if (method == FEM_SOLVER_BICGSTAB) {
+        // bicgstab.hh:56-222: right preconditioning iterates on y with
+        // x = M^-1 y; the initial guess enters as y (r = b - A M^-1 x) and
+        // the result is mapped back at the end (x = M^-1 y)
+        T *r = (T *)S.workbuf(0, vb), *rs = (T *)S.workbuf(1, vb), *pp = (T *)S.workbuf(2, vb);
+        T *ss = (T *)S.workbuf(3, vb), *tmp = (T *)S.workbuf(4, vb);
+        T *z = haspre ? (T *)S.workbuf(5, vb) : nullptr;
+        if (haspre) {
+            p->apply_raw(x, z, nullptr);
+            o->apply_raw(z, r, nullptr);
+        } else {
+            o->apply_raw(x, r, nullptr);
+        }
+        hipLaunchKernelGGL(k_bicg_res<T>, dim3(G), dim3(256), 0, s, b, r, N);   // r = b - A x
+        FEM_HIP(hipMemcpyAsync(pp, r, vb, hipMemcpyDeviceToDevice, s));
+        FEM_HIP(hipMemcpyAsync(rs, r, vb, hipMemcpyDeviceToDevice, s));
+        dot_async(*S.ctx, N, r, rs, &st->nu, cnt + 1, part);
+        dot_async(*S.ctx, N, b, b, &st->bnorm2, cnt + 1, part);
+        hipLaunchKernelGGL(k_bicg_init, dim3(1), dim3(1), 0, s, st, eps, criteria, maxit);
+        FEM_HIP(hipGetLastError());
+        const int *skip = &st->done;
+        int chunk = 4;
+        while (true) {
+            for (int c = 0; c < chunk; ++c) {
+                if (haspre) {
+                    p->apply_raw(pp, z, skip);
+                    o->apply_raw(z, tmp, skip);
+                } else {
+                    o->apply_raw(pp, tmp, skip);
+                }
+                hipLaunchKernelGGL(k_bicg_alpha<T>, dim3(RED_DOT_BLOCKS), dim3(RED_THREADS), 0, s, tmp, rs, N, st, part, cnt);
+                hipLaunchKernelGGL(k_bicg_s<T>, dim3(G), dim3(256), 0, s, r, tmp, ss, N, st);
+                if (haspre) {
+                    p->apply_raw(ss, z, skip);
+                    o->apply_raw(z, r, skip);
+                } else {
+                    o->apply_raw(ss, r, skip);
+                }
+                hipLaunchKernelGGL(k_bicg_dots<T>, dim3(RED_DOT_BLOCKS), dim3(RED_THREADS), 0, s, r, ss, rs, N, st, part, cnt);
+                hipLaunchKernelGGL(k_bicg_update<T>, dim3(G), dim3(256), 0, s, x, pp, r, ss, tmp, N, st);
+                hipLaunchKernelGGL(k_bicg_commit, dim3(1), dim3(1), 0, s, st);
+            }
+            FEM_HIP(hipGetLastError());
+            S.fetch_state();
+            if (verbose) { std::printf("Fem::BiCGstab it: %d : %g\n", S.hstate->it, S.hstate->bres); std::fflush(stdout); }
+            if (S.hstate->done) break;
+            if (chunk < 64) chunk *= 2;
+        }
+        if (haspre) {
+            p->apply_raw(x, z, nullptr);
+            FEM_HIP(hipMemcpyAsync(x, z, vb, hipMemcpyDeviceToDevice, s));
         }
         const int it = S.hstate->it;
         return it < maxit ? it : -it;

@@ -24,6 +24,9 @@ struct KState {
     int jlast;       // inner steps done in the current cycle
     int converged;   // cycle ended by |g| < tol
     int cycle_done;  // stop inner loop (converged or maxit)
+    // BiCGStab
+    double nu, omega, bres;
+    int fin;         // the iteration just reduced is the last one
 };
 
 struct Solver {

@@ -29,7 +29,7 @@ OP_DG_SIPG_3D, OP_CG_Q2_2D = 1, 2
 KAPPA_CONST, KAPPA_QP, KAPPA_CELL = 0, 1, 2
 PRE_NONE, PRE_JACOBI, PRE_BLOCK_JACOBI, PRE_LEARNED, PRE_MULTIGRID = 0, 1, 2, 3, 4
 LEARNED_OFF, LEARNED_NET, LEARNED_COARSE_ONLY = 0, 1, 2
-CG, GMRES, FGMRES = 0, 1, 2
+CG, GMRES, FGMRES, BICGSTAB = 0, 1, 2, 3
 ABSOLUTE, RELATIVE, REDUCTION = 0, 1, 2
 
 # every symbol declared in include/dunefem_amd.h

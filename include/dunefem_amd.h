@@ -209,6 +209,7 @@ int fem_precond_apply(fem_precond *p, const fem_vec *x, fem_vec *y);
 #define FEM_SOLVER_CG 0
 #define FEM_SOLVER_GMRES 1
 #define FEM_SOLVER_FGMRES 2
+#define FEM_SOLVER_BICGSTAB 3   /* LinearSolver::bicgstab (dune/fem/solver/linear/bicgstab.hh:56-222) */
 
 #define FEM_ERRMEASURE_ABSOLUTE 0
 #define FEM_ERRMEASURE_RELATIVE 1
@@ -261,7 +262,7 @@ void fem_solver_default_params(fem_solver_params *p);
  * (dune/fem/solver/parameter.hh:71-250).  Accepted (prefix "fem.solver.",
  * optionally followed by "linear.", "newton.linear." or "nonlinear.linear.",
  * or one of those prefixes alone, as scheme parameters are written):
- *   method                 cg | gmres | fgmres              (parameter.hh:170-195)
+ *   method                 cg | gmres | fgmres | bicgstab   (parameter.hh:170-195)
  *   tolerance, absolutetol, reductiontol   (:118-146; the last two deprecated)
  *   errormeasure           absolute | relative | residualreduction (:108-116)
  *   maxiterations, maxlineariterations (deprecated)           (:148-162)

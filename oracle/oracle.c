@@ -686,6 +686,53 @@ int ora_cg(int64_t n, ora_apply_fn op, void *opctx, ora_apply_fn pre, void *prec
     return it < maxit ? it : -it;
 }
 
+/* BiCGStab, right preconditioned, as LinearSolver::bicgstab
+ * (dune/fem/solver/linear/bicgstab.hh:56-222): the iterate is y with
+ * x = M^-1 y; the initial guess enters as y and the result is mapped back */
+int ora_bicgstab(int64_t n, ora_apply_fn op, void *opctx, ora_apply_fn pre, void *prectx,
+                 double *x, const double *b, double eps, int maxit, int criteria)
+{
+    double *r = (double *)malloc(sizeof(double) * n);
+    double *rs = (double *)malloc(sizeof(double) * n);
+    double *p = (double *)malloc(sizeof(double) * n);
+    double *s = (double *)malloc(sizeof(double) * n);
+    double *t = (double *)malloc(sizeof(double) * n);
+    double *z = pre ? (double *)malloc(sizeof(double) * n) : NULL;
+    double tol = eps;
+    if (criteria == 1) tol *= sqrt(dot(n, b, b));
+    if (pre) { pre(prectx, x, z); op(opctx, z, r); }
+    else op(opctx, x, r);
+    for (int64_t i = 0; i < n; ++i) r[i] = b[i] - r[i];
+    memcpy(p, r, sizeof(double) * n);
+    memcpy(rs, r, sizeof(double) * n);
+    double nu = dot(n, r, rs);
+    if (criteria == 2) tol *= sqrt(nu);
+    int it = 0;
+    while (1) {
+        if (pre) { pre(prectx, p, z); op(opctx, z, t); }
+        else op(opctx, p, t);
+        double alpha = nu / dot(n, t, rs);
+        for (int64_t i = 0; i < n; ++i) s[i] = r[i] - alpha * t[i];
+        if (pre) { pre(prectx, s, z); op(opctx, z, r); }
+        else op(opctx, s, r);
+        double g0 = dot(n, r, s), g1 = dot(n, r, r), g2 = dot(n, s, s), g3 = dot(n, s, rs), g4 = dot(n, r, rs);
+        double omega = g0 / g1;
+        double res = sqrt(g2 - omega * (2.0 * g0 - omega * g1));
+        double beta = (g3 - omega * g4) * alpha / (omega * nu);
+        nu = g3 - omega * g4;
+        for (int64_t i = 0; i < n; ++i) x[i] += alpha * p[i] + omega * s[i];
+        ++it;
+        if (res < tol || it >= maxit) break;
+        for (int64_t i = 0; i < n; ++i) {
+            r[i] = s[i] - omega * r[i];
+            p[i] = beta * (p[i] - omega * t[i]) + r[i];
+        }
+    }
+    if (pre) { pre(prectx, x, z); memcpy(x, z, sizeof(double) * n); }
+    free(r); free(rs); free(p); free(s); free(t); free(z);
+    return it >= maxit ? -it : it;
+}
+
 /* ------------------------------------------------------------------------ */
 /* GMRES / FGMRES                                                            */
 /* ------------------------------------------------------------------------ */

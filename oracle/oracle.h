@@ -126,6 +126,9 @@ int ora_cg(int64_t n, ora_apply_fn op, void *opctx, ora_apply_fn pre, void *prec
            double *x, const double *b, double eps, int maxit, int criteria);
 /* LinearSolver::gmres (dune/fem/solver/linear/gmres.hh:117-299): right
  * preconditioned, restarted, classical Gram-Schmidt, Givens on the host. */
+/* BiCGStab (bicgstab.hh:56-222), right preconditioned; signed iterations */
+int ora_bicgstab(int64_t n, ora_apply_fn op, void *opctx, ora_apply_fn pre, void *prectx,
+                 double *x, const double *b, double eps, int maxit, int criteria);
 int ora_gmres(int64_t n, ora_apply_fn op, void *opctx, ora_apply_fn pre, void *prectx,
               double *x, const double *b, int restart, double eps, int maxit, int criteria);
 /* Flexible GMRES(m) (Saad 1993): stores z_j = M_j^-1 v_j; the same

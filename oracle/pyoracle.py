@@ -80,7 +80,7 @@ def lib():
         L.ora_cg_apply.argtypes = [C.POINTER(CG), _dp, _dp]
         L.ora_cg_rhs.argtypes = [C.POINTER(CG), _dp, _dp]
         L.ora_cg_diag.argtypes = [C.POINTER(CG), _dp]
-        for f in ("ora_cg",):
+        for f in ("ora_cg", "ora_bicgstab"):
             getattr(L, f).argtypes = [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                       _dp, _dp, C.c_double, C.c_int, C.c_int]
         for f in ("ora_gmres", "ora_fgmres"):
@@ -282,8 +282,8 @@ def _solve(kind, op, b, x0=None, pre=None, tol=1e-10, maxit=1000, criteria=1, re
     cb = op.callback(nthreads) if isinstance(op, DGOperator) else op.callback()
     pcb = pre.cb if pre is not None else None
     L = lib()
-    if kind == "cg":
-        it = L.ora_cg(b.size, C.cast(cb, C.c_void_p), None,
+    if kind in ("cg", "bicgstab"):
+        it = (L.ora_cg if kind == "cg" else L.ora_bicgstab)(b.size, C.cast(cb, C.c_void_p), None,
                       C.cast(pcb, C.c_void_p) if pcb else None, None,
                       _ptr(x), _ptr(b), tol, maxit, criteria)
     else:
@@ -296,6 +296,10 @@ def _solve(kind, op, b, x0=None, pre=None, tol=1e-10, maxit=1000, criteria=1, re
 
 def cg(op, b, **kw):
     return _solve("cg", op, b, **kw)
+
+
+def bicgstab(op, b, **kw):
+    return _solve("bicgstab", op, b, **kw)
 
 
 def gmres(op, b, **kw):

@@ -208,3 +208,51 @@ def test_mixed_precision_refinement_reaches_fp64_residual(ctx, method):
     print("IR", method, info, flush=True)
     assert info["converged"] and info["true_rel_residual"] <= 1e-8
     assert info["outer_iterations"] >= 2
+
+
+def test_bicgstab_matches_oracle(ctx):
+    # LinearSolver::bicgstab restated (bicgstab.hh:56-222): nonsymmetric SIPG
+    # with advection, unpreconditioned; iteration count and solution against
+    # the oracle
+    rng = np.random.default_rng(12)
+    n, p = (5, 4, 3), 2
+    kc = rng.uniform(0.5, 2.0, size=60)
+    oop = ora.DGOperator(n, p, 40.0, (1.0, -0.5, 0.25), 0.0, kappa=("cell", kc))
+    b = rng.normal(size=oop.N)
+    xo, ito = ora.bicgstab(oop, b, tol=1e-10, maxit=3000, criteria=1)
+    op = fa.GalerkinOperator(ctx, fa.OP_DG_SIPG_3D, n, p, 40.0, (1.0, -0.5, 0.25), 0.0, fa.KAPPA_CELL)
+    op.set_kappa_cell(kc)
+    inv = fa.InverseOperator(ctx, method=fa.BICGSTAB, tolerance=1e-10, errormeasure=fa.RELATIVE, maxiterations=3000)
+    inv.bind(op)
+    X = op.vector()
+    inv(op.vector(b), X)
+    info = inv.info()
+    print("bicgstab gpu its", info["iterations"], "oracle its", ito, "true rel res", info["true_rel_residual"], flush=True)
+    assert info["converged"] and abs(info["iterations"] - ito) <= 1
+    assert np.abs(X.download() - xo).max() <= 1e-7 * np.abs(xo).max()
+    # non-convergence is a negative count
+    inv2 = fa.InverseOperator(ctx, method=fa.BICGSTAB, tolerance=1e-14, errormeasure=fa.RELATIVE, maxiterations=3)
+    inv2.bind(op)
+    X.clear()
+    inv2(op.vector(b), X)
+    assert inv2.iterations() == -3 and not inv2.info()["converged"]
+
+
+def test_bicgstab_block_jacobi_diffusion_matches_oracle(ctx):
+    rng = np.random.default_rng(13)
+    n, p = (5, 4, 3), 3
+    kc = 10.0 ** rng.uniform(-1, 1, size=60)
+    oop = ora.DGOperator(n, p, 90.0, (0, 0, 0), 0.0, kappa=("cell", kc))
+    b = rng.normal(size=oop.N)
+    xo, ito = ora.bicgstab(oop, b, pre=ora.block_jacobi(np.linalg.inv(oop.block_diag())), tol=1e-10, maxit=3000,
+                           criteria=1, nthreads=8)
+    op = fa.GalerkinOperator(ctx, fa.OP_DG_SIPG_3D, n, p, 90.0, (0, 0, 0), 0.0, fa.KAPPA_CELL)
+    op.set_kappa_cell(kc)
+    inv = fa.InverseOperator(ctx, method=fa.BICGSTAB, tolerance=1e-10, errormeasure=fa.RELATIVE, maxiterations=3000)
+    inv.bind(op, fa.Preconditioner(op, fa.PRE_BLOCK_JACOBI))
+    X = op.vector(np.zeros(op.N))
+    inv(op.vector(b), X)
+    info = inv.info()
+    print("bicgstab+bj gpu its", info["iterations"], "oracle its", ito, flush=True)
+    assert info["converged"] and abs(info["iterations"] - ito) <= 1
+    assert np.abs(X.download() - xo).max() <= 1e-7 * np.abs(xo).max()

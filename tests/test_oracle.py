@@ -514,3 +514,24 @@ def test_sipg_consistency_with_pointwise_kappa_and_boundary_data(p):
     assert np.abs(lhs - rhs).max() <= 1e-11 * np.abs(rhs).max()
     # the identity needs the boundary data: without it, it fails
     assert np.abs(lhs - op.rhs(M["fq"])).max() > 1e-3 * np.abs(rhs).max()
+
+
+def test_bicgstab_oracle_reaches_the_dense_solution():
+    # nonsymmetric SIPG with advection, with and without block Jacobi: the
+    # BiCGStab restatement converges to the dense solution; its residual
+    # estimate matches the true residual at the end (no preconditioner)
+    rng = np.random.default_rng(40)
+    n, p = (3, 3, 2), 2
+    op = ora.DGOperator(n, p, 40.0, (1.0, -0.5, 0.25), 0.0, kappa=("cell", rng.uniform(0.5, 2.0, size=18)))
+    A = assemble(op)
+    b = rng.normal(size=op.N)
+    xd = np.linalg.solve(A, b)
+    x, it = ora.bicgstab(op, b, tol=1e-10, maxit=2000, criteria=1)
+    assert it > 0 and np.abs(x - xd).max() <= 1e-7 * np.abs(xd).max()
+    assert np.linalg.norm(b - A @ x) <= 2e-10 * np.linalg.norm(b)
+    inv = np.linalg.inv(op.block_diag())
+    x2, it2 = ora.bicgstab(op, b, pre=ora.block_jacobi(inv), tol=1e-10, maxit=2000, criteria=1)
+    assert 0 < it2 < it and np.abs(x2 - xd).max() <= 1e-6 * np.abs(xd).max()
+    # non-convergence: negative count
+    _, it3 = ora.bicgstab(op, b, tol=1e-14, maxit=3, criteria=1)
+    assert it3 == -3
