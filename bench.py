@@ -199,8 +199,9 @@ def solve_section(fa, wl, ctx, n=48, pairs=2000, epochs=3, lr=1e-4, learned=True
     out["multigrid_setup_s_three_precisions"] = round(time.perf_counter() - t0, 3)
     run("fp64_cg_bj", fa.CG, bj64)
     run("fp64_fgmres_bj", fa.FGMRES, bj64, maxit=3000)
-    run("fp64_cg_mg", fa.CG, mg64)
-    run("mixed_fp32_cg_mg_ir", fa.CG, mg32, op32, refinement=1, inner_tolerance=1e-3, inner_maxiterations=2000)
+    run("fp64_cg_mg", fa.CG, mg64, graph=1)
+    run("mixed_fp32_cg_mg_ir", fa.CG, mg32, op32, refinement=1, inner_tolerance=1e-3, inner_maxiterations=2000,
+        graph=1)
     # the V-cycle on the BF16-MFMA operator: the inner Krylov solve keeps the
     # FP32 operator; FGMRES, since BF16 rounding makes the V-cycle slightly
     # nonlinear
@@ -245,10 +246,10 @@ def combined_section(fa, wl, ctx, n=96, learned=True, pairs=400, epochs=2, lr=1e
     mg64 = fa.Preconditioner(op64, fa.PRE_MULTIGRID)
     ctx.synchronize()
     out["multigrid_setup_s_fp64"] = round(time.perf_counter() - t0, 3)
-    run("fp64_cg_mg", fa.CG, mg64)
+    run("fp64_cg_mg", fa.CG, mg64, graph=1)
     op32 = _dg_cell(fa, ctx, P, fa.F32)
     run("mixed_fp32_cg_mg_ir", fa.CG, fa.Preconditioner(op32, fa.PRE_MULTIGRID), op32, refinement=1,
-        inner_tolerance=1e-3, inner_maxiterations=2000)
+        inner_tolerance=1e-3, inner_maxiterations=2000, graph=1)
     opbf = _dg_cell(fa, ctx, P, fa.BF16MFMA)
     run("mixed_bf16mg_fgmres_ir", fa.FGMRES, fa.Preconditioner(opbf, fa.PRE_MULTIGRID), op32, refinement=1,
         inner_tolerance=1e-3, inner_maxiterations=2000)

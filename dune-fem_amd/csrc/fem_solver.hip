@@ -697,8 +697,21 @@ static int run_krylov(Solver &S, int method, Operator *o, Precond *p, const T *b
         double *dq = fuse ? (double *)S.workbuf(5, sizeof(double) * nbd) : nullptr;
         double *dp = fuse ? (double *)S.workbuf(6, 2 * sizeof(double) * nbd) : nullptr;
         int chunk = S.snapshot ? S.snap_chunk : 4;
-        while (true) {
-            for (int c = 0; c < chunk; ++c) {
+        // one CG step as a HIP graph: every kernel of the step reads the
+        // device done flag, so replays after convergence do nothing; the step
+        // holds no memset node.  Captured per solve (the vectors differ).
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t gexec = nullptr;
+        struct GraphGuard {
+            hipGraph_t &g;
+            hipGraphExec_t &e;
+            ~GraphGuard()
+            {
+                if (e) (void)hipGraphExecDestroy(e);
+                if (g) (void)hipGraphDestroy(g);
+            }
+        } gguard{ graph, gexec };
+        auto cg_step = [&]() {
                 hipLaunchKernelGGL(k_cg_qupd<T>, dim3(G), dim3(256), 0, s, q, ss, pp, r, N, st, haspre ? 1 : 0);
                 if (fuse) {
                     o->apply_raw(q, h, &st->done, dq);
@@ -706,7 +719,7 @@ static int run_krylov(Solver &S, int method, Operator *o, Precond *p, const T *b
                     hipLaunchKernelGGL((k_cg_step<T, 2>), dim3(RED_DOT_BLOCKS), dim3(RED_THREADS), 0, s, x, pp, q, h, ss, r, dinv, N, st, part, cnt);
                     p->apply_raw(pp, ss, &st->done, dp);
                     hipLaunchKernelGGL(k_cg_fin_parts<3>, dim3(GF), dim3(RED_THREADS), 0, s, dp, nbd, st, part, cnt);
-                    continue;
+                    return;
                 }
                 o->apply_raw(q, h, &st->done);
                 hipLaunchKernelGGL((k_cg_step<T, 0>), dim3(RED_DOT_BLOCKS), dim3(RED_THREADS), 0, s, x, pp, q, h, ss, r, dinv, N, st, part, cnt);
@@ -719,6 +732,17 @@ static int run_krylov(Solver &S, int method, Operator *o, Precond *p, const T *b
                     p->apply_raw(pp, ss, &st->done);
                     hipLaunchKernelGGL((k_cg_step<T, 3>), dim3(RED_DOT_BLOCKS), dim3(RED_THREADS), 0, s, x, pp, q, h, ss, r, dinv, N, st, part, cnt);
                 }
+        };
+        if (S.prm.graph && !S.snapshot) {
+            FEM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            cg_step();
+            FEM_HIP(hipStreamEndCapture(s, &graph));
+            FEM_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+        }
+        while (true) {
+            for (int c = 0; c < chunk; ++c) {
+                if (gexec) FEM_HIP(hipGraphLaunch(gexec, s));
+                else cg_step();
             }
             FEM_HIP(hipGetLastError());
             S.fetch_state();

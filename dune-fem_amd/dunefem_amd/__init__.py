@@ -69,7 +69,8 @@ class SolverParams(C.Structure):
                 ("maxiterations", C.c_int), ("restart", C.c_int), ("verbose", C.c_int),
                 ("refinement", C.c_int), ("inner_tolerance", C.c_double),
                 ("inner_maxiterations", C.c_int), ("residual_check", C.c_int),
-                ("preconditioning", C.c_int), ("relaxation", C.c_double), ("precond_iterations", C.c_int)]
+                ("preconditioning", C.c_int), ("relaxation", C.c_double), ("precond_iterations", C.c_int),
+                ("graph", C.c_int)]
 
 
 class BatchInfo(C.Structure):
@@ -458,7 +459,7 @@ def solver_parameters(**kw):
     load_library().fem_solver_default_params(C.byref(p))
     for k, v in kw.items():
         if k in ("method", "errormeasure", "maxiterations", "restart", "verbose", "refinement",
-                 "inner_maxiterations", "residual_check", "preconditioning", "precond_iterations"):
+                 "inner_maxiterations", "residual_check", "preconditioning", "precond_iterations", "graph"):
             setattr(p, k, int(v))
         elif k in ("tolerance", "inner_tolerance", "relaxation"):
             setattr(p, k, float(v))

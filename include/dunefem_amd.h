@@ -245,6 +245,10 @@ typedef struct {
      * preconditioner applies d^-1 once and does not use them */
     double relaxation;
     int precond_iterations;
+    /* CG: capture one iteration (operator, preconditioner, updates) in a HIP
+     * graph and replay it; for preconditioners of many small launches (the
+     * multigrid V-cycle).  No reference counterpart; default 0 */
+    int graph;
 } fem_solver_params;
 
 typedef struct {

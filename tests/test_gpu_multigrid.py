@@ -116,3 +116,28 @@ def test_multigrid_solves_full_size_workloads(ctx, prob):
     info = inv.info()
     print(prob, "full size CG+MG:", info, flush=True)
     assert info["converged"] and info["true_rel_residual"] < 1.01e-8
+
+
+def test_cg_step_graph_replays_match_eager_solve(ctx):
+    # one CG + multigrid step captured as a HIP graph and replayed for every
+    # iteration (host synchronisation between chunks of replays): the same
+    # kernels in the same order, so iterations and iterate equal the eager
+    # solve bitwise; a second solve re-captures on the same solver
+    P = wl.problem3(n=16)
+    op = dg(ctx, P["n"], P["p"], P["beta"], ("cell", P["kcell"]))
+    pre = fa.Preconditioner(op, fa.PRE_MULTIGRID)
+    F = op.rhs(P["fq"])
+    res = {}
+    for graph in (0, 1):
+        inv = fa.InverseOperator(ctx, method=fa.CG, tolerance=1e-9, errormeasure=fa.RELATIVE, maxiterations=500,
+                                 graph=graph)
+        inv.bind(op, pre)
+        out = []
+        for rep in range(2):
+            X = op.vector()
+            inv(F, X)
+            out.append((inv.info()["iterations"], X.download()))
+        res[graph] = out
+    for rep in range(2):
+        assert res[0][rep][0] == res[1][rep][0] > 0
+        assert np.array_equal(res[0][rep][1], res[1][rep][1])
