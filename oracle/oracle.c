@@ -392,6 +392,213 @@ void ora_dg_apply(const ora_dg_t *op, const double *u, double *w, int nthreads)
     dg_tables_free(&T);
 }
 
+/* ---- sum-factorised apply (the CPU counterpart of the GPU algorithm) -------
+ * The same operator as ora_dg_apply, evaluated with 1D contractions per
+ * direction instead of the reference's dense nq x nb tables: u and its three
+ * reference gradients at the volume points by K x K passes, fluxes, and the
+ * transposed passes back; faces "owner computes": every element evaluates
+ * its own and its neighbour's traces (value and normal derivative) on its six
+ * faces and writes only its own DoFs (no locks, no deferred writes).  Used
+ * only to time a sum-factorised CPU apply beside the reference algorithm. */
+#define SF_KMAX 8
+
+/* out = M along dimension dim (0: a outermost, 2: c innermost) of a K^3 tensor */
+static void sf_pass(int K, const double *M /* [K][K]: out index, in index */, int dim, const double *in,
+                    double *out)
+{
+    const int s[3] = { K * K, K, 1 };
+    for (int a = 0; a < K; ++a)
+        for (int b = 0; b < K; ++b)
+            for (int c = 0; c < K; ++c) {
+                const int idx[3] = { a, b, c };
+                int base = 0;
+                for (int d = 0; d < 3; ++d)
+                    if (d != dim) base += idx[d] * s[d];
+                double acc = 0;
+                for (int k = 0; k < K; ++k) acc += M[idx[dim] * K + k] * in[base + k * s[dim]];
+                out[a * K * K + b * K + c] = acc;
+            }
+}
+
+typedef struct {
+    int K;
+    double B[SF_KMAX * SF_KMAX], Bt[SF_KMAX * SF_KMAX];     /* B[q][a] = phi_a(x_q) and transpose */
+    double D[SF_KMAX * SF_KMAX], Dt[SF_KMAX * SF_KMAX];     /* phi_a'(x_q) and transpose */
+    double P[2][SF_KMAX], dP[2][SF_KMAX], w[SF_KMAX];
+} sf_tab;
+
+static void sf_tab_init(sf_tab *t, int p)
+{
+    int K = p + 1;
+    double x[SF_KMAX];
+    t->K = K;
+    ora_gauss(K, x, t->w);
+    for (int q = 0; q < K; ++q)
+        for (int a = 0; a < K; ++a) {
+            t->B[q * K + a] = ora_legendre(a, x[q]);
+            t->Bt[a * K + q] = t->B[q * K + a];
+            t->D[q * K + a] = ora_legendre_d(a, x[q]);
+            t->Dt[a * K + q] = t->D[q * K + a];
+        }
+    for (int s = 0; s < 2; ++s)
+        for (int a = 0; a < K; ++a) {
+            t->P[s][a] = ora_legendre(a, (double)s);
+            t->dP[s][a] = ora_legendre_d(a, (double)s);
+        }
+}
+
+/* value and d/dxi_d (reference) of the modal tensor u at the K^2 points of
+ * face side s normal to d; point (q1, q2) over the tangential dims t1 < t2 */
+static void sf_trace(const sf_tab *t, const double *u, int d, int s, double *val, double *der)
+{
+    int K = t->K;
+    int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+    const int st[3] = { K * K, K, 1 };
+    double v2[SF_KMAX * SF_KMAX], d2[SF_KMAX * SF_KMAX];
+    /* normal contraction -> [a_t1][a_t2] */
+    for (int i = 0; i < K; ++i)
+        for (int j = 0; j < K; ++j) {
+            double a0 = 0, a1 = 0;
+            for (int m = 0; m < K; ++m) {
+                double um = u[m * st[d] + i * st[t1] + j * st[t2]];
+                a0 += t->P[s][m] * um;
+                a1 += t->dP[s][m] * um;
+            }
+            v2[i * K + j] = a0;
+            d2[i * K + j] = a1;
+        }
+    /* tangential contractions -> [q1][q2] stored q1 + K q2 */
+    for (int q1 = 0; q1 < K; ++q1)
+        for (int q2 = 0; q2 < K; ++q2) {
+            double a0 = 0, a1 = 0;
+            for (int i = 0; i < K; ++i) {
+                double bi = t->B[q1 * K + i];
+                for (int j = 0; j < K; ++j) {
+                    double bij = bi * t->B[q2 * K + j];
+                    a0 += bij * v2[i * K + j];
+                    a1 += bij * d2[i * K + j];
+                }
+            }
+            val[q1 + K * q2] = a0;
+            der[q1 + K * q2] = a1;
+        }
+}
+
+/* r += sum_q c0(q) phi(q) + c1(q) dphi/dxi_d(q) over the face points */
+static void sf_lift(const sf_tab *t, int d, int s, const double *c0, const double *c1, double *r)
+{
+    int K = t->K;
+    int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+    const int st[3] = { K * K, K, 1 };
+    double a0[SF_KMAX * SF_KMAX], a1[SF_KMAX * SF_KMAX];
+    for (int i = 0; i < K; ++i)
+        for (int j = 0; j < K; ++j) {
+            double x0 = 0, x1 = 0;
+            for (int q1 = 0; q1 < K; ++q1)
+                for (int q2 = 0; q2 < K; ++q2) {
+                    double bb = t->B[q1 * K + i] * t->B[q2 * K + j];
+                    x0 += bb * c0[q1 + K * q2];
+                    x1 += bb * c1[q1 + K * q2];
+                }
+            a0[i * K + j] = x0;
+            a1[i * K + j] = x1;
+        }
+    for (int m = 0; m < K; ++m)
+        for (int i = 0; i < K; ++i)
+            for (int j = 0; j < K; ++j)
+                r[m * st[d] + i * st[t1] + j * st[t2]] += t->P[s][m] * a0[i * K + j] + t->dP[s][m] * a1[i * K + j];
+}
+
+void ora_dg_apply_sf(const ora_dg_t *op, const double *u, double *w, int nthreads)
+{
+    sf_tab T;
+    sf_tab_init(&T, op->p);
+    const int K = T.K, K3 = K * K * K;
+    const int64_t E = (int64_t)op->n[0] * op->n[1] * op->n[2];
+    const int64_t nx = op->n[0], ny = op->n[1];
+    const double h[3] = { 1.0 / op->n[0], 1.0 / op->n[1], 1.0 / op->n[2] };
+    const double detJ = h[0] * h[1] * h[2];
+    if (nthreads < 1) nthreads = 1;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+#endif
+    for (int64_t e = 0; e < E; ++e) {
+        double t0[SF_KMAX * SF_KMAX * SF_KMAX], t1b[SF_KMAX * SF_KMAX * SF_KMAX];
+        double uq[SF_KMAX * SF_KMAX * SF_KMAX], g[3][SF_KMAX * SF_KMAX * SF_KMAX];
+        double r[SF_KMAX * SF_KMAX * SF_KMAX];
+        const double *ue = u + e * K3;
+        const int c[3] = { (int)(e % nx), (int)((e / nx) % ny), (int)(e / (nx * ny)) };
+        /* u and reference gradients at the volume points: tensor (qa, qb, qc) = (x, y, z) */
+        for (int d = -1; d < 3; ++d) {
+            sf_pass(K, d == 0 ? T.D : T.B, 0, ue, t0);
+            sf_pass(K, d == 1 ? T.D : T.B, 1, t0, t1b);
+            sf_pass(K, d == 2 ? T.D : T.B, 2, t1b, d < 0 ? uq : g[d]);
+        }
+        /* fluxes: W |K| (kappa g_d / h_d - b_d u) / h_d against dphi/dxi_d, W |K| c u against phi */
+        double F[3][SF_KMAX * SF_KMAX * SF_KMAX], S0[SF_KMAX * SF_KMAX * SF_KMAX];
+        for (int qa = 0; qa < K; ++qa)
+            for (int qb = 0; qb < K; ++qb)
+                for (int qc = 0; qc < K; ++qc) {
+                    int i = qa * K * K + qb * K + qc;
+                    double W = T.w[qa] * T.w[qb] * T.w[qc] * detJ;
+                    double kap = op->kappa_mode == ORA_KAPPA_QP ? op->kvol[e * K3 + qa + K * qb + K * K * qc]
+                               : op->kappa_mode == ORA_KAPPA_CELL ? op->kcell[e] : op->kappa_const;
+                    for (int d = 0; d < 3; ++d) F[d][i] = W * (kap * g[d][i] / h[d] - op->b[d] * uq[i]) / h[d];
+                    S0[i] = W * op->c * uq[i];
+                }
+        memset(r, 0, sizeof(double) * K3);
+        for (int d = -1; d < 3; ++d) {
+            const double *src = d < 0 ? S0 : F[d];
+            sf_pass(K, d == 0 ? T.Dt : T.Bt, 0, src, t0);
+            sf_pass(K, d == 1 ? T.Dt : T.Bt, 1, t0, t1b);
+            sf_pass(K, d == 2 ? T.Dt : T.Bt, 2, t1b, t0);
+            for (int i = 0; i < K3; ++i) r[i] += t0[i];
+        }
+        /* faces */
+        for (int d = 0; d < 3; ++d) {
+            int td1 = (d == 0) ? 1 : 0, td2 = (d == 2) ? 1 : 2;
+            double area = h[td1] * h[td2];
+            for (int s = 0; s < 2; ++s) {
+                double nrm = s ? 1.0 : -1.0;
+                int nbc = c[d] + (s ? 1 : -1);
+                int interior = nbc >= 0 && nbc < op->n[d];
+                int64_t fidx = dg_face_index(op, d, c, s);
+                double vin[SF_KMAX * SF_KMAX], din[SF_KMAX * SF_KMAX], vout[SF_KMAX * SF_KMAX], dout[SF_KMAX * SF_KMAX];
+                double c0[SF_KMAX * SF_KMAX], c1[SF_KMAX * SF_KMAX];
+                sf_trace(&T, ue, d, s, vin, din);
+                int64_t en = e;
+                if (interior) {
+                    int cn[3] = { c[0], c[1], c[2] };
+                    cn[d] = nbc;
+                    en = cn[0] + nx * (cn[1] + ny * (int64_t)cn[2]);
+                    sf_trace(&T, u + en * K3, d, 1 - s, vout, dout);
+                }
+                double bn = op->b[d] * nrm, bp = bn > 0 ? bn : 0.0, bm = bn < 0 ? -bn : 0.0;
+                for (int q = 0; q < K * K; ++q) {
+                    int q1 = q % K, q2 = q / K;
+                    double Wf = T.w[q1] * T.w[q2] * area;
+                    double kin, kout, kp;
+                    if (op->kappa_mode == ORA_KAPPA_QP) kin = kout = kp = op->kface[d][fidx * K * K + q];
+                    else if (op->kappa_mode == ORA_KAPPA_CELL) {
+                        kin = op->kcell[e];
+                        kout = interior ? op->kcell[en] : kin;
+                        kp = interior ? 0.5 * (kin + kout) : kin;
+                    } else kin = kout = kp = op->kappa_const;
+                    double dnin = din[q] * nrm / h[d];
+                    double uo = interior ? vout[q] : 0.0, dno = interior ? dout[q] * nrm / h[d] : 0.0;
+                    double wgt = interior ? 0.5 : 1.0;
+                    double jump = vin[q] - uo;
+                    double sigma = wgt * (kin * dnin + kout * dno);
+                    c0[q] = Wf * (kp * op->beta / h[d] * jump - sigma + bp * vin[q] - bm * uo);
+                    c1[q] = Wf * (-wgt * kin * jump) * nrm / h[d];
+                }
+                sf_lift(&T, d, s, c0, c1, r);
+            }
+        }
+        memcpy(w + e * K3, r, sizeof(double) * K3);
+    }
+}
+
 void ora_dg_rhs(const ora_dg_t *op, const double *fq, double *F)
 {
     dg_tables T;

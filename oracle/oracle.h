@@ -86,6 +86,9 @@ typedef struct {
 int64_t ora_dg_ndofs(const ora_dg_t *op);
 /* w = A u (w overwritten).  nthreads <= 1: serial. */
 void ora_dg_apply(const ora_dg_t *op, const double *u, double *w, int nthreads);
+/* the same w = A u evaluated by sum factorisation (1D contractions, faces
+ * owner-computes); the labelled sum-factorised CPU baseline */
+void ora_dg_apply_sf(const ora_dg_t *op, const double *u, double *w, int nthreads);
 /* F_i = int f phi_i with f given at the volume quadrature points [E][m^3] */
 void ora_dg_rhs(const ora_dg_t *op, const double *fq, double *F);
 /* F += the boundary-data part of the load vector (Nitsche and inflow terms

@@ -72,6 +72,7 @@ def lib():
         L.ora_dg_ndofs.argtypes = [C.POINTER(DG)]
         L.ora_dg_ndofs.restype = C.c_int64
         L.ora_dg_apply.argtypes = [C.POINTER(DG), _dp, _dp, C.c_int]
+        L.ora_dg_apply_sf.argtypes = [C.POINTER(DG), _dp, _dp, C.c_int]
         L.ora_dg_rhs.argtypes = [C.POINTER(DG), _dp, _dp]
         L.ora_dg_block_diag.argtypes = [C.POINTER(DG), _dp]
         L.ora_dg_rhs_bnd.argtypes = [C.POINTER(DG), C.POINTER(_dp), _dp]
@@ -159,6 +160,13 @@ class DGOperator:
         u = np.ascontiguousarray(u, dtype=np.float64)
         w = np.empty(self.N)
         lib().ora_dg_apply(C.byref(self.s), _ptr(u), _ptr(w), nthreads)
+        return w
+
+    def apply_sf(self, u, nthreads=1):
+        """the same apply evaluated by sum factorisation (timing baseline)"""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        w = np.empty(self.N)
+        lib().ora_dg_apply_sf(C.byref(self.s), _ptr(u), _ptr(w), nthreads)
         return w
 
     def rhs(self, fq):

@@ -535,3 +535,29 @@ def test_bicgstab_oracle_reaches_the_dense_solution():
     # non-convergence: negative count
     _, it3 = ora.bicgstab(op, b, tol=1e-14, maxit=3, criteria=1)
     assert it3 == -3
+
+
+@pytest.mark.parametrize("p,kmode,b,c", [(1, "qp", (1.0, 0.5, 0.25), 0.0), (2, "cell", (0, 0, 0), 0.3),
+                                         (3, "qp", (1.0, -0.5, 0.25), 0.0), (4, "cell", (0.3, 0.2, -1.0), 1.0),
+                                         (3, "const", (0, 0, 0), 0.0)])
+def test_sum_factorised_cpu_apply_equals_dense_oracle(p, kmode, b, c):
+    # the sum-factorised CPU apply (timing baseline) is the same operator as
+    # the dense restatement of the reference algorithm
+    rng = np.random.default_rng(50 + p)
+    n = (4, 3, 5)
+    E, m = 60, p + 1
+    if kmode == "qp":
+        kvol = rng.uniform(0.2, 5.0, size=(E, m ** 3))
+        kf = tuple(rng.uniform(0.2, 5.0, size=((n[0] + (d == 0)) * (n[1] + (d == 1)) * (n[2] + (d == 2)), m * m))
+                   for d in range(3))
+        kappa = ("qp", kvol, kf)
+    elif kmode == "cell":
+        kappa = ("cell", rng.uniform(0.1, 10.0, size=E))
+    else:
+        kappa = 1.7
+    op = ora.DGOperator(n, p, 10.0 * p * p, b, c, kappa=kappa)
+    u = rng.normal(size=op.N)
+    w = op.apply(u)
+    for nt in (1, 4):
+        ws = op.apply_sf(u, nthreads=nt)
+        assert np.abs(ws - w).max() <= 1e-12 * np.abs(w).max()
