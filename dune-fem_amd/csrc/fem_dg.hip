@@ -72,6 +72,16 @@ struct DGShape {
     static constexpr int SZ = ((SZ0 + 15) / 16) * 16 + (((SZ0 + 15) / 16) % 2 == 0 ? 16 : 0);
 };
 
+// Workgroup of the apply: one wavefront of EPB = 64 / K^2 elements, except
+// at K = 5, where two wavefronts hold five elements (125 of 128 lanes busy
+// instead of 50 of 64); the pencil exchanges of an element may then cross
+// the two wavefronts and synchronise the workgroup.
+template <int K>
+struct ApplyShape {
+    static constexpr int WG = (K == 5) ? 128 : 64;
+    static constexpr int EPB = WG / (K * K);
+};
+
 template <int K>
 __device__ __forceinline__ int lidx(int a, int b, int c)
 {
@@ -88,6 +98,15 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// the exchange barrier of a workgroup of WG threads: a wavefront fence for
+// one wavefront, the workgroup barrier for more
+template <int WG>
+__device__ __forceinline__ void lds_sync()
+{
+    if constexpr (WG > 64) __syncthreads();
+    else wave_lds_sync();
 }
 
 // Cross-lane moves inside a 16-lane DPP row (one K = 4 element of the FP32
@@ -299,12 +318,12 @@ struct Ctr {
 // KQ: kappa given at the quadrature points (FEM_KAPPA_QP); otherwise one
 // value per element (FEM_KAPPA_CELL) or one constant (FEM_KAPPA_CONST)
 template <int K, typename T, bool MF, bool KQ>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(ApplyShape<K>::WG)
     k_dg_apply(const DGArgs<T> a, const T *__restrict__ u, T *__restrict__ w, const int *__restrict__ skip,
                double *__restrict__ dot)
 {
     using S = DGShape<K>;
-    constexpr int K2 = S::K2, K3 = S::K3, EPB = S::EPB;
+    constexpr int K2 = S::K2, K3 = S::K3, EPB = ApplyShape<K>::EPB, WG = ApplyShape<K>::WG;
     if (skip && *skip) return;
     // FP64 MFMA path (K = 4, lane 16 i + 4 el + j): a 16-lane LDS access
     // group holds one i of all four elements.  The layout (a, b, c) ->
@@ -464,16 +483,16 @@ __global__ void __launch_bounds__(64)
     if constexpr (KLDS)
 #pragma unroll
         for (int q = 0; q < K; ++q) KA[L(q, i, j)] = kx[q];
-    wave_lds_sync();
+    lds_sync<WG>();
     // Y layout: thread (i=a, j=qz); each thread rewrites the pencil it read
 #pragma unroll
     for (int b = 0; b < K; ++b) v[b] = A[L(i, b, j)];
     C::B(tb, mr, v, tmp);
-    wave_lds_sync();
+    lds_sync<WG>();
     if (wr)
 #pragma unroll
         for (int q = 0; q < K; ++q) A[L(i, q, j)] = tmp[q];
-    wave_lds_sync();
+    lds_sync<WG>();
 #pragma unroll
     for (int d = 0; d < 3; ++d)
 #pragma unroll
@@ -510,7 +529,7 @@ __global__ void __launch_bounds__(64)
                 nds[s] = row_contract(tb.B[i], i, quad_contract(tb.B[j], nder[d][s]));
             }
         } else {
-            wave_lds_sync();
+            lds_sync<WG>();
             if (wr) {
 #pragma unroll
                 for (int s = 0; s < 2; ++s) {
@@ -518,7 +537,7 @@ __global__ void __launch_bounds__(64)
                     Sx[(s * 2 + 1) * SXQ + i * SXI + j] = nder[d][s];
                 }
             }
-            wave_lds_sync();
+            lds_sync<WG>();
             // second tangential index, in place: thread (i=m1, j=q2)
             T t2v[4], w2[K];
 #pragma unroll
@@ -527,11 +546,11 @@ __global__ void __launch_bounds__(64)
                 for (int m = 0; m < K; ++m) w2[m] = Sx[sq * SXQ + i * SXI + m];
                 t2v[sq] = dotk<K>(tb.B[j], w2);
             }
-            wave_lds_sync();
+            lds_sync<WG>();
             if (wr)
 #pragma unroll
                 for (int sq = 0; sq < 4; ++sq) Sx[sq * SXQ + i * SXI + j] = t2v[sq];
-            wave_lds_sync();
+            lds_sync<WG>();
             // first tangential index: thread (i=q1, j=q2)
 #pragma unroll
             for (int s = 0; s < 2; ++s) {
@@ -613,7 +632,7 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
     for (int b = 0; b < K; ++b) v[b] = A[L(b, i, j)];
     C::B(tb, mr, v, uq);
-    wave_lds_sync();
+    lds_sync<WG>();
     if (wr)
 #pragma unroll
         for (int q = 0; q < K; ++q) A[L(q, i, j)] = uq[q];
@@ -636,7 +655,7 @@ __global__ void __launch_bounds__(64)
         }
         face_terms(0, uq, Rx);
     }
-    wave_lds_sync();
+    lds_sync<WG>();
     // Y layout of u_q: thread (i=qx, j=qz) -> R = y part
     {
         const T sc = wij * a.vsc[1], bh = a.bh[1];
@@ -657,7 +676,7 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
             for (int q = 0; q < K; ++q) R[L(i, q, j)] = tmp[q];
     }
-    wave_lds_sync();
+    lds_sync<WG>();
     // Z layout of u_q: thread (i=qx, j=qy) -> R += z part
     {
         const T sc = wij * a.vsc[2], bh = a.bh[2];
@@ -680,7 +699,7 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
             for (int q = 0; q < K; ++q) R[L(i, j, q)] = tmp[q];
     }
-    wave_lds_sync();
+    lds_sync<WG>();
 
     // ---- backward: quadrature points -> modal test functions ---------------
     // X layout: thread (i=qy, j=qz)
@@ -691,16 +710,16 @@ __global__ void __launch_bounds__(64)
     if (wr)
 #pragma unroll
         for (int q = 0; q < K; ++q) A[L(q, i, j)] = tmp[q];
-    wave_lds_sync();
+    lds_sync<WG>();
     // Y layout: thread (i=a, j=qz)
 #pragma unroll
     for (int q = 0; q < K; ++q) v[q] = A[L(i, q, j)];
     C::BT(tb, mr, v, tmp);
-    wave_lds_sync();
+    lds_sync<WG>();
     if (wr)
 #pragma unroll
         for (int q = 0; q < K; ++q) A[L(i, q, j)] = tmp[q];
-    wave_lds_sync();
+    lds_sync<WG>();
     // Z layout: thread (i=a, j=b)
 #pragma unroll
     for (int q = 0; q < K; ++q) v[q] = A[L(i, j, q)];
@@ -718,7 +737,18 @@ __global__ void __launch_bounds__(64)
             for (int c = 0; c < K; ++c) acc += (double)ue[i * K2 + j * K + c] * (double)tmp[c];
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-        if (tid == 0) dot[blockIdx.x] = acc;
+        if constexpr (WG > 64) {
+            __shared__ double wsum[WG / 64];
+            if ((tid & 63) == 0) wsum[tid >> 6] = acc;
+            __syncthreads();
+            if (tid == 0) {
+                double t = 0;
+                for (int k = 0; k < WG / 64; ++k) t += wsum[k];
+                dot[blockIdx.x] = t;
+            }
+        } else {
+            if (tid == 0) dot[blockIdx.x] = acc;
+        }
     }
 }
 
@@ -726,13 +756,14 @@ template <int K, typename T, bool MF = false>
 static void launch_apply(const DGArgs<T> &a, const T *u, T *w, const int *skip, hipStream_t s, double *dot = nullptr)
 {
     using S = DGShape<K>;
+    using A = ApplyShape<K>;
     const int64_t E = (int64_t)a.n[0] * a.n[1] * a.n[2];
     FEM_CHECK(E * S::K3 < (int64_t)1 << 31, FEM_ERR_UNSUPPORTED, "DG apply: more than 2^31 DoFs");
-    const unsigned nb = (unsigned)((E + S::EPB - 1) / S::EPB);
+    const unsigned nb = (unsigned)((E + A::EPB - 1) / A::EPB);
     if (a.kappa_mode == FEM_KAPPA_QP)
-        hipLaunchKernelGGL((k_dg_apply<K, T, MF, true>), dim3(nb), dim3(64), 0, s, a, u, w, skip, dot);
+        hipLaunchKernelGGL((k_dg_apply<K, T, MF, true>), dim3(nb), dim3(A::WG), 0, s, a, u, w, skip, dot);
     else
-        hipLaunchKernelGGL((k_dg_apply<K, T, MF, false>), dim3(nb), dim3(64), 0, s, a, u, w, skip, dot);
+        hipLaunchKernelGGL((k_dg_apply<K, T, MF, false>), dim3(nb), dim3(A::WG), 0, s, a, u, w, skip, dot);
     FEM_HIP(hipGetLastError());
 }
 
@@ -760,10 +791,22 @@ void dg_apply_f32(const DGArgs<float> &a, int k, const float *u, float *w, const
     dispatch_apply<float>(a, k, u, w, skip, s, dot);
 }
 
+int dg_apply_epb(int k)
+{
+    FEM_CHECK(k >= 2 && k <= 5, FEM_ERR_UNSUPPORTED, "DG: order must be 1..4");
+    return k == 5 ? ApplyShape<5>::EPB : 64 / (k * k);
+}
+
 int64_t dg_blocks(int k, int64_t E)
 {
-    const int epb = 64 / (k * k);
+    const int epb = dg_apply_epb(k);
+    return (E + epb - 1) / epb;
+}
+
+int64_t dg_bj_blocks(int k, int64_t E)
+{
     FEM_CHECK(k >= 2 && k <= 5, FEM_ERR_UNSUPPORTED, "DG: order must be 1..4");
+    const int epb = 64 / (k * k);
     return (E + epb - 1) / epb;
 }
 

@@ -50,12 +50,15 @@ struct DGArgs {
     UDiv dv_slab, dv_zrow, dv_rpb;   // by rpb ty nz, rpb ty, rpb
 };
 
+// elements per workgroup of the apply for k points per direction
+int dg_apply_epb(int k);
+
 // fills the per-direction constants and the traversal of DGArgs from n, h,
 // ih, beta, b and the number of 1D points k
 template <typename T>
 inline void dg_derived(DGArgs<T> &a, int k)
 {
-    const int epb = 64 / (k * k);
+    const int epb = dg_apply_epb(k);
     a.slabs = a.n[0] % epb == 0;
     a.rpb = a.n[0] / epb;
     const int ny = a.n[1];
@@ -86,8 +89,10 @@ void dg_apply_f32(const DGArgs<float> &a, int k, const float *u, float *w, const
                   double *dot = nullptr);
 // BF16 MFMA contraction variant (FP32 data and accumulation)
 void dg_apply_bf16(const DGArgs<float> &a, int k, const float *u, float *w, const int *skip, hipStream_t s);
-// workgroups of the DG apply and block-Jacobi launches for order k-1
+// workgroups of the DG apply launch for order k-1 (its per-workgroup dot
+// partials), and of the block-Jacobi launch
 int64_t dg_blocks(int k, int64_t E);
+int64_t dg_bj_blocks(int k, int64_t E);
 // operator diagonal
 void dg_diagonal(const DGArgs<double> &a, int k, double *d, hipStream_t s);
 // load vector from f at volume quadrature points
@@ -103,7 +108,7 @@ void dg_rhs_bnd(const DGArgs<double> &a, int k, const double *const g[3], double
 void dg_block_setup(const DGArgs<double> &a, int k, double *Q /*[E][3][K][K]*/, double *lam /*[E][3][K]*/,
                     hipStream_t s);
 // `dot` (may be null): each workgroup writes its FP64 partials of <x, y> to
-// dot[workgroup] and of <x, x> to dot[dg_blocks(k, E) + workgroup]
+// dot[workgroup] and of <x, x> to dot[dg_bj_blocks(k, E) + workgroup]
 template <typename T>
 void dg_block_apply(int k, int64_t E, const T *Q, const T *lam, const T *x, T *y, const int *skip, hipStream_t s,
                     double *dot = nullptr);

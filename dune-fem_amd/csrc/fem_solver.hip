@@ -691,11 +691,15 @@ static int run_krylov(Solver &S, int method, Operator *o, Precond *p, const T *b
         // write per-workgroup partials of the two CG dots, which a small
         // kernel finishes, instead of two more passes over the vectors
         const bool fuse = o->is_dg() && haspre && p->fused_dot() && o->desc.precision != FEM_BF16MFMA;
-        const int64_t nbd = fuse ? o->dot_blocks() : 0;
+        const int64_t nbd = fuse ? o->dot_blocks() : 0;        // apply workgroups
+        const int64_t nbj = fuse ? dg_bj_blocks(p->op->k, p->op->E) : 0;   // block-Jacobi workgroups
         // about 4 partials per thread: a small launch finishes quickly
-        const unsigned GF = (unsigned)std::min<int64_t>(RED_DOT_BLOCKS, std::max<int64_t>(1, (nbd + 4 * RED_THREADS - 1) / (4 * RED_THREADS)));
+        auto fin_grid = [](int64_t n) {
+            return (unsigned)std::min<int64_t>(RED_DOT_BLOCKS, std::max<int64_t>(1, (n + 4 * RED_THREADS - 1) / (4 * RED_THREADS)));
+        };
+        const unsigned GF = fin_grid(nbd), GJ = fin_grid(nbj);
         double *dq = fuse ? (double *)S.workbuf(5, sizeof(double) * nbd) : nullptr;
-        double *dp = fuse ? (double *)S.workbuf(6, 2 * sizeof(double) * nbd) : nullptr;
+        double *dp = fuse ? (double *)S.workbuf(6, 2 * sizeof(double) * nbj) : nullptr;
         int chunk = S.snapshot ? S.snap_chunk : 4;
         // one CG step as a HIP graph: every kernel of the step reads the
         // device done flag, so replays after convergence do nothing; the step
@@ -718,7 +722,7 @@ static int run_krylov(Solver &S, int method, Operator *o, Precond *p, const T *b
                     hipLaunchKernelGGL(k_cg_fin_parts<0>, dim3(GF), dim3(RED_THREADS), 0, s, dq, nbd, st, part, cnt);
                     hipLaunchKernelGGL((k_cg_step<T, 2>), dim3(RED_DOT_BLOCKS), dim3(RED_THREADS), 0, s, x, pp, q, h, ss, r, dinv, N, st, part, cnt);
                     p->apply_raw(pp, ss, &st->done, dp);
-                    hipLaunchKernelGGL(k_cg_fin_parts<3>, dim3(GF), dim3(RED_THREADS), 0, s, dp, nbd, st, part, cnt);
+                    hipLaunchKernelGGL(k_cg_fin_parts<3>, dim3(GJ), dim3(RED_THREADS), 0, s, dp, nbj, st, part, cnt);
                     return;
                 }
                 o->apply_raw(q, h, &st->done);

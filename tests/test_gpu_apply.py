@@ -63,6 +63,11 @@ CASES = [
     # slabs of 2 rows, 5 or 7 layers)
     ((12, 6, 5), 3, (-1, .5, .25), 0.0, "qp"),
     ((6, 10, 7), 4, (0, 0, 0), 0.0, "cell"),
+    # Q4 in workgroups of five elements: nx a multiple of five (x-row
+    # traversal in slabs) and not (element order), pointwise and cellwise kappa
+    ((10, 4, 3), 4, (1, .5, .25), 0.0, "qp"),
+    ((5, 6, 4), 4, (0, 0, 0), 0.2, "cell"),
+    ((7, 3, 2), 4, (-.5, 1, .25), 0.0, "qp"),
 ]
 
 
