@@ -43,4 +43,4 @@ struct Batch {
 
 }  // namespace femamd
 
-struct fem_batch { femamd::Batch b; };
+struct fem_batch { femamd::Batch b; femamd::Context *held = nullptr; };

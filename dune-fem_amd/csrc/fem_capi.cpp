@@ -1,6 +1,8 @@
 // extern "C" boundary: opaque handles, int status codes, no exceptions
 // across the ABI (the reference throws Dune exceptions; non-convergence is a
 // negative iteration count, not an error).
+#include <cstddef>
+#include <type_traits>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -70,21 +72,36 @@ int fem_ctx_create(int device, fem_ctx **out)
     });
 }
 
+static_assert(std::is_standard_layout<fem_ctx>::value && offsetof(fem_ctx, c) == 0,
+              "a Context pointer converts back to its fem_ctx");
+
+// Every handle made on a context holds a reference to it, so a handle that
+// outlives fem_ctx_destroy still finds its stream and reduction workspace.
+static Context *ctx_retain(Context *c)
+{
+    c->refs.fetch_add(1);
+    return c;
+}
+
+static void ctx_release(Context *c)
+{
+    if (!c || c->refs.fetch_sub(1) != 1) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->red_partial) (void)hipFree(c->red_partial);
+    if (c->red_counter) (void)hipFree(c->red_counter);
+    if (c->red_result) (void)hipFree(c->red_result);
+    if (c->host_result) (void)hipHostFree(c->host_result);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete reinterpret_cast<fem_ctx *>(c);
+}
+
 int fem_ctx_destroy(fem_ctx *h)
 {
     return guarded([&] {
-        if (!h) return;
-        Context &c = h->c;
-        (void)hipSetDevice(c.device);
-        if (c.stream) (void)hipStreamSynchronize(c.stream);
-        if (c.red_partial) (void)hipFree(c.red_partial);
-        if (c.red_counter) (void)hipFree(c.red_counter);
-        if (c.red_result) (void)hipFree(c.red_result);
-        if (c.host_result) (void)hipHostFree(c.host_result);
-        if (c.ev0) (void)hipEventDestroy(c.ev0);
-        if (c.ev1) (void)hipEventDestroy(c.ev1);
-        if (c.stream) (void)hipStreamDestroy(c.stream);
-        delete h;
+        if (h) ctx_release(&h->c);
     });
 }
 
@@ -124,10 +141,12 @@ int fem_vec_create(fem_ctx *ctx, int64_t n, int precision, fem_vec **out)
         h->v.ctx = &ctx->c;
         h->v.n = n;
         h->v.prec = precision;
+        h->held = ctx_retain(&ctx->c);
         if (n > 0) {
             hipError_t e = hipMalloc(&h->v.data, h->v.bytes());
             if (e != hipSuccess) {
                 delete h;
+                ctx_release(&ctx->c);
                 throw Error(FEM_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
             }
             FEM_HIP(hipMemsetAsync(h->v.data, 0, h->v.bytes(), ctx->c.stream));
@@ -155,6 +174,7 @@ int fem_vec_wrap(fem_ctx *ctx, int64_t n, int precision, void *device_ptr, fem_v
         h->v.prec = precision;
         h->v.data = n > 0 ? device_ptr : nullptr;
         h->v.owner = false;
+        h->held = ctx_retain(&ctx->c);
         *out = h;
     });
 }
@@ -167,7 +187,9 @@ int fem_vec_destroy(fem_vec *v)
             (void)hipStreamSynchronize(v->v.ctx->stream);
             (void)hipFree(v->v.data);
         }
+        Context *c = v->held;
         delete v;
+        ctx_release(c);
     });
 }
 
@@ -292,6 +314,7 @@ int fem_op_create(fem_ctx *ctx, const fem_op_desc *desc, fem_op **out)
             delete h;
             throw;
         }
+        h->held = ctx_retain(&ctx->c);
         *out = h;
     });
 }
@@ -301,7 +324,9 @@ int fem_op_destroy(fem_op *op)
     return guarded([&] {
         if (op) {
             (void)hipStreamSynchronize(op->o.ctx->stream);
+            Context *c = op->held;
             delete op;
+            ctx_release(c);
         }
     });
 }
@@ -387,13 +412,19 @@ int fem_op_assemble_csr(fem_op *op, fem_csr **out)
             delete h;
             throw;
         }
+        h->held = ctx_retain(&*op->o.ctx);
         *out = h;
     });
 }
 
 int fem_csr_destroy(fem_csr *A)
 {
-    return guarded([&] { delete A; });
+    return guarded([&] {
+        if (!A) return;
+        Context *c = A->held;
+        delete A;
+        ctx_release(c);
+    });
 }
 
 int64_t fem_csr_rows(const fem_csr *A) { return A ? A->m.N : -1; }
@@ -435,13 +466,19 @@ int fem_precond_create(fem_op *op, int kind, fem_precond **out)
             delete h;
             throw;
         }
+        h->held = ctx_retain(&*op->o.ctx);
         *out = h;
     });
 }
 
 int fem_precond_destroy(fem_precond *p)
 {
-    return guarded([&] { delete p; });
+    return guarded([&] {
+        if (!p) return;
+        Context *c = p->held;
+        delete p;
+        ctx_release(c);
+    });
 }
 
 int fem_precond_apply(fem_precond *p, const fem_vec *x, fem_vec *y)
@@ -578,13 +615,19 @@ int fem_solver_create(fem_ctx *ctx, const fem_solver_params *params, fem_solver 
             delete h;
             throw;
         }
+        h->held = ctx_retain(&ctx->c);
         *out = h;
     });
 }
 
 int fem_solver_destroy(fem_solver *s)
 {
-    return guarded([&] { delete s; });
+    return guarded([&] {
+        if (!s) return;
+        Context *c = s->held;
+        delete s;
+        ctx_release(c);
+    });
 }
 
 int fem_solver_bind(fem_solver *s, fem_op *op, fem_precond *pre, fem_op *iop, fem_precond *ipre)
@@ -626,6 +669,7 @@ int fem_cnn_create(fem_ctx *ctx, int nx, int ny, int nz, const int *dil, fem_cnn
             delete h;
             throw;
         }
+        h->held = ctx_retain(&ctx->c);
         *out = h;
     });
 }
@@ -635,7 +679,9 @@ int fem_cnn_destroy(fem_cnn *c)
     return guarded([&] {
         if (c) {
             (void)hipStreamSynchronize(c->c.ctx->stream);
+            Context *cx = c->held;
             delete c;
+            ctx_release(cx);
         }
     });
 }
@@ -772,6 +818,7 @@ int fem_precond_create_learned_ex(fem_op *op, fem_cnn *cnn, int coarse_steps, in
             delete h;
             throw;
         }
+        h->held = ctx_retain(&*op->o.ctx);
         *out = h;
     });
 }
@@ -803,6 +850,7 @@ int fem_batch_create(fem_ctx *ctx, int count, const int *nx, const int *ny, doub
             delete h;
             throw;
         }
+        h->held = ctx_retain(&ctx->c);
         *out = h;
     });
 }
@@ -812,7 +860,9 @@ int fem_batch_destroy(fem_batch *b)
     return guarded([&] {
         if (b) {
             (void)hipStreamSynchronize(b->b.ctx->stream);
+            Context *c = b->held;
             delete b;
+            ctx_release(c);
         }
     });
 }

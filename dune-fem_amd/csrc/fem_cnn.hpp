@@ -54,4 +54,4 @@ struct CNN {
 
 }  // namespace femamd
 
-struct fem_cnn { femamd::CNN c; };
+struct fem_cnn { femamd::CNN c; femamd::Context *held = nullptr; };

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <cstring>
 #include <stdexcept>
@@ -86,6 +87,9 @@ struct Context {
     double *red_result = nullptr;      // device scalar results [RED_MAX_SLOTS]
     double *host_result = nullptr;     // pinned mirror
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // one reference for the fem_ctx handle and one for every handle made on
+    // it: the stream and workspaces live until the last of them is destroyed
+    std::atomic<int> refs{1};
 };
 
 constexpr int RED_BLOCKS = 512;      // fixed grid of the reduction kernels
@@ -125,4 +129,4 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned bid, unsigned nblocks)
 }  // namespace femamd
 
 struct fem_ctx { femamd::Context c; };
-struct fem_vec { femamd::Vec v; };
+struct fem_vec { femamd::Vec v; femamd::Context *held = nullptr; };

@@ -20,4 +20,4 @@ struct CSR {
 
 }  // namespace femamd
 
-struct fem_csr { femamd::CSR m; };
+struct fem_csr { femamd::CSR m; femamd::Context *held = nullptr; };

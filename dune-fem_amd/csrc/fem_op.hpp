@@ -95,5 +95,5 @@ struct Precond {
 
 }  // namespace femamd
 
-struct fem_op { femamd::Operator o; };
-struct fem_precond { femamd::Precond p; };
+struct fem_op { femamd::Operator o; femamd::Context *held = nullptr; };
+struct fem_precond { femamd::Precond p; femamd::Context *held = nullptr; };

@@ -61,4 +61,4 @@ struct Solver {
 
 }  // namespace femamd
 
-struct fem_solver { femamd::Solver s; };
+struct fem_solver { femamd::Solver s; femamd::Context *held = nullptr; };

@@ -48,6 +48,9 @@ int fem_version(void);
  * One device + stream (replaces the MPIManager thread pool as the place where
  * parallel work runs: dune/fem/misc/mpimanager.hh:155-198). */
 int fem_ctx_create(int device, fem_ctx **out);
+/* Drops the caller's reference: every handle made on the context (vectors,
+ * operators, preconditioners, solvers, matrices, networks, batches) holds
+ * its own, and the stream is released with the last of them. */
 int fem_ctx_destroy(fem_ctx *ctx);
 int fem_ctx_synchronize(fem_ctx *ctx);
 /* elapsed milliseconds between two event marks recorded on the context stream */

@@ -225,6 +225,33 @@ def test_dg_apply_bf16_mfma_error_bound(ctx, n, p, kind):
     assert err < 1e-2
 
 
+def test_handles_outlive_their_context_handle():
+    # fem_ctx_destroy drops one reference; vectors, operators and solvers
+    # made on the context keep its stream and workspaces until they go
+    # (a test that fails keeps its handles alive past the session context)
+    c = fa.Context(0)
+    rng = np.random.default_rng(43)
+    n, p = (3, 3, 2), 2
+    kappa = ("cell", rng.uniform(0.5, 2.0, size=18))
+    gop = make_dg(c, n, p, 40.0, (0, 0, 0), 0.0, kappa)
+    u = rng.normal(size=gop.N)
+    A, W, X = gop.vector(u), gop.vector(), gop.vector()
+    inv = fa.InverseOperator(c, method=fa.CG, tolerance=1e-10, errormeasure=fa.RELATIVE, maxiterations=500)
+    inv.bind(gop)
+    gop(A, W)
+    w0 = W.download()
+    c.close()
+    W.clear()
+    gop(A, W)
+    assert np.array_equal(W.download(), w0)
+    inv(W, X)
+    assert inv.info()["converged"]
+    assert np.abs(X.download() - u).max() <= 1e-7 * np.abs(u).max()
+    del inv, X, W, A, gop
+    import gc
+    gc.collect()
+
+
 def test_vec_wrap_is_a_view(ctx):
     # fem_vec_wrap: a handle over DoFs the caller already holds on the device
     # (here the memory of another vector); applies and BLAS act in place and

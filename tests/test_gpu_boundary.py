@@ -91,7 +91,7 @@ def test_solver_builds_preconditioner_from_parameter_key(ctx):
     op = dg_cell(ctx, n, p, kc)
     f = rng.normal(size=op.N)
     B = op.vector(f)
-    its = {}
+    its, res = {}, {}
     for name, pre, keys in (("object", fa.Preconditioner(op, fa.PRE_BLOCK_JACOBI), {}),
                             ("key", None, {"fem.solver.preconditioning.method": "blockjacobi"})):
         inv = fa.InverseOperator(ctx, method=fa.CG, tolerance=1e-9, errormeasure=fa.RELATIVE, maxiterations=3000,
@@ -100,9 +100,12 @@ def test_solver_builds_preconditioner_from_parameter_key(ctx):
         X = op.vector()
         inv(B, X)
         info = inv.info()
-        assert info["converged"] and info["true_rel_residual"] < 1e-8, name
-        its[name] = info["iterations"]
+        # CG stops on sqrt(<r, M^-1 r>); with kappa in {1, 100} the true
+        # residual sits up to two orders above the tolerance
+        assert info["converged"] and info["true_rel_residual"] < 1e-6, name
+        its[name], res[name] = info["iterations"], info["true_rel_residual"]
     assert its["object"] == its["key"], its
+    assert abs(res["object"] - res["key"]) <= 1e-6 * res["object"], res
     # the continuous Q2 problem through "jacobi", against the oracle's CG + Jacobi
     P = wl.problem0(n=32)
     gop = fa.GalerkinOperator(ctx, fa.OP_CG_Q2_2D, (32, 32), 2, c=1.0)

@@ -210,6 +210,35 @@ def test_mixed_precision_refinement_reaches_fp64_residual(ctx, method):
     assert info["outer_iterations"] >= 2
 
 
+def bicgstab_count_slack(ito):
+    # BiCGStab's residual is not monotone and the reference's stopping value
+    # sqrt(<s,s> - w(2<r,s> - w<r,r>)) cancels near convergence, so the
+    # order of the dot-product sums moves the step at which it first falls
+    # below the tolerance (measured: 215 vs 204, 146 vs 149 steps). The
+    # iterates themselves are compared step by step in
+    # bicgstab_iterates_agree.
+    return max(2, ito // 10)
+
+
+def bicgstab_iterates_agree(ctx, op, oop, b, pre=False):
+    # after a fixed number of steps (the solver stops at maxiterations and
+    # reports a negative count) the GPU iterate equals the oracle's to
+    # rounding: a dropped term or a wrong sign in any update shows here
+    inv_blocks = np.linalg.inv(oop.block_diag()) if pre else None
+    for k in (1, 2, 5, 12):
+        xo, ito = ora.bicgstab(oop, b, pre=ora.block_jacobi(inv_blocks) if pre else None, tol=1e-15, maxit=k,
+                               criteria=1, nthreads=8)
+        inv = fa.InverseOperator(ctx, method=fa.BICGSTAB, tolerance=1e-15, errormeasure=fa.RELATIVE, maxiterations=k)
+        inv.bind(op, fa.Preconditioner(op, fa.PRE_BLOCK_JACOBI) if pre else None)
+        X = op.vector(np.zeros(op.N))
+        inv(op.vector(b), X)
+        assert ito == -k and inv.iterations() == -k
+        x = X.download()
+        err = np.abs(x - xo).max() / np.abs(xo).max()
+        print("bicgstab step", k, "iterate rel diff", err, flush=True)
+        assert err < 1e-10, (k, err)
+
+
 def test_bicgstab_matches_oracle(ctx):
     # LinearSolver::bicgstab restated (bicgstab.hh:56-222): nonsymmetric SIPG
     # with advection, unpreconditioned; iteration count and solution against
@@ -228,8 +257,10 @@ def test_bicgstab_matches_oracle(ctx):
     inv(op.vector(b), X)
     info = inv.info()
     print("bicgstab gpu its", info["iterations"], "oracle its", ito, "true rel res", info["true_rel_residual"], flush=True)
-    assert info["converged"] and abs(info["iterations"] - ito) <= 1
+    assert info["converged"] and info["true_rel_residual"] < 1e-9
+    assert abs(info["iterations"] - ito) <= bicgstab_count_slack(ito)
     assert np.abs(X.download() - xo).max() <= 1e-7 * np.abs(xo).max()
+    bicgstab_iterates_agree(ctx, op, oop, b)
     # non-convergence is a negative count
     inv2 = fa.InverseOperator(ctx, method=fa.BICGSTAB, tolerance=1e-14, errormeasure=fa.RELATIVE, maxiterations=3)
     inv2.bind(op)
@@ -254,5 +285,6 @@ def test_bicgstab_block_jacobi_diffusion_matches_oracle(ctx):
     inv(op.vector(b), X)
     info = inv.info()
     print("bicgstab+bj gpu its", info["iterations"], "oracle its", ito, flush=True)
-    assert info["converged"] and abs(info["iterations"] - ito) <= 1
+    assert info["converged"] and abs(info["iterations"] - ito) <= bicgstab_count_slack(ito)
     assert np.abs(X.download() - xo).max() <= 1e-7 * np.abs(xo).max()
+    bicgstab_iterates_agree(ctx, op, oop, b, pre=True)
