@@ -82,6 +82,14 @@ struct ApplyShape {
     static constexpr int EPB = WG / (K * K);
 };
 
+// waves per SIMD the compiler is asked to fit: five for the FP64 Q3 apply
+// (at most 96 VGPRs), whose loads wait on memory most of the time
+template <int K, typename T, bool MF>
+constexpr int apply_min_waves()
+{
+    return (K == 4 && std::is_same<T, double>::value && !MF) ? 5 : 1;
+}
+
 template <int K>
 __device__ __forceinline__ int lidx(int a, int b, int c)
 {
@@ -318,7 +326,7 @@ struct Ctr {
 // KQ: kappa given at the quadrature points (FEM_KAPPA_QP); otherwise one
 // value per element (FEM_KAPPA_CELL) or one constant (FEM_KAPPA_CONST)
 template <int K, typename T, bool MF, bool KQ>
-__global__ void __launch_bounds__(ApplyShape<K>::WG)
+__global__ void __launch_bounds__(ApplyShape<K>::WG, (apply_min_waves<K, T, MF>()))
     k_dg_apply(const DGArgs<T> a, const T *__restrict__ u, T *__restrict__ w, const int *__restrict__ skip,
                double *__restrict__ dot)
 {
@@ -404,6 +412,15 @@ __global__ void __launch_bounds__(ApplyShape<K>::WG)
     }
     const int ec[3] = { ex, ey, ez };
     const int stride[3] = { 1, nx, nx * ny };
+    // FP64 at K = 4: the four elements of a wavefront are consecutive
+    // elements of one x-row (lane 16 i + 4 el + j), so an x-face between two
+    // of them takes the neighbour's trace from the neighbour's own lanes
+    // (lane +- 4), where it is the neighbour's own extrapolated trace, instead
+    // of contracting the neighbour's DoFs.  Only the first element of the
+    // wavefront loads its -x neighbour and the last its +x neighbour: one
+    // x pencil per lane instead of two.
+    constexpr bool XSH = (K == 4) && std::is_same<T, double>::value && !MF;
+    const int xs = (el0 == 0) ? 0 : 1;   // the x side whose neighbour this lane loads
 
     T *A = smem + el * SZ;
     T *R = A + K * S1;
@@ -436,6 +453,7 @@ __global__ void __launch_bounds__(ApplyShape<K>::WG)
 #pragma unroll
         for (int s2 = 0; s2 < 2; ++s2) {
             has[d][s2] = s2 ? ec[d] + 1 < a.n[d] : ec[d] > 0;
+            if (XSH && d == 0) continue;   // one x pencil, below
             const T *un = u + (size_t)(has[d][s2] ? e + (s2 ? stride[d] : -stride[d]) : e) * K3;
 #pragma unroll
             for (int m = 0; m < K; ++m) {
@@ -443,6 +461,14 @@ __global__ void __launch_bounds__(ApplyShape<K>::WG)
                 nbr[d][s2][m] = un[off];
             }
         }
+    T nbx[K];
+    if constexpr (XSH) {
+        // the middle elements need no x pencil: their own DoFs (cached) stand in
+        const bool xload = has[0][xs] && (el0 == 0 || el0 == EPB - 1);
+        const T *un = u + (size_t)(xload ? e + (xs ? 1 : -1) : e) * K3;
+#pragma unroll
+        for (int m = 0; m < K; ++m) nbx[m] = un[m * K2 + i * K + j];
+    }
     const bool kcell = a.kappa_mode == FEM_KAPPA_CELL;
     T kx[K], ky[K], kz[K], kf[3][2];
     {
@@ -493,10 +519,23 @@ __global__ void __launch_bounds__(ApplyShape<K>::WG)
 #pragma unroll
         for (int q = 0; q < K; ++q) A[L(i, q, j)] = tmp[q];
     lds_sync<WG>();
+    T nvx = 0, ndx = 0;   // XSH: the loaded x neighbour's normal contraction
+    if constexpr (XSH) {
+        T val = 0, der = 0;
+#pragma unroll
+        for (int m = 0; m < K; ++m) {
+            const T sg = (xs == 1 && (m & 1)) ? T(-1) : T(1);
+            val = fma(sg * tb.P[1][m], nbx[m], val);
+            der = fma(xs == 1 ? -sg * tb.dP[1][m] : tb.dP[1][m], nbx[m], der);
+        }
+        nvx = has[0][xs] ? val : T(0);
+        ndx = has[0][xs] ? der : T(0);
+    }
 #pragma unroll
     for (int d = 0; d < 3; ++d)
 #pragma unroll
         for (int s2 = 0; s2 < 2; ++s2) {
+            if (XSH && d == 0) continue;
             // the neighbour below (s2 = 0) is evaluated at its face xi = 1,
             // the one above at xi = 0: P(0) = (-1)^m P(1), P'(0) = -(-1)^m P'(1)
             T val = 0, der = 0;
@@ -516,6 +555,12 @@ __global__ void __launch_bounds__(ApplyShape<K>::WG)
         if constexpr (F64MF) {
             // Z = B N B^T on the matrix cores: two chained 4x4x4 products per
             // quantity, no LDS and no cross-lane moves
+            if (XSH && d == 0) {
+                // the one loaded x neighbour (both entries; face_terms picks)
+                nvs[0] = nvs[1] = mf64_tangential(nvx, bji);
+                nds[0] = nds[1] = mf64_tangential(ndx, bji);
+                return;
+            }
 #pragma unroll
             for (int s = 0; s < 2; ++s) {
                 nvs[s] = mf64_tangential(nval[d][s], bji);
@@ -577,17 +622,38 @@ __global__ void __launch_bounds__(ApplyShape<K>::WG)
         // E[1][q] = E[0][K-1-q], dE[1][q] = -dE[0][K-1-q]
         auto Es = [&](int s, int m) { return s ? tb.E[0][K - 1 - m] : tb.E[0][m]; };
         auto dEs = [&](int s, int m) { return s ? -tb.dE[0][K - 1 - m] : tb.dE[0][m]; };
+        // own traces (value, d/dxi) on both faces normal to d
+        T ut[2], dt[2];
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
-            const T nrm = s ? T(1) : T(-1);
-            const bool interior = has[d][s];
             T uin = 0, din = 0;
 #pragma unroll
             for (int m = 0; m < K; ++m) {
                 uin = fma(Es(s, m), up[m], uin);
                 din = fma(dEs(s, m), up[m], din);
             }
-            din *= ihd * nrm;
+            ut[s] = uin;
+            dt[s] = din;
+        }
+        if (XSH && d == 0) {
+            // the x neighbours inside the wavefront: element el + 1 (lane + 4)
+            // holds the trace of the shared face as its own lower face, el - 1
+            // (lane - 4) as its upper face; all lanes take part in the moves
+            const int lane = threadIdx.x & 63;
+            const T pv = __shfl(ut[0], (lane + 4) & 63, 64), pd = __shfl(dt[0], (lane + 4) & 63, 64);
+            const T mv = __shfl(ut[1], (lane + 60) & 63, 64), md = __shfl(dt[1], (lane + 60) & 63, 64);
+            if (el0 < EPB - 1) { nvs[1] = pv; nds[1] = pd; }
+            if (el0 > 0) { nvs[0] = mv; nds[0] = md; }
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+                if (!has[0][s]) { nvs[s] = T(0); nds[s] = T(0); }
+        }
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const T nrm = s ? T(1) : T(-1);
+            const bool interior = has[d][s];
+            const T uin = ut[s];
+            const T din = dt[s] * (ihd * nrm);
             // upwind split of b.n for the outward normal nrm e_d
             const T bp = s ? a.bpos[d] : a.bneg[d], bm = s ? a.bneg[d] : a.bpos[d];
             T kin, kout, kp;
