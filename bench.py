@@ -175,10 +175,11 @@ def solve_section(fa, wl, ctx, n=48, pairs=2000, epochs=3, lr=1e-4, learned=True
     The plain FP64 GPU path (CG + exact block Jacobi, the reference's Krylov
     recurrence with element-block preconditioning), block-Jacobi FGMRES(30)
     (the configuration's comparison), CG with the multigrid preconditioner
-    in FP64, FP64 refinement around FP32 inner solves with an FP32 multigrid
-    and with a multigrid on the BF16-MFMA operator, and the learned
-    correction after the V-cycle in FGMRES(30), with and without the network
-    and inside FP64 refinement."""
+    in FP64, FP64 CG with an FP32 V-cycle, FP64 refinement around FP32 inner
+    solves with an FP32 multigrid and with a multigrid on the BF16-MFMA
+    operator, and the learned correction after the V-cycle in FGMRES(30):
+    with and without the network (FP64), with an FP32 V-cycle in FP64
+    FGMRES, and inside FP64 refinement."""
     P = wl.problem3(n=n)
     E = n ** 3
     N = E * (P["p"] + 1) ** 3
@@ -200,6 +201,8 @@ def solve_section(fa, wl, ctx, n=48, pairs=2000, epochs=3, lr=1e-4, learned=True
     run("fp64_cg_bj", fa.CG, bj64)
     run("fp64_fgmres_bj", fa.FGMRES, bj64, maxit=3000)
     run("fp64_cg_mg", fa.CG, mg64, graph=1)
+    # FP64 CG whose V-cycle runs in FP32 (rounded in, widened out)
+    run("mixed_fp64cg_fp32mg", fa.CG, mg32, graph=1)
     run("mixed_fp32_cg_mg_ir", fa.CG, mg32, op32, refinement=1, inner_tolerance=1e-3, inner_maxiterations=2000,
         graph=1)
     # the V-cycle on the BF16-MFMA operator: the inner Krylov solve keeps the
@@ -213,9 +216,11 @@ def solve_section(fa, wl, ctx, n=48, pairs=2000, epochs=3, lr=1e-4, learned=True
             fa.LearnedPreconditioner(op64, net, coarse_steps=0, mode=fa.LEARNED_OFF, smoother=fa.PRE_MULTIGRID))
         run("fp64_fgmres_mg_learned", fa.FGMRES,
             fa.LearnedPreconditioner(op64, net, coarse_steps=0, mode=fa.LEARNED_NET, smoother=fa.PRE_MULTIGRID))
-        run("mixed_fp32_fgmres_mg_learned_ir", fa.FGMRES,
-            fa.LearnedPreconditioner(op32, net, coarse_steps=0, mode=fa.LEARNED_NET, smoother=fa.PRE_MULTIGRID),
-            op32, refinement=1, inner_tolerance=1e-3, inner_maxiterations=2000)
+        lp32 = fa.LearnedPreconditioner(op32, net, coarse_steps=0, mode=fa.LEARNED_NET, smoother=fa.PRE_MULTIGRID)
+        # FP64 FGMRES with the network and the V-cycle in FP32
+        run("mixed_fp64fgmres_fp32mg_learned", fa.FGMRES, lp32)
+        run("mixed_fp32_fgmres_mg_learned_ir", fa.FGMRES, lp32, op32, refinement=1, inner_tolerance=1e-3,
+            inner_maxiterations=2000)
     out["runs"] = runs
     base = runs["fp64_cg_bj"]["time_s"]
     out["speedup_vs_fp64_cg_bj"] = {k: round(base / v["time_s"], 2) for k, v in runs.items()}
@@ -226,10 +231,10 @@ def solve_section(fa, wl, ctx, n=48, pairs=2000, epochs=3, lr=1e-4, learned=True
 
 def combined_section(fa, wl, ctx, n=96, learned=True, pairs=400, epochs=2, lr=1e-4):
     """Q4 log-normal problem (110.6 M DoFs): the plain FP64 GPU path (CG +
-    block Jacobi) against CG + multigrid in FP64, FP64 refinement around FP32
-    inner solves with FP32 and BF16-MFMA multigrid, and the learned correction
-    after the V-cycle inside FP64 refinement (trained on a reduced budget of
-    pairs at this size)."""
+    block Jacobi) against CG + multigrid in FP64, FP64 CG with an FP32
+    V-cycle, FP64 refinement around FP32 inner solves, and the learned
+    correction after an FP32 V-cycle in FP64 FGMRES and inside FP64
+    refinement (trained on a reduced budget of pairs at this size)."""
     P = wl.problem4(n=n)
     E = n ** 3
     N = E * (P["p"] + 1) ** 3
@@ -248,17 +253,18 @@ def combined_section(fa, wl, ctx, n=96, learned=True, pairs=400, epochs=2, lr=1e
     out["multigrid_setup_s_fp64"] = round(time.perf_counter() - t0, 3)
     run("fp64_cg_mg", fa.CG, mg64, graph=1)
     op32 = _dg_cell(fa, ctx, P, fa.F32)
-    run("mixed_fp32_cg_mg_ir", fa.CG, fa.Preconditioner(op32, fa.PRE_MULTIGRID), op32, refinement=1,
-        inner_tolerance=1e-3, inner_maxiterations=2000, graph=1)
-    opbf = _dg_cell(fa, ctx, P, fa.BF16MFMA)
-    run("mixed_bf16mg_fgmres_ir", fa.FGMRES, fa.Preconditioner(opbf, fa.PRE_MULTIGRID), op32, refinement=1,
-        inner_tolerance=1e-3, inner_maxiterations=2000)
-    del opbf
+    mg32 = fa.Preconditioner(op32, fa.PRE_MULTIGRID)
+    run("mixed_fp64cg_fp32mg", fa.CG, mg32, graph=1)
+    run("mixed_fp32_cg_mg_ir", fa.CG, mg32, op32, refinement=1, inner_tolerance=1e-3, inner_maxiterations=2000,
+        graph=1)
+    # (a V-cycle on the BF16-MFMA operator does not converge at this size:
+    # BF16 rounding noise swamps the smooth residual; not run here)
     if learned:
         net = _train_learned(fa, wl, ctx, P, op64, mg64, pairs, epochs, lr, out, "learned_training")
-        run("mixed_fp32_fgmres_mg_learned_ir", fa.FGMRES,
-            fa.LearnedPreconditioner(op32, net, coarse_steps=0, mode=fa.LEARNED_NET, smoother=fa.PRE_MULTIGRID),
-            op32, refinement=1, inner_tolerance=1e-3, inner_maxiterations=2000)
+        lp32 = fa.LearnedPreconditioner(op32, net, coarse_steps=0, mode=fa.LEARNED_NET, smoother=fa.PRE_MULTIGRID)
+        run("mixed_fp64fgmres_fp32mg_learned", fa.FGMRES, lp32)
+        run("mixed_fp32_fgmres_mg_learned_ir", fa.FGMRES, lp32, op32, refinement=1, inner_tolerance=1e-3,
+            inner_maxiterations=2000)
     out["runs"] = runs
     base = runs["fp64_cg_bj"]["time_s"]
     out["speedup_vs_fp64_cg_bj"] = {k: round(base / v["time_s"], 2) for k, v in runs.items()}

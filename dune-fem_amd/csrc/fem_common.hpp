@@ -34,6 +34,9 @@ struct Error : std::runtime_error {
 // Maximum number of 1D points/modes per direction supported by the kernels
 // (polynomial order <= 7).
 constexpr int KMAX = 8;
+// internal preconditioner kind (not in the C header): an FP32 preconditioner
+// applied to the FP64 vectors of an FP64 Krylov solve
+constexpr int FEM_PRE_WIDENED = 100;
 
 // 1D tables of one polynomial order, on the m = p+1 point Gauss rule.
 //   B[q][a]   modal Legendre basis a at Gauss point q

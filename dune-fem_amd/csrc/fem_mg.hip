@@ -83,8 +83,9 @@ __global__ void k_mg_restrict_modes(int64_t E, int Kf, int Kc, const T *__restri
     MG_SKIP;
     const int Kc3 = Kc * Kc * Kc, Kf3 = Kf * Kf * Kf;
     const int64_t n = E * Kc3;
+    const bool narrow = n < (int64_t(1) << 31);
     for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t e = q / Kc3;
+        const int64_t e = narrow ? (int64_t)((unsigned)q / (unsigned)Kc3) : q / Kc3;
         const int l = (int)(q - e * Kc3);
         const int a = l / (Kc * Kc), b = (l / Kc) % Kc, c = l % Kc;
         const int64_t f = e * Kf3 + a * Kf * Kf + b * Kf + c;
@@ -100,8 +101,9 @@ __global__ void k_mg_prolong_modes(int64_t E, int Kf, int Kc, const T *__restric
     MG_SKIP;
     const int Kc3 = Kc * Kc * Kc, Kf3 = Kf * Kf * Kf;
     const int64_t n = E * Kc3;
+    const bool narrow = n < (int64_t(1) << 31);
     for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t e = q / Kc3;
+        const int64_t e = narrow ? (int64_t)((unsigned)q / (unsigned)Kc3) : q / Kc3;
         const int l = (int)(q - e * Kc3);
         const int a = l / (Kc * Kc), b = (l / Kc) % Kc, c = l % Kc;
         xf[e * Kf3 + a * Kf * Kf + b * Kf + c] += xc[q];
@@ -128,24 +130,23 @@ __global__ void k_mg_dg1_to_vertex(int nx, int ny, int nz, const T *__restrict__
     for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < V; v += (int64_t)gridDim.x * blockDim.x) {
         const int vx = (int)(v % nvx), vy = (int)((v / nvx) % nvy), vz = (int)(v / ((int64_t)nvx * nvy));
         double acc = 0;
-        for (int k = 0; k < 2; ++k) {
-            const int cz = vz - k;
-            if (cz < 0 || cz >= nz) continue;
-            for (int j = 0; j < 2; ++j) {
-                const int cy = vy - j;
-                if (cy < 0 || cy >= ny) continue;
-                for (int i = 0; i < 2; ++i) {
-                    const int cx = vx - i;
-                    if (cx < 0 || cx >= nx) continue;
-                    const int64_t e = cx + (int64_t)nx * (cy + (int64_t)ny * cz);
+        // the (up to) eight elements around the vertex; an element outside
+        // the mesh reads element 0 with weight 0, so all loads are issued
+        // together
 #pragma unroll
-                    for (int l = 0; l < 8; ++l) {
-                        const int a = l >> 2, b = (l >> 1) & 1, c = l & 1;
-                        const double val = (double)r[e * 8 + l] - (t ? (double)t[e * 8 + l] : 0.0);
-                        acc += tq1(a, i) * tq1(b, j) * tq1(c, k) * val;
-                    }
-                }
+        for (int q = 0; q < 8; ++q) {
+            const int i = q & 1, j = (q >> 1) & 1, k = q >> 2;
+            const int cx = vx - i, cy = vy - j, cz = vz - k;
+            const bool in = cx >= 0 && cx < nx && cy >= 0 && cy < ny && cz >= 0 && cz < nz;
+            const int64_t e = in ? cx + (int64_t)nx * (cy + (int64_t)ny * cz) : 0;
+            double part = 0;
+#pragma unroll
+            for (int l = 0; l < 8; ++l) {
+                const int a = l >> 2, b = (l >> 1) & 1, c = l & 1;
+                const double val = (double)r[e * 8 + l] - (t ? (double)t[e * 8 + l] : 0.0);
+                part = fma(tq1(a, i) * tq1(b, j) * tq1(c, k), val, part);
             }
+            acc += in ? part : 0.0;
         }
         rv[v] = (T)acc;
     }
@@ -178,7 +179,11 @@ struct VDims {
     int n[3];
 };
 
-// t = sg * (A x) + (r ? r : 0) with sg = -1 (residual r - A x) or +1 (A x, r null)
+// t = sg * (A x) + (r ? r : 0) with sg = -1 (residual r - A x) or +1 (A x,
+// r null).  The stencil is stored by offset, A[s * V + v] with s = (dx+1) +
+// 3 (dy+1) + 9 (dz+1), so each of the 27 coefficient loads is coalesced; the
+// 27 terms are unrolled, and a neighbour outside the lattice reads x[v] with
+// weight 0, so all loads are issued together.
 template <typename T>
 __global__ void k_mg_v_apply(VDims d, const T *__restrict__ A, const T *__restrict__ x, const T *__restrict__ r,
                              T *__restrict__ t, int sg, const int *__restrict__ skip)
@@ -188,19 +193,27 @@ __global__ void k_mg_v_apply(VDims d, const T *__restrict__ A, const T *__restri
     const int64_t sy = d.n[0], sz = (int64_t)d.n[0] * d.n[1];
     for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < V; v += (int64_t)gridDim.x * blockDim.x) {
         const int vx = (int)(v % d.n[0]), vy = (int)((v / d.n[0]) % d.n[1]), vz = (int)(v / sz);
-        const T *a = A + v * 27;
         T acc = 0;
-        for (int dz = -1; dz <= 1; ++dz) {
-            if (vz + dz < 0 || vz + dz >= d.n[2]) continue;
-            for (int dy = -1; dy <= 1; ++dy) {
-                if (vy + dy < 0 || vy + dy >= d.n[1]) continue;
-                for (int dx = -1; dx <= 1; ++dx) {
-                    if (vx + dx < 0 || vx + dx >= d.n[0]) continue;
-                    acc = fma(a[(dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)], x[v + dx + dy * sy + dz * sz], acc);
-                }
-            }
+#pragma unroll
+        for (int s = 0; s < 27; ++s) {
+            const int dx = s % 3 - 1, dy = (s / 3) % 3 - 1, dz = s / 9 - 1;
+            const bool in = vx + dx >= 0 && vx + dx < d.n[0] && vy + dy >= 0 && vy + dy < d.n[1] && vz + dz >= 0 &&
+                            vz + dz < d.n[2];
+            const T a = A[s * V + v];
+            const T xv = x[in ? v + dx + dy * sy + dz * sz : v];
+            acc = fma(in ? a : T(0), xv, acc);
         }
         t[v] = (r ? r[v] : T(0)) + (sg < 0 ? -acc : acc);
+    }
+}
+
+// the stencil from one row per vertex (A[v * 27 + s]) to one row per offset
+__global__ void k_mg_by_offset(int64_t V, const double *__restrict__ Av, double *__restrict__ As)
+{
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < V * 27; q += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t v = q / 27;
+        const int s = (int)(q - v * 27);
+        As[s * V + v] = Av[q];
     }
 }
 
@@ -226,17 +239,6 @@ __device__ __forceinline__ double w1(int i, int J, int nf, int nc)
     return (J == (i >> 1) || J == (i >> 1) + 1) ? 0.5 : 0.0;
 }
 
-// fine vertices with a nonzero weight for coarse J: {2J-1, 2J, 2J+1} and the
-// last fine vertex, in range and without repeats
-__device__ __forceinline__ int cands(int J, int nf, int *c)
-{
-    int m = 0;
-    for (int i = 2 * J - 1; i <= 2 * J + 1; ++i)
-        if (i >= 0 && i < nf) c[m++] = i;
-    if (nf - 1 > 2 * J + 1) c[m++] = nf - 1;
-    return m;
-}
-
 // rc = P^T tf
 template <typename T>
 __global__ void k_mg_v_restrict(VDims f, VDims cdim, const T *__restrict__ tf, T *__restrict__ rc,
@@ -247,22 +249,32 @@ __global__ void k_mg_v_restrict(VDims f, VDims cdim, const T *__restrict__ tf, T
     for (int64_t J = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; J < Vc; J += (int64_t)gridDim.x * blockDim.x) {
         const int Jx = (int)(J % cdim.n[0]), Jy = (int)((J / cdim.n[0]) % cdim.n[1]),
                   Jz = (int)(J / ((int64_t)cdim.n[0] * cdim.n[1]));
-        int cx[4], cy[4], cz[4];
-        const int mx = cands(Jx, f.n[0], cx), my = cands(Jy, f.n[1], cy), mz = cands(Jz, f.n[2], cz);
-        double acc = 0;
-        for (int c = 0; c < mz; ++c) {
-            const double wz = w1(cz[c], Jz, f.n[2], cdim.n[2]);
-            if (wz == 0.0) continue;
-            for (int b = 0; b < my; ++b) {
-                const double wy = w1(cy[b], Jy, f.n[1], cdim.n[1]);
-                if (wy == 0.0) continue;
-                for (int a = 0; a < mx; ++a) {
-                    const double wx = w1(cx[a], Jx, f.n[0], cdim.n[0]);
-                    if (wx == 0.0) continue;
-                    acc += wx * wy * wz * (double)tf[cx[a] + (int64_t)f.n[0] * (cy[b] + (int64_t)f.n[1] * cz[c])];
-                }
-            }
+        // per direction the fine vertices 2J-1, 2J, 2J+1 (in range) and
+        // their weights; the last fine vertex, the only other candidate,
+        // has weight 0 unless it is one of them (coarse1 puts the last
+        // coarse vertex at or beyond it)
+        int ix[3], iy[3], iz[3];
+        double wx[3], wy[3], wz[3];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            const int fx = 2 * Jx - 1 + m, fy = 2 * Jy - 1 + m, fz = 2 * Jz - 1 + m;
+            const bool okx = fx >= 0 && fx < f.n[0], oky = fy >= 0 && fy < f.n[1], okz = fz >= 0 && fz < f.n[2];
+            ix[m] = okx ? fx : 0;
+            iy[m] = oky ? fy : 0;
+            iz[m] = okz ? fz : 0;
+            wx[m] = okx ? w1(fx, Jx, f.n[0], cdim.n[0]) : 0.0;
+            wy[m] = oky ? w1(fy, Jy, f.n[1], cdim.n[1]) : 0.0;
+            wz[m] = okz ? w1(fz, Jz, f.n[2], cdim.n[2]) : 0.0;
         }
+        double acc = 0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+                    acc = fma(wx[a] * wy[b] * wz[c],
+                              (double)tf[ix[a] + (int64_t)f.n[0] * (iy[b] + (int64_t)f.n[1] * iz[c])], acc);
         rc[J] = (T)acc;
     }
 }
@@ -300,18 +312,25 @@ __global__ void k_mg_v_prolong(VDims f, VDims cdim, const T *__restrict__ xc, T 
     }
 }
 
-// x = Ainv r, Ainv stored column-major (thread v reads row v of Ainv
-// coalesced across threads)
+// x = Ainv r with Ainv row-major: one wavefront per row (four per block),
+// the lanes striding its columns (coalesced), then a butterfly sum.  The
+// row index is the same on all lanes of a wavefront, so a wavefront past
+// the last row leaves as a whole.
 template <typename T>
-__global__ void k_mg_dense(int V, const T *__restrict__ AinvT, const T *__restrict__ r, T *__restrict__ x,
-                           const int *__restrict__ skip)
+__global__ void __launch_bounds__(256)
+    k_mg_dense(int V, const T *__restrict__ Ainv, const T *__restrict__ r, T *__restrict__ x,
+               const int *__restrict__ skip)
 {
     MG_SKIP;
-    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) {
-        T acc = 0;
-        for (int w = 0; w < V; ++w) acc = fma(AinvT[(int64_t)w * V + v], r[w], acc);
-        x[v] = acc;
-    }
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= V) return;
+    const T *a = Ainv + (int64_t)row * V;
+    double acc = 0;
+    for (int w = lane; w < V; w += 64) acc = fma((double)a[w], (double)r[w], acc);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (lane == 0) x[row] = (T)acc;
 }
 
 // ---- set-up by probing (FP64) -------------------------------------------------------------
@@ -405,6 +424,12 @@ void Multigrid::init(Operator *fine)
         std::unique_ptr<MGDGLevel> L(new MGDGLevel());
         if (dg.empty()) {
             L->op = fine;
+            // BF16 rounds the quadrature values to 2^-8 of their size, which
+            // is noise of order 4e-3 * lambda_max |x| in A x: far above the
+            // smooth part of a smoothed residual, which is what the coarse
+            // levels must see. The smoothers tolerate it; the restricted
+            // residual is formed in FP32.
+            if (d0.precision == FEM_BF16MFMA) L->resop = make_op(d0.order, FEM_F32);
         } else {
             L->own = make_op(p, lp);
             L->op = L->own.get();
@@ -466,9 +491,13 @@ void Multigrid::init(Operator *fine)
         yc.alloc(C->V * 8);
         xf.alloc(F.V * 8);
         yf.alloc(F.V * 8);
+        DevBuf Afs;   // the fine level's stencil by offset
+        Afs.alloc(F.V * 27 * 8);
+        hipLaunchKernelGGL(k_mg_by_offset, dim3(mg_grid(F.V * 27)), dim3(MGT), 0, s, F.V, A64.back()->as<double>(),
+                           Afs.as<double>());
         A64.emplace_back(new DevBuf());
         A64.back()->alloc(C->V * 27 * 8);
-        const double *Af = A64[A64.size() - 2]->as<double>();
+        const double *Af = Afs.as<double>();
         for (int col = 0; col < 27; ++col) {
             hipLaunchKernelGGL(k_mg_probe, dim3(mg_grid(C->V)), dim3(MGT), 0, s, dc, xc.as<double>(), col);
             FEM_HIP(hipMemsetAsync(xf.p, 0, xf.bytes, s));
@@ -492,14 +521,19 @@ void Multigrid::init(Operator *fine)
         d64.alloc(L.V * 8);
         hipLaunchKernelGGL(k_mg_dinv, dim3(mg_grid(L.V)), dim3(MGT), 0, s, L.V, A64[l]->as<double>(), d64.as<double>());
         FEM_HIP(hipGetLastError());
-        store_as(lp, L.A, *A64[l], L.V * 27, ctx);
+        DevBuf As64;
+        As64.alloc(L.V * 27 * 8);
+        hipLaunchKernelGGL(k_mg_by_offset, dim3(mg_grid(L.V * 27)), dim3(MGT), 0, s, L.V, A64[l]->as<double>(),
+                           As64.as<double>());
+        FEM_HIP(hipGetLastError());
+        store_as(lp, L.A, As64, L.V * 27, ctx);
         store_as(lp, L.dinv, d64, L.V, ctx);
         L.x.alloc(L.V * es);
         L.r.alloc(L.V * es);
         L.t.alloc(L.V * es);
     }
     // dense inverse of the coarsest level (host, FP64, Gauss-Jordan with
-    // partial pivoting), stored column-major
+    // partial pivoting), stored row-major
     {
         const MGVertexLevel &L = *vx.back();
         const int V = (int)L.V;
@@ -541,10 +575,8 @@ void Multigrid::init(Operator *fine)
                 }
             }
         }
-        // column-major: element (row v, column w) at w * V + v
-        std::vector<double> cm((size_t)V * V);
-        for (int v = 0; v < V; ++v)
-            for (int w = 0; w < V; ++w) cm[(size_t)w * V + v] = I[(size_t)v * V + w];
+        // row-major: element (row v, column w) at v * V + w
+        const std::vector<double> &cm = I;
         DevBuf c64;
         c64.alloc(cm.size() * 8);
         FEM_HIP(hipMemcpy(c64.p, cm.data(), cm.size() * 8, hipMemcpyHostToDevice));
@@ -560,7 +592,7 @@ void Multigrid::cycle_vertex(size_t l, const T *r, T *x, const int *skip)
     hipStream_t s = op->ctx->stream;
     const unsigned G = mg_grid(L.V);
     if (l + 1 == vx.size()) {
-        hipLaunchKernelGGL(k_mg_dense<T>, dim3((unsigned)((L.V + 255) / 256)), dim3(256), 0, s, (int)L.V,
+        hipLaunchKernelGGL(k_mg_dense<T>, dim3((unsigned)((L.V + 3) / 4)), dim3(256), 0, s, (int)L.V,
                            coarse_inv.as<T>(), r, x, skip);
         return;
     }
@@ -615,7 +647,7 @@ void Multigrid::cycle_dg(size_t l, const T *r, T *x, const int *skip)
         L.op->apply_raw(x, t, skip);
         mg_bj<T>(L, r, t, x, om, 1, skip, s);
     }
-    L.op->apply_raw(x, t, skip);
+    (L.resop ? L.resop.get() : L.op)->apply_raw(x, t, skip);
     if (l + 1 < dg.size()) {
         MGDGLevel &C = *dg[l + 1];
         hipLaunchKernelGGL(k_mg_restrict_modes<T>, dim3(mg_grid(E * C.k * C.k * C.k)), dim3(MGT), 0, s, E, L.k, C.k, r,

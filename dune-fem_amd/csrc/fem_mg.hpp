@@ -13,6 +13,9 @@ namespace femamd {
 struct MGDGLevel {
     std::unique_ptr<Operator> own;   // coarser DG levels own their operator
     Operator *op = nullptr;          // level 0: the preconditioner's operator
+    // level 0 of a BF16-MFMA operator: its FP32 twin forms the residual that
+    // is restricted to the coarse levels (the smoothing sweeps keep BF16)
+    std::unique_ptr<Operator> resop;
     Precond bj;
     int k = 0;                       // points/modes per direction, p + 1
     DevBuf x, r, t;                  // correction, residual, A x

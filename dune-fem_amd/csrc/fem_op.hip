@@ -426,6 +426,36 @@ void Precond::init(Operator *o, int k_)
     }
 }
 
+// y (FP64) = widen(M32 round(x)): the FP32 preconditioner inside an FP64
+// Krylov solve
+__global__ void k_round32(const double *__restrict__ x, float *__restrict__ y, int64_t n, const int *__restrict__ skip)
+{
+    if (skip && *skip) return;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = (float)x[i];
+}
+
+__global__ void k_widen64(const float *__restrict__ x, double *__restrict__ y, int64_t n, const int *__restrict__ skip)
+{
+    if (skip && *skip) return;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = (double)x[i];
+}
+
+void Precond::init_widening(Operator *o64, const Precond *p32)
+{
+    FEM_CHECK(o64 && p32, FEM_ERR_ARG, "null argument");
+    FEM_CHECK(o64->vec_prec == FEM_F64 && p32->prec == FEM_F32, FEM_ERR_ARG,
+              "an FP32 preconditioner widened to an FP64 operator");
+    FEM_CHECK(p32->op->N == o64->N, FEM_ERR_ARG, "preconditioner does not match the operator");
+    op = o64;
+    kind = FEM_PRE_WIDENED;
+    prec = FEM_F64;
+    lowp = p32;
+    cx.alloc(o64->N * sizeof(float));
+    cy.alloc(o64->N * sizeof(float));
+}
+
 Precond::~Precond()
 {
     delete learned;
@@ -437,7 +467,13 @@ void Precond::apply_raw(const void *x, void *y, const int *skip, double *dot) co
     hipStream_t s = op->ctx->stream;
     const int64_t N = op->N;
     FEM_CHECK(!dot || fused_dot(), FEM_ERR_UNSUPPORTED, "fused dot: block Jacobi only");
-    if (kind == FEM_PRE_LEARNED) {
+    if (lowp) {
+        const unsigned G = grid_for(N, 256, 8192);
+        hipLaunchKernelGGL(k_round32, dim3(G), dim3(256), 0, s, (const double *)x, cx.as<float>(), N, skip);
+        lowp->apply_raw(cx.p, cy.p, skip);
+        hipLaunchKernelGGL(k_widen64, dim3(G), dim3(256), 0, s, (const float *)cy.as<float>(), (double *)y, N, skip);
+        FEM_HIP(hipGetLastError());
+    } else if (kind == FEM_PRE_LEARNED) {
         learned->apply_raw(x, y, skip);
     } else if (kind == FEM_PRE_MULTIGRID) {
         mg->apply_raw(x, y, skip);

@@ -84,7 +84,14 @@ struct Precond {
     DevBuf Q, lam;     // block Jacobi (tensor-product eigen-decomposition)
     Learned *learned = nullptr;   // FEM_PRE_LEARNED (owned)
     Multigrid *mg = nullptr;      // FEM_PRE_MULTIGRID (owned)
+    // an FP32 preconditioner applied to the FP64 vectors of an FP64 Krylov
+    // solve (not owned): x is rounded to FP32, M32 applied, the result
+    // widened back; cx, cy hold the FP32 vectors
+    const Precond *lowp = nullptr;
+    DevBuf cx, cy;
     void init(Operator *o, int kind);
+    void init_widening(Operator *o64, const Precond *p32);
+    Learned *learned_part() const { return lowp ? lowp->learned : learned; }
     // `dot` (block Jacobi only, may be null): per-workgroup FP64 partials of
     // <x, y> and then of <x, x>, op->dot_blocks() entries each
     void apply_raw(const void *x, void *y, const int *skip, double *dot = nullptr) const;

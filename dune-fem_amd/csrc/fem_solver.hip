@@ -597,12 +597,15 @@ void Solver::bind(Operator *o, Precond *p, Operator *io, Precond *ip)
     pre = ipre = nullptr;
     FEM_CHECK(o, FEM_ERR_ARG, "bind: null operator");
     FEM_CHECK(!p || p->op == o || p->op->N == o->N, FEM_ERR_ARG, "preconditioner does not match the operator");
-    FEM_CHECK(!p || p->prec == o->vec_prec, FEM_ERR_ARG, "preconditioner precision does not match the operator");
+    // an FP32 preconditioner may serve an FP64 solve (rounded in, widened
+    // out); otherwise the precisions agree
+    FEM_CHECK(!p || p->prec == o->vec_prec || (p->prec == FEM_F32 && o->vec_prec == FEM_F64 && !prm.refinement),
+              FEM_ERR_ARG, "preconditioner precision does not match the operator");
     // the learned preconditioner is nonlinear in its argument (the coarse
     // scaling depends on it): only the flexible GMRES, which stores
     // z_j = M(v_j), stays a valid Krylov method with it
     auto flexible_only = [&](const Precond *q) {
-        FEM_CHECK(!q || q->kind != FEM_PRE_LEARNED || prm.method == FEM_SOLVER_FGMRES, FEM_ERR_UNSUPPORTED,
+        FEM_CHECK(!q || !q->learned_part() || prm.method == FEM_SOLVER_FGMRES, FEM_ERR_UNSUPPORTED,
                   "the learned preconditioner is nonlinear: it needs fem.solver.method = fgmres");
     };
     flexible_only(p);
@@ -621,6 +624,12 @@ void Solver::bind(Operator *o, Precond *p, Operator *io, Precond *ip)
     // way, krylovinverseoperators.hh:86-97)
     own_pre.reset();
     own_ipre.reset();
+    widened.reset();
+    if (p && p->prec == FEM_F32 && o->vec_prec == FEM_F64) {
+        widened.reset(new Precond());
+        widened->init_widening(o, p);
+        p = widened.get();
+    }
     if (!p && prm.preconditioning != FEM_PRE_NONE && !prm.refinement) {
         own_pre.reset(new Precond());
         own_pre->init(o, prm.preconditioning);
@@ -643,6 +652,7 @@ void Solver::unbind()
     pre = ipre = nullptr;
     own_pre.reset();
     own_ipre.reset();
+    widened.reset();
     for (DevBuf *b : work) delete b;
     work.clear();
 }
@@ -838,7 +848,7 @@ static int run_krylov(Solver &S, int method, Operator *o, Precond *p, const T *b
         FEM_HIP(hipMemcpyAsync(st, S.hstate, sizeof(KState), hipMemcpyHostToDevice, s));
     }
     bool first = true;
-    Learned *lp = (p && p->kind == FEM_PRE_LEARNED) ? p->learned : nullptr;
+    Learned *lp = p ? p->learned_part() : nullptr;
     T *xsave = lp ? (T *)S.workbuf(4, vb) : nullptr;
     while (true) {
         if (lp && lp->active()) FEM_HIP(hipMemcpyAsync(xsave, x, vb, hipMemcpyDeviceToDevice, s));
@@ -921,8 +931,8 @@ void Solver::solve(const Vec &b, Vec &x, fem_solver_info &info)
         Learned *l;
         int mode;
         ~ModeGuard() { if (l) l->mode = mode; }
-    } g_outer{ pre && pre->learned ? pre->learned : nullptr, pre && pre->learned ? pre->learned->mode : 0 },
-        g_inner{ ipre && ipre->learned ? ipre->learned : nullptr, ipre && ipre->learned ? ipre->learned->mode : 0 };
+    } g_outer{ pre ? pre->learned_part() : nullptr, pre && pre->learned_part() ? pre->learned_part()->mode : 0 },
+        g_inner{ ipre ? ipre->learned_part() : nullptr, ipre && ipre->learned_part() ? ipre->learned_part()->mode : 0 };
     const int64_t N = op->N;
     FEM_HIP(hipStreamSynchronize(ctx->stream));
     auto t0 = std::chrono::steady_clock::now();

@@ -37,6 +37,7 @@ struct Solver {
     // preconditioners built at bind from fem.solver.preconditioning.method
     // when the caller passes none
     std::unique_ptr<Precond> own_pre, own_ipre;
+    std::unique_ptr<Precond> widened;   // an FP32 preconditioner serving the FP64 solve
     DevBuf state, partials, counters;
     KState *hstate = nullptr;   // pinned mirror
     std::vector<DevBuf *> work;

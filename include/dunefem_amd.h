@@ -197,7 +197,7 @@ int fem_csr_download(const fem_csr *A, int64_t *rowptr, int *col, double *val);
 #define FEM_PRE_JACOBI 1        /* d^-1 with |d| < 16 eps -> 1 (fempreconditioning.hh:99-106) */
 #define FEM_PRE_BLOCK_JACOBI 2  /* DG element blocks A_KK^-1                        */
 #define FEM_PRE_LEARNED 3       /* CNN coarse correction + block Jacobi (FGMRES only) */
-#define FEM_PRE_MULTIGRID 4     /* DG: p-multigrid to the element means, then h-multigrid (V-cycle) */
+#define FEM_PRE_MULTIGRID 4     /* DG, b = 0: V-cycle over DG p-levels, the conforming trilinear space and vertex h-levels */
 
 int fem_precond_create(fem_op *op, int kind, fem_precond **out);
 int fem_precond_destroy(fem_precond *p);
@@ -291,7 +291,11 @@ int fem_solver_params_set(fem_solver_params *p, const char *key, const char *val
 int fem_solver_create(fem_ctx *ctx, const fem_solver_params *params, fem_solver **out);
 int fem_solver_destroy(fem_solver *s);
 /* bind(op[, precond]) / unbind() (inverseoperatorinterface.hh:127-151).
- * For refinement, inner_op is the low-precision operator (may be NULL). */
+ * For refinement, inner_op is the low-precision operator (may be NULL).
+ * Without refinement an FP64 operator also takes a preconditioner built on
+ * the FP32 (or BF16-MFMA) operator of the same problem: each application
+ * rounds its argument to FP32 and widens the result, so the Krylov vectors,
+ * dots and the stopping test stay FP64 while the V-cycle runs in FP32. */
 int fem_solver_bind(fem_solver *s, fem_op *op, fem_precond *pre, fem_op *inner_op,
                     fem_precond *inner_pre);
 int fem_solver_unbind(fem_solver *s);
