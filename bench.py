@@ -177,9 +177,10 @@ def solve_section(fa, wl, ctx, n=48, pairs=2000, epochs=3, lr=1e-4, learned=True
     (the configuration's comparison), CG with the multigrid preconditioner
     in FP64, FP64 CG with an FP32 V-cycle, FP64 refinement around FP32 inner
     solves with an FP32 multigrid and with a multigrid on the BF16-MFMA
-    operator, and the learned correction after the V-cycle in FGMRES(30):
-    with and without the network (FP64), with an FP32 V-cycle in FP64
-    FGMRES, and inside FP64 refinement."""
+    operator, and the learned correction after the V-cycle in FGMRES(30)
+    (V-cycle, network correction, one fine block-Jacobi sweep): with and
+    without the network (FP64), with an FP32 V-cycle in FP64 FGMRES, and
+    inside FP64 refinement."""
     P = wl.problem3(n=n)
     E = n ** 3
     N = E * (P["p"] + 1) ** 3

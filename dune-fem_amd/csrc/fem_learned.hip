@@ -32,6 +32,7 @@
 #include "fem_blas.hpp"
 #include "fem_cnn.hpp"
 #include "fem_learned.hpp"
+#include "fem_mg.hpp"
 #include "fem_solver.hpp"
 
 namespace femamd {
@@ -529,17 +530,25 @@ static void learned_apply_t(Learned &L, const T *x, T *y, const int *skip)
     const int E = (int)o.E;
     const int64_t N = o.N;
     const unsigned G = grid_for(N, 256, 8192);
-    if (!L.active()) {
-        L.bj.apply_raw(x, y, skip);
-        return;
-    }
     const bool net = L.mode == FEM_LEARNED_NET;
     if (L.smoother == FEM_PRE_MULTIGRID) {
         // y = MG(x); t = x - A y; c from the network on R t (and coarse CG
-        // steps); y += alpha P c with the energy-optimal alpha
+        // steps); y += alpha P c with the energy-optimal alpha; then one
+        // damped block-Jacobi sweep of the fine level on x - A y, which takes
+        // out the jumps a piecewise-constant correction leaves at element
+        // faces.  Without the network (FEM_LEARNED_OFF, or no coarse steps)
+        // the V-cycle and the same sweep, so the two differ by the network
+        // alone.
         T *t1 = L.t1.as<T>(), *w1 = L.w1.as<T>();
         L.bj.apply_raw(x, y, skip);
-        if (!net && L.coarse_steps == 0) return;   // the method without the network: the V-cycle alone
+        auto post_sweep = [&]() {
+            o.apply_raw(y, w1, skip);
+            L.bj.mg->smooth_fine(x, w1, y, skip);
+        };
+        if (!(L.active() && (net || L.coarse_steps > 0))) {
+            post_sweep();
+            return;
+        }
         o.apply_raw(y, w1, skip);
         hipLaunchKernelGGL(k_sub<T>, dim3(G), dim3(256), 0, s, x, (const T *)w1, t1, N, skip);
         if (net) {
@@ -559,6 +568,11 @@ static void learned_apply_t(Learned &L, const T *x, T *y, const int *skip)
         hipLaunchKernelGGL(k_add_coarse<T>, dim3((E + 255) / 256), dim3(256), 0, s, y, L.ccg.x.as<float>(), nb, E,
                            L.red.as<float>(), skip);
         FEM_HIP(hipGetLastError());
+        post_sweep();
+        return;
+    }
+    if (!L.active()) {
+        L.bj.apply_raw(x, y, skip);
         return;
     }
     if (net) {

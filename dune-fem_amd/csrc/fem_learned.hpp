@@ -35,7 +35,8 @@ struct Learned {
     // smoother kind: FEM_PRE_BLOCK_JACOBI (coarse correction first, then the
     // block-Jacobi smoother on the remaining residual) or FEM_PRE_MULTIGRID
     // (a multigrid V-cycle first, then the coarse correction of what it
-    // leaves); FEM_LEARNED_OFF applies the smoother alone
+    // leaves, then one block-Jacobi sweep of the fine level); FEM_LEARNED_OFF
+    // applies the smoother alone (with the multigrid: the V-cycle and the sweep)
     int smoother = FEM_PRE_BLOCK_JACOBI;
     Precond bj;                 // exact element-block inverse, or the multigrid
     CoarseCG ccg;

@@ -118,11 +118,33 @@ __device__ __forceinline__ double tq1(int a, int i)
     return a == 0 ? 0.5 : (i ? 0.28867513459481287 : -0.28867513459481287);   // 1 / (2 sqrt 3)
 }
 
-// rv = P_c^T (r - t) (t may be null): per vertex, the sum over the (up to 8)
-// elements around it of its trilinear coefficients times the element's Q1 modes
+// rv = P_c^T (r - t) (t may be null), in two passes.  First, per element,
+// the trilinear functional of its Q1 modes at each of its 8 vertices,
+// c[e][q] with q = i + 2 j + 4 k for the vertex (cx + i, cy + j, cz + k),
+// kept in FP64; then per vertex the sum over the (up to) 8 elements around
+// it.  A single per-vertex pass would gather 8 x 8 mode values per vertex.
 template <typename T>
-__global__ void k_mg_dg1_to_vertex(int nx, int ny, int nz, const T *__restrict__ r, const T *__restrict__ t,
-                                   T *__restrict__ rv, const int *__restrict__ skip)
+__global__ void k_mg_dg1_corners(int64_t E, const T *__restrict__ r, const T *__restrict__ t,
+                                 double *__restrict__ c, const int *__restrict__ skip)
+{
+    MG_SKIP;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < E * 8; q += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t e = q >> 3;
+        const int cq = (int)(q & 7), i = cq & 1, j = (cq >> 1) & 1, k = cq >> 2;
+        double acc = 0;
+#pragma unroll
+        for (int l = 0; l < 8; ++l) {
+            const int a = l >> 2, b = (l >> 1) & 1, cc = l & 1;
+            const double val = (double)r[e * 8 + l] - (t ? (double)t[e * 8 + l] : 0.0);
+            acc = fma(tq1(a, i) * tq1(b, j) * tq1(cc, k), val, acc);
+        }
+        c[q] = acc;
+    }
+}
+
+template <typename T>
+__global__ void k_mg_corners_to_vertex(int nx, int ny, int nz, const double *__restrict__ c, T *__restrict__ rv,
+                                       const int *__restrict__ skip)
 {
     MG_SKIP;
     const int nvx = nx + 1, nvy = ny + 1;
@@ -130,26 +152,28 @@ __global__ void k_mg_dg1_to_vertex(int nx, int ny, int nz, const T *__restrict__
     for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < V; v += (int64_t)gridDim.x * blockDim.x) {
         const int vx = (int)(v % nvx), vy = (int)((v / nvx) % nvy), vz = (int)(v / ((int64_t)nvx * nvy));
         double acc = 0;
-        // the (up to) eight elements around the vertex; an element outside
-        // the mesh reads element 0 with weight 0, so all loads are issued
-        // together
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             const int i = q & 1, j = (q >> 1) & 1, k = q >> 2;
             const int cx = vx - i, cy = vy - j, cz = vz - k;
             const bool in = cx >= 0 && cx < nx && cy >= 0 && cy < ny && cz >= 0 && cz < nz;
             const int64_t e = in ? cx + (int64_t)nx * (cy + (int64_t)ny * cz) : 0;
-            double part = 0;
-#pragma unroll
-            for (int l = 0; l < 8; ++l) {
-                const int a = l >> 2, b = (l >> 1) & 1, c = l & 1;
-                const double val = (double)r[e * 8 + l] - (t ? (double)t[e * 8 + l] : 0.0);
-                part = fma(tq1(a, i) * tq1(b, j) * tq1(c, k), val, part);
-            }
-            acc += in ? part : 0.0;
+            const double val = c[e * 8 + q];
+            acc += in ? val : 0.0;
         }
         rv[v] = (T)acc;
     }
+}
+
+template <typename T>
+void dg1_to_vertex(const int *n, const T *r, const T *t, double *corners, T *rv, const int *skip, hipStream_t s)
+{
+    const int64_t E = (int64_t)n[0] * n[1] * n[2];
+    const int64_t V = (int64_t)(n[0] + 1) * (n[1] + 1) * (n[2] + 1);
+    hipLaunchKernelGGL(k_mg_dg1_corners<T>, dim3(mg_grid(E * 8)), dim3(MGT), 0, s, E, r, t, corners, skip);
+    hipLaunchKernelGGL(k_mg_corners_to_vertex<T>, dim3(mg_grid(V)), dim3(MGT), 0, s, n[0], n[1], n[2],
+                       (const double *)corners, rv, skip);
+    FEM_HIP(hipGetLastError());
 }
 
 // x += P_c xv: the Q1 modes of the trilinear interpolant of the vertex values
@@ -449,6 +473,7 @@ void Multigrid::init(Operator *fine)
         q164 = make_op(1, FEM_F64);
         q1 = q164.get();
     }
+    corners.alloc(E * 8 * sizeof(double));
     std::vector<std::unique_ptr<DevBuf>> A64;   // FP64 stencils per vertex level
     {
         std::unique_ptr<MGVertexLevel> V0(new MGVertexLevel());
@@ -470,8 +495,7 @@ void Multigrid::init(Operator *fine)
             hipLaunchKernelGGL(k_mg_vertex_to_dg1<double>, dim3(mg_grid(E * 8)), dim3(MGT), 0, s, n[0], n[1], n[2],
                                xp.as<double>(), dgx.as<double>(), (const int *)nullptr);
             q1->apply_raw(dgx.p, dgy.p, nullptr);
-            hipLaunchKernelGGL(k_mg_dg1_to_vertex<double>, dim3(mg_grid(V0->V)), dim3(MGT), 0, s, n[0], n[1], n[2],
-                               dgy.as<double>(), (const double *)nullptr, y.as<double>(), (const int *)nullptr);
+            dg1_to_vertex<double>(n, dgy.as<double>(), nullptr, corners.as<double>(), y.as<double>(), nullptr, s);
             hipLaunchKernelGGL(k_mg_extract, dim3(mg_grid(V0->V)), dim3(MGT), 0, s, dv, y.as<double>(),
                                A64.back()->as<double>(), col);
             FEM_HIP(hipGetLastError());
@@ -657,8 +681,7 @@ void Multigrid::cycle_dg(size_t l, const T *r, T *x, const int *skip)
                            (const T *)C.x.as<T>(), x, skip);
     } else {
         MGVertexLevel &V0 = *vx[0];
-        hipLaunchKernelGGL(k_mg_dg1_to_vertex<T>, dim3(mg_grid(V0.V)), dim3(MGT), 0, s, n[0], n[1], n[2], r,
-                           (const T *)t, V0.r.as<T>(), skip);
+        dg1_to_vertex<T>(n, r, (const T *)t, corners.as<double>(), V0.r.as<T>(), skip, s);
         cycle_vertex<T>(0, V0.r.as<T>(), V0.x.as<T>(), skip);
         hipLaunchKernelGGL(k_mg_vertex_to_dg1<T>, dim3(mg_grid(E * 8)), dim3(MGT), 0, s, n[0], n[1], n[2],
                            (const T *)V0.x.as<T>(), x, skip);
@@ -667,6 +690,16 @@ void Multigrid::cycle_dg(size_t l, const T *r, T *x, const int *skip)
         L.op->apply_raw(x, t, skip);
         mg_bj<T>(L, r, t, x, om, 1, skip, s);
     }
+    FEM_HIP(hipGetLastError());
+}
+
+void Multigrid::smooth_fine(const void *r, const void *t, void *x, const int *skip)
+{
+    hipStream_t s = op->ctx->stream;
+    if (prec == FEM_F64)
+        mg_bj<double>(*dg[0], (const double *)r, (const double *)t, (double *)x, (double)omega_dg, 1, skip, s);
+    else
+        mg_bj<float>(*dg[0], (const float *)r, (const float *)t, (float *)x, (float)omega_dg, 1, skip, s);
     FEM_HIP(hipGetLastError());
 }
 

@@ -35,12 +35,16 @@ struct Multigrid {
     std::vector<std::unique_ptr<MGDGLevel>> dg;
     std::vector<std::unique_ptr<MGVertexLevel>> vx;
     DevBuf coarse_inv;               // dense inverse of the coarsest vertex level
+    DevBuf corners;                  // [E][8] FP64: Q1 modes -> vertex functionals
     int nu_dg = 2, nu_vertex = 1;    // smoothing sweeps before and after the coarse correction
     double omega_dg = 0.8, omega_vertex = 0.7;
 
     void init(Operator *fine);
     // z = V-cycle(r); `skip` (device flag, may be null) skips all work
     void apply_raw(const void *r, void *z, const int *skip);
+    // one damped block-Jacobi sweep of the finest level: x += omega B^-1 (r - t)
+    // with t = A x given
+    void smooth_fine(const void *r, const void *t, void *x, const int *skip);
 
 private:
     template <typename T> void cycle_dg(size_t l, const T *r, T *x, const int *skip);

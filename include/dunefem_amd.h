@@ -357,8 +357,9 @@ int fem_cnn_train(fem_cnn *c, int epochs, int batch, float lr, float val_frac, u
 int fem_precond_create_learned(fem_op *op, fem_cnn *cnn, int coarse_steps, fem_precond **out);
 /* the same with a choice of smoother: FEM_PRE_BLOCK_JACOBI (the form above:
  * coarse correction, then block Jacobi on what remains) or FEM_PRE_MULTIGRID
- * (a multigrid V-cycle, then the learned coarse correction of the residual
- * it leaves; FEM_LEARNED_OFF is the V-cycle alone).  Train such a network on
+ * (a multigrid V-cycle, the learned coarse correction of the residual it
+ * leaves, then one damped block-Jacobi sweep of the fine level;
+ * FEM_LEARNED_OFF is the V-cycle and the sweep).  Train such a network on
  * pairs collected with a multigrid preconditioner (fem_cnn_collect_pairs). */
 int fem_precond_create_learned_ex(fem_op *op, fem_cnn *cnn, int coarse_steps, int smoother, fem_precond **out);
 int fem_precond_set_learned_mode(fem_precond *p, int mode);
