@@ -383,13 +383,13 @@ __global__ void __launch_bounds__(ApplyShape<K>::WG, (apply_min_waves<K, T, MF>(
     const int E = nx * ny * nz;
     const int blk = (int)xcd_remap(blockIdx.x, gridDim.x);
     const int e_raw = blk * EPB + el;
-    const bool active = wr && e_raw < E;
+    bool active = wr && e_raw < E;
     // shadow lanes (el = 0 when !wr) take the block's first element, so e
     // and its coordinates below always describe the same in-range element:
     // every neighbour index formed from them stays inside the grid
     int e = e_raw < E ? e_raw : 0;
     int ex, ey, ez;
-    if (a.slabs && blk * EPB < E) {
+    if (a.slabs) {
         // the workgroup's elements share one x-row.  Rows are visited in
         // slabs of ty (16, 8, 4, 2 or 1, dividing ny) y-rows, each slab
         // through all z, so the rows below and above (z -+ 1) are read a few
@@ -402,6 +402,11 @@ __global__ void __launch_bounds__(ApplyShape<K>::WG, (apply_min_waves<K, T, MF>(
         const int z = (int)udiv((unsigned)rem, a.dv_zrow), rem2 = rem - z * (rpb * ty);
         const int yl = (int)udiv((unsigned)rem2, a.dv_rpb), xu = rem2 - yl * rpb;
         ex = xu * EPB + el;
+        // past the end of the row (the last group of a padded row): a
+        // shadow of the row's last element, computed and not written
+        const bool inrow = ex < nx;
+        ex = inrow ? ex : nx - 1;
+        active = wr && inrow;
         ey = t * ty + yl;
         ez = z;
         e = ex + nx * (ey + ny * ez);
@@ -413,8 +418,10 @@ __global__ void __launch_bounds__(ApplyShape<K>::WG, (apply_min_waves<K, T, MF>(
     const int ec[3] = { ex, ey, ez };
     const int stride[3] = { 1, nx, nx * ny };
     // FP64 at K = 4: the four elements of a wavefront are consecutive
-    // elements of one x-row (lane 16 i + 4 el + j), so an x-face between two
-    // of them takes the neighbour's trace from the neighbour's own lanes
+    // elements of one x-row (lane 16 i + 4 el + j; in a padded last group the
+    // shadows follow the row's last element, whose +x face is a boundary
+    // face), so an x-face between two of them takes the neighbour's trace
+    // from the neighbour's own lanes
     // (lane +- 4), where it is the neighbour's own extrapolated trace, instead
     // of contracting the neighbour's DoFs.  Only the first element of the
     // wavefront loads its -x neighbour and the last its +x neighbour: one
@@ -825,7 +832,7 @@ static void launch_apply(const DGArgs<T> &a, const T *u, T *w, const int *skip, 
     using A = ApplyShape<K>;
     const int64_t E = (int64_t)a.n[0] * a.n[1] * a.n[2];
     FEM_CHECK(E * S::K3 < (int64_t)1 << 31, FEM_ERR_UNSUPPORTED, "DG apply: more than 2^31 DoFs");
-    const unsigned nb = (unsigned)((E + A::EPB - 1) / A::EPB);
+    const unsigned nb = (unsigned)dg_blocks(K, a.n);
     if (a.kappa_mode == FEM_KAPPA_QP)
         hipLaunchKernelGGL((k_dg_apply<K, T, MF, true>), dim3(nb), dim3(A::WG), 0, s, a, u, w, skip, dot);
     else
@@ -863,10 +870,11 @@ int dg_apply_epb(int k)
     return k == 5 ? ApplyShape<5>::EPB : 64 / (k * k);
 }
 
-int64_t dg_blocks(int k, int64_t E)
+int64_t dg_blocks(int k, const int *n)
 {
+    // the slab traversal: rpb groups per x-row, one workgroup each
     const int epb = dg_apply_epb(k);
-    return (E + epb - 1) / epb;
+    return (int64_t)((n[0] + epb - 1) / epb) * n[1] * n[2];
 }
 
 int64_t dg_bj_blocks(int k, int64_t E)

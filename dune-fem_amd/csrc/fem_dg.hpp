@@ -43,9 +43,11 @@ struct DGArgs {
     // per direction d, set on the host: beta / h_d, the face area of the
     // faces normal to d, |K| / h_d^2, b_d h_d, max(b_d, 0), max(-b_d, 0)
     T pen[3], farea[3], vsc[3], bh[3], bpos[3], bneg[3];
-    // element traversal of the apply (order k-1, epb = 64 / k^2 elements per
-    // workgroup): rows of epb elements in slabs of ty y-rows through all z;
-    // `slabs` is 0 when nx is not a multiple of epb (element order instead)
+    // element traversal of the apply (order k-1, epb elements per
+    // workgroup): x-rows cut into rpb = ceil(nx / epb) groups of epb
+    // elements (the last group of a row padded with shadow lanes that
+    // repeat the row's last element and write nothing), visited in slabs of
+    // ty y-rows through all z; `slabs` = 0 selects plain element order
     int slabs, rpb, ty;
     UDiv dv_slab, dv_zrow, dv_rpb;   // by rpb ty nz, rpb ty, rpb
 };
@@ -59,8 +61,8 @@ template <typename T>
 inline void dg_derived(DGArgs<T> &a, int k)
 {
     const int epb = dg_apply_epb(k);
-    a.slabs = a.n[0] % epb == 0;
-    a.rpb = a.n[0] / epb;
+    a.slabs = 1;
+    a.rpb = (a.n[0] + epb - 1) / epb;
     const int ny = a.n[1];
     a.ty = (ny % 16 == 0) ? 16 : (ny % 8 == 0) ? 8 : (ny % 4 == 0) ? 4 : (ny % 2 == 0) ? 2 : 1;
     if (a.slabs) {
@@ -89,9 +91,9 @@ void dg_apply_f32(const DGArgs<float> &a, int k, const float *u, float *w, const
                   double *dot = nullptr);
 // BF16 MFMA contraction variant (FP32 data and accumulation)
 void dg_apply_bf16(const DGArgs<float> &a, int k, const float *u, float *w, const int *skip, hipStream_t s);
-// workgroups of the DG apply launch for order k-1 (its per-workgroup dot
-// partials), and of the block-Jacobi launch
-int64_t dg_blocks(int k, int64_t E);
+// workgroups of the DG apply launch for order k-1 on n[0] x n[1] x n[2]
+// elements (its per-workgroup dot partials), and of the block-Jacobi launch
+int64_t dg_blocks(int k, const int *n);
 int64_t dg_bj_blocks(int k, int64_t E);
 // operator diagonal
 void dg_diagonal(const DGArgs<double> &a, int k, double *d, hipStream_t s);

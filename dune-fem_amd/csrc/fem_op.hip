@@ -219,7 +219,7 @@ void Operator::constrain(Vec &x, const Vec *add) const
 
 int64_t Operator::dot_blocks() const
 {
-    return is_dg() ? dg_blocks(k, E) : 0;
+    return is_dg() ? dg_blocks(k, desc.n) : 0;
 }
 
 void Operator::apply_raw(const void *x, void *y, const int *skip, double *dot) const
