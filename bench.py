@@ -49,9 +49,11 @@ def parse():
     ap.add_argument("--prec", choices=["f64", "f32", "bf16"], default="f64", help="precision of --apply-only")
     ap.add_argument("--calib", action="store_true",
                     help="also run a plain FP64 axpy of known size (HBM counter calibration)")
-    ap.add_argument("--mg-solve-only", choices=["c3", "c4"], default=None,
-                    help="only FP64 CG with the multigrid preconditioner on one workload, twice (for profiling); "
-                         "--prec f32 runs the V-cycle in FP32")
+    ap.add_argument("--solve-only", choices=["c3", "c4"], default=None,
+                    help="only FP64 CG on one workload, --reps times (for profiling); --pre mg (multigrid, --prec "
+                         "f32 runs the V-cycle in FP32) or bj (exact element block Jacobi)")
+    ap.add_argument("--pre", choices=["mg", "bj"], default="mg", help="preconditioner of --solve-only")
+    ap.add_argument("--reps", type=int, default=2, help="solves of --solve-only")
     return ap.parse_args()
 
 
@@ -397,24 +399,25 @@ def apply_only(a, fa, wl):
                       "achieved_gbs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4)}), flush=True)
 
 
-def mg_solve_only(a, fa, wl):
-    """FP64 CG + multigrid on the C3 or C4 workload, solved twice without
-    graph capture (the profiling driver for the V-cycle's kernels)."""
+def solve_only(a, fa, wl):
+    """FP64 CG with the multigrid or the block-Jacobi preconditioner on the
+    C3 or C4 workload, solved --reps times without graph capture (the
+    profiling driver of the solve kernels)."""
     ctx = fa.Context(0)
-    P = wl.problem3() if a.mg_solve_only == "c3" else wl.problem4()
+    P = wl.problem3() if a.solve_only == "c3" else wl.problem4()
     op64 = _dg_cell(fa, ctx, P, fa.F64)
     F = op64.rhs(P["fq"])
     del P["fq"]
     opm = op64 if a.prec == "f64" else _dg_cell(fa, ctx, P, fa.F32)
-    mg = fa.Preconditioner(opm, fa.PRE_MULTIGRID)
-    for rep in range(2):
+    pre = fa.Preconditioner(opm, fa.PRE_MULTIGRID if a.pre == "mg" else fa.PRE_BLOCK_JACOBI)
+    for rep in range(a.reps):
         X = op64.vector()
-        inv = fa.InverseOperator(ctx, method=fa.CG, tolerance=1e-8, errormeasure=fa.RELATIVE, maxiterations=1000,
+        inv = fa.InverseOperator(ctx, method=fa.CG, tolerance=1e-8, errormeasure=fa.RELATIVE, maxiterations=20000,
                                  residual_check=1)
-        inv.bind(op64, mg)
+        inv.bind(op64, pre)
         inv(F, X)
         info = inv.info()
-        print(json.dumps({"mg_solve_only": a.mg_solve_only, "vcycle_precision": a.prec, "rep": rep,
+        print(json.dumps({"solve_only": a.solve_only, "preconditioner": a.pre, "precision": a.prec, "rep": rep,
                           "iterations": info["iterations"], "time_s": round(info["timing"][0], 4),
                           "true_rel_residual": info["true_rel_residual"]}), flush=True)
 
@@ -430,8 +433,8 @@ def main():
     if a.apply_only:
         apply_only(a, fa, wl)
         return
-    if a.mg_solve_only:
-        mg_solve_only(a, fa, wl)
+    if a.solve_only:
+        solve_only(a, fa, wl)
         return
     if not a.no_cpu:
         # the CPU baseline runs the oracle built for this host (-march=native)

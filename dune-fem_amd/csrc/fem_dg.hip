@@ -82,14 +82,6 @@ struct ApplyShape {
     static constexpr int EPB = WG / (K * K);
 };
 
-// waves per SIMD the compiler is asked to fit: five for the FP64 Q3 apply
-// (at most 96 VGPRs), whose loads wait on memory most of the time
-template <int K, typename T, bool MF>
-constexpr int apply_min_waves()
-{
-    return (K == 4 && std::is_same<T, double>::value && !MF) ? 5 : 1;
-}
-
 template <int K>
 __device__ __forceinline__ int lidx(int a, int b, int c)
 {
@@ -326,7 +318,7 @@ struct Ctr {
 // KQ: kappa given at the quadrature points (FEM_KAPPA_QP); otherwise one
 // value per element (FEM_KAPPA_CELL) or one constant (FEM_KAPPA_CONST)
 template <int K, typename T, bool MF, bool KQ>
-__global__ void __launch_bounds__(ApplyShape<K>::WG, (apply_min_waves<K, T, MF>()))
+__global__ void __launch_bounds__(ApplyShape<K>::WG)
     k_dg_apply(const DGArgs<T> a, const T *__restrict__ u, T *__restrict__ w, const int *__restrict__ skip,
                double *__restrict__ dot)
 {
