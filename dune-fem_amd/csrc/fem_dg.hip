@@ -316,11 +316,14 @@ struct Ctr {
 };
 
 // KQ: kappa given at the quadrature points (FEM_KAPPA_QP); otherwise one
-// value per element (FEM_KAPPA_CELL) or one constant (FEM_KAPPA_CONST)
-template <int K, typename T, bool MF, bool KQ>
+// value per element (FEM_KAPPA_CELL) or one constant (FEM_KAPPA_CONST).
+// SM: the damped block-Jacobi sweep of a multigrid level in the same pass,
+// xout = u + omega B^-1 (r - A u), instead of w = A u (sm carries r, the
+// element blocks' eigen data, omega and xout; w and dot are not used)
+template <int K, typename T, bool MF, bool KQ, bool SM = false>
 __global__ void __launch_bounds__(ApplyShape<K>::WG)
     k_dg_apply(const DGArgs<T> a, const T *__restrict__ u, T *__restrict__ w, const int *__restrict__ skip,
-               double *__restrict__ dot)
+               double *__restrict__ dot, const DGSmooth<T> sm)
 {
     using S = DGShape<K>;
     constexpr int K2 = S::K2, K3 = S::K3, EPB = ApplyShape<K>::EPB, WG = ApplyShape<K>::WG;
@@ -789,7 +792,24 @@ __global__ void __launch_bounds__(ApplyShape<K>::WG)
 #pragma unroll
     for (int q = 0; q < K; ++q) v[q] = A[L(i, j, q)];
     C::BT(tb, mr, v, tmp);
-    if (active) {
+    if constexpr (SM) {
+        // the element's residual pencil d = r - A u, then the exact element
+        // block inverse through the element's LDS slice (K^3 values, then
+        // its eigen data), every lane of the workgroup taking part
+        static_assert(SZ >= K3 + 3 * K2 + 3 * K, "the element's LDS slice holds the block-inverse scratch");
+        const T *re = sm.r + (size_t)e * K3;
+#pragma unroll
+        for (int c = 0; c < K; ++c) tmp[c] = re[i * K2 + j * K + c] - tmp[c];
+        __syncthreads();   // the slice's last reads above are done
+        T *Ab = smem + el * SZ;
+        bj_element<K>(sm.Q + (size_t)e * 3 * K2, sm.lam + (size_t)e * 3 * K, Ab, Ab + K3, i, j, tmp);
+        if (active) {
+            T *xe = sm.xout + (size_t)e * K3;
+#pragma unroll
+            for (int c = 0; c < K; ++c) xe[i * K2 + j * K + c] = ue[i * K2 + j * K + c] + sm.omega * tmp[c];
+        }
+        return;
+    } else if (active) {
         T *we = w + (size_t)e * K3;
 #pragma unroll
         for (int c = 0; c < K; ++c) we[i * K2 + j * K + c] = tmp[c];
@@ -826,10 +846,53 @@ static void launch_apply(const DGArgs<T> &a, const T *u, T *w, const int *skip, 
     FEM_CHECK(E * S::K3 < (int64_t)1 << 31, FEM_ERR_UNSUPPORTED, "DG apply: more than 2^31 DoFs");
     const unsigned nb = (unsigned)dg_blocks(K, a.n);
     if (a.kappa_mode == FEM_KAPPA_QP)
-        hipLaunchKernelGGL((k_dg_apply<K, T, MF, true>), dim3(nb), dim3(A::WG), 0, s, a, u, w, skip, dot);
+        hipLaunchKernelGGL((k_dg_apply<K, T, MF, true>), dim3(nb), dim3(A::WG), 0, s, a, u, w, skip, dot,
+                           DGSmooth<T>{});
     else
-        hipLaunchKernelGGL((k_dg_apply<K, T, MF, false>), dim3(nb), dim3(A::WG), 0, s, a, u, w, skip, dot);
+        hipLaunchKernelGGL((k_dg_apply<K, T, MF, false>), dim3(nb), dim3(A::WG), 0, s, a, u, w, skip, dot,
+                           DGSmooth<T>{});
     FEM_HIP(hipGetLastError());
+}
+
+template <int K, typename T>
+static void launch_smooth(const DGArgs<T> &a, const T *u, const DGSmooth<T> &sm, const int *skip, hipStream_t s)
+{
+    using S = DGShape<K>;
+    using A = ApplyShape<K>;
+    const int64_t E = (int64_t)a.n[0] * a.n[1] * a.n[2];
+    FEM_CHECK(E * S::K3 < (int64_t)1 << 31, FEM_ERR_UNSUPPORTED, "DG apply: more than 2^31 DoFs");
+    FEM_CHECK(a.kappa_mode != FEM_KAPPA_QP, FEM_ERR_UNSUPPORTED, "smoothing sweep: cellwise or constant kappa");
+    FEM_CHECK(sm.r && sm.Q && sm.lam && sm.xout && sm.xout != u, FEM_ERR_ARG,
+              "smoothing sweep: r, eigen data and an output other than the input");
+    const unsigned nb = (unsigned)dg_blocks(K, a.n);
+    hipLaunchKernelGGL((k_dg_apply<K, T, false, false, true>), dim3(nb), dim3(A::WG), 0, s, a, u, (T *)nullptr, skip,
+                       (double *)nullptr, sm);
+    FEM_HIP(hipGetLastError());
+}
+
+template <typename T>
+static void dispatch_smooth(const DGArgs<T> &a, int k, const T *u, const DGSmooth<T> &sm, const int *skip,
+                            hipStream_t s)
+{
+    switch (k) {
+    case 2: launch_smooth<2, T>(a, u, sm, skip, s); break;
+    case 3: launch_smooth<3, T>(a, u, sm, skip, s); break;
+    case 4: launch_smooth<4, T>(a, u, sm, skip, s); break;
+    case 5: launch_smooth<5, T>(a, u, sm, skip, s); break;
+    default: throw Error(FEM_ERR_UNSUPPORTED, "DG apply: order must be 1..4");
+    }
+}
+
+void dg_smooth_f64(const DGArgs<double> &a, int k, const double *u, const DGSmooth<double> &sm, const int *skip,
+                   hipStream_t s)
+{
+    dispatch_smooth<double>(a, k, u, sm, skip, s);
+}
+
+void dg_smooth_f32(const DGArgs<float> &a, int k, const float *u, const DGSmooth<float> &sm, const int *skip,
+                   hipStream_t s)
+{
+    dispatch_smooth<float>(a, k, u, sm, skip, s);
 }
 
 template <typename T>

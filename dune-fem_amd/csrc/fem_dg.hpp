@@ -52,6 +52,18 @@ struct DGArgs {
     UDiv dv_slab, dv_zrow, dv_rpb;   // by rpb ty nz, rpb ty, rpb
 };
 
+// the damped block-Jacobi sweep fused into the apply (multigrid smoothing):
+// xout = u + omega B^-1 (r - A u) with B^-1 the exact element-block inverse
+// from the eigen data Q [E][3][k][k], lam [E][3][k]
+template <typename T>
+struct DGSmooth {
+    const T *r = nullptr;
+    const T *Q = nullptr;
+    const T *lam = nullptr;
+    T omega = T(0);
+    T *xout = nullptr;
+};
+
 // elements per workgroup of the apply for k points per direction
 int dg_apply_epb(int k);
 
@@ -89,6 +101,12 @@ void dg_apply_f64(const DGArgs<double> &a, int k, const double *u, double *w, co
                   double *dot = nullptr);
 void dg_apply_f32(const DGArgs<float> &a, int k, const float *u, float *w, const int *skip, hipStream_t s,
                   double *dot = nullptr);
+// the sweep xout = u + omega B^-1 (r - A u) in one pass (cellwise or
+// constant kappa; xout must not alias u, whose neighbours the pass reads)
+void dg_smooth_f64(const DGArgs<double> &a, int k, const double *u, const DGSmooth<double> &sm, const int *skip,
+                   hipStream_t s);
+void dg_smooth_f32(const DGArgs<float> &a, int k, const float *u, const DGSmooth<float> &sm, const int *skip,
+                   hipStream_t s);
 // BF16 MFMA contraction variant (FP32 data and accumulation)
 void dg_apply_bf16(const DGArgs<float> &a, int k, const float *u, float *w, const int *skip, hipStream_t s);
 // workgroups of the DG apply launch for order k-1 on n[0] x n[1] x n[2]

@@ -28,6 +28,7 @@
 // on a 12^3 channel problem: 69 PCG iterations with an exact Q_0 coarse
 // solve against 27 with the exact conforming one).
 #include <cmath>
+#include <type_traits>
 
 #include "fem_blas.hpp"
 #include "fem_dg_bj.hpp"
@@ -657,6 +658,29 @@ static void mg_bj(const MGDGLevel &L, const T *r, const T *t, T *x, T omega, int
 #undef MGBJ_CASE
 }
 
+// one damped block-Jacobi sweep on a level, out = in + omega B^-1 (r - A in),
+// in one pass of the apply kernel (FP64 and FP32 operators) or as an apply
+// followed by the block-Jacobi kernel (BF16-MFMA operators, whose sweep
+// keeps the BF16 apply; then out == in and t is the scratch for A in)
+template <typename T>
+static void mg_sweep(const MGDGLevel &L, const T *r, const T *in, T *out, T *t, T omega, const int *skip, hipStream_t s)
+{
+    const Operator &o = *L.op;
+    if (o.desc.precision == FEM_BF16MFMA) {
+        L.op->apply_raw(in, t, skip);
+        mg_bj<T>(L, r, t, out, omega, 1, skip, s);
+        return;
+    }
+    DGSmooth<T> sm;
+    sm.r = r;
+    sm.Q = L.bj.Q.as<T>();
+    sm.lam = L.bj.lam.as<T>();
+    sm.omega = omega;
+    sm.xout = out;
+    if constexpr (std::is_same<T, double>::value) dg_smooth_f64(o.a64, o.k, in, sm, skip, s);
+    else dg_smooth_f32(o.a32, o.k, in, sm, skip, s);
+}
+
 template <typename T>
 void Multigrid::cycle_dg(size_t l, const T *r, T *x, const int *skip)
 {
@@ -666,30 +690,44 @@ void Multigrid::cycle_dg(size_t l, const T *r, T *x, const int *skip)
     const int64_t E = op->E;
     T *t = L.t.as<T>();
     const T om = (T)omega_dg;
-    mg_bj<T>(L, r, nullptr, x, om, 0, skip, s);
-    for (int it = 1; it < nu_dg; ++it) {
-        L.op->apply_raw(x, t, skip);
-        mg_bj<T>(L, r, t, x, om, 1, skip, s);
-    }
-    (L.resop ? L.resop.get() : L.op)->apply_raw(x, t, skip);
+    // the fused sweeps write a second buffer (a sweep reads its input's
+    // neighbours): cur is the buffer holding the current iterate, and the
+    // first sweep is arranged so that the last one ends in x
+    const bool fused = L.op->desc.precision != FEM_BF16MFMA;
+    T *cur = x;
+    auto sweep = [&]() {
+        if (!fused) {
+            mg_sweep<T>(L, r, cur, cur, t, om, skip, s);
+            return;
+        }
+        T *nxt = cur == x ? t : x;
+        mg_sweep<T>(L, r, cur, nxt, t, om, skip, s);
+        cur = nxt;
+    };
+    // the pre-smoothing leaves its result in t when the remaining sweeps
+    // (nu - 1 before, nu after) are odd in number, so that they end in x
+    if (fused && ((2 * nu_dg - 1) & 1)) cur = t;
+    mg_bj<T>(L, r, nullptr, cur, om, 0, skip, s);
+    for (int it = 1; it < nu_dg; ++it) sweep();
+    // A cur for the restriction, in whichever buffer cur is not
+    T *ax = fused ? (cur == x ? t : x) : t;
+    (L.resop ? L.resop.get() : L.op)->apply_raw(cur, ax, skip);
     if (l + 1 < dg.size()) {
         MGDGLevel &C = *dg[l + 1];
         hipLaunchKernelGGL(k_mg_restrict_modes<T>, dim3(mg_grid(E * C.k * C.k * C.k)), dim3(MGT), 0, s, E, L.k, C.k, r,
-                           (const T *)t, C.r.as<T>(), skip);
+                           (const T *)ax, C.r.as<T>(), skip);
         cycle_dg<T>(l + 1, C.r.as<T>(), C.x.as<T>(), skip);
         hipLaunchKernelGGL(k_mg_prolong_modes<T>, dim3(mg_grid(E * C.k * C.k * C.k)), dim3(MGT), 0, s, E, L.k, C.k,
-                           (const T *)C.x.as<T>(), x, skip);
+                           (const T *)C.x.as<T>(), cur, skip);
     } else {
         MGVertexLevel &V0 = *vx[0];
-        dg1_to_vertex<T>(n, r, (const T *)t, corners.as<double>(), V0.r.as<T>(), skip, s);
+        dg1_to_vertex<T>(n, r, (const T *)ax, corners.as<double>(), V0.r.as<T>(), skip, s);
         cycle_vertex<T>(0, V0.r.as<T>(), V0.x.as<T>(), skip);
         hipLaunchKernelGGL(k_mg_vertex_to_dg1<T>, dim3(mg_grid(E * 8)), dim3(MGT), 0, s, n[0], n[1], n[2],
-                           (const T *)V0.x.as<T>(), x, skip);
+                           (const T *)V0.x.as<T>(), cur, skip);
     }
-    for (int it = 0; it < nu_dg; ++it) {
-        L.op->apply_raw(x, t, skip);
-        mg_bj<T>(L, r, t, x, om, 1, skip, s);
-    }
+    for (int it = 0; it < nu_dg; ++it) sweep();
+    FEM_CHECK(cur == x, FEM_ERR_STATE, "multigrid: the sweeps end outside the output buffer");
     FEM_HIP(hipGetLastError());
 }
 
