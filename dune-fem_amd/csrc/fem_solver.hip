@@ -936,10 +936,37 @@ void Solver::solve(const Vec &b, Vec &x, fem_solver_info &info)
     const int64_t N = op->N;
     FEM_HIP(hipStreamSynchronize(ctx->stream));
     auto t0 = std::chrono::steady_clock::now();
+    Vec r;
+    r.ctx = ctx;
+    r.n = N;
+    r.prec = op->vec_prec;
+    r.data = workbuf(20, x.bytes());
+    r.owner = false;
     if (!prm.refinement) {
         info.iterations = krylov(prm.method, op, pre, b.data, x.data, prm.tolerance, prm.errormeasure,
                                  prm.maxiterations, prm.restart, N, op->vec_prec);
         info.converged = info.iterations >= 0;
+        // residualcheck holds CG to the true residual as well: when the
+        // recursive residual has converged but b - A x has drifted above the
+        // tolerance, CG restarts from x (at most twice)
+        if (prm.residual_check && prm.method == FEM_SOLVER_CG && prm.errormeasure != FEM_ERRMEASURE_REDUCTION) {
+            for (int rs = 0; rs < 2 && info.converged; ++rs) {
+                const double rel = true_residual(b, x, r);
+                const double bn = std::sqrt(vec_dot(b, b));
+                const double res = prm.errormeasure == FEM_ERRMEASURE_RELATIVE ? rel : rel * (bn > 0 ? bn : 1.0);
+                if (!(res > prm.tolerance)) break;
+                const int left = prm.maxiterations - info.iterations;
+                if (left <= 0) {
+                    info.converged = 0;
+                    info.iterations = -info.iterations;
+                    break;
+                }
+                const int more = krylov(prm.method, op, pre, b.data, x.data, prm.tolerance, prm.errormeasure, left,
+                                        prm.restart, N, op->vec_prec);
+                info.iterations = more >= 0 ? info.iterations + more : -(info.iterations - more);
+                info.converged = more >= 0;
+            }
+        }
     } else {
         // FP64 iterative refinement around a low-precision Krylov solve
         hipStream_t s = ctx->stream;
@@ -982,12 +1009,6 @@ void Solver::solve(const Vec &b, Vec &x, fem_solver_info &info)
     FEM_HIP(hipStreamSynchronize(ctx->stream));
     auto t1 = std::chrono::steady_clock::now();
     info.time_s = std::chrono::duration<double>(t1 - t0).count();
-    Vec r;
-    r.ctx = ctx;
-    r.n = N;
-    r.prec = op->vec_prec;
-    r.data = workbuf(20, x.bytes());
-    r.owner = false;
     info.true_rel_residual = true_residual(b, x, r);
     info.fallback = fallback_used;
 }

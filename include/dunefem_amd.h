@@ -234,7 +234,9 @@ typedef struct {
     int inner_maxiterations;
     /* CG stopping rule: 0 = the reference rule <r, M^-1 r> <= eps^2 ref
      * (dune/fem/solver/linear/cg.hh:62-70); 1 = on the unpreconditioned
-     * residual ||r||_2^2 <= eps^2 ref (GMRES always stops on ||r||_2) */
+     * residual ||r||_2^2 <= eps^2 ref, and at convergence on the true
+     * residual b - A x (CG restarts from x, at most twice, while it is above
+     * the tolerance); GMRES always stops on ||r||_2 */
     int residual_check;
     /* fem.solver.preconditioning.method (dune/fem/solver/parameter.hh:206-217):
      * the preconditioner the solver builds for itself at bind when no
@@ -284,7 +286,8 @@ void fem_solver_default_params(fem_solver_params *p);
  *   preconditioning.level  0 only                            (:235-238)
  *   threading              true | false (accepted; one stream per context)
  *   knolltrick             false only                        (:245-248)
- *   residualcheck          true | false (this library: CG stops on ||r||)
+ *   residualcheck          true | false (this library: CG stops on ||r||, and
+ *                          restarts from x while ||b - A x|| is above the tolerance)
  * Newton keys (nonlinear.*, newton.* without linear.) are not linear-solver
  * keys: FEM_ERR_ARG. */
 int fem_solver_params_set(fem_solver_params *p, const char *key, const char *value);
