@@ -115,7 +115,8 @@ def test_multigrid_solves_full_size_workloads(ctx, prob):
     inv(F, X)
     info = inv.info()
     print(prob, "full size CG+MG:", info, flush=True)
-    assert info["converged"] and info["true_rel_residual"] < 1.01e-8
+    # residual_check: the solve also holds b - A x to the tolerance
+    assert info["converged"] and info["true_rel_residual"] <= 1e-8
 
 
 def test_cg_step_graph_replays_match_eager_solve(ctx):
