@@ -1,6 +1,7 @@
 // extern "C" boundary: opaque handles, int status codes, no exceptions
 // across the ABI (the reference throws Dune exceptions; non-convergence is a
 // negative iteration count, not an error).
+#include <algorithm>
 #include <cstddef>
 #include <type_traits>
 #include <cstdio>
@@ -243,6 +244,142 @@ int fem_vec_download(const fem_vec *v, double *host)
             FEM_HIP(hipStreamSynchronize(x.ctx->stream));
             for (int64_t i = 0; i < x.n; ++i) host[i] = tmp[i];
         }
+    });
+}
+
+// ---- DoF vectors in the reference's binary stream format ---------------------------
+}  // extern "C"
+
+namespace {
+
+constexpr unsigned kDuneFemVersionId = (2u << 24) | (12u << 16);   // DUNE_VERSION_ID(2,12,0)
+constexpr unsigned kOldestVersionId = (0u << 24) | (9u << 16) | 1u;  // DUNE_VERSION_ID(0,9,1)
+constexpr unsigned kSpaceIdVersionId = (1u << 24) | (5u << 16);     // DUNE_VERSION_ID(1,5,0)
+
+// little-endian primitives, byte by byte (the stream's byte order whatever
+// the host's)
+template <typename U>
+void put_le(std::vector<unsigned char> &out, U v)
+{
+    unsigned char b[sizeof(U)];
+    std::memcpy(b, &v, sizeof(U));
+    const uint16_t probe = 1;
+    const bool little = *reinterpret_cast<const unsigned char *>(&probe) == 1;
+    for (size_t i = 0; i < sizeof(U); ++i) out.push_back(b[little ? i : sizeof(U) - 1 - i]);
+}
+
+template <typename U>
+U get_le(const unsigned char *&p, const unsigned char *end)
+{
+    FEM_CHECK(end - p >= (ptrdiff_t)sizeof(U), FEM_ERR_ARG, "DoF stream: truncated");
+    unsigned char b[sizeof(U)];
+    const uint16_t probe = 1;
+    const bool little = *reinterpret_cast<const unsigned char *>(&probe) == 1;
+    for (size_t i = 0; i < sizeof(U); ++i) b[little ? i : sizeof(U) - 1 - i] = p[i];
+    p += sizeof(U);
+    U v;
+    std::memcpy(&v, b, sizeof(U));
+    return v;
+}
+
+std::vector<unsigned char> dofs_encode(const double *dofs, int64_t n, const char *name, int space_id)
+{
+    FEM_CHECK(n >= 0 && n <= INT32_MAX, FEM_ERR_ARG, "DoF stream: the size is written as a 32-bit integer");
+    FEM_CHECK(n == 0 || dofs, FEM_ERR_ARG, "null DoFs");
+    const std::string nm = name ? name : "";
+    std::vector<unsigned char> out;
+    out.reserve(16 + nm.size() + 8 * (size_t)n);
+    put_le<uint32_t>(out, kDuneFemVersionId);
+    put_le<int32_t>(out, space_id);
+    put_le<uint32_t>(out, (uint32_t)nm.size());
+    for (char c : nm) out.push_back((unsigned char)c);
+    put_le<int32_t>(out, (int32_t)n);
+    for (int64_t i = 0; i < n; ++i) put_le<double>(out, dofs[i]);
+    return out;
+}
+
+// the DoFs into dofs[n]; returns the name
+std::string dofs_decode(const unsigned char *buf, int64_t len, int space_id, double *dofs, int64_t n)
+{
+    FEM_CHECK(buf || len == 0, FEM_ERR_ARG, "null buffer");
+    const unsigned char *p = buf, *end = buf + len;
+    const uint32_t version = get_le<uint32_t>(p, end);
+    FEM_CHECK(version >= kOldestVersionId, FEM_ERR_ARG, "DoF stream: trying to read outdated file");
+    if (version >= kSpaceIdVersionId) {
+        const int32_t sid = get_le<int32_t>(p, end);
+        FEM_CHECK(sid == space_id, FEM_ERR_ARG, "DoF stream: written from a different space");
+    }
+    const uint32_t nl = get_le<uint32_t>(p, end);
+    FEM_CHECK(end - p >= (ptrdiff_t)nl, FEM_ERR_ARG, "DoF stream: truncated");
+    std::string nm(reinterpret_cast<const char *>(p), nl);
+    p += nl;
+    const int32_t size = get_le<int32_t>(p, end);
+    FEM_CHECK(size == n, FEM_ERR_ARG, "DoF stream: trying to read a discrete function of different size");
+    FEM_CHECK(n == 0 || dofs, FEM_ERR_ARG, "null DoFs");
+    for (int64_t i = 0; i < n; ++i) dofs[i] = get_le<double>(p, end);
+    return nm;
+}
+
+void copy_name(const std::string &nm, char *name, int64_t cap)
+{
+    if (!name || cap <= 0) return;
+    const size_t k = std::min<size_t>(nm.size(), (size_t)(cap - 1));
+    std::memcpy(name, nm.data(), k);
+    name[k] = 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fem_dofs_encode(const double *dofs, int64_t n, const char *name, int space_id, unsigned char *buf, int64_t cap,
+                    int64_t *len)
+{
+    return guarded([&] {
+        FEM_CHECK(len, FEM_ERR_ARG, "null length");
+        const std::vector<unsigned char> out = dofs_encode(dofs, n, name, space_id);
+        *len = (int64_t)out.size();
+        FEM_CHECK(buf && cap >= (int64_t)out.size(), FEM_ERR_ARG, "DoF stream: buffer too small");
+        std::memcpy(buf, out.data(), out.size());
+    });
+}
+
+int fem_dofs_decode(const unsigned char *buf, int64_t len, int space_id, double *dofs, int64_t n, char *name,
+                    int64_t name_cap)
+{
+    return guarded([&] { copy_name(dofs_decode(buf, len, space_id, dofs, n), name, name_cap); });
+}
+
+int fem_vec_write(const fem_vec *v, const char *path, const char *name, int space_id)
+{
+    return guarded([&] {
+        FEM_CHECK(v && path, FEM_ERR_ARG, "null argument");
+        std::vector<double> host(v->v.n);
+        FEM_CHECK(fem_vec_download(v, host.data()) == FEM_OK, FEM_ERR_HIP, g_last_error);
+        const std::vector<unsigned char> out = dofs_encode(host.data(), v->v.n, name, space_id);
+        std::FILE *f = std::fopen(path, "wb");
+        FEM_CHECK(f, FEM_ERR_ARG, std::string("cannot open for writing: ") + path);
+        const size_t w = std::fwrite(out.data(), 1, out.size(), f);
+        const int rc = std::fclose(f);
+        FEM_CHECK(w == out.size() && rc == 0, FEM_ERR_ARG, std::string("write failed: ") + path);
+    });
+}
+
+int fem_vec_read(fem_vec *v, const char *path, int space_id, char *name, int64_t name_cap)
+{
+    return guarded([&] {
+        FEM_CHECK(v && path, FEM_ERR_ARG, "null argument");
+        std::FILE *f = std::fopen(path, "rb");
+        FEM_CHECK(f, FEM_ERR_ARG, std::string("cannot open for reading: ") + path);
+        std::vector<unsigned char> buf;
+        unsigned char chunk[1 << 16];
+        size_t k;
+        while ((k = std::fread(chunk, 1, sizeof(chunk), f)) > 0) buf.insert(buf.end(), chunk, chunk + k);
+        std::fclose(f);
+        std::vector<double> host(v->v.n);
+        const std::string nm = dofs_decode(buf.data(), (int64_t)buf.size(), space_id, host.data(), v->v.n);
+        FEM_CHECK(fem_vec_upload(v, host.data()) == FEM_OK, FEM_ERR_HIP, g_last_error);
+        copy_name(nm, name, name_cap);
     });
 }
 

@@ -31,6 +31,7 @@ PRE_NONE, PRE_JACOBI, PRE_BLOCK_JACOBI, PRE_LEARNED, PRE_MULTIGRID = 0, 1, 2, 3,
 LEARNED_OFF, LEARNED_NET, LEARNED_COARSE_ONLY = 0, 1, 2
 CG, GMRES, FGMRES, BICGSTAB = 0, 1, 2, 3
 ABSOLUTE, RELATIVE, REDUCTION = 0, 1, 2
+SPACE_LAGRANGE, SPACE_LEGENDRE_DG = 6, 8
 
 # every symbol declared in include/dunefem_amd.h
 EXPORTS = [
@@ -51,6 +52,7 @@ EXPORTS = [
     "fem_cnn_train", "fem_precond_create_learned", "fem_precond_create_learned_ex", "fem_precond_set_learned_mode", "fem_cnn_coarse_apply",
     "fem_cnn_coarse_solve", "fem_op_assemble_csr", "fem_csr_destroy", "fem_csr_rows", "fem_csr_nnz",
     "fem_csr_apply", "fem_csr_diagonal", "fem_csr_download",
+    "fem_dofs_encode", "fem_dofs_decode", "fem_vec_write", "fem_vec_read",
 ]
 
 
@@ -115,6 +117,12 @@ def load_library(path=LIBPATH):
         "fem_vec_upload": ([_vp, _dp], C.c_int),
         "fem_vec_download": ([_vp, _dp], C.c_int),
         "fem_vec_upload_f32": ([_vp, C.POINTER(C.c_float)], C.c_int),
+        "fem_dofs_encode": ([C.POINTER(C.c_double), C.c_int64, C.c_char_p, C.c_int, C.c_void_p, C.c_int64,
+                             C.POINTER(C.c_int64)], C.c_int),
+        "fem_dofs_decode": ([C.c_void_p, C.c_int64, C.c_int, C.POINTER(C.c_double), C.c_int64, C.c_char_p,
+                             C.c_int64], C.c_int),
+        "fem_vec_write": ([_vp, C.c_char_p, C.c_char_p, C.c_int], C.c_int),
+        "fem_vec_read": ([_vp, C.c_char_p, C.c_int, C.c_char_p, C.c_int64], C.c_int),
         "fem_vec_fill": ([_vp, C.c_double], C.c_int),
         "fem_vec_copy": ([_vp, _vp], C.c_int),
         "fem_vec_axpy": ([_vp, C.c_double, _vp], C.c_int),
@@ -190,6 +198,29 @@ def load_library(path=LIBPATH):
 def _check(rc):
     if rc != 0:
         raise FemError("dunefem_amd error %d: %s" % (rc, _lib.fem_last_error().decode()))
+
+
+def encode_dofs(dofs, name="", space=SPACE_LEGENDRE_DG):
+    """The bytes DiscreteFunction::write puts on a binary stream for these
+    DoFs (host only, no device work)."""
+    L = load_library()
+    d = np.ascontiguousarray(dofs, dtype=np.float64)
+    n = C.c_int64()
+    L.fem_dofs_encode(_dptr(d), d.size, name.encode(), int(space), None, 0, C.byref(n))
+    buf = C.create_string_buffer(n.value)
+    _check(L.fem_dofs_encode(_dptr(d), d.size, name.encode(), int(space), buf, n.value, C.byref(n)))
+    return buf.raw[:n.value]
+
+
+def decode_dofs(data, n, space=SPACE_LEGENDRE_DG):
+    """DiscreteFunction::read of n DoFs from the bytes of a binary stream:
+    (dofs, name)."""
+    L = load_library()
+    raw = C.create_string_buffer(bytes(data), len(data))
+    out = np.empty(int(n))
+    name = C.create_string_buffer(4096)
+    _check(L.fem_dofs_decode(raw, len(data), int(space), _dptr(out), int(n), name, len(name)))
+    return out, name.value.decode()
 
 
 def _dptr(a):
@@ -269,6 +300,18 @@ class DofVector:
 
     def numpy(self):
         return self.download()
+
+    def write(self, path, name="", space=SPACE_LEGENDRE_DG):
+        """DiscreteFunction::write through a binary file stream
+        (function/common/discretefunction_inline.hh:270-305)."""
+        _check(_lib.fem_vec_write(self.h, os.fsencode(path), name.encode(), int(space)))
+
+    def read(self, path, space=SPACE_LEGENDRE_DG):
+        """DiscreteFunction::read (discretefunction_inline.hh:213-267); returns
+        the stored name."""
+        buf = C.create_string_buffer(4096)
+        _check(_lib.fem_vec_read(self.h, os.fsencode(path), int(space), buf, len(buf)))
+        return buf.value.decode()
 
     def clear(self):
         _check(_lib.fem_vec_fill(self.h, 0.0))

@@ -77,6 +77,27 @@ void *fem_vec_device_ptr(fem_vec *v);
 int fem_vec_upload(fem_vec *v, const double *host);
 int fem_vec_download(const fem_vec *v, double *host);
 int fem_vec_upload_f32(fem_vec *v, const float *host);
+/* DoF vectors in the reference's binary stream format:
+ * DiscreteFunctionDefault::write / read (function/common/discretefunction_inline.hh:213-305)
+ * through a BinaryFileOutStream / StandardOutStream (io/streams/standardstreams.hh:60-170):
+ * little-endian unsigned version id (written: DUNE_VERSION_ID(2,12,0) =
+ * 2<<24 | 12<<16; files older than 0.9.1 are refused), int space id (checked
+ * for files from 1.5.0 on), the name as unsigned length + bytes, int32 DoF
+ * count, then the DoFs as doubles in DoF order.  The space ids are the
+ * DFSpaceIdentifier values (space/common/discretefunctionspace.hh:95-108).
+ * Legendre DG DoFs are in the reference's order (element-major, the element
+ * index x fastest); the Q2 DoFs are in this library's node-lattice order, a
+ * permutation of the reference's IndexSetDofMapper order.
+ * fem_dofs_encode / fem_dofs_decode act on host arrays (no device work);
+ * *len receives the encoded size (also when cap is too small: FEM_ERR_ARG). */
+#define FEM_SPACE_LAGRANGE 6
+#define FEM_SPACE_LEGENDRE_DG 8
+int fem_dofs_encode(const double *dofs, int64_t n, const char *name, int space_id, unsigned char *buf, int64_t cap,
+                    int64_t *len);
+int fem_dofs_decode(const unsigned char *buf, int64_t len, int space_id, double *dofs, int64_t n, char *name,
+                    int64_t name_cap);
+int fem_vec_write(const fem_vec *v, const char *path, const char *name, int space_id);
+int fem_vec_read(fem_vec *v, const char *path, int space_id, char *name, int64_t name_cap);
 /* DiscreteFunctionInterface BLAS-1 (dune/fem/function/common/discretefunction.hh:731-835,
  * defaultblockvectors.hh:110-135): clear, assign, axpy, scaling, scalarProductDofs,
  * normSquaredDofs (scalarproducts.hh:101-127).  Mixed precisions convert. */
