@@ -133,6 +133,19 @@ public:
         check(fem_vec_norm2sq(h_, &d));
         return d;
     }
+    // write / read in the binary stream format of DiscreteFunctionDefault
+    // (discretefunction_inline.hh:213-305) to / from a file; read takes the
+    // stored name, as the reference does
+    void write(const std::string &path, int spaceId = FEM_SPACE_LEGENDRE_DG) const
+    {
+        check(fem_vec_write(h_, path.c_str(), name_.c_str(), spaceId));
+    }
+    void read(const std::string &path, int spaceId = FEM_SPACE_LEGENDRE_DG)
+    {
+        std::vector<char> nm(4096, 0);
+        check(fem_vec_read(h_, path.c_str(), spaceId, nm.data(), (int64_t)nm.size()));
+        name_ = nm.data();
+    }
     // host <-> device copies of the dof vector
     void upload(const std::vector<double> &dofs)
     {

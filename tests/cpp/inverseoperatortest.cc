@@ -289,6 +289,14 @@ static void scheme_check(Context &ctx)
     report(info.converged && info.nonlinearIterations == 1, "FemScheme::solve converged in one Newton step",
            (double)info.linearIterations);
     report(err <= 1e-9 * mx, "FemScheme::solve reproduces u* in Q2 with Dirichlet data", err / mx);
+    // backup and restore of the solution (DiscreteFunction::write / read)
+    const std::string path = "/tmp/dunefem_amd_inverseoperatortest_u.bin";
+    u.write(path, FEM_SPACE_LAGRANGE);
+    DiscreteFunction v = A.function("v");
+    v.read(path, FEM_SPACE_LAGRANGE);
+    const std::vector<double> vh = v.download();
+    std::remove(path.c_str());
+    report(vh == uh && v.name() == "u", "DiscreteFunction write / read restores DoFs and name", 0.0);
 }
 
 int main()
