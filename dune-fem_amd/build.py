@@ -15,7 +15,7 @@ OUT_DIR = os.path.join(HERE, "dunefem_amd")
 OBJ_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libdunefem_amd.so")
 
-SOURCES = ["fem_tables.cpp", "fem_blas.hip", "fem_dg.hip", "fem_cg.hip",
+SOURCES = ["fem_tables.cpp", "fem_blas.hip", "fem_dg.hip", "fem_cg.hip", "fem_newton.cpp",
            "fem_op.hip", "fem_mg.hip", "fem_csr.hip", "fem_noalloc.hip", "fem_solver.hip", "fem_batch.hip", "fem_cnn.hip", "fem_learned.hip",
            "fem_capi.cpp"]
 

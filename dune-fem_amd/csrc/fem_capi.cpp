@@ -2,6 +2,7 @@
 // across the ABI (the reference throws Dune exceptions; non-convergence is a
 // negative iteration count, not an error).
 #include <algorithm>
+#include <climits>
 #include <cstddef>
 #include <type_traits>
 #include <cstdio>
@@ -14,6 +15,7 @@
 #include "fem_cnn.hpp"
 #include "fem_csr.hpp"
 #include "fem_learned.hpp"
+#include "fem_newton.hpp"
 #include "fem_op.hpp"
 #include "fem_solver.hpp"
 
@@ -534,6 +536,61 @@ int fem_op_block_diagonal(fem_op *op, fem_vec *blocks)
     });
 }
 
+// ---- pointwise and nonlinear reaction ----------------------------------------------
+int fem_op_set_reaction_qp(fem_op *op, const double *cvol)
+{
+    return guarded([&] {
+        FEM_CHECK(op, FEM_ERR_ARG, "null operator");
+        op->o.set_reaction_qp(cvol);
+    });
+}
+
+int fem_op_set_reaction_law(fem_op *op, double coef, int pw)
+{
+    return guarded([&] {
+        FEM_CHECK(op, FEM_ERR_ARG, "null operator");
+        op->o.set_reaction_law(coef, pw);
+    });
+}
+
+int fem_op_nonlinear(const fem_op *op)
+{
+    if (!op) return FEM_ERR_ARG;
+    return op->o.nonlinear() ? 1 : 0;
+}
+
+int fem_op_eval_qp(fem_op *op, const fem_vec *u, fem_vec *uq)
+{
+    return guarded([&] {
+        FEM_CHECK(op && u && uq, FEM_ERR_ARG, "null argument");
+        op->o.eval_qp(u->v, uq->v);
+    });
+}
+
+int fem_op_jacobian_create(const fem_op *op, fem_op **jop)
+{
+    return guarded([&] {
+        FEM_CHECK(op && jop, FEM_ERR_ARG, "null argument");
+        fem_op *h = new fem_op();
+        try {
+            op->o.make_jacobian(h->o);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        h->held = ctx_retain(&*op->o.ctx);
+        *jop = h;
+    });
+}
+
+int fem_op_jacobian(const fem_op *op, const fem_vec *u, fem_op *jop)
+{
+    return guarded([&] {
+        FEM_CHECK(op && u && jop, FEM_ERR_ARG, "null argument");
+        op->o.jacobian(u->v, jop->o);
+    });
+}
+
 // ---- assembled Jacobian ----------------------------------------------------------
 int fem_op_assemble_csr(fem_op *op, fem_csr **out)
 {
@@ -542,6 +599,7 @@ int fem_op_assemble_csr(fem_op *op, fem_csr **out)
         FEM_CHECK(op->o.is_dg() ? op->o.a64.kappa_mode >= 0
                                 : (op->o.desc.kappa_mode == FEM_KAPPA_CONST || op->o.kvol64.p),
                   FEM_ERR_STATE, "coefficient data not set");
+        FEM_CHECK(!op->o.nonlinear(), FEM_ERR_STATE, "assemble: the operator is nonlinear (take its Jacobian)");
         fem_csr *h = new fem_csr();
         try {
             h->m.assemble(&op->o);
@@ -789,6 +847,124 @@ int fem_solver_solve(fem_solver *s, const fem_vec *b, fem_vec *x, fem_solver_inf
         FEM_CHECK(s && b && x, FEM_ERR_ARG, "null argument");
         fem_solver_info tmp;
         s->s.solve(b->v, x->v, info ? *info : tmp);
+    });
+}
+
+// ---- Newton ----------------------------------------------------------------------
+void fem_newton_default_params(fem_newton_params *p)
+{
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->tolerance = 1e-6;                  // newtoninverseoperator.hh:203-207
+    p->maxiterations = INT_MAX;           // :231-235
+    p->maxlineariterations = -1;          // the linear solver's maxiterations (:246-250)
+    p->linesearch = FEM_LINESEARCH_NONE;  // :277-288
+    p->maxlinesearchiterations = INT_MAX; // :259-264
+    p->forcing = FEM_FORCING_NONE;        // :298-315
+    p->eisenstatwalker_etamax = 0.99;     // :326-331
+    p->eisenstatwalker_gamma = 0.1;       // :343-348
+    p->simplified = 0;                    // :359-362
+    p->verbose = 0;
+}
+
+int fem_newton_params_set(fem_newton_params *p, const char *key, const char *value)
+{
+    return guarded([&] {
+        FEM_CHECK(p && key && value, FEM_ERR_ARG, "null argument");
+        std::string k(key), v(value);
+        auto strip = [&](const char *pre) {
+            const size_t n = std::strlen(pre);
+            if (k.compare(0, n, pre) == 0) { k = k.substr(n); return true; }
+            return false;
+        };
+        strip("fem.solver.");
+        FEM_CHECK(strip("nonlinear.") || strip("newton."), FEM_ERR_ARG,
+                  std::string("not a Newton parameter (nonlinear.*): ") + key);
+        FEM_CHECK(k.compare(0, 7, "linear.") != 0 || k == "linear.tolerance.strategy", FEM_ERR_ARG,
+                  std::string("a linear-solver parameter, set with fem_solver_params_set: ") + key);
+        if (k == "tolerance") {
+            p->tolerance = parse_double(k, v);
+            FEM_CHECK(p->tolerance > 0, FEM_ERR_ARG, "tolerance must be > 0");
+        } else if (k == "maxiterations") {
+            p->maxiterations = parse_int(k, v);
+            FEM_CHECK(p->maxiterations >= 0, FEM_ERR_ARG, "maxiterations must be >= 0");
+        } else if (k == "maxlineariterations") {
+            p->maxlineariterations = parse_int(k, v);
+            FEM_CHECK(p->maxlineariterations >= 0, FEM_ERR_ARG, "maxlineariterations must be >= 0");
+        } else if (k == "linesearch" || k == "lineSearch") {
+            if (v == "none") p->linesearch = FEM_LINESEARCH_NONE;
+            else if (v == "simple") p->linesearch = FEM_LINESEARCH_SIMPLE;
+            else throw Error(FEM_ERR_ARG, "linesearch: none or simple, got " + v);
+        } else if (k == "maxlinesearchiterations") {
+            p->maxlinesearchiterations = parse_int(k, v);
+            FEM_CHECK(p->maxlinesearchiterations >= 0, FEM_ERR_ARG, "maxlinesearchiterations must be >= 0");
+        } else if (k == "forcing" || k == "linear.tolerance.strategy") {
+            if (v == "none") p->forcing = FEM_FORCING_NONE;
+            else if (v == "eisenstatwalker") p->forcing = FEM_FORCING_EISENSTATWALKER;
+            else throw Error(FEM_ERR_ARG, "forcing: none or eisenstatwalker, got " + v);
+        } else if (k == "eisenstatwalker.etamax") {
+            p->eisenstatwalker_etamax = parse_double(k, v);
+        } else if (k == "eisenstatwalker.gamma") {
+            p->eisenstatwalker_gamma = parse_double(k, v);
+        } else if (k == "simplified") {
+            p->simplified = parse_bool(k, v);
+        } else if (k == "verbose") {
+            p->verbose = parse_bool(k, v);
+        } else {
+            throw Error(FEM_ERR_ARG, std::string("unknown Newton parameter: ") + key);
+        }
+    });
+}
+
+int fem_newton_create(fem_ctx *ctx, const fem_newton_params *params, const fem_solver_params *linear,
+                      fem_newton **out)
+{
+    return guarded([&] {
+        FEM_CHECK(ctx && params && linear && out, FEM_ERR_ARG, "null argument");
+        fem_newton *h = new fem_newton();
+        try {
+            h->n.init(&ctx->c, *params, *linear);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        h->held = ctx_retain(&ctx->c);
+        *out = h;
+    });
+}
+
+int fem_newton_destroy(fem_newton *n)
+{
+    return guarded([&] {
+        if (!n) return;
+        Context *c = n->held;
+        delete n;
+        ctx_release(c);
+    });
+}
+
+int fem_newton_bind(fem_newton *n, fem_op *op, fem_precond *pre)
+{
+    return guarded([&] {
+        FEM_CHECK(n && op, FEM_ERR_ARG, "null argument");
+        n->n.bind(&op->o, pre ? &pre->p : nullptr);
+    });
+}
+
+int fem_newton_unbind(fem_newton *n)
+{
+    return guarded([&] {
+        FEM_CHECK(n, FEM_ERR_ARG, "null argument");
+        n->n.unbind();
+    });
+}
+
+int fem_newton_solve(fem_newton *n, const fem_vec *u, fem_vec *w, fem_newton_info *info)
+{
+    return guarded([&] {
+        FEM_CHECK(n && w, FEM_ERR_ARG, "null argument");
+        fem_newton_info tmp;
+        n->n.solve(u ? &u->v : nullptr, w->v, info ? *info : tmp);
     });
 }
 

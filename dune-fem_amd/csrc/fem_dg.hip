@@ -349,6 +349,10 @@ __global__ void __launch_bounds__(ApplyShape<K>::WG)
     // values: loaded once as x-pencils, the y- and z-pencils come from LDS
     // (K = 4 only: the other orders keep the face-trace scratch there)
     constexpr bool KLDS = (K == 4) && KQ;
+    // the pointwise/nonlinear reaction is compiled into the plain applies
+    // with constant or cellwise kappa only (not the BF16-MFMA, pointwise
+    // kappa or fused-smoother variants)
+    constexpr bool CV = !KQ && !MF && !SM;
     constexpr int SZ = F64MF_ ? (KLDS ? 244 : 164) : D35 ? (K == 3 ? 107 : 361) : S::SZ;
     static_assert(!F64MF_ || SZ >= (KLDS ? 3 : 2) * K * S1, "tensors per element");
     static_assert(F64MF_ || SZ >= 2 * K * S1 + 4 * SXQ, "tensors and face-trace scratch per element");
@@ -721,6 +725,24 @@ __global__ void __launch_bounds__(ApplyShape<K>::WG)
 #pragma unroll
             for (int q = 0; q < K; ++q) Rx[q] = fma(tb.w[q] * cm, uq[q], Rx[q]);
         }
+        if constexpr (CV) {
+            // pointwise and nonlinear reaction at this lane's K volume points
+            // (qx = q, qy = i, qz = j): (cvol + nl_coef u^nl_pow) u
+            if (a.cvol || a.nl_pow) {
+                const T cm = wij * detJ;
+                const T *cv = a.cvol ? a.cvol + ((size_t)e * K3 + K * i + K2 * j) : nullptr;
+#pragma unroll
+                for (int q = 0; q < K; ++q) {
+                    T r = cv ? cv[q] : T(0);
+                    if (a.nl_pow) {
+                        T pw = a.nl_coef;
+                        for (int t = 0; t < a.nl_pow; ++t) pw *= uq[q];
+                        r += pw;
+                    }
+                    Rx[q] = fma(tb.w[q] * cm * r, uq[q], Rx[q]);
+                }
+            }
+        }
         face_terms(0, uq, Rx);
     }
     lds_sync<WG>();
@@ -1007,6 +1029,80 @@ void dg_rhs(const DGArgs<double> &a, int k, const double *fq, double *F, hipStre
 }
 
 // ---------------------------------------------------------------------------
+// a DG function at the volume quadrature points, out[e][qx + K qy + K^2 qz]:
+// out = base + coef u^pw (base may be null; pw = 1, coef = 1 gives u itself)
+// ---------------------------------------------------------------------------
+template <int K, typename T>
+__global__ void __launch_bounds__(DGShape<K>::THREADS)
+    k_dg_eval_qp(const DGArgs<T> a, const T *__restrict__ u, const T *__restrict__ base, T coef, int pw,
+                 T *__restrict__ out)
+{
+    using S = DGShape<K>;
+    constexpr int K2 = S::K2, K3 = S::K3, EPB = S::EPB;
+    __shared__ T smem[EPB * K3];
+    const Tab1D<T> &tb = *a.tab;
+    const int tid = threadIdx.x, el = tid / K2, t = tid % K2, i = t / K, j = t % K;
+    const int64_t E = (int64_t)a.n[0] * a.n[1] * a.n[2];
+    const int64_t e_raw = (int64_t)blockIdx.x * EPB + el;
+    const bool active = e_raw < E;
+    const int64_t e = active ? e_raw : 0;
+    T *A = smem + el * K3;
+    T v[K], tmp[K];
+    // z modes -> qz: lane (i = a, j = b)
+#pragma unroll
+    for (int c = 0; c < K; ++c) v[c] = u[e * K3 + i * K2 + j * K + c];
+    mvB<K>(tb.B, v, tmp);
+#pragma unroll
+    for (int q = 0; q < K; ++q) A[i * K2 + j * K + q] = tmp[q];
+    __syncthreads();
+    // y modes -> qy: lane (i = a, j = qz)
+#pragma unroll
+    for (int b = 0; b < K; ++b) v[b] = A[i * K2 + b * K + j];
+    mvB<K>(tb.B, v, tmp);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < K; ++q) A[i * K2 + q * K + j] = tmp[q];
+    __syncthreads();
+    // x modes -> qx: lane (i = qy, j = qz)
+#pragma unroll
+    for (int b = 0; b < K; ++b) v[b] = A[b * K2 + i * K + j];
+    mvB<K>(tb.B, v, tmp);
+    if (active)
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            const int64_t o = e * K3 + q + K * i + K2 * j;
+            T r = coef;
+            for (int p = 0; p < pw; ++p) r *= tmp[q];
+            out[o] = base ? base[o] + r : r;
+        }
+}
+
+template <typename T>
+void dg_eval_qp(const DGArgs<T> &a, int k, const T *u, const T *base, T coef, int pw, T *out, hipStream_t s)
+{
+    const int64_t E = (int64_t)a.n[0] * a.n[1] * a.n[2];
+    FEM_CHECK(pw >= 0, FEM_ERR_ARG, "quadrature-point evaluation: power >= 0");
+#define EQ_CASE(KK)                                                                                        \
+    case KK: {                                                                                             \
+        unsigned nb = (unsigned)((E + DGShape<KK>::EPB - 1) / DGShape<KK>::EPB);                           \
+        hipLaunchKernelGGL((k_dg_eval_qp<KK, T>), dim3(nb), dim3(DGShape<KK>::THREADS), 0, s, a, u, base, coef, \
+                           pw, out);                                                                       \
+        break;                                                                                             \
+    }
+    switch (k) {
+        EQ_CASE(2) EQ_CASE(3) EQ_CASE(4) EQ_CASE(5)
+    default: throw Error(FEM_ERR_UNSUPPORTED, "DG: order must be 1..4");
+    }
+#undef EQ_CASE
+    FEM_HIP(hipGetLastError());
+}
+
+template void dg_eval_qp<double>(const DGArgs<double> &, int, const double *, const double *, double, int, double *,
+                                 hipStream_t);
+template void dg_eval_qp<float>(const DGArgs<float> &, int, const float *, const float *, float, int, float *,
+                                hipStream_t);
+
+// ---------------------------------------------------------------------------
 // boundary data of the load vector: F_i += sum over the element's boundary
 // faces of int (kappa beta/h + max(-b.n, 0)) g phi_i - kappa g dphi_i/dn,
 // the g-dependent part of the Nitsche and inflow boundary terms
@@ -1105,7 +1201,8 @@ __global__ void k_dg_diag(const DGArgs<double> a, int K, double *__restrict__ do
                 const double W = tb.w[qx] * tb.w[qy] * tb.w[qz] * detJ;
                 const double kap = kappa_vol(a, e, qx + K * qy + K2 * qz, K3);
                 acc += W * (kap * (gr[0] * gr[0] + gr[1] * gr[1] + gr[2] * gr[2]) -
-                            (a.b[0] * gr[0] + a.b[1] * gr[1] + a.b[2] * gr[2]) * phi + a.c * phi * phi);
+                            (a.b[0] * gr[0] + a.b[1] * gr[1] + a.b[2] * gr[2]) * phi +
+                            (a.c + (a.cvol ? a.cvol[e * K3 + qx + K * qy + K2 * qz] : 0.0)) * phi * phi);
             }
     for (int d = 0; d < 3; ++d) {
         const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
@@ -1234,8 +1331,21 @@ __global__ void k_dg_block_setup(const DGArgs<double> a, int K, double *__restri
                                  symw * kin * nrm / a.h[d] * (tb.dP[s][c] * tb.P[s][r] + tb.P[s][c] * tb.dP[s][r]));
     }
     sym_eig(K, M, Q);
+    // the reaction c + cvol (orthonormal basis: mass block |K| I) shifts
+    // every eigenvalue sum; taken into the x direction, with the element's
+    // weighted mean of cvol (exact for an element-constant reaction)
+    double shift = 0;
+    if (d == 0) {
+        double cm = a.c;
+        if (a.cvol)
+            for (int qz = 0; qz < K; ++qz)
+                for (int qy = 0; qy < K; ++qy)
+                    for (int qx = 0; qx < K; ++qx)
+                        cm += tb.w[qx] * tb.w[qy] * tb.w[qz] * a.cvol[e * K * K * K + qx + K * (qy + K * qz)];
+        shift = cm * a.h[0] * a.h[1] * a.h[2];
+    }
     for (int r = 0; r < K; ++r) {
-        lam[(e * 3 + d) * K + r] = M[r][r];
+        lam[(e * 3 + d) * K + r] = M[r][r] + shift;
         for (int c = 0; c < K; ++c) Qout[((e * 3 + d) * K + r) * K + c] = Q[r][c];
     }
 }

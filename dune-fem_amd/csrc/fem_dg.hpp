@@ -39,6 +39,14 @@ struct DGArgs {
     const T *kvol;
     const T *kface[3];
     const T *kcell;
+    // reaction at the volume quadrature points added to c (DG operators with
+    // constant or cellwise kappa): a pointwise coefficient cvol [E][k^3]
+    // (x fastest, may be null) and the nonlinear law nl_coef u^nl_pow
+    // evaluated from the input at each point (nl_pow = 0: none); the volume
+    // term is int (c + cvol + nl_coef u^nl_pow) u v
+    const T *cvol;
+    T nl_coef;
+    int nl_pow;
     const Tab1D<T> *tab;   // device copy of the 1D tables
     // per direction d, set on the host: beta / h_d, the face area of the
     // faces normal to d, |K| / h_d^2, b_d h_d, max(b_d, 0), max(-b_d, 0)
@@ -115,6 +123,10 @@ int64_t dg_blocks(int k, const int *n);
 int64_t dg_bj_blocks(int k, int64_t E);
 // operator diagonal
 void dg_diagonal(const DGArgs<double> &a, int k, double *d, hipStream_t s);
+// a DG function at the volume quadrature points [E][k^3] (x fastest):
+// out = base + coef u^pw (base may be null)
+template <typename T>
+void dg_eval_qp(const DGArgs<T> &a, int k, const T *u, const T *base, T coef, int pw, T *out, hipStream_t s);
 // load vector from f at volume quadrature points
 void dg_rhs(const DGArgs<double> &a, int k, const double *fq, double *F, hipStream_t s);
 // F += boundary-data part of the load vector (u = g on the boundary, g at

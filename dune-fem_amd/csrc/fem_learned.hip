@@ -499,6 +499,7 @@ void Learned::init(Operator *o, CNN *c, int steps, int smoother_kind)
     FEM_CHECK(steps >= 0 && steps <= 1000, FEM_ERR_ARG, "learned preconditioner: coarse_steps in [0, 1000]");
     coarse_steps = steps;
     FEM_CHECK(o->is_dg(), FEM_ERR_UNSUPPORTED, "learned preconditioner needs a DG operator");
+    FEM_CHECK(!o->has_reaction(), FEM_ERR_UNSUPPORTED, "learned preconditioner: no pointwise or nonlinear reaction");
     FEM_CHECK(c->V == o->E, FEM_ERR_ARG, "learned preconditioner: network grid must match the element grid");
     FEM_CHECK((int64_t)c->n[0] == o->desc.n[0] && c->n[1] == o->desc.n[1] && c->n[2] == o->desc.n[2], FEM_ERR_ARG,
               "learned preconditioner: network grid must match the element grid");

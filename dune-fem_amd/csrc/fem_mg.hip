@@ -420,6 +420,7 @@ void Multigrid::init(Operator *fine)
     const fem_op_desc &d0 = fine->desc;
     FEM_CHECK(d0.b[0] == 0 && d0.b[1] == 0 && d0.b[2] == 0 && d0.c == 0, FEM_ERR_UNSUPPORTED,
               "multigrid: diffusion operators only (b = 0, c = 0)");
+    FEM_CHECK(!fine->has_reaction(), FEM_ERR_UNSUPPORTED, "multigrid: no pointwise or nonlinear reaction");
     FEM_CHECK(d0.kappa_mode == FEM_KAPPA_CELL || d0.kappa_mode == FEM_KAPPA_CONST, FEM_ERR_UNSUPPORTED,
               "multigrid: cellwise or constant kappa");
     FEM_CHECK(fine->a64.kappa_mode >= 0, FEM_ERR_STATE, "coefficient data not set");

@@ -159,6 +159,134 @@ void Operator::set_kappa_cell(const double *kc)
     a64.kappa_mode = a32.kappa_mode = FEM_KAPPA_CELL;
 }
 
+// ---- reaction beyond the constant c: pointwise and nonlinear ---------------------
+static void copy_buf(DevBuf &dst, const DevBuf &src, hipStream_t s)
+{
+    dst.free();
+    if (!src.p) return;
+    dst.alloc(src.bytes);
+    FEM_HIP(hipMemcpyAsync(dst.p, src.p, src.bytes, hipMemcpyDeviceToDevice, s));
+}
+
+// a raw device array as a (non-owning) vector
+static Vec raw_vec(Context *c, void *p, int64_t n, int prec)
+{
+    Vec v;
+    v.ctx = c; v.n = n; v.prec = prec; v.data = p; v.owner = false;
+    return v;
+}
+
+static void reaction_checks(const Operator &o)
+{
+    FEM_CHECK(o.is_dg(), FEM_ERR_UNSUPPORTED, "pointwise/nonlinear reaction: DG operators only");
+    FEM_CHECK(o.desc.kappa_mode != FEM_KAPPA_QP, FEM_ERR_UNSUPPORTED,
+              "pointwise/nonlinear reaction: constant or cellwise kappa");
+    FEM_CHECK(o.desc.precision != FEM_BF16MFMA, FEM_ERR_UNSUPPORTED,
+              "pointwise/nonlinear reaction: FP64 or FP32 operators");
+}
+
+void Operator::set_reaction_qp(const double *cvol)
+{
+    reaction_checks(*this);
+    if (!cvol) {
+        cvol64.free();
+        cvol32.free();
+    } else {
+        const size_t nq = (size_t)E * k * k * k;
+        upload_as<double>(cvol64, cvol, nq, ctx->stream);
+        upload_as<float>(cvol32, cvol, nq, ctx->stream);
+    }
+    a64.cvol = cvol64.as<double>();
+    a32.cvol = cvol32.as<float>();
+}
+
+void Operator::set_reaction_law(double coef, int pw)
+{
+    reaction_checks(*this);
+    FEM_CHECK(coef == 0 || (pw >= 1 && pw <= 4), FEM_ERR_ARG, "reaction law: power in 1..4");
+    nl_coef = coef;
+    nl_pow = coef == 0 ? 0 : pw;
+    a64.nl_coef = nl_coef;
+    a32.nl_coef = (float)nl_coef;
+    a64.nl_pow = a32.nl_pow = nl_pow;
+}
+
+void Operator::eval_qp(const Vec &u, Vec &out) const
+{
+    FEM_CHECK(is_dg(), FEM_ERR_UNSUPPORTED, "quadrature-point evaluation: DG operators only");
+    FEM_CHECK(u.n == N && out.n == E * k * k * k, FEM_ERR_ARG, "quadrature-point evaluation: vector sizes");
+    FEM_CHECK(u.prec == vec_prec && out.prec == vec_prec, FEM_ERR_ARG,
+              "quadrature-point evaluation: vectors of the operator's precision");
+    if (vec_prec == FEM_F64)
+        dg_eval_qp<double>(a64, k, (const double *)u.data, nullptr, 1.0, 1, (double *)out.data, ctx->stream);
+    else
+        dg_eval_qp<float>(a32, k, (const float *)u.data, nullptr, 1.0f, 1, (float *)out.data, ctx->stream);
+}
+
+void Operator::make_jacobian(Operator &jop) const
+{
+    reaction_checks(*this);
+    hipStream_t s = ctx->stream;
+    jop.init(ctx, desc);
+    for (int d = 0; d < 3; ++d) {
+        copy_buf(jop.kface64[d], kface64[d], s);
+        copy_buf(jop.kface32[d], kface32[d], s);
+        copy_buf(jop.gbnd64[d], gbnd64[d], s);
+    }
+    copy_buf(jop.kvol64, kvol64, s);
+    copy_buf(jop.kvol32, kvol32, s);
+    copy_buf(jop.kcell64, kcell64, s);
+    copy_buf(jop.kcell32, kcell32, s);
+    copy_buf(jop.cvol64, cvol64, s);
+    copy_buf(jop.cvol32, cvol32, s);
+    jop.has_bnd = has_bnd;
+    jop.a64.kappa_mode = a64.kappa_mode;
+    jop.a32.kappa_mode = a32.kappa_mode;
+    jop.a64.kcell = jop.kcell64.as<double>();
+    jop.a32.kcell = jop.kcell32.as<float>();
+    jop.a64.cvol = jop.cvol64.as<double>();
+    jop.a32.cvol = jop.cvol32.as<float>();
+    FEM_HIP(hipStreamSynchronize(s));
+}
+
+void Operator::jacobian(const Vec &u, Operator &jop) const
+{
+    reaction_checks(*this);
+    FEM_CHECK(jop.is_dg() && jop.desc.precision == desc.precision && jop.k == k && jop.E == E &&
+                  jop.desc.n[0] == desc.n[0] && jop.desc.n[1] == desc.n[1] && jop.desc.n[2] == desc.n[2] &&
+                  jop.desc.kappa_mode == desc.kappa_mode && jop.a64.kappa_mode == a64.kappa_mode,
+              FEM_ERR_ARG, "jacobian: the Jacobian operator must be made from this operator");
+    FEM_CHECK(&jop != this, FEM_ERR_ARG, "jacobian: an operator other than this one");
+    FEM_CHECK(u.n == N && u.prec == vec_prec, FEM_ERR_ARG, "jacobian: vector size or precision");
+    hipStream_t s = ctx->stream;
+    const int64_t nq = E * k * k * k;
+    jop.set_reaction_law(0.0, 0);
+    if (!nonlinear()) {
+        // linear: the Jacobian is the operator (its cvol, if any)
+        copy_buf(jop.cvol64, cvol64, s);
+        copy_buf(jop.cvol32, cvol32, s);
+    } else {
+        // d/du (nl_coef u^pw u) = (pw + 1) nl_coef u^pw
+        if (!jop.cvol64.p || jop.cvol64.bytes != (size_t)nq * 8) jop.cvol64.alloc((size_t)nq * 8);
+        if (!jop.cvol32.p || jop.cvol32.bytes != (size_t)nq * 4) jop.cvol32.alloc((size_t)nq * 4);
+        const double cj = (nl_pow + 1) * nl_coef;
+        if (vec_prec == FEM_F64) {
+            dg_eval_qp<double>(a64, k, (const double *)u.data, cvol64.as<double>(), cj, nl_pow, jop.cvol64.as<double>(),
+                               s);
+            Vec lo = raw_vec(ctx, jop.cvol32.p, nq, FEM_F32), hi = raw_vec(ctx, jop.cvol64.p, nq, FEM_F64);
+            vec_copy(lo, hi);
+        } else {
+            dg_eval_qp<float>(a32, k, (const float *)u.data, cvol32.as<float>(), (float)cj, nl_pow,
+                              jop.cvol32.as<float>(), s);
+            Vec lo = raw_vec(ctx, jop.cvol32.p, nq, FEM_F32), hi = raw_vec(ctx, jop.cvol64.p, nq, FEM_F64);
+            vec_copy(hi, lo);
+        }
+    }
+    jop.a64.cvol = jop.cvol64.as<double>();
+    jop.a32.cvol = jop.cvol32.as<float>();
+    FEM_HIP(hipStreamSynchronize(s));
+}
+
 // Dirichlet nodes of the (2 nx + 1) x (2 ny + 1) Q2 node lattice
 __device__ __forceinline__ bool q2_boundary(int64_t i, int nx, int ny)
 {
@@ -289,6 +417,7 @@ void Operator::rhs(const double *fq_host, Vec &F) const
 void Operator::diagonal(Vec &d) const
 {
     FEM_CHECK(d.n == N, FEM_ERR_ARG, "diagonal: vector size");
+    FEM_CHECK(!nonlinear(), FEM_ERR_STATE, "diagonal: the operator is nonlinear (take its Jacobian)");
     hipStream_t s = ctx->stream;
     DevBuf out;
     double *dst;
@@ -348,6 +477,7 @@ void Operator::block_diagonal(Vec &blocks) const
 {
     FEM_CHECK(is_dg(), FEM_ERR_UNSUPPORTED, "block diagonal: DG operators only");
     FEM_CHECK(a64.kappa_mode >= 0, FEM_ERR_STATE, "coefficient data not set");
+    FEM_CHECK(!nonlinear(), FEM_ERR_STATE, "block diagonal: the operator is nonlinear (take its Jacobian)");
     const int nb = k * k * k;
     FEM_CHECK(blocks.prec == FEM_F64 && blocks.n == E * nb * nb, FEM_ERR_ARG,
               "block diagonal: an FP64 vector of E nb^2 entries");
@@ -394,8 +524,9 @@ void Precond::init(Operator *o, int k_)
         FEM_HIP(hipStreamSynchronize(s));
     } else if (kind == FEM_PRE_BLOCK_JACOBI) {
         FEM_CHECK(o->is_dg(), FEM_ERR_UNSUPPORTED, "block Jacobi needs a DG operator");
-        FEM_CHECK(o->desc.b[0] == 0 && o->desc.b[1] == 0 && o->desc.b[2] == 0 && o->desc.c == 0, FEM_ERR_UNSUPPORTED,
-                  "block Jacobi: diffusion operators only (b = 0, c = 0)");
+        FEM_CHECK(o->desc.b[0] == 0 && o->desc.b[1] == 0 && o->desc.b[2] == 0, FEM_ERR_UNSUPPORTED,
+                  "block Jacobi: operators without advection (b = 0)");
+        FEM_CHECK(!o->nonlinear(), FEM_ERR_STATE, "block Jacobi: the operator is nonlinear (take its Jacobian)");
         FEM_CHECK(o->a64.kappa_mode >= 0, FEM_ERR_STATE, "coefficient data not set");
         const int k = o->k;
         DevBuf Q64, l64;

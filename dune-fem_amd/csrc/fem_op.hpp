@@ -59,6 +59,28 @@ struct Operator {
     void constrain(Vec &x, const Vec *add = nullptr) const;
     DevBuf gbnd64[3];   // boundary data as set (DG: per direction; CG: gbnd64[0])
     bool has_bnd = false;
+    // DG reaction beyond the constant c: a pointwise coefficient at the
+    // volume quadrature points (cvol, FP64 and FP32 copies) and the
+    // nonlinear law nl_coef u^nl_pow (the volume term int (c + cvol +
+    // nl_coef u^nl_pow) u v); constant or cellwise kappa only
+    DevBuf cvol64, cvol32;
+    double nl_coef = 0;
+    int nl_pow = 0;
+    bool nonlinear() const { return nl_pow > 0; }
+    bool has_reaction() const { return cvol64.p != nullptr || nonlinear(); }
+    // cvol from host values [E][k^3] (null clears)
+    void set_reaction_qp(const double *cvol);
+    // the law nl_coef u^pw, pw in 1..4 (coef = 0 clears)
+    void set_reaction_law(double coef, int pw);
+    // u (operator precision) at the volume quadrature points, into `out`
+    // of the same precision [E][k^3]
+    void eval_qp(const Vec &u, Vec &out) const;
+    // the Jacobian at u into jop (an operator made by make_jacobian from
+    // this one): cvol_J = cvol + (pw + 1) nl_coef u^pw, no law
+    void jacobian(const Vec &u, Operator &jop) const;
+    // jop: a linear operator of the same description and coefficient data
+    // (kappa, boundary data, cvol), device copies
+    void make_jacobian(Operator &jop) const;
     // y = A x on raw device pointers of the operator's vector precision;
     // `dot` (DG only, may be null): per-workgroup FP64 partials of <x, y>,
     // dot_blocks() entries

@@ -596,6 +596,8 @@ void Solver::bind(Operator *o, Precond *p, Operator *io, Precond *ip)
     op = iop = nullptr;
     pre = ipre = nullptr;
     FEM_CHECK(o, FEM_ERR_ARG, "bind: null operator");
+    FEM_CHECK(!o->nonlinear() && (!io || !io->nonlinear()), FEM_ERR_STATE,
+              "bind: the operator is nonlinear (bind its Jacobian; the Newton solve does)");
     FEM_CHECK(!p || p->op == o || p->op->N == o->N, FEM_ERR_ARG, "preconditioner does not match the operator");
     // an FP32 preconditioner may serve an FP64 solve (rounded in, widened
     // out); otherwise the precisions agree

@@ -32,6 +32,12 @@ LEARNED_OFF, LEARNED_NET, LEARNED_COARSE_ONLY = 0, 1, 2
 CG, GMRES, FGMRES, BICGSTAB = 0, 1, 2, 3
 ABSOLUTE, RELATIVE, REDUCTION = 0, 1, 2
 SPACE_LAGRANGE, SPACE_LEGENDRE_DG = 6, 8
+LINESEARCH_NONE, LINESEARCH_SIMPLE = 0, 1
+FORCING_NONE, FORCING_EISENSTATWALKER = 0, 1
+# NewtonFailure (dune/fem/solver/newtoninverseoperator.hh:389-400)
+NEWTON_FAILURE = {0: "Success", 1: "InvalidResidual", 2: "IterationsExceeded", 3: "LinearIterationsExceeded",
+                  4: "LineSearchFailed", 5: "TooManyIterations", 6: "TooManyLinearIterations",
+                  7: "LinearSolverFailed"}
 
 # every symbol declared in include/dunefem_amd.h
 EXPORTS = [
@@ -53,6 +59,10 @@ EXPORTS = [
     "fem_cnn_coarse_solve", "fem_op_assemble_csr", "fem_csr_destroy", "fem_csr_rows", "fem_csr_nnz",
     "fem_csr_apply", "fem_csr_diagonal", "fem_csr_download",
     "fem_dofs_encode", "fem_dofs_decode", "fem_vec_write", "fem_vec_read",
+    "fem_op_set_reaction_qp", "fem_op_set_reaction_law", "fem_op_nonlinear", "fem_op_eval_qp",
+    "fem_op_jacobian_create", "fem_op_jacobian",
+    "fem_newton_default_params", "fem_newton_params_set", "fem_newton_create", "fem_newton_destroy",
+    "fem_newton_bind", "fem_newton_unbind", "fem_newton_solve",
 ]
 
 
@@ -85,6 +95,18 @@ class BatchInfo(C.Structure):
 class SolverInfo(C.Structure):
     _fields_ = [("converged", C.c_int), ("iterations", C.c_int), ("outer_iterations", C.c_int),
                 ("true_rel_residual", C.c_double), ("time_s", C.c_double), ("fallback", C.c_int)]
+
+
+class NewtonParams(C.Structure):
+    _fields_ = [("tolerance", C.c_double), ("maxiterations", C.c_int), ("maxlineariterations", C.c_int),
+                ("linesearch", C.c_int), ("maxlinesearchiterations", C.c_int), ("forcing", C.c_int),
+                ("eisenstatwalker_etamax", C.c_double), ("eisenstatwalker_gamma", C.c_double),
+                ("simplified", C.c_int), ("verbose", C.c_int)]
+
+
+class NewtonInfo(C.Structure):
+    _fields_ = [("converged", C.c_int), ("failure", C.c_int), ("iterations", C.c_int),
+                ("linear_iterations", C.c_int), ("residual", C.c_double), ("timing", C.c_double * 3)]
 
 
 _lib = None
@@ -186,6 +208,19 @@ def load_library(path=LIBPATH):
         "fem_precond_create_learned": ([_vp, _vp, C.c_int, C.POINTER(_vp)], C.c_int),
         "fem_precond_create_learned_ex": ([_vp, _vp, C.c_int, C.c_int, C.POINTER(_vp)], C.c_int),
         "fem_precond_set_learned_mode": ([_vp, C.c_int], C.c_int),
+        "fem_op_set_reaction_qp": ([_vp, _dp], C.c_int),
+        "fem_op_set_reaction_law": ([_vp, C.c_double, C.c_int], C.c_int),
+        "fem_op_nonlinear": ([_vp], C.c_int),
+        "fem_op_eval_qp": ([_vp, _vp, _vp], C.c_int),
+        "fem_op_jacobian_create": ([_vp, C.POINTER(_vp)], C.c_int),
+        "fem_op_jacobian": ([_vp, _vp, _vp], C.c_int),
+        "fem_newton_default_params": ([C.POINTER(NewtonParams)], None),
+        "fem_newton_params_set": ([C.POINTER(NewtonParams), C.c_char_p, C.c_char_p], C.c_int),
+        "fem_newton_create": ([_vp, C.POINTER(NewtonParams), C.POINTER(SolverParams), C.POINTER(_vp)], C.c_int),
+        "fem_newton_destroy": ([_vp], C.c_int),
+        "fem_newton_bind": ([_vp, _vp, _vp], C.c_int),
+        "fem_newton_unbind": ([_vp], C.c_int),
+        "fem_newton_solve": ([_vp, _vp, _vp, C.POINTER(NewtonInfo)], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -426,8 +461,45 @@ class GalerkinOperator:
         _check(_lib.fem_op_block_diagonal(self.h, B.h))
         return B.download().reshape(E, nb, nb)
 
+    def set_reaction_qp(self, cvol):
+        """pointwise reaction coefficient at the volume quadrature points
+        [E][m^3] (x fastest), added to c; None clears (DG operators)"""
+        if cvol is None:
+            _check(_lib.fem_op_set_reaction_qp(self.h, None))
+        else:
+            cvol = np.ascontiguousarray(cvol, dtype=np.float64)
+            assert cvol.size == self.N, "one value per volume quadrature point"
+            _check(_lib.fem_op_set_reaction_qp(self.h, _dptr(cvol)))
+
+    def set_reaction_law(self, coef, pw):
+        """the nonlinear reaction coef u^pw u (pw in 1..4; coef = 0 clears):
+        the operator becomes S[u] = A u + int coef u^(pw+1) v"""
+        _check(_lib.fem_op_set_reaction_law(self.h, float(coef), int(pw)))
+
     def nonlinear(self):
-        return False
+        """Operator::nonlinear() (dune/fem/operator/common/operator.hh:64)"""
+        r = _lib.fem_op_nonlinear(self.h)
+        if r < 0:
+            _check(r)
+        return bool(r)
+
+    def eval_qp(self, u, out=None):
+        """u at the volume quadrature points [E][m^3] (x fastest)"""
+        out = out if out is not None else DofVector(self.ctx, self.N, self.vec_precision)
+        _check(_lib.fem_op_eval_qp(self.h, u.h, out.h))
+        return out
+
+    def jacobian(self, u, jop=None):
+        """DifferentiableOperator::jacobian(u, jOp): the linear operator
+        S'(u), matrix-free (a GalerkinOperator; pass jop to refill one)"""
+        if jop is None:
+            h = _vp()
+            _check(_lib.fem_op_jacobian_create(self.h, C.byref(h)))
+            jop = GalerkinOperator.__new__(GalerkinOperator)
+            jop.ctx, jop.h, jop.desc = self.ctx, h, self.desc
+            jop.precision, jop.vec_precision, jop.N = self.precision, self.vec_precision, self.N
+        _check(_lib.fem_op_jacobian(self.h, u.h, jop.h))
+        return jop
 
     def __del__(self):
         try:
@@ -723,44 +795,131 @@ class LearnedPreconditioner(Preconditioner):
         _check(_lib.fem_precond_set_learned_mode(self.h, int(mode)))
 
 
-class Scheme:
-    """The scheme-level solve of the reference, FemScheme::solve/_solve
-    (dune/fem/schemes/femscheme.hh:195-253), for the affine operators of
-    this library: R(u) = A u - F, the boundary data inside F (and, for the
-    continuous Q2 operator, the Dirichlet rows u_D - g_D).
+def _is_newton_key(k):
+    """a fem.solver.nonlinear.* (or newton.*) key that is not a linear-solver key"""
+    k = k[len("fem.solver."):] if k.startswith("fem.solver.") else k
+    for pre in ("nonlinear.", "newton."):
+        if k.startswith(pre):
+            rest = k[len(pre):]
+            return not rest.startswith("linear.") or rest == "linear.tolerance.strategy"
+    return False
 
-    ``solve(target, rhs=None)``: setConstraints (u_D = g_D) and
-    addConstraints (u_D += rhs_D) on the target, then the Newton step of
-    NewtonInverseOperator::operator() (dune/fem/solver/newtoninverseoperator.hh:688-802;
-    one step, the operator is linear): residual = R(u) - rhs, the linear
-    solver on J du = residual from du = 0, u -= du.  Returns the info dict
-    of scheme.solve (dune/fempy/py/scheme.hh:156-187): converged,
-    iterations (Newton steps), linear_iterations, timing."""
 
-    def __init__(self, op, F, preconditioner=None, **solver_parameters):
-        self.op, self.F = op, F
-        self.pre = preconditioner
-        self.inv = InverseOperator(op.ctx, **solver_parameters)
+def newton_parameters(parameters=None, **kw):
+    """NewtonParameter (dune/fem/solver/newtoninverseoperator.hh:101-380) from
+    fem.solver.nonlinear.* keys or the fields of NewtonParams"""
+    p = NewtonParams()
+    load_library().fem_newton_default_params(C.byref(p))
+    fields = {f for f, _ in NewtonParams._fields_}
+    for k, v in dict(parameters or {}, **kw).items():
+        if k in fields:
+            setattr(p, k, type(getattr(p, k))(v))
+        else:
+            _check(_lib.fem_newton_params_set(C.byref(p), k.encode(), str(v).encode()))
+    return p
+
+
+class NewtonInverseOperator:
+    """NewtonInverseOperator (dune/fem/solver/newtoninverseoperator.hh:423-802):
+    ``bind(op[, pre])``, ``inv(u, w)`` solves S[w] = u (u None: zero) from the
+    initial guess w, ``iterations()``, ``linearIterations()``, ``info()``.
+    ``parameters`` holds reference keys: fem.solver.nonlinear.* for Newton,
+    the others (fem.solver.*, fem.solver.nonlinear.linear.*) for its linear
+    solver; ``linear`` (a dict of solver_parameters fields) adds to those."""
+
+    def __init__(self, ctx, parameters=None, linear=None):
+        self.ctx = ctx
+        parameters = dict(parameters or {})
+        nl = {k: v for k, v in parameters.items() if _is_newton_key(k)}
+        lin = {k: v for k, v in parameters.items() if not _is_newton_key(k)}
+        lin.update(linear or {})
+        self.params = newton_parameters(nl)
+        if self.params.forcing == FORCING_EISENSTATWALKER and not any(
+                k in ("errormeasure",) or k.endswith(".errormeasure") for k in lin):
+            # the reference's default linear error measure under Eisenstat-Walker
+            # forcing (newtoninverseoperator.hh:175-177)
+            lin["errormeasure"] = REDUCTION
+        self.linear_params = solver_parameters(**lin)
+        h = _vp()
+        _check(_lib.fem_newton_create(ctx.h, C.byref(self.params), C.byref(self.linear_params), C.byref(h)))
+        self.h = h
+        self._info = NewtonInfo()
+        self._bound = None
+
+    def bind(self, op, pre=None):
+        _check(_lib.fem_newton_bind(self.h, op.h, pre.h if pre is not None else None))
+        self._bound = (op, pre)
+
+    def unbind(self):
+        _check(_lib.fem_newton_unbind(self.h))
+        self._bound = None
 
     def __call__(self, u, w):
-        """w = R(u) = A u - F"""
+        _check(_lib.fem_newton_solve(self.h, u.h if u is not None else None, w.h, C.byref(self._info)))
+
+    def iterations(self):
+        return self._info.iterations
+
+    def linearIterations(self):
+        return self._info.linear_iterations
+
+    def info(self):
+        i = self._info
+        return {"converged": bool(i.converged), "iterations": i.iterations,
+                "linear_iterations": i.linear_iterations, "timing": list(i.timing),
+                "failure": NEWTON_FAILURE.get(i.failure, str(i.failure)), "residual": i.residual}
+
+    def __del__(self):
+        try:
+            if self.h:
+                _lib.fem_newton_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class Scheme:
+    """The scheme-level solve of the reference, FemScheme::solve/_solve
+    (dune/fem/schemes/femscheme.hh:195-253), for the operators of this
+    library: S[u] = A u + N(u) with the affine part F, R(u) = S[u] - F, the
+    boundary data inside F (and, for the continuous Q2 operator, the
+    Dirichlet rows u_D - g_D).
+
+    ``solve(target, rhs=None)``: setConstraints (u_D = g_D) and
+    addConstraints (u_D += rhs_D) on the target, then
+    NewtonInverseOperator::operator() (newtoninverseoperator.hh:688-802) on
+    S[u] = F + rhs: one step for a linear operator (no nonlinear iteration
+    counted, :786-790), Newton iterations with matrix-free Jacobians for a
+    nonlinear one.  Returns the info dict of scheme.solve
+    (dune/fempy/py/scheme.hh:156-187): converged, iterations (nonlinear),
+    linear_iterations, timing [total, jacobian, solve].
+
+    ``parameters``: reference keys (fem.solver.*, fem.solver.nonlinear.*);
+    keyword arguments: fields of solver_parameters for the linear solver."""
+
+    def __init__(self, op, F, preconditioner=None, parameters=None, **solver_parameters):
+        self.op, self.F = op, F
+        self.pre = preconditioner
+        self.inv = NewtonInverseOperator(op.ctx, parameters, linear=solver_parameters)
+
+    def __call__(self, u, w):
+        """w = R(u) = S[u] - F"""
         self.op(u, w)
         w.axpy(-1.0, self.F)
 
     def solve(self, target, rhs=None):
-        import time as _time
-        t0 = _time.perf_counter()
         op = self.op
         op.constrain(target, rhs)                 # u_D = g_D + rhs_D
-        res, du = op.vector(), op.vector()
-        self(target, res)                         # residual R(u)
+        u = self.F
         if rhs is not None:
-            res.axpy(-1.0, rhs)                   #          - rhs
-        self.inv.bind(op, self.pre) if self.pre is not None else self.inv.bind(op)
-        du.clear()
-        self.inv(res, du)
-        lin = self.inv.info()
-        target.axpy(-1.0, du)
-        self.inv.unbind()
-        return {"converged": bool(lin["converged"]), "iterations": 1 if lin["converged"] else 0,
-                "linear_iterations": lin["iterations"], "timing": [_time.perf_counter() - t0]}
+            u = op.vector()
+            u.assign(self.F)
+            u.axpy(1.0, rhs)                      # S[w] = F + rhs
+        self.inv.bind(op, self.pre)
+        try:
+            self.inv(u, target)
+        finally:
+            self.inv.unbind()
+        i = self.inv.info()
+        return {"converged": i["converged"], "iterations": i["iterations"],
+                "linear_iterations": i["linear_iterations"], "timing": i["timing"]}

@@ -192,6 +192,42 @@ int fem_op_constrain(fem_op *op, fem_vec *x, const fem_vec *add);
  * supported (computed by probing the apply). */
 int fem_op_block_diagonal(fem_op *op, fem_vec *blocks);
 
+/* ---- pointwise and nonlinear reaction (DG operators) -----------------------------
+ * Integrands whose source term is pointwise at the volume quadrature points,
+ * as the reference's GalerkinOperator evaluates every integrand
+ * (dune/fem/schemes/galerkin.hh:333-360, interior term of the integrands,
+ * dune/fem/schemes/integrands.hh:412-664): the volume term of the operator
+ * becomes int (c + cvol(x) + coef u^pw) u v, so that -div(kappa grad u) +
+ * coef u^(pw+1) = f is the nonlinear problem of a Newton solve.  DG operators
+ * with constant or cellwise kappa, FP64 or FP32 (FEM_ERR_UNSUPPORTED
+ * otherwise).
+ *
+ * fem_op_set_reaction_qp: cvol at the volume quadrature points [E][m^3]
+ *   (x fastest, host array, copied; NULL clears).
+ * fem_op_set_reaction_law: coef u^pw, pw in 1..4 (coef = 0 clears).  With a
+ *   law set the operator is nonlinear (the reference's Operator::nonlinear(),
+ *   dune/fem/operator/common/operator.hh:64): fem_op_apply evaluates S[u],
+ *   while the diagonal, the element blocks, the CSR assembly and binding to
+ *   a linear solver are refused (FEM_ERR_STATE): they belong to the Jacobian.
+ * fem_op_nonlinear: 1 if a law is set, else 0 (negative: error).
+ * fem_op_eval_qp: u at the volume quadrature points [E][m^3] (x fastest),
+ *   vectors of the operator's precision.
+ * fem_op_jacobian_create: a linear operator of the same description and
+ *   coefficient data (device copies), the Jacobian operator object
+ *   (NewtonInverseOperator's jOp, newtoninverseoperator.hh:704).
+ * fem_op_jacobian: DifferentiableOperator::jacobian(u, jOp)
+ *   (dune/fem/operator/common/differentiableoperator.hh:54): jop = S'(u),
+ *   the linear operator with cvol_J = cvol + (pw + 1) coef u^pw; a linear
+ *   operator's Jacobian is the operator itself.  Matrix-free: jop is applied
+ *   by the same kernel, and fem_op_assemble_csr(jop) gives the assembled
+ *   matrix when one is needed. */
+int fem_op_set_reaction_qp(fem_op *op, const double *cvol);
+int fem_op_set_reaction_law(fem_op *op, double coef, int pw);
+int fem_op_nonlinear(const fem_op *op);
+int fem_op_eval_qp(fem_op *op, const fem_vec *u, fem_vec *uq);
+int fem_op_jacobian_create(const fem_op *op, fem_op **jop);
+int fem_op_jacobian(const fem_op *op, const fem_vec *u, fem_op *jop);
+
 /* ---- assembled Jacobian -------------------------------------------------------
  * DifferentiableGalerkinOperator::jacobian / assemble into a SparseRowMatrix
  * (dune/fem/schemes/galerkin.hh:1195-1314, 1548-1643;
@@ -325,6 +361,79 @@ int fem_solver_bind(fem_solver *s, fem_op *op, fem_precond *pre, fem_op *inner_o
 int fem_solver_unbind(fem_solver *s);
 /* operator()(b, x): x is the initial guess on entry, the solution on exit */
 int fem_solver_solve(fem_solver *s, const fem_vec *b, fem_vec *x, fem_solver_info *info);
+
+/* ---- Newton ---------------------------------------------------------------------
+ * Replaces NewtonInverseOperator (dune/fem/solver/newtoninverseoperator.hh:
+ * 423-802) for the operators of this library: solve S[w] = u.  Each step
+ * binds S'(w) (fem_op_jacobian, matrix-free) to an owned linear solver with
+ * the preconditioner its fem.solver.preconditioning.method builds (rebuilt
+ * per step), or the one given at bind, solves S'(w) dw = S[w] - u from
+ * dw = 0 and sets w -= dw (:688-802).  A linear S takes one step and counts
+ * no nonlinear iteration (:786-790).  Stopping and failure follow
+ * NewtonInverseOperator::failed (:568-584) with the NewtonFailure codes
+ * (:389-400). */
+#define FEM_NEWTON_SUCCESS 0
+#define FEM_NEWTON_INVALID_RESIDUAL 1
+#define FEM_NEWTON_ITERATIONS_EXCEEDED 2
+#define FEM_NEWTON_LINEAR_ITERATIONS_EXCEEDED 3
+#define FEM_NEWTON_LINESEARCH_FAILED 4
+#define FEM_NEWTON_TOO_MANY_ITERATIONS 5
+#define FEM_NEWTON_TOO_MANY_LINEAR_ITERATIONS 6
+#define FEM_NEWTON_LINEAR_SOLVER_FAILED 7
+
+#define FEM_LINESEARCH_NONE 0
+#define FEM_LINESEARCH_SIMPLE 1
+#define FEM_FORCING_NONE 0
+#define FEM_FORCING_EISENSTATWALKER 1
+
+typedef struct fem_newton fem_newton;
+
+/* NewtonParameter (newtoninverseoperator.hh:101-380), keys under
+ * fem.solver.nonlinear. */
+typedef struct {
+    double tolerance;              /* nonlinear.tolerance, default 1e-6 (:203-207)      */
+    int maxiterations;             /* nonlinear.maxiterations, default INT_MAX (:231-235) */
+    int maxlineariterations;       /* total over all steps; < 0: the linear solver's
+                                      maxiterations (:246-250)                          */
+    int linesearch;                /* nonlinear.linesearch none | simple (:273-288)       */
+    int maxlinesearchiterations;   /* default INT_MAX (:259-264)                         */
+    int forcing;                   /* nonlinear.forcing none | eisenstatwalker (:294-315);
+                                      eisenstatwalker needs the linear errormeasure
+                                      residualreduction (:173-181)                      */
+    double eisenstatwalker_etamax; /* default 0.99 (:326-331)                            */
+    double eisenstatwalker_gamma;  /* default 0.1 (:343-348)                             */
+    int simplified;                /* nonlinear.simplified: the Jacobian of the first
+                                      step only (:359-362)                              */
+    int verbose;                   /* nonlinear.verbose                                  */
+} fem_newton_params;
+
+typedef struct {
+    int converged;
+    int failure;             /* FEM_NEWTON_*                                          */
+    int iterations;          /* nonlinear steps (0 for a linear operator)            */
+    int linear_iterations;   /* total; negative when a linear solve failed           */
+    double residual;         /* |S[w] - u|_2 when last evaluated (the initial one for
+                                a linear operator)                                   */
+    double timing[3];        /* seconds: total, Jacobian (and bind), solve (:798-801) */
+} fem_newton_info;
+
+void fem_newton_default_params(fem_newton_params *p);
+/* keys (prefix "fem.solver." optional, then "nonlinear." or, deprecated in
+ * the reference and accepted here, "newton."): tolerance, maxiterations,
+ * maxlineariterations, linesearch (or lineSearch), maxlinesearchiterations,
+ * forcing (or linear.tolerance.strategy), eisenstatwalker.etamax,
+ * eisenstatwalker.gamma, simplified, verbose.  Linear-solver keys
+ * (nonlinear.linear.*) go to fem_solver_params_set: FEM_ERR_ARG here. */
+int fem_newton_params_set(fem_newton_params *p, const char *key, const char *value);
+int fem_newton_create(fem_ctx *ctx, const fem_newton_params *params, const fem_solver_params *linear,
+                      fem_newton **out);
+int fem_newton_destroy(fem_newton *n);
+/* bind(op[, precond]) / unbind() (newtoninverseoperator.hh:540-551) */
+int fem_newton_bind(fem_newton *n, fem_op *op, fem_precond *pre);
+int fem_newton_unbind(fem_newton *n);
+/* operator()(u, w): u the right-hand side (NULL: zero), w the initial guess
+ * on entry, the solution on exit */
+int fem_newton_solve(fem_newton *n, const fem_vec *u, fem_vec *w, fem_newton_info *info);
 
 /* ---- learned coarse-space preconditioner ----------------------------------------
  * Two-level preconditioner for SIPG DG operators with an element-mean coarse

@@ -9,8 +9,7 @@
  *   GalerkinOperator   Operator<DF,DF>::operator()(u, w), w overwritten
  *                      (dune/fem/operator/common/operator.hh:55,
  *                       dune/fem/schemes/galerkin.hh:1460-1496); nonlinear()
- *                      (operator.hh:64); jacobian(u, jOp) of the linear
- *                      operator is the operator itself
+ *                      (operator.hh:64); jacobian(u, jOp), matrix-free
  *                      (operator/common/differentiableoperator.hh:54)
  *   SolverParameter    fem.solver.* keys (dune/fem/solver/parameter.hh:71-250)
  *   SolverInfo         {converged, linearIterations, nonlinearIterations,
@@ -21,9 +20,10 @@
  *                      (inverseoperatorinterface.hh:66-166); CgInverseOperator,
  *                      GmresInverseOperator (krylovinverseoperators.hh:280-295)
  *                      and FgmresInverseOperator
- *   AffineGalerkinOperator, NewtonInverseOperator
- *                      R(u) = A u - F and the Newton solve of R(w) = u
- *                      (dune/fem/solver/newtoninverseoperator.hh:688-802)
+ *   AffineGalerkinOperator, NewtonParameter, NewtonInverseOperator, FemScheme
+ *                      R(u) = S[u] - F and the Newton solve of R(w) = u
+ *                      (dune/fem/solver/newtoninverseoperator.hh:101-802,
+ *                       dune/fem/schemes/femscheme.hh:195-253)
  * Errors: every non-zero status of the C ABI becomes a dunefem_amd::Exception
  * (the reference throws Dune::Exception subclasses); non-convergence is not an
  * exception but a negative iteration count, as in the reference.
@@ -177,15 +177,39 @@ public:
     {
         check(fem_op_create(ctx.handle(), &desc, &h_));
     }
+    // the Jacobian operator object of `op` (NewtonInverseOperator's jOp):
+    // a linear operator with op's description and coefficient data
+    struct JacobianOf {};
+    GalerkinOperator(const GalerkinOperator &op, JacobianOf) : ctx_(op.ctx_), desc_(op.desc_)
+    {
+        check(fem_op_jacobian_create(op.h_, &h_));
+    }
     ~GalerkinOperator() { if (h_) fem_op_destroy(h_); }
     GalerkinOperator(const GalerkinOperator &) = delete;
     GalerkinOperator &operator=(const GalerkinOperator &) = delete;
 
     // w = A u (w overwritten)
     void operator()(const DiscreteFunction &u, DiscreteFunction &w) const { check(fem_op_apply(h_, u.handle(), w.handle())); }
-    bool nonlinear() const { return false; }
-    // the linearisation of the affine operator is its linear part: this operator
-    void jacobian(const DiscreteFunction &, const JacobianOperatorType *&jOp) const { jOp = this; }
+    bool nonlinear() const
+    {
+        const int r = fem_op_nonlinear(h_);
+        if (r < 0) check(r);
+        return r != 0;
+    }
+    // jOp = S'(u), jOp made by GalerkinOperator(op, JacobianOf{})
+    void jacobian(const DiscreteFunction &u, JacobianOperatorType &jOp) const
+    {
+        check(fem_op_jacobian(h_, u.handle(), jOp.handle()));
+    }
+    // DG: reaction int (cvol + coef u^pw) u v beyond c: cvol at the volume
+    // quadrature points [E][m^3] (empty clears), the law coef u^pw (coef 0 clears)
+    void setReactionQp(const std::vector<double> &cvol)
+    {
+        check(fem_op_set_reaction_qp(h_, cvol.empty() ? nullptr : cvol.data()));
+    }
+    void setReactionLaw(double coef, int pw) { check(fem_op_set_reaction_law(h_, coef, pw)); }
+    // u at the volume quadrature points [E][m^3] (x fastest)
+    void evalQp(const DiscreteFunction &u, DiscreteFunction &uq) const { check(fem_op_eval_qp(h_, u.handle(), uq.handle())); }
 
     void setKappaQp(const std::vector<double> &kvol, const std::vector<double> &kx, const std::vector<double> &ky,
                     const std::vector<double> &kz = {})
@@ -284,6 +308,8 @@ struct SolverInfo {
 
 class InverseOperator {
 public:
+    // the Krylov method a subclass fixes (-1: fem.solver.method decides)
+    static constexpr int fixedMethod = -1;
     using DomainFunctionType = DiscreteFunction;
     using RangeFunctionType = DiscreteFunction;
     using OperatorType = GalerkinOperator;
@@ -336,6 +362,7 @@ public:
     }
     SolverParameter &parameter() { return param_; }
     bool verbose() const { return param_.verbose(); }
+    Context &ctx() const { return *ctx_; }
 
 protected:
     InverseOperator(Context &ctx, SolverParameter param, int method) : ctx_(&ctx), param_(std::move(param)), fixed_method_(true)
@@ -377,144 +404,239 @@ private:
 // KrylovInverseOperator<DF, cg | gmres> and a flexible GMRES
 class CgInverseOperator : public InverseOperator {
 public:
+    static constexpr int fixedMethod = FEM_SOLVER_CG;
     explicit CgInverseOperator(Context &ctx, const SolverParameter &p = SolverParameter())
         : InverseOperator(ctx, p, FEM_SOLVER_CG) {}
 };
 class GmresInverseOperator : public InverseOperator {
 public:
+    static constexpr int fixedMethod = FEM_SOLVER_GMRES;
     explicit GmresInverseOperator(Context &ctx, const SolverParameter &p = SolverParameter())
         : InverseOperator(ctx, p, FEM_SOLVER_GMRES) {}
 };
 class FgmresInverseOperator : public InverseOperator {
 public:
+    static constexpr int fixedMethod = FEM_SOLVER_FGMRES;
     explicit FgmresInverseOperator(Context &ctx, const SolverParameter &p = SolverParameter())
         : InverseOperator(ctx, p, FEM_SOLVER_FGMRES) {}
 };
 
-// R(u) = A u - F: a Galerkin operator with its source term, as the
-// reference's GalerkinOperator evaluates it (galerkin.hh:333-360); its
-// jacobian is the linear operator A (nonlinear() == false)
+// R(u) = S[u] - F: a Galerkin operator with its source term, as the
+// reference's GalerkinOperator evaluates it (galerkin.hh:333-360); nonlinear
+// and differentiable as S is
 class AffineGalerkinOperator {
 public:
     using DomainFunctionType = DiscreteFunction;
     using RangeFunctionType = DiscreteFunction;
     using JacobianOperatorType = GalerkinOperator;
 
-    AffineGalerkinOperator(const GalerkinOperator &A, const DiscreteFunction &F) : A_(&A), F_(&F) {}
+    AffineGalerkinOperator(const GalerkinOperator &S, const DiscreteFunction &F) : S_(&S), F_(&F) {}
     void operator()(const DiscreteFunction &u, DiscreteFunction &w) const
     {
-        (*A_)(u, w);
+        (*S_)(u, w);
         w -= *F_;
     }
-    bool nonlinear() const { return false; }
-    const JacobianOperatorType &jacobian(const DiscreteFunction &) const { return *A_; }
+    bool nonlinear() const { return S_->nonlinear(); }
+    void jacobian(const DiscreteFunction &u, JacobianOperatorType &jOp) const { S_->jacobian(u, jOp); }
+    const GalerkinOperator &op() const { return *S_; }
+    const DiscreteFunction &source() const { return *F_; }
 
 private:
-    const GalerkinOperator *A_;
+    const GalerkinOperator *S_;
     const DiscreteFunction *F_;
 };
 
+// NewtonParameter (newtoninverseoperator.hh:101-380): fem.solver.nonlinear.*
+// keys for Newton, the other fem.solver.* keys (and nonlinear.linear.*) for
+// its linear solver
+class NewtonParameter {
+public:
+    NewtonParameter() { fem_newton_default_params(&p_); }
+    // the linear solver's parameters with Newton's defaults
+    explicit NewtonParameter(const SolverParameter &linear) : linear_(linear) { fem_newton_default_params(&p_); }
+    NewtonParameter &set(const std::string &key, const std::string &value)
+    {
+        if (newtonKey(key)) check(fem_newton_params_set(&p_, key.c_str(), value.c_str()));
+        else linear_.set(key, value);
+        return *this;
+    }
+    double tolerance() const { return p_.tolerance; }
+    void setTolerance(double tol) { p_.tolerance = tol; }
+    int maxIterations() const { return p_.maxiterations; }
+    void setMaxIterations(int it) { p_.maxiterations = it; }
+    int maxLinearIterations() const { return p_.maxlineariterations >= 0 ? p_.maxlineariterations : linear_.maxIterations(); }
+    void setMaxLinearIterations(int it) { p_.maxlineariterations = it; }
+    int lineSearch() const { return p_.linesearch; }
+    int forcing() const { return p_.forcing; }
+    bool simplified() const { return p_.simplified != 0; }
+    bool verbose() const { return p_.verbose != 0; }
+    SolverParameter &linear() { return linear_; }
+    const SolverParameter &linear() const { return linear_; }
+    fem_newton_params &raw() { return p_; }
+    const fem_newton_params &raw() const { return p_; }
+
+private:
+    static bool newtonKey(std::string k)
+    {
+        const std::string fs = "fem.solver.";
+        if (k.compare(0, fs.size(), fs) == 0) k = k.substr(fs.size());
+        for (const char *pre : { "nonlinear.", "newton." }) {
+            const std::string ps(pre);
+            if (k.compare(0, ps.size(), ps) == 0) {
+                const std::string rest = k.substr(ps.size());
+                return rest.compare(0, 7, "linear.") != 0 || rest == "linear.tolerance.strategy";
+            }
+        }
+        return false;
+    }
+    fem_newton_params p_;
+    SolverParameter linear_;
+};
+
 // NewtonInverseOperator<JacobianOperator, LInvOp>::operator()(u, w): solves
-// R(w) = u (dune/fem/solver/newtoninverseoperator.hh:688-802): residual
-// R(w) - u, dw = DR^-1 residual, w -= dw; for an operator with
-// nonlinear() == false one step, else until |residual| <= tolerance
-template <class LinearInverseOperator>
+// S[w] = u (newtoninverseoperator.hh:688-802) on the device, with the
+// Jacobian S'(w) applied matrix-free.  The linear solver is the library's,
+// with LinearInverseOperator's Krylov method (CgInverseOperator: cg, ...,
+// InverseOperator: fem.solver.method).  A linear operator takes one step and
+// counts no nonlinear iteration; failed() returns the NewtonFailure code
+// (:389-400, :568-584).
+template <class LinearInverseOperator = InverseOperator>
 class NewtonInverseOperator {
 public:
-    NewtonInverseOperator(Context &ctx, const SolverParameter &linear = SolverParameter(), double tolerance = 1e-6,
-                          int maxIterations = 100, int maxLinearIterations = 1 << 30)
-        : jInv_(ctx, linear), tolerance_(tolerance), maxIterations_(maxIterations),
-          maxLinearIterations_(maxLinearIterations) {}
+    using DomainFunctionType = DiscreteFunction;
+    using RangeFunctionType = DiscreteFunction;
 
-    void bind(const AffineGalerkinOperator &op) { op_ = &op; }
-    void unbind() { op_ = nullptr; }
+    explicit NewtonInverseOperator(Context &ctx, const NewtonParameter &param = NewtonParameter())
+        : ctx_(&ctx), param_(param) { create(); }
+    // the reference's (epsilon, parameter) constructor
+    NewtonInverseOperator(Context &ctx, double epsilon, const NewtonParameter &param = NewtonParameter())
+        : ctx_(&ctx), param_(param)
+    {
+        param_.setTolerance(epsilon);
+        create();
+    }
+    // a linear solver's parameters with Newton's defaults
+    NewtonInverseOperator(Context &ctx, const SolverParameter &linear) : ctx_(&ctx), param_(linear) { create(); }
+    ~NewtonInverseOperator() { if (h_) fem_newton_destroy(h_); }
+    NewtonInverseOperator(const NewtonInverseOperator &) = delete;
+    NewtonInverseOperator &operator=(const NewtonInverseOperator &) = delete;
+
+    void bind(const GalerkinOperator &op) { bind_(&op, nullptr, nullptr); }
+    void bind(const GalerkinOperator &op, const Preconditioner &pre) { bind_(&op, &pre, nullptr); }
+    // R(w) = S[w] - F = u is S[w] = u + F
+    void bind(const AffineGalerkinOperator &op) { bind_(&op.op(), nullptr, &op.source()); }
+    void bind(const AffineGalerkinOperator &op, const Preconditioner &pre) { bind_(&op.op(), &pre, &op.source()); }
+    void unbind()
+    {
+        check(fem_newton_unbind(h_));
+        op_ = nullptr;
+        pre_ = nullptr;
+        F_ = nullptr;
+    }
 
     void operator()(const DiscreteFunction &u, DiscreteFunction &w) const
     {
         if (!op_) throw Exception(FEM_ERR_STATE, "NewtonInverseOperator: bind an operator first");
-        DiscreteFunction residual(u), dw(w);
-        iterations_ = 0;
-        linearIterations_ = 0;
-        (*op_)(w, residual);
-        residual -= u;
-        delta_ = std::sqrt(residual.normSquaredDofs());
-        const bool nonlinear = op_->nonlinear();
-        nonlinear_ = nonlinear;
-        while (true) {
-            jInv_.bind(op_->jacobian(w));
-            const int left = maxLinearIterations_ - linearIterations_;
-            if (left <= 0) break;
-            jInv_.setMaxIterations(left);
-            dw.clear();
-            jInv_(residual, dw);
-            if (jInv_.iterations() < 0) {   // the linear solve did not converge
-                linearIterations_ = jInv_.iterations();
-                break;
-            }
-            linearIterations_ += jInv_.iterations();
-            w -= dw;
-            (*op_)(w, residual);
-            residual -= u;
-            delta_ = std::sqrt(residual.normSquaredDofs());
-            ++iterations_;
-            if (!nonlinear || delta_ <= tolerance_ || iterations_ >= maxIterations_) break;
+        if (F_) {
+            DiscreteFunction rhs(u);
+            rhs += *F_;
+            check(fem_newton_solve(h_, rhs.handle(), w.handle(), &info_));
+        } else {
+            check(fem_newton_solve(h_, u.handle(), w.handle(), &info_));
         }
     }
-    int iterations() const { return iterations_; }
-    int linearIterations() const { return linearIterations_; }
-    double residualNorm() const { return delta_; }
-    // the linear solves converged and, for a nonlinear operator, |residual| <= tolerance
-    bool converged() const { return linearIterations_ >= 0 && iterations_ > 0 && (!nonlinear_ || delta_ <= tolerance_); }
+    int iterations() const { return info_.iterations; }
+    int linearIterations() const { return info_.linear_iterations; }
+    double residual() const { return info_.residual; }
+    int failed() const { return info_.failure; }
+    bool converged() const { return info_.converged != 0; }
+    std::vector<double> timing() const { return { info_.timing[0], info_.timing[1], info_.timing[2] }; }
+    SolverInfo info() const
+    {
+        SolverInfo i;
+        i.converged = converged();
+        i.linearIterations = linearIterations();
+        i.nonlinearIterations = iterations();
+        i.timing = timing();
+        return i;
+    }
+    void setMaxIterations(int it)
+    {
+        param_.setMaxIterations(it);
+        recreate();
+    }
+    void setMaxLinearIterations(int it)
+    {
+        param_.setMaxLinearIterations(it);
+        recreate();
+    }
+    NewtonParameter &parameter() { return param_; }
 
 private:
-    const AffineGalerkinOperator *op_ = nullptr;
-    mutable LinearInverseOperator jInv_;
-    double tolerance_;
-    int maxIterations_, maxLinearIterations_;
-    mutable int iterations_ = 0, linearIterations_ = 0;
-    mutable double delta_ = 0;
-    mutable bool nonlinear_ = false;
+    void create()
+    {
+        if (LinearInverseOperator::fixedMethod >= 0) param_.linear().setMethod(LinearInverseOperator::fixedMethod);
+        check(fem_newton_create(ctx_->handle(), &param_.raw(), &param_.linear().raw(), &h_));
+    }
+    void recreate()
+    {
+        fem_newton *old = h_;
+        h_ = nullptr;
+        create();
+        fem_newton_destroy(old);
+        if (op_) bind_(op_, pre_, F_);
+    }
+    void bind_(const GalerkinOperator *op, const Preconditioner *pre, const DiscreteFunction *F)
+    {
+        check(fem_newton_bind(h_, op->handle(), pre ? pre->handle() : nullptr));
+        op_ = op;
+        pre_ = pre;
+        F_ = F;
+    }
+    Context *ctx_;
+    NewtonParameter param_;
+    fem_newton *h_ = nullptr;
+    const GalerkinOperator *op_ = nullptr;
+    const Preconditioner *pre_ = nullptr;
+    const DiscreteFunction *F_ = nullptr;
+    mutable fem_newton_info info_{};
 };
 
-// FemScheme::solve / _solve (dune/fem/schemes/femscheme.hh:195-253) for the
-// affine operator R(u) = A u - F (boundary data inside F and, for Q2, in the
-// Dirichlet rows): setConstraints and addConstraints on the solution, then
-// the Newton step of NewtonInverseOperator over R with the linear inverse
-// operator (one step: R is affine)
+// FemScheme::solve / _solve (dune/fem/schemes/femscheme.hh:195-253) for
+// R(u) = S[u] - F (boundary data inside F and, for Q2, in the Dirichlet
+// rows): setConstraints and addConstraints on the solution, then
+// NewtonInverseOperator over R with the linear inverse operator's
+// parameters (one step when S is linear)
 template <class LinearInverseOperator>
 class FemScheme {
 public:
-    FemScheme(const GalerkinOperator &A, const DiscreteFunction &F, LinearInverseOperator &inv)
-        : A_(&A), R_(A, F), inv_(&inv) {}
+    FemScheme(const GalerkinOperator &S, const DiscreteFunction &F, LinearInverseOperator &inv,
+              const NewtonParameter &nonlinear = NewtonParameter())
+        : S_(&S), R_(S, F), newton_(inv.ctx(), withLinear(nonlinear, inv.parameter())) {}
     SolverInfo solve(const DiscreteFunction &rhs, DiscreteFunction &solution) const
     {
-        A_->constrain(solution, &rhs);          // sol = g + rhs on the boundary
-        return newton(&rhs, solution);
+        S_->constrain(solution, &rhs);          // sol = g + rhs on the boundary
+        newton_.bind(R_);
+        newton_(rhs, solution);
+        newton_.unbind();
+        return newton_.info();
     }
     SolverInfo solve(DiscreteFunction &solution) const
     {
-        A_->constrain(solution);                // sol = g on the boundary
-        return newton(nullptr, solution);
+        DiscreteFunction zero(solution);
+        zero.clear();
+        return solve(zero, solution);
     }
 
 private:
-    SolverInfo newton(const DiscreteFunction *rhs, DiscreteFunction &w) const
+    static NewtonParameter withLinear(NewtonParameter p, const SolverParameter &linear)
     {
-        DiscreteFunction residual(w), dw(w);
-        R_(w, residual);
-        if (rhs) residual -= *rhs;
-        inv_->bind(*A_);
-        dw.clear();
-        (*inv_)(residual, dw);
-        SolverInfo info = inv_->info();
-        w -= dw;
-        inv_->unbind();
-        info.nonlinearIterations = info.converged ? 1 : 0;
-        return info;
+        p.linear() = linear;
+        return p;
     }
-    const GalerkinOperator *A_;
+    const GalerkinOperator *S_;
     AffineGalerkinOperator R_;
-    LinearInverseOperator *inv_;
+    mutable NewtonInverseOperator<LinearInverseOperator> newton_;
 };
 
 // Gauss-Legendre rule with m points on [0,1] (the reference's GaussPts

@@ -253,7 +253,7 @@ static void dg_interior(const ora_dg_t *op, const dg_tables *T, int64_t e, const
         double kap = dg_kappa_vol(op, e, q, nq);
         double W = T->wvol[q] * detJ;
         /* integrand interior: (source, flux) = (c u, kappa grad u - b u) */
-        double s = W * op->c * u;
+        double s = W * (op->c + (op->cvol ? op->cvol[e * nq + q] : 0.0)) * u;
         double F[3];
         for (int d = 0; d < 3; ++d) F[d] = W * (kap * g[d] - op->b[d] * u) / h[d];
         for (int i = 0; i < nb; ++i) wl[i] += s * v[i] + F[0] * G0[i] + F[1] * G1[i] + F[2] * G2[i];
@@ -544,7 +544,8 @@ void ora_dg_apply_sf(const ora_dg_t *op, const double *u, double *w, int nthread
                     double kap = op->kappa_mode == ORA_KAPPA_QP ? op->kvol[e * K3 + qa + K * qb + K * K * qc]
                                : op->kappa_mode == ORA_KAPPA_CELL ? op->kcell[e] : op->kappa_const;
                     for (int d = 0; d < 3; ++d) F[d][i] = W * (kap * g[d][i] / h[d] - op->b[d] * uq[i]) / h[d];
-                    S0[i] = W * op->c * uq[i];
+                    const double cv = op->cvol ? op->cvol[e * K3 + qa + K * qb + K * K * qc] : 0.0;
+                    S0[i] = W * (op->c + cv) * uq[i];
                 }
         memset(r, 0, sizeof(double) * K3);
         for (int d = -1; d < 3; ++d) {

@@ -81,6 +81,11 @@ typedef struct {
     const double *kvol;   /* ORA_KAPPA_QP: [E][m^3] */
     const double *kface[3]; /* ORA_KAPPA_QP: [faces_d][m^2] */
     const double *kcell;  /* ORA_KAPPA_CELL: [E] */
+    /* pointwise reaction coefficient at the volume quadrature points
+     * [E][m^3] (x fastest), added to c; NULL for none.  With c(x) =
+     * c3 u_lin(x)^2 it gives the residual of the nonlinear term c3 u^3
+     * (u_lin = u) and, with 3 c3 u_lin^2, its Jacobian at u_lin */
+    const double *cvol;
 } ora_dg_t;
 
 int64_t ora_dg_ndofs(const ora_dg_t *op);

@@ -22,7 +22,7 @@ class DG(C.Structure):
     _fields_ = [("n", C.c_int * 3), ("p", C.c_int), ("beta", C.c_double),
                 ("b", C.c_double * 3), ("c", C.c_double), ("kappa_mode", C.c_int),
                 ("kappa_const", C.c_double), ("kvol", _dp), ("kface", _dp * 3),
-                ("kcell", _dp)]
+                ("kcell", _dp), ("cvol", _dp)]
 
 
 class CG(C.Structure):
@@ -161,6 +161,29 @@ class DGOperator:
         w = np.empty(self.N)
         lib().ora_dg_apply(C.byref(self.s), _ptr(u), _ptr(w), nthreads)
         return w
+
+    def set_reaction_qp(self, cvol):
+        """pointwise reaction coefficient at the volume quadrature points
+        [E][m^3] (x fastest), added to c; None to clear"""
+        if cvol is None:
+            self.s.cvol = None
+            return
+        cv = np.ascontiguousarray(cvol, dtype=np.float64).ravel()
+        m = self.p + 1
+        assert cv.size == self.N // m ** 3 * m ** 3
+        self._cvol = cv
+        self.s.cvol = _ptr(cv)
+
+    def eval_qp(self, u):
+        """u at the volume quadrature points [E][m^3] (x fastest), by the
+        dense basis tables (numpy: phi_a(x) phi_b(y) phi_c(z))"""
+        m = self.p + 1
+        x, _ = gauss(m)
+        B = np.array([[legendre(a, xq) for xq in x] for a in range(m)])   # [mode][point]
+        E = self.N // m ** 3
+        U = np.asarray(u, dtype=np.float64).reshape(E, m, m, m)   # [e][a][b][c]
+        # uq[e][qz][qy][qx] = sum_abc U[a][b][c] B[a][qx] B[b][qy] B[c][qz]
+        return np.einsum("eabc,ax,by,cz->ezyx", U, B, B, B).reshape(E, m ** 3)
 
     def apply_sf(self, u, nthreads=1):
         """the same apply evaluated by sum factorisation (timing baseline)"""

@@ -148,7 +148,8 @@ def test_sipg_boundary_data_consistency_on_gpu(ctx, p):
 def test_q2_scheme_solve_with_dirichlet_data_reproduces_polynomial(ctx):
     # continuous Q2, -div(kappa grad u) + u = f, kappa = 1 + x + 2y, u = u* on
     # the boundary with u* in Q2: the scheme solve (constraints, one Newton
-    # step with CG + Jacobi) returns u* at every node
+    # step with CG + Jacobi) returns u* at every node; a linear operator counts
+    # no nonlinear iteration (newtoninverseoperator.hh:786-790)
     from manufactured import q2_problem
     n = (12, 9)
     M = q2_problem(n, c=1.0, seed=3)
@@ -162,7 +163,7 @@ def test_q2_scheme_solve_with_dirichlet_data_reproduces_polynomial(ctx):
     info = scheme.solve(U)
     print("Q2 scheme solve:", info, flush=True)
     assert set(info) == {"converged", "iterations", "linear_iterations", "timing"}
-    assert info["converged"] and info["iterations"] == 1
+    assert info["converged"] and info["iterations"] == 0 and info["linear_iterations"] > 0
     u = U.download()
     assert np.abs(u - M["nodes"]).max() <= 1e-9 * np.abs(M["nodes"]).max()
     # R(u*) = A u* - F vanishes, Dirichlet rows included (u_D - g_D)
