@@ -12,7 +12,8 @@ Also reported on the same line:
                 per-element tables, element/face loop, OpenMP element chunks)
                 on the host cores, same 64^3 workload
   mixed         FP32 and BF16-MFMA applies: GDoF/s and relative error vs FP64
-  solve         time-to-solve comparisons when --solve is given
+  solve         time-to-solve comparisons (48^3 Q2 channels, 96^3 Q4 log-normal)
+  nonlinear     Newton solves of -div(kappa grad u) + c3 u^3 = f (48^3 Q2)
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--solve]
 """
@@ -43,6 +44,8 @@ def parse():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="solve section without the learned preconditioner")
     ap.add_argument("--no-combined", action="store_true", help="skip the 96^3 Q4 solves")
+    ap.add_argument("--no-nonlinear", action="store_true", help="skip the Newton solves")
+    ap.add_argument("--nonlinear-only", action="store_true", help="only the Newton section (profiling)")
     ap.add_argument("--apply-only", choices=["c1", "c3", "c4"], default=None,
                     help="time only the operator apply of one workload (for profiling): c1 = 64^3 Q3 pointwise "
                          "kappa with advection, c3 = 48^3 Q2 channels, c4 = 96^3 Q4 log-normal")
@@ -279,6 +282,102 @@ def combined_section(fa, wl, ctx, n=96, learned=True, pairs=400, epochs=2, lr=1e
     return out
 
 
+def nonlinear_section(fa, wl, ctx, n=48, p=2, c3=1.0e4, amp=50.0, cpu=True):
+    """Newton on -div(kappa grad u) + c3 u^3 = f: SIPG DG Q2 on n^3 hexes,
+    log-normal cellwise kappa (sigma 1), f = amp x (32 Fourier modes), the
+    cubic reaction at the volume quadrature points.  Newton (the reference's
+    NewtonInverseOperator loop) to |S[w] - F| < 1e-8 |F| from w = 0 with
+    matrix-free Jacobians: CG + block Jacobi rebuilt on each Jacobian, the
+    same with Eisenstat-Walker forcing, and CG with a multigrid built once on
+    the diffusion part (FP64, and FP32 widened into the FP64 CG).  Also the
+    apply rates of S[u], of the Jacobian and of the linear part, with their
+    algorithmic bytes against 8 TB/s, and the oracle's S[u] on the host."""
+    P = wl.problem4(n=n, p=p, sigma=1.0)
+    E = n ** 3
+    N = E * (p + 1) ** 3
+    out = {"workload": "3D SIPG DG Q%d, %d^3 hexes (%d DoFs), -div(kappa grad u) + %g u^3 = f, log-normal kappa "
+                       "(sigma 1, per element), f = %g x 32 Fourier modes; Newton to |S[w] - F| < 1e-8 |F| from "
+                       "w = 0" % (p, n, N, c3, amp), "dofs": N}
+    lin64 = _dg_cell(fa, ctx, P, fa.F64)       # the diffusion part alone
+    S = _dg_cell(fa, ctx, P, fa.F64)
+    S.set_reaction_law(c3, 2)
+    F = lin64.rhs(amp * P["fq"])
+    fnorm = float(np.sqrt(F.normSquaredDofs()))
+    tol = 1e-8 * fnorm
+    runs = {}
+
+    def run(name, nl, linear, pre=None):
+        inv = fa.NewtonInverseOperator(ctx, dict(nl, **{"nonlinear.tolerance": tol}), linear=linear)
+        inv.bind(S, pre)
+        W = S.vector()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        inv(F, W)
+        ctx.synchronize()
+        wall = time.perf_counter() - t0
+        inv.unbind()
+        i = inv.info()
+        R = S.vector()
+        S(W, R)
+        R.axpy(-1.0, F)
+        runs[name] = {"converged": i["converged"], "failure": i["failure"], "newton_iterations": i["iterations"],
+                      "linear_iterations": i["linear_iterations"], "time_s": round(wall, 4),
+                      "timing_total_jacobian_solve": [round(t, 4) for t in i["timing"]],
+                      "rel_residual": float(np.sqrt(R.normSquaredDofs()) / fnorm)}
+        log("nonlinear", name, runs[name])
+        return W
+
+    exact = dict(method=fa.CG, tolerance=1e-10, errormeasure=fa.RELATIVE, maxiterations=20000)
+    W = run("newton_cg_bj", {}, dict(exact, preconditioning=fa.PRE_BLOCK_JACOBI))
+    run("newton_ew_cg_bj", {"nonlinear.forcing": "eisenstatwalker"},
+        dict(method=fa.CG, maxiterations=20000, preconditioning=fa.PRE_BLOCK_JACOBI))
+    t0 = time.perf_counter()
+    mg64 = fa.Preconditioner(lin64, fa.PRE_MULTIGRID)
+    lin32 = _dg_cell(fa, ctx, P, fa.F32)
+    mg32 = fa.Preconditioner(lin32, fa.PRE_MULTIGRID)
+    ctx.synchronize()
+    out["multigrid_setup_s"] = round(time.perf_counter() - t0, 3)
+    run("newton_cg_mg_diffusion", {}, dict(exact, graph=1), mg64)
+    run("newton_ew_cg_mg32_diffusion", {"nonlinear.forcing": "eisenstatwalker"},
+        dict(method=fa.CG, maxiterations=20000, graph=1), mg32)
+    out["runs"] = runs
+    base = runs["newton_cg_bj"]["time_s"]
+    out["speedup_vs_newton_cg_bj"] = {k: round(base / v["time_s"], 2) for k, v in runs.items()}
+
+    # apply rates at the solution: S[u] (law in the kernel), J(u) w (cvol
+    # read), the linear part; cellwise kappa: E values
+    J = S.jacobian(W)
+    rates = {}
+    for name, op, nbytes in (("S_apply", S, 8 * (2 * N + E)), ("jacobian_apply", J, 8 * (3 * N + E)),
+                             ("linear_apply", lin64, 8 * (2 * N + E))):
+        Y = op.vector()
+        wall, kms = time_applies(fa, ctx, op, W, Y, 50, 5)
+        rates[name] = {"gdofs": round(N / (wall / 50) / 1e9, 3), "kernel_ms": round(kms, 5),
+                       "alg_bytes": nbytes, "hbm_frac": round(nbytes / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    out["applies"] = rates
+    log("nonlinear applies", rates)
+
+    if cpu:
+        # the oracle's S[u] at the solution (the reference algorithm restated:
+        # cvol = c3 u^2 at the quadrature points, then the dense-table apply),
+        # one apply on the host cores
+        import pyoracle as ora
+        threads = cpu_threads()
+        oop = ora.DGOperator(P["n"], p, P["beta"], kappa=("cell", P["kcell"]))
+        w = W.download()
+        t0 = time.perf_counter()
+        uq = oop.eval_qp(w)
+        oop.set_reaction_qp(c3 * uq * uq)
+        so = oop.apply(w, nthreads=threads)
+        tc = time.perf_counter() - t0
+        Y = S.vector()
+        S(W, Y)
+        out["cpu_oracle_S_apply"] = {"seconds": round(tc, 3), "gdofs": round(N / tc / 1e9, 5), "cores": threads,
+                                     "gpu_vs_oracle_rel_err": float(np.abs(Y.download() - so).max() /
+                                                                    np.abs(so).max())}
+    return out
+
+
 def batch_section(fa, wl, ctx, count=256):
     """256 uneven 2D Q2 problems: one batched FP32-CG + FP64-refinement solve
     vs the same problems solved one by one with the FP64 GPU CG."""
@@ -436,6 +535,13 @@ def main():
     if a.solve_only:
         solve_only(a, fa, wl)
         return
+    if a.nonlinear_only:
+        if not a.no_cpu:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import pyoracle as ora
+            ora.select_native()
+        print(json.dumps({"nonlinear": nonlinear_section(fa, wl, fa.Context(0), cpu=not a.no_cpu)}), flush=True)
+        return
     if not a.no_cpu:
         # the CPU baseline runs the oracle built for this host (-march=native)
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -532,6 +638,9 @@ def main():
 
     if not a.no_combined and not a.no_solve:
         result["combined"] = combined_section(fa, wl, ctx, learned=not a.no_learned)
+
+    if not a.no_nonlinear and not a.no_solve:
+        result["nonlinear"] = nonlinear_section(fa, wl, ctx, cpu=not a.no_cpu)
 
     # ---- CPU baseline: the oracle (reference algorithm) on the host cores ----------
     if not a.no_cpu:

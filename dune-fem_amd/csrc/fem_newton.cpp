@@ -80,7 +80,7 @@ void Newton::unbind()
 
 int Newton::max_linear() const
 {
-    return prm.maxlineariterations >= 0 ? prm.maxlineariterations : lin.prm.maxiterations;
+    return linear_limit;
 }
 
 // NewtonInverseOperator::failed (newtoninverseoperator.hh:568-584)
@@ -179,6 +179,9 @@ void Newton::solve(const Vec *u, Vec &w, fem_newton_info &info)
             try { s.unbind(); } catch (...) {}
         }
     } restore{ lin, lin.prm.tolerance, lin.prm.maxiterations };
+    // the total over all steps (newtoninverseoperator.hh:246-250), fixed
+    // before the per-step limits below change the linear solver's own
+    linear_limit = prm.maxlineariterations >= 0 ? prm.maxlineariterations : lin.prm.maxiterations;
     ew = EisenstatWalker{ prm.eisenstatwalker_etamax, prm.eisenstatwalker_gamma, prm.tolerance };
     step_completed = true;
     iterations = 0;

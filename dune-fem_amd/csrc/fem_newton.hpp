@@ -26,6 +26,7 @@ struct Newton {
     // state of the last solve
     double delta = 0;
     int iterations = 0, linear_iterations = 0;
+    int linear_limit = 0;   // total linear iterations allowed in this solve
     bool step_completed = true;
     double timing[3] = { 0, 0, 0 };
 
