@@ -553,6 +553,14 @@ int fem_op_set_reaction_law(fem_op *op, double coef, int pw)
     });
 }
 
+int fem_op_set_reaction_plugin(fem_op *op, const char *path)
+{
+    return guarded([&] {
+        FEM_CHECK(op, FEM_ERR_ARG, "null operator");
+        op->o.set_reaction_plugin(path);
+    });
+}
+
 int fem_op_nonlinear(const fem_op *op)
 {
     if (!op) return FEM_ERR_ARG;

@@ -973,9 +973,10 @@ void dg_apply_bf16(const DGArgs<float> &a, int k, const float *u, float *w, cons
 }
 
 // ---------------------------------------------------------------------------
-// load vector F_i = int_K f phi_i (f at volume quadrature points)
+// load vector F_i = int_K f phi_i (f at volume quadrature points); ADD:
+// F_i += int_K f phi_i
 // ---------------------------------------------------------------------------
-template <int K>
+template <int K, bool ADD>
 __global__ void __launch_bounds__(DGShape<K>::THREADS)
     k_dg_rhs(const DGArgs<double> a, const double *__restrict__ fq, double *__restrict__ Fv)
 {
@@ -1008,16 +1009,22 @@ __global__ void __launch_bounds__(DGShape<K>::THREADS)
     mvT<K>(tb.B, v, tmp);
     if (active)
 #pragma unroll
-        for (int c = 0; c < K; ++c) Fv[e * K3 + i * K2 + j * K + c] = tmp[c];
+        for (int c = 0; c < K; ++c) {
+            double *f = Fv + e * K3 + i * K2 + j * K + c;
+            *f = ADD ? *f + tmp[c] : tmp[c];
+        }
 }
 
-void dg_rhs(const DGArgs<double> &a, int k, const double *fq, double *F, hipStream_t s)
+void dg_rhs(const DGArgs<double> &a, int k, const double *fq, double *F, hipStream_t s, bool add)
 {
     const int64_t E = (int64_t)a.n[0] * a.n[1] * a.n[2];
 #define RHS_CASE(KK)                                                                                   \
     case KK: {                                                                                         \
         unsigned nb = (unsigned)((E + DGShape<KK>::EPB - 1) / DGShape<KK>::EPB);                       \
-        hipLaunchKernelGGL(k_dg_rhs<KK>, dim3(nb), dim3(DGShape<KK>::THREADS), 0, s, a, fq, F);         \
+        if (add)                                                                                       \
+            hipLaunchKernelGGL((k_dg_rhs<KK, true>), dim3(nb), dim3(DGShape<KK>::THREADS), 0, s, a, fq, F); \
+        else                                                                                           \
+            hipLaunchKernelGGL((k_dg_rhs<KK, false>), dim3(nb), dim3(DGShape<KK>::THREADS), 0, s, a, fq, F); \
         break;                                                                                         \
     }
     switch (k) {

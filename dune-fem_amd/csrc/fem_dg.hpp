@@ -128,7 +128,7 @@ void dg_diagonal(const DGArgs<double> &a, int k, double *d, hipStream_t s);
 template <typename T>
 void dg_eval_qp(const DGArgs<T> &a, int k, const T *u, const T *base, T coef, int pw, T *out, hipStream_t s);
 // load vector from f at volume quadrature points
-void dg_rhs(const DGArgs<double> &a, int k, const double *fq, double *F, hipStream_t s);
+void dg_rhs(const DGArgs<double> &a, int k, const double *fq, double *F, hipStream_t s, bool add = false);
 // F += boundary-data part of the load vector (u = g on the boundary, g at
 // the face quadrature points [faces_d][m^2] per normal direction)
 void dg_rhs_bnd(const DGArgs<double> &a, int k, const double *const g[3], double *F, hipStream_t s);

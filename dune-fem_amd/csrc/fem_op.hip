@@ -1,5 +1,7 @@
 // Operator and preconditioner objects: set-up of coefficient data and
 // dispatch to the DG / CG kernels.
+#include <dlfcn.h>
+
 #include <cmath>
 
 #include "fem_blas.hpp"
@@ -200,6 +202,51 @@ void Operator::set_reaction_qp(const double *cvol)
     a32.cvol = cvol32.as<float>();
 }
 
+Operator::~Operator()
+{
+    if (plugin_dl) dlclose(plugin_dl);
+}
+
+fem_qp_geometry Operator::qp_geometry() const
+{
+    fem_qp_geometry g{};
+    for (int d = 0; d < 3; ++d) {
+        g.n[d] = desc.n[d];
+        g.h[d] = a64.h[d];
+    }
+    g.m = k;
+    double w[KMAX];
+    gauss_legendre01(k, g.gx, w);
+    g.nq = E * k * k * k;
+    return g;
+}
+
+void Operator::set_reaction_plugin(const char *path)
+{
+    reaction_checks(*this);
+    FEM_CHECK(desc.precision == FEM_F64, FEM_ERR_UNSUPPORTED, "compiled reaction: FP64 operators");
+    if (plugin_dl) {
+        dlclose(plugin_dl);
+        plugin_dl = nullptr;
+        plugin = nullptr;
+    }
+    pq.free();
+    ps.free();
+    if (!path) return;
+    void *h = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+    FEM_CHECK(h, FEM_ERR_ARG, std::string("compiled reaction: cannot load ") + path + ": " + dlerror());
+    auto fn = (fem_reaction_eval_fn)dlsym(h, "fem_reaction_eval");
+    if (!fn) {
+        dlclose(h);
+        throw Error(FEM_ERR_ARG, std::string("compiled reaction: no fem_reaction_eval in ") + path);
+    }
+    const size_t nq = (size_t)E * k * k * k;
+    pq.alloc(nq * sizeof(double));
+    ps.alloc(nq * sizeof(double));
+    plugin_dl = h;
+    plugin = fn;
+}
+
 void Operator::set_reaction_law(double coef, int pw)
 {
     reaction_checks(*this);
@@ -260,9 +307,10 @@ void Operator::jacobian(const Vec &u, Operator &jop) const
     FEM_CHECK(u.n == N && u.prec == vec_prec, FEM_ERR_ARG, "jacobian: vector size or precision");
     hipStream_t s = ctx->stream;
     const int64_t nq = E * k * k * k;
+    FEM_CHECK(!jop.plugin, FEM_ERR_ARG, "jacobian: the Jacobian operator must be linear");
     jop.set_reaction_law(0.0, 0);
-    if (!nonlinear()) {
-        // linear: the Jacobian is the operator (its cvol, if any)
+    if (nl_pow == 0) {
+        // no law: the linear part's cvol, if any
         copy_buf(jop.cvol64, cvol64, s);
         copy_buf(jop.cvol32, cvol32, s);
     } else {
@@ -281,6 +329,25 @@ void Operator::jacobian(const Vec &u, Operator &jop) const
             Vec lo = raw_vec(ctx, jop.cvol32.p, nq, FEM_F32), hi = raw_vec(ctx, jop.cvol64.p, nq, FEM_F64);
             vec_copy(hi, lo);
         }
+    }
+    if (plugin) {
+        // + dr/du(u) from the compiled reaction
+        dg_eval_qp<double>(a64, k, (const double *)u.data, nullptr, 1.0, 1, pq.as<double>(), s);
+        const fem_qp_geometry g = qp_geometry();
+        FEM_CHECK(plugin(&g, pq.as<double>(), nullptr, ps.as<double>(), (void *)s) == 0, FEM_ERR_HIP,
+                  "compiled reaction: launch failed");
+        Vec dr = raw_vec(ctx, ps.p, nq, FEM_F64);
+        if (jop.cvol64.p) {
+            Vec c = raw_vec(ctx, jop.cvol64.p, nq, FEM_F64);
+            vec_axpy(c, 1.0, dr);
+        } else {
+            jop.cvol64.alloc((size_t)nq * 8);
+            jop.cvol32.alloc((size_t)nq * 4);
+            Vec c = raw_vec(ctx, jop.cvol64.p, nq, FEM_F64);
+            vec_copy(c, dr);
+        }
+        Vec lo = raw_vec(ctx, jop.cvol32.p, nq, FEM_F32), hi = raw_vec(ctx, jop.cvol64.p, nq, FEM_F64);
+        vec_copy(lo, hi);
     }
     jop.a64.cvol = jop.cvol64.as<double>();
     jop.a32.cvol = jop.cvol32.as<float>();
@@ -356,9 +423,20 @@ void Operator::apply_raw(const void *x, void *y, const int *skip, double *dot) c
     if (is_dg()) {
         FEM_CHECK(a64.kappa_mode >= 0, FEM_ERR_STATE, "coefficient data not set");
         FEM_CHECK(!dot || desc.precision != FEM_BF16MFMA, FEM_ERR_UNSUPPORTED, "fused dot: FP64 or FP32 operator");
+        FEM_CHECK(!plugin || (!dot && !skip), FEM_ERR_STATE,
+                  "apply: an operator with a compiled reaction is nonlinear (no fused dot, no captured solve)");
         if (desc.precision == FEM_F64) dg_apply_f64(a64, k, (const double *)x, (double *)y, skip, s, dot);
         else if (desc.precision == FEM_F32) dg_apply_f32(a32, k, (const float *)x, (float *)y, skip, s, dot);
         else dg_apply_bf16(a32, k, (const float *)x, (float *)y, skip, s);
+        if (plugin) {
+            // y += int r(u, x) v: u at the points, r by the compiled code,
+            // then the load-vector kernel accumulating into y
+            dg_eval_qp<double>(a64, k, (const double *)x, nullptr, 1.0, 1, pq.as<double>(), s);
+            const fem_qp_geometry g = qp_geometry();
+            FEM_CHECK(plugin(&g, pq.as<double>(), ps.as<double>(), nullptr, (void *)s) == 0, FEM_ERR_HIP,
+                      "compiled reaction: launch failed");
+            dg_rhs(a64, k, ps.as<double>(), (double *)y, s, true);
+        }
     } else {
         FEM_CHECK(!dot, FEM_ERR_UNSUPPORTED, "fused dot: DG operators only");
         FEM_CHECK(desc.kappa_mode == FEM_KAPPA_CONST || kvol64.p, FEM_ERR_STATE, "coefficient data not set");

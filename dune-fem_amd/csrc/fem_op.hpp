@@ -66,8 +66,19 @@ struct Operator {
     DevBuf cvol64, cvol32;
     double nl_coef = 0;
     int nl_pow = 0;
-    bool nonlinear() const { return nl_pow > 0; }
+    // a compiled pointwise reaction r(u, x) (shared library, owned handle)
+    // and its scratch: u and r at the volume quadrature points
+    void *plugin_dl = nullptr;
+    fem_reaction_eval_fn plugin = nullptr;
+    mutable DevBuf pq, ps;
+    bool nonlinear() const { return nl_pow > 0 || plugin != nullptr; }
     bool has_reaction() const { return cvol64.p != nullptr || nonlinear(); }
+    void set_reaction_plugin(const char *path);
+    fem_qp_geometry qp_geometry() const;
+    Operator() = default;
+    Operator(const Operator &) = delete;
+    Operator &operator=(const Operator &) = delete;
+    ~Operator();
     // cvol from host values [E][k^3] (null clears)
     void set_reaction_qp(const double *cvol);
     // the law nl_coef u^pw, pw in 1..4 (coef = 0 clears)

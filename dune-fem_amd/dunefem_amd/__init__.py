@@ -60,7 +60,7 @@ EXPORTS = [
     "fem_csr_apply", "fem_csr_diagonal", "fem_csr_download",
     "fem_dofs_encode", "fem_dofs_decode", "fem_vec_write", "fem_vec_read",
     "fem_op_set_reaction_qp", "fem_op_set_reaction_law", "fem_op_nonlinear", "fem_op_eval_qp",
-    "fem_op_jacobian_create", "fem_op_jacobian",
+    "fem_op_jacobian_create", "fem_op_jacobian", "fem_op_set_reaction_plugin",
     "fem_newton_default_params", "fem_newton_params_set", "fem_newton_create", "fem_newton_destroy",
     "fem_newton_bind", "fem_newton_unbind", "fem_newton_solve",
 ]
@@ -211,6 +211,7 @@ def load_library(path=LIBPATH):
         "fem_op_set_reaction_qp": ([_vp, _dp], C.c_int),
         "fem_op_set_reaction_law": ([_vp, C.c_double, C.c_int], C.c_int),
         "fem_op_nonlinear": ([_vp], C.c_int),
+        "fem_op_set_reaction_plugin": ([_vp, C.c_char_p], C.c_int),
         "fem_op_eval_qp": ([_vp, _vp, _vp], C.c_int),
         "fem_op_jacobian_create": ([_vp, C.POINTER(_vp)], C.c_int),
         "fem_op_jacobian": ([_vp, _vp, _vp], C.c_int),
@@ -475,6 +476,34 @@ class GalerkinOperator:
         """the nonlinear reaction coef u^pw u (pw in 1..4; coef = 0 clears):
         the operator becomes S[u] = A u + int coef u^(pw+1) v"""
         _check(_lib.fem_op_set_reaction_law(self.h, float(coef), int(pw)))
+
+    def set_reaction(self, r, build=True):
+        """a pointwise reaction r(u, x): the volume term int r(u_h, x) v with
+        r a sympy expression (or string) of u, x, y, z, compiled to HIP by
+        dunefem_amd.forms (cached; build=False requires the compiled library
+        to exist); None removes it.  FP64 DG operators."""
+        if r is None:
+            _check(_lib.fem_op_set_reaction_plugin(self.h, None))
+            self.reaction = None
+            return
+        from . import forms
+        path = forms.compile_reaction(r, build=build)
+        _check(_lib.fem_op_set_reaction_plugin(self.h, path.encode()))
+        self.reaction = forms.as_expr(r)
+
+    def qp_coordinates(self):
+        """x, y, z of the volume quadrature points [E][m^3] (DG)"""
+        from . import forms
+        m = self.desc.order + 1
+        return forms.qp_coordinates(tuple(self.desc.n), m, gauss_points(m)[0])
+
+    def rhs_expr(self, f, F=None):
+        """load vector of a source f(x, y, z) (a sympy expression or string
+        of x, y, z), evaluated at the volume quadrature points"""
+        from . import forms
+        X, Y, Z = self.qp_coordinates()
+        fq, _ = forms.evaluate(f, np.zeros_like(X), X, Y, Z)
+        return self.rhs(fq, F)
 
     def nonlinear(self):
         """Operator::nonlinear() (dune/fem/operator/common/operator.hh:64)"""

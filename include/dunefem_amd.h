@@ -223,6 +223,32 @@ int fem_op_block_diagonal(fem_op *op, fem_vec *blocks);
  *   matrix when one is needed. */
 int fem_op_set_reaction_qp(fem_op *op, const double *cvol);
 int fem_op_set_reaction_law(fem_op *op, double coef, int pw);
+
+/* A compiled pointwise reaction r(u, x): the volume term int r(u, x) v, with
+ * r any expression of u and the coordinates (the counterpart of the
+ * reference's generated integrands, python/dune/models/integrands/ufl.py:
+ * 197-348 and dune/fem/schemes/integrands.hh:412-664, for the source part of
+ * the interior integrand).  The Python module dunefem_amd.forms writes r and
+ * dr/du (symbolic derivative) as HIP code and compiles it into a shared
+ * library exporting
+ *     int fem_reaction_eval(const fem_qp_geometry *g, const double *uq,
+ *                           double *r, double *dr, void *stream);
+ * which writes r(u(x_q), x_q) and dr/du at every volume quadrature point q
+ * (either output may be NULL; launched on `stream`; 0 on success).
+ * fem_op_set_reaction_plugin loads it (NULL unloads): the operator is then
+ * nonlinear, S[u] = A u + int r(u, x) v (plus any law above), and
+ * fem_op_jacobian adds dr/du(u) to the Jacobian's cvol.  FP64 DG operators
+ * with constant or cellwise kappa. */
+typedef struct {
+    int n[3];          /* cells per direction                               */
+    int m;             /* points per direction (p + 1)                      */
+    double h[3];       /* cell size per direction                           */
+    double gx[8];      /* Gauss points on [0, 1]                            */
+    int64_t nq;        /* E m^3 quadrature points, [E][m^3], x fastest      */
+} fem_qp_geometry;
+typedef int (*fem_reaction_eval_fn)(const fem_qp_geometry *g, const double *uq, double *r, double *dr,
+                                    void *stream);
+int fem_op_set_reaction_plugin(fem_op *op, const char *path);
 int fem_op_nonlinear(const fem_op *op);
 int fem_op_eval_qp(fem_op *op, const fem_vec *u, fem_vec *uq);
 int fem_op_jacobian_create(const fem_op *op, fem_op **jop);

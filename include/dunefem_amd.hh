@@ -208,6 +208,12 @@ public:
         check(fem_op_set_reaction_qp(h_, cvol.empty() ? nullptr : cvol.data()));
     }
     void setReactionLaw(double coef, int pw) { check(fem_op_set_reaction_law(h_, coef, pw)); }
+    // a compiled pointwise reaction r(u, x) (a library written by
+    // dunefem_amd.forms); empty path unloads it
+    void setReactionPlugin(const std::string &path)
+    {
+        check(fem_op_set_reaction_plugin(h_, path.empty() ? nullptr : path.c_str()));
+    }
     // u at the volume quadrature points [E][m^3] (x fastest)
     void evalQp(const DiscreteFunction &u, DiscreteFunction &uq) const { check(fem_op_eval_qp(h_, u.handle(), uq.handle())); }
 

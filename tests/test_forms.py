@@ -1,0 +1,187 @@
+"""Compiled pointwise integrands (dunefem_amd.forms): a reaction r(u, x) of
+-div(kappa grad u) + r(u, x) = f written as HIP code with its symbolic
+derivative, compiled into a library the operator loads.
+
+CPU: the symbolic derivative against central differences of the numpy
+evaluation, the quadrature-point coordinates against the workload layout,
+and the compiled libraries of forms.PREBUILT (they load and name their
+expression).  GPU: S[u] = A u + int r(u_h, x) v and the Jacobian against the
+oracle (A u by the oracle's apply, the reaction by numpy at the oracle's
+quadrature-point values and the oracle's load-vector assembly), the compiled
+u^3 against the built-in law, and Newton / scheme solves of manufactured
+problems.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import pyoracle as ora
+from dunefem_amd import forms
+from dunefem_amd import workloads as wl
+
+TOL64 = 1e-12
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("r", forms.PREBUILT)
+def test_symbolic_derivative_matches_central_difference(r):
+    rng = np.random.default_rng(0)
+    uq, px, py, pz = rng.uniform(-1.5, 1.5, 50), rng.uniform(0, 1, 50), rng.uniform(0, 1, 50), rng.uniform(0, 1, 50)
+    h = 1e-5
+    rp, _ = forms.evaluate(r, uq + h, px, py, pz)
+    rm, _ = forms.evaluate(r, uq - h, px, py, pz)
+    _, dr = forms.evaluate(r, uq, px, py, pz)
+    assert np.abs((rp - rm) / (2 * h) - dr).max() < 1e-6 * max(1.0, np.abs(dr).max())
+
+
+def test_unknown_symbols_are_rejected():
+    with pytest.raises(ValueError):
+        forms.as_expr("u**2 + t")
+
+
+def test_qp_coordinates_match_the_workload_layout():
+    n, m = (3, 4, 2), 3
+    gx, _ = ora.gauss(m)
+    X, Y, Z = forms.qp_coordinates(n, m, gx)
+    cx, cy, cz = wl.qp_coords(n[0], m), wl.qp_coords(n[1], m), wl.qp_coords(n[2], m)
+    zz, yy, xx = np.meshgrid(cz, cy, cx, indexing="ij")
+    for got, grid in ((X, xx), (Y, yy), (Z, zz)):
+        assert np.abs(got - wl.vol_layout3(grid, n, m)).max() < 1e-15
+
+
+@pytest.mark.parametrize("r", forms.PREBUILT)
+def test_prebuilt_libraries_load_and_name_their_expression(r):
+    path = forms.compile_reaction(r, build=False)
+    lib = C.CDLL(path)
+    f = lib.fem_reaction_expression
+    f.restype = C.c_char_p
+    assert f().decode() == str(forms.as_expr(r))
+    assert hasattr(lib, "fem_reaction_eval")
+    with pytest.raises(FileNotFoundError):
+        forms.compile_reaction("u**5 + 7*u", build=False)
+
+
+# ---- GPU --------------------------------------------------------------------------------
+def _oracle_S_J(n, p, kc, r, u, w):
+    """S[u] and J(u) w on the oracle: A u + int r(u_h, x) v and A w + int r'(u_h) w v"""
+    oop = ora.DGOperator(n, p, 10.0 * p * p, kappa=("cell", kc))
+    m = p + 1
+    X, Y, Z = forms.qp_coordinates(n, m, ora.gauss(m)[0])
+    uq = oop.eval_qp(u)
+    rq, drq = forms.evaluate(r, uq, X, Y, Z)
+    S = oop.apply(u) + oop.rhs(rq)
+    oop.set_reaction_qp(drq)
+    J = oop.apply(w)
+    return S, J
+
+
+def _dg(ctx, fa, n, p, kc, precision=None):
+    precision = fa.F64 if precision is None else precision
+    op = fa.GalerkinOperator(ctx, fa.OP_DG_SIPG_3D, n, p, 10.0 * p * p, (0, 0, 0), 0.0, fa.KAPPA_CELL,
+                             precision=precision)
+    op.set_kappa_cell(kc)
+    return op
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("r", forms.PREBUILT)
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_gpu_compiled_reaction_and_jacobian_match_oracle(ctx, r, p):
+    import dunefem_amd as fa
+    n, rng = (4, 3, 5), np.random.default_rng(30 + p)
+    kc = rng.uniform(0.5, 2.0, size=60)
+    op = _dg(ctx, fa, n, p, kc)
+    op.set_reaction(r, build=False)
+    assert op.nonlinear()
+    u, w = rng.normal(size=op.N), rng.normal(size=op.N)
+    S_ref, J_ref = _oracle_S_J(n, p, kc, r, u, w)
+    U, Wv, Y = op.vector(u), op.vector(w), op.vector()
+    op(U, Y)
+    errS = rel(Y.download(), S_ref)
+    J = op.jacobian(U)
+    J(Wv, Y)
+    errJ = rel(Y.download(), J_ref)
+    print("r", r, "p", p, "S rel err", errS, "J rel err", errJ, flush=True)
+    assert errS < TOL64 and errJ < TOL64
+
+
+@pytest.mark.gpu
+def test_gpu_compiled_cubic_equals_builtin_law(ctx):
+    # the generated u^3 and the law fused in the apply kernel give the same S
+    # and J (different code paths: quadrature-point kernel + load-vector
+    # assembly against the term inside the apply)
+    import dunefem_amd as fa
+    n, p, rng = (5, 4, 3), 2, np.random.default_rng(5)
+    kc = rng.uniform(0.5, 2.0, size=60)
+    gen, law = _dg(ctx, fa, n, p, kc), _dg(ctx, fa, n, p, kc)
+    gen.set_reaction("u**3", build=False)
+    law.set_reaction_law(1.0, 2)
+    U, Wv = gen.vector(rng.normal(size=gen.N)), gen.vector(rng.normal(size=gen.N))
+    a, b = gen.vector(), gen.vector()
+    gen(U, a)
+    law(U, b)
+    assert rel(a.download(), b.download()) < 1e-13
+    gen.jacobian(U)(Wv, a)
+    law.jacobian(U)(Wv, b)
+    assert rel(a.download(), b.download()) < 1e-13
+
+
+@pytest.mark.gpu
+def test_gpu_compiled_reaction_errors(ctx):
+    import dunefem_amd as fa
+    n, p = (3, 3, 3), 2
+    kc = np.ones(27)
+    with pytest.raises(fa.FemError):
+        _dg(ctx, fa, n, p, kc, fa.F32).set_reaction("exp(u)", build=False)
+    op = _dg(ctx, fa, n, p, kc)
+    with pytest.raises(fa.FemError):   # a library without fem_reaction_eval
+        fa._check(fa._lib.fem_op_set_reaction_plugin(op.h, fa.LIBPATH.encode()))
+    op.set_reaction("exp(u)", build=False)
+    op.set_reaction(None)
+    assert not op.nonlinear()
+
+
+@pytest.mark.gpu
+def test_gpu_newton_with_compiled_reaction_returns_manufactured_solution(ctx):
+    # -div(kappa grad u) + sinh(u) (2 + cos cos cos) = f: with F = S[u*] for
+    # a given u*, Newton from 0 returns u*; then the scheme with the source
+    # given as an expression (its load vector against the oracle's) ends
+    # with a residual below its tolerance
+    import dunefem_amd as fa
+    r = forms.PREBUILT[3]
+    n, p = (6, 5, 4), 2
+    rng = np.random.default_rng(6)
+    kc = rng.uniform(0.5, 2.0, size=120)
+    op = _dg(ctx, fa, n, p, kc)
+    op.set_reaction(r, build=False)
+    ustar = op.vector(0.2 * rng.normal(size=op.N))
+    F = op.vector()
+    op(ustar, F)
+    tol = 1e-10 * np.sqrt(F.normSquaredDofs())
+    inv = fa.NewtonInverseOperator(ctx, {"nonlinear.tolerance": tol},
+                                   linear=dict(method=fa.CG, tolerance=1e-12, errormeasure=fa.RELATIVE,
+                                               preconditioning=fa.PRE_BLOCK_JACOBI))
+    inv.bind(op)
+    W = op.vector()
+    inv(F, W)
+    info = inv.info()
+    print("newton with the compiled reaction", info, flush=True)
+    assert info["converged"] and 2 <= info["iterations"] <= 12
+    assert rel(W.download(), ustar.download()) < 1e-8
+    # scheme: source f = 1 + x y z as an expression, S[u] = F
+    Fx = op.rhs_expr("1 + x*y*z")
+    oop = ora.DGOperator(n, p, 10.0 * p * p, kappa=("cell", kc))
+    X, Y, Z = op.qp_coordinates()
+    assert rel(Fx.download(), oop.rhs(1 + X * Y * Z)) < 1e-13
+    scheme = fa.Scheme(op, Fx, parameters={"fem.solver.nonlinear.tolerance": 1e-10 * np.sqrt(Fx.normSquaredDofs())},
+                       method=fa.CG, tolerance=1e-12, errormeasure=fa.RELATIVE,
+                       preconditioning=fa.PRE_BLOCK_JACOBI)
+    Us = op.vector()
+    si = scheme.solve(Us)
+    R = op.vector()
+    scheme(Us, R)
+    assert si["converged"] and np.sqrt(R.normSquaredDofs()) < 1e-10 * np.sqrt(Fx.normSquaredDofs())
