@@ -29,6 +29,9 @@ sys.path.insert(0, os.path.join(ROOT, "dune-fem_amd"))
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
+# CG steps captured in a HIP graph (off under --no-graph, for profiler runs:
+# rocprofv3's kernel trace does not follow graph launches reliably)
+GRAPH = 1
 
 
 def parse():
@@ -46,6 +49,7 @@ def parse():
     ap.add_argument("--no-combined", action="store_true", help="skip the 96^3 Q4 solves")
     ap.add_argument("--no-nonlinear", action="store_true", help="skip the Newton solves")
     ap.add_argument("--nonlinear-only", action="store_true", help="only the Newton section (profiling)")
+    ap.add_argument("--no-graph", action="store_true", help="no HIP graph capture of CG steps (profiler runs)")
     ap.add_argument("--apply-only", choices=["c1", "c3", "c4"], default=None,
                     help="time only the operator apply of one workload (for profiling): c1 = 64^3 Q3 pointwise "
                          "kappa with advection, c3 = 48^3 Q2 channels, c4 = 96^3 Q4 log-normal")
@@ -209,11 +213,11 @@ def solve_section(fa, wl, ctx, n=48, pairs=2000, epochs=3, lr=1e-4, learned=True
     out["multigrid_setup_s_three_precisions"] = round(time.perf_counter() - t0, 3)
     run("fp64_cg_bj", fa.CG, bj64)
     run("fp64_fgmres_bj", fa.FGMRES, bj64, maxit=3000)
-    run("fp64_cg_mg", fa.CG, mg64, graph=1)
+    run("fp64_cg_mg", fa.CG, mg64, graph=GRAPH)
     # FP64 CG whose V-cycle runs in FP32 (rounded in, widened out)
-    run("mixed_fp64cg_fp32mg", fa.CG, mg32, graph=1)
+    run("mixed_fp64cg_fp32mg", fa.CG, mg32, graph=GRAPH)
     run("mixed_fp32_cg_mg_ir", fa.CG, mg32, op32, refinement=1, inner_tolerance=1e-3, inner_maxiterations=2000,
-        graph=1)
+        graph=GRAPH)
     # the V-cycle on the BF16-MFMA operator: the inner Krylov solve keeps the
     # FP32 operator; FGMRES, since BF16 rounding makes the V-cycle slightly
     # nonlinear
@@ -260,12 +264,12 @@ def combined_section(fa, wl, ctx, n=96, learned=True, pairs=400, epochs=2, lr=1e
     mg64 = fa.Preconditioner(op64, fa.PRE_MULTIGRID)
     ctx.synchronize()
     out["multigrid_setup_s_fp64"] = round(time.perf_counter() - t0, 3)
-    run("fp64_cg_mg", fa.CG, mg64, graph=1)
+    run("fp64_cg_mg", fa.CG, mg64, graph=GRAPH)
     op32 = _dg_cell(fa, ctx, P, fa.F32)
     mg32 = fa.Preconditioner(op32, fa.PRE_MULTIGRID)
-    run("mixed_fp64cg_fp32mg", fa.CG, mg32, graph=1)
+    run("mixed_fp64cg_fp32mg", fa.CG, mg32, graph=GRAPH)
     run("mixed_fp32_cg_mg_ir", fa.CG, mg32, op32, refinement=1, inner_tolerance=1e-3, inner_maxiterations=2000,
-        graph=1)
+        graph=GRAPH)
     # (a V-cycle on the BF16-MFMA operator does not converge at this size:
     # BF16 rounding noise swamps the smooth residual; not run here)
     if learned:
@@ -337,9 +341,9 @@ def nonlinear_section(fa, wl, ctx, n=48, p=2, c3=1.0e4, amp=50.0, cpu=True):
     mg32 = fa.Preconditioner(lin32, fa.PRE_MULTIGRID)
     ctx.synchronize()
     out["multigrid_setup_s"] = round(time.perf_counter() - t0, 3)
-    run("newton_cg_mg_diffusion", {}, dict(exact, graph=1), mg64)
+    run("newton_cg_mg_diffusion", {}, dict(exact, graph=GRAPH), mg64)
     run("newton_ew_cg_mg32_diffusion", {"nonlinear.forcing": "eisenstatwalker"},
-        dict(method=fa.CG, maxiterations=20000, graph=1), mg32)
+        dict(method=fa.CG, maxiterations=20000, graph=GRAPH), mg32)
     out["runs"] = runs
     base = runs["newton_cg_bj"]["time_s"]
     out["speedup_vs_newton_cg_bj"] = {k: round(base / v["time_s"], 2) for k, v in runs.items()}
@@ -522,7 +526,10 @@ def solve_only(a, fa, wl):
 
 
 def main():
+    global GRAPH
     a = parse()
+    if a.no_graph:
+        GRAPH = 0
     if a.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X (--gpus %d is not supported)"
               % a.gpus, file=sys.stderr)
