@@ -107,3 +107,38 @@ def test_every_reference_solver_key_is_accepted():
             fa.solver_parameters(**{k: v})
     d = fa.solver_parameters()
     assert d.preconditioning == fa.PRE_NONE and d.relaxation == 1.1 and d.precond_iterations == 1
+
+
+def test_every_newton_key_is_accepted():
+    # NewtonParameter's keys (dune/fem/solver/newtoninverseoperator.hh:101-380)
+    # with their defaults, the deprecated prefix newton. and lineSearch
+    # spelling, and the split of scheme parameters between Newton and its
+    # linear solver
+    import pytest
+    d = fa.newton_parameters()
+    assert d.tolerance == 1e-6 and d.maxiterations == 2 ** 31 - 1 and d.maxlineariterations < 0
+    assert d.linesearch == fa.LINESEARCH_NONE and d.forcing == fa.FORCING_NONE and d.simplified == 0
+    assert d.eisenstatwalker_etamax == 0.99 and d.eisenstatwalker_gamma == 0.1
+    p = fa.newton_parameters({
+        "fem.solver.nonlinear.tolerance": "1e-9", "fem.solver.nonlinear.maxiterations": "25",
+        "fem.solver.nonlinear.maxlineariterations": "4000", "fem.solver.nonlinear.linesearch": "simple",
+        "fem.solver.nonlinear.maxlinesearchiterations": "7", "nonlinear.forcing": "eisenstatwalker",
+        "nonlinear.eisenstatwalker.etamax": "0.5", "nonlinear.eisenstatwalker.gamma": "0.3",
+        "nonlinear.simplified": "true", "newton.verbose": "false"})
+    assert (p.tolerance, p.maxiterations, p.maxlineariterations, p.linesearch, p.maxlinesearchiterations) == \
+        (1e-9, 25, 4000, fa.LINESEARCH_SIMPLE, 7)
+    assert p.forcing == fa.FORCING_EISENSTATWALKER and p.eisenstatwalker_etamax == 0.5
+    assert p.eisenstatwalker_gamma == 0.3 and p.simplified == 1 and p.verbose == 0
+    assert fa.newton_parameters({"nonlinear.lineSearch": "simple"}).linesearch == fa.LINESEARCH_SIMPLE
+    assert fa.newton_parameters({"nonlinear.linear.tolerance.strategy": "eisenstatwalker"}).forcing == \
+        fa.FORCING_EISENSTATWALKER
+    for k, v in (("nonlinear.tolerance", "0"), ("nonlinear.linesearch", "wolfe"), ("nonlinear.forcing", "x"),
+                 ("nonlinear.linear.tolerance", "1e-8"), ("fem.solver.tolerance", "1e-8"),
+                 ("nonlinear.bogus", "1"), ("nonlinear.maxiterations", "-1")):
+        with pytest.raises(fa.FemError):
+            fa.newton_parameters({k: v})
+    # scheme parameters: nonlinear.* to Newton, the rest (and nonlinear.linear.*) to the linear solver
+    assert fa._is_newton_key("fem.solver.nonlinear.tolerance") and fa._is_newton_key("newton.linesearch")
+    assert fa._is_newton_key("fem.solver.nonlinear.linear.tolerance.strategy")
+    assert not fa._is_newton_key("fem.solver.nonlinear.linear.method")
+    assert not fa._is_newton_key("fem.solver.method") and not fa._is_newton_key("linear.tolerance")
