@@ -292,8 +292,9 @@ def nonlinear_section(fa, wl, ctx, n=48, p=2, c3=1.0e4, amp=50.0, cpu=True):
     cubic reaction at the volume quadrature points.  Newton (the reference's
     NewtonInverseOperator loop) to |S[w] - F| < 1e-8 |F| from w = 0 with
     matrix-free Jacobians: CG + block Jacobi rebuilt on each Jacobian, the
-    same with Eisenstat-Walker forcing, and CG with a multigrid built once on
-    the diffusion part (FP64, and FP32 widened into the FP64 CG).  Also the
+    same with Eisenstat-Walker forcing, CG with a multigrid built once on
+    the diffusion part (FP64, and FP32 widened into the FP64 CG), and FP64
+    refinement around FP32 inner CG on an FP32 twin of each Jacobian.  Also the
     apply rates of S[u], of the Jacobian and of the linear part, with their
     algorithmic bytes against 8 TB/s, and the oracle's S[u] on the host."""
     P = wl.problem4(n=n, p=p, sigma=1.0)
@@ -344,6 +345,11 @@ def nonlinear_section(fa, wl, ctx, n=48, p=2, c3=1.0e4, amp=50.0, cpu=True):
     run("newton_cg_mg_diffusion", {}, dict(exact, graph=GRAPH), mg64)
     run("newton_ew_cg_mg32_diffusion", {"nonlinear.forcing": "eisenstatwalker"},
         dict(method=fa.CG, maxiterations=20000, graph=GRAPH), mg32)
+    # each Newton system by FP64 refinement around FP32 inner CG on an FP32
+    # twin of the Jacobian, the FP32 multigrid of the diffusion part inside
+    run("newton_ew_ir_fp32cg_mg32_diffusion", {"nonlinear.forcing": "eisenstatwalker"},
+        dict(method=fa.CG, maxiterations=20000, refinement=1, inner_tolerance=1e-2, inner_maxiterations=2000,
+             graph=GRAPH), mg32)
     out["runs"] = runs
     base = runs["newton_cg_bj"]["time_s"]
     out["speedup_vs_newton_cg_bj"] = {k: round(base / v["time_s"], 2) for k, v in runs.items()}

@@ -55,7 +55,6 @@ void Newton::init(Context *c, const fem_newton_params &p, const fem_solver_param
     FEM_CHECK(p.forcing != FEM_FORCING_EISENSTATWALKER ||
                   (p.eisenstatwalker_etamax > 0 && p.eisenstatwalker_etamax < 1 && p.eisenstatwalker_gamma > 0),
               FEM_ERR_ARG, "newton: Eisenstat-Walker etamax in (0, 1), gamma > 0");
-    FEM_CHECK(!linear.refinement, FEM_ERR_UNSUPPORTED, "newton: the linear solver runs without refinement");
     prm = p;
     lin.init(c, linear);
 }
@@ -166,6 +165,19 @@ void Newton::solve(const Vec *u, Vec &w, fem_newton_info &info)
         jop.reset(new Operator());
         op->make_jacobian(*jop);
     }
+    // a linear solver with FP64 refinement solves each Newton system by FP32
+    // inner solves on an FP32 twin of S'(w) (the true residual and the
+    // update stay FP64); a preconditioner given at bind serves the inner
+    // solves
+    const bool ir = lin.prm.refinement != 0;
+    if (ir) {
+        FEM_CHECK(op->is_dg() && op->desc.precision == FEM_F64, FEM_ERR_UNSUPPORTED,
+                  "newton with refinement: FP64 DG operators");
+        jop32.reset(new Operator());
+        op->make_jacobian(*jop32, FEM_F32);
+    } else {
+        jop32.reset();
+    }
     // the linear solver's tolerance and iteration limit change per step:
     // restored, and the solver unbound, however the solve ends
     struct Restore {
@@ -198,11 +210,13 @@ void Newton::solve(const Vec *u, Vec &w, fem_newton_info &info)
         const auto tj = clk::now();
         if (compute_jacobian && nonlinear) {
             op->jacobian(w, *J);
+            if (ir) op->jacobian(w, *jop32);
             compute_jacobian = !prm.simplified;
         }
         // bind S'(w) (and the preconditioner it builds from
         // fem.solver.preconditioning.method, or the one given)
-        lin.bind(J, pre, nullptr, nullptr);
+        if (ir) lin.bind(J, nullptr, jop32.get(), pre);
+        else lin.bind(J, pre, nullptr, nullptr);
         FEM_HIP(hipStreamSynchronize(ctx->stream));
         t_jac += secs(tj);
         const int left = max_linear() - linear_iterations;

@@ -21,6 +21,7 @@ struct Newton {
     Operator *op = nullptr;           // bound operator S
     Precond *pre = nullptr;           // optional preconditioner of every linear solve
     std::unique_ptr<Operator> jop;    // S'(w) of a nonlinear S
+    std::unique_ptr<Operator> jop32;  // its FP32 twin (linear solver with refinement)
     DevBuf res, dw;
     EisenstatWalker ew;
     // state of the last solve

@@ -270,11 +270,15 @@ void Operator::eval_qp(const Vec &u, Vec &out) const
         dg_eval_qp<float>(a32, k, (const float *)u.data, nullptr, 1.0f, 1, (float *)out.data, ctx->stream);
 }
 
-void Operator::make_jacobian(Operator &jop) const
+void Operator::make_jacobian(Operator &jop, int precision) const
 {
     reaction_checks(*this);
+    FEM_CHECK(precision < 0 || precision == desc.precision || (desc.precision == FEM_F64 && precision == FEM_F32),
+              FEM_ERR_ARG, "jacobian operator: the operator's precision or FP32 for an FP64 operator");
     hipStream_t s = ctx->stream;
-    jop.init(ctx, desc);
+    fem_op_desc d = desc;
+    if (precision >= 0) d.precision = precision;
+    jop.init(ctx, d);
     for (int d = 0; d < 3; ++d) {
         copy_buf(jop.kface64[d], kface64[d], s);
         copy_buf(jop.kface32[d], kface32[d], s);
@@ -299,7 +303,9 @@ void Operator::make_jacobian(Operator &jop) const
 void Operator::jacobian(const Vec &u, Operator &jop) const
 {
     reaction_checks(*this);
-    FEM_CHECK(jop.is_dg() && jop.desc.precision == desc.precision && jop.k == k && jop.E == E &&
+    FEM_CHECK(jop.is_dg() &&
+                  (jop.desc.precision == desc.precision || (desc.precision == FEM_F64 && jop.desc.precision == FEM_F32)) &&
+                  jop.k == k && jop.E == E &&
                   jop.desc.n[0] == desc.n[0] && jop.desc.n[1] == desc.n[1] && jop.desc.n[2] == desc.n[2] &&
                   jop.desc.kappa_mode == desc.kappa_mode && jop.a64.kappa_mode == a64.kappa_mode,
               FEM_ERR_ARG, "jacobian: the Jacobian operator must be made from this operator");

@@ -87,11 +87,14 @@ struct Operator {
     // of the same precision [E][k^3]
     void eval_qp(const Vec &u, Vec &out) const;
     // the Jacobian at u into jop (an operator made by make_jacobian from
-    // this one): cvol_J = cvol + (pw + 1) nl_coef u^pw, no law
+    // this one, in its precision or FP32): cvol_J = cvol + (pw + 1) nl_coef
+    // u^pw (+ dr/du of a compiled reaction), no law
     void jacobian(const Vec &u, Operator &jop) const;
     // jop: a linear operator of the same description and coefficient data
-    // (kappa, boundary data, cvol), device copies
-    void make_jacobian(Operator &jop) const;
+    // (kappa, boundary data, cvol), device copies; precision >= 0 makes it
+    // in that precision (FP32 for an FP64 operator: the inner operator of a
+    // refinement solve)
+    void make_jacobian(Operator &jop, int precision = -1) const;
     // y = A x on raw device pointers of the operator's vector precision;
     // `dot` (DG only, may be null): per-workgroup FP64 partials of <x, y>,
     // dot_blocks() entries

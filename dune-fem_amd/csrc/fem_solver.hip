@@ -983,6 +983,7 @@ void Solver::solve(const Vec &b, Vec &x, fem_solver_info &info)
         const double bn = std::sqrt(vec_dot(b, b));
         int total = 0, outer = 0;
         bool ok = false;
+        double rn0 = 0;   // the first residual (residualreduction)
         while (true) {
             op->apply_raw(x.data, h, nullptr);
             hipLaunchKernelGGL(k_residual64, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, (const double *)b.data, h, r, N, part, cnt + 2, dres);
@@ -992,7 +993,10 @@ void Solver::solve(const Vec &b, Vec &x, fem_solver_info &info)
             const double rn = std::sqrt(rn2);
             const double rel = bn > 0 ? rn / bn : rn;
             if (prm.verbose) { std::printf("IR outer %d: rel residual %g (inner its %d)\n", outer, rel, total); std::fflush(stdout); }
-            const double target = prm.errormeasure == FEM_ERRMEASURE_RELATIVE ? prm.tolerance * bn : prm.tolerance;
+            if (outer == 0) rn0 = rn;
+            const double target = prm.errormeasure == FEM_ERRMEASURE_RELATIVE    ? prm.tolerance * bn
+                                  : prm.errormeasure == FEM_ERRMEASURE_REDUCTION ? prm.tolerance * rn0
+                                                                                 : prm.tolerance;
             if (rn <= target || !(rn == rn)) { ok = rn == rn; break; }
             if (outer >= 100 || total >= prm.maxiterations) break;
             hipLaunchKernelGGL(k_scale_to32, dim3(G), dim3(256), 0, s, r, r32, N, 1.0 / rn);

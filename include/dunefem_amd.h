@@ -311,7 +311,8 @@ typedef struct {
     /* mixed precision: 0 = plain (solve in the operator's precision);
      * 1 = FP64 iterative refinement around an inner solve in `inner_op`'s
      * precision; the true residual, inner products of the outer loop and the
-     * update are FP64 */
+     * update are FP64 (errormeasure residualreduction: relative to the
+     * first FP64 residual) */
     int refinement;
     double inner_tolerance;   /* relative reduction of each inner solve        */
     int inner_maxiterations;
@@ -397,7 +398,11 @@ int fem_solver_solve(fem_solver *s, const fem_vec *b, fem_vec *x, fem_solver_inf
  * dw = 0 and sets w -= dw (:688-802).  A linear S takes one step and counts
  * no nonlinear iteration (:786-790).  Stopping and failure follow
  * NewtonInverseOperator::failed (:568-584) with the NewtonFailure codes
- * (:389-400). */
+ * (:389-400).  With a linear solver in FP64 refinement (`refinement` = 1;
+ * FP64 DG operators with constant or cellwise kappa) each Newton system is
+ * solved by FP32 inner solves on an FP32 twin of S'(w), the residual and the
+ * update staying FP64; a preconditioner given at bind must then be FP32 and
+ * serves the inner solves. */
 #define FEM_NEWTON_SUCCESS 0
 #define FEM_NEWTON_INVALID_RESIDUAL 1
 #define FEM_NEWTON_ITERATIONS_EXCEEDED 2

@@ -368,6 +368,51 @@ def test_gpu_newton_options_and_failures(ctx):
 
 
 @pytest.mark.gpu
+def test_gpu_newton_with_fp32_refinement_solves(ctx):
+    # each Newton system solved by FP64 refinement around FP32 inner CG on an
+    # FP32 twin of the Jacobian: the same Newton steps as with FP64 solves,
+    # the same solution; with Eisenstat-Walker forcing (residualreduction in
+    # the refinement loop) too, and with a multigrid of the diffusion part
+    # (FP32) as the inner preconditioner given at bind
+    import dunefem_amd as fa
+    n, p, c3 = (6, 6, 6), 2, 10.0
+    rng = np.random.default_rng(12)
+    kc = rng.uniform(0.5, 2.0, size=216)
+    op = _dg(ctx, fa, n, p, kc)
+    op.set_reaction_law(c3, 2)
+    ustar = op.vector(rng.normal(size=op.N))
+    F = op.vector()
+    op(ustar, F)
+    tol = 1e-9 * np.sqrt(F.normSquaredDofs())
+
+    def run(nl, linear, pre=None):
+        inv = fa.NewtonInverseOperator(ctx, dict(nl, **{"nonlinear.tolerance": tol}), linear=linear)
+        inv.bind(op, pre)
+        W = op.vector()
+        inv(F, W)
+        inv.unbind()
+        return inv.info(), W.download()
+
+    exact = dict(method=fa.CG, tolerance=1e-12, errormeasure=fa.RELATIVE, preconditioning=fa.PRE_BLOCK_JACOBI)
+    ref, w_ref = run({}, exact)
+    ir, w_ir = run({}, dict(exact, refinement=1, inner_tolerance=1e-4, inner_maxiterations=2000))
+    print("newton fp64 solves", ref, "with fp32 refinement", ir, flush=True)
+    assert ref["converged"] and ir["converged"] and ir["iterations"] == ref["iterations"]
+    assert rel(w_ir, ustar.download()) < 1e-8 and rel(w_ref, ustar.download()) < 1e-8
+    ew, w_ew = run({"nonlinear.forcing": "eisenstatwalker"},
+                   dict(method=fa.CG, refinement=1, inner_tolerance=1e-3, inner_maxiterations=2000,
+                        preconditioning=fa.PRE_BLOCK_JACOBI))
+    print("eisenstat-walker with fp32 refinement", ew, flush=True)
+    assert ew["converged"] and ew["residual"] < tol and rel(w_ew, ustar.download()) < 1e-6
+    lin32 = _dg(ctx, fa, n, p, kc, precision=fa.F32)
+    mg32 = fa.Preconditioner(lin32, fa.PRE_MULTIGRID)
+    mg, w_mg = run({"nonlinear.forcing": "eisenstatwalker"},
+                   dict(method=fa.CG, refinement=1, inner_tolerance=1e-3, inner_maxiterations=2000), mg32)
+    print("eisenstat-walker, fp32 refinement, fp32 multigrid of the diffusion part", mg, flush=True)
+    assert mg["converged"] and mg["residual"] < tol and rel(w_mg, ustar.download()) < 1e-6
+
+
+@pytest.mark.gpu
 def test_gpu_scheme_solves_nonlinear_problem(ctx):
     # the scheme path: S[u] = A u + c3 u^3 with the affine part F of a
     # manufactured u*: scheme.solve from u = 0 returns u*, the info dict has
