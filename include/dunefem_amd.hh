@@ -18,8 +18,9 @@
  *                      with x the initial guess, iterations() < 0 when not
  *                      converged, info(), setMaxIterations, setParameters
  *                      (inverseoperatorinterface.hh:66-166); CgInverseOperator,
- *                      GmresInverseOperator (krylovinverseoperators.hh:280-295)
- *                      and FgmresInverseOperator
+ *                      BicgstabInverseOperator, GmresInverseOperator
+ *                      (krylovinverseoperators.hh:280-317) and
+ *                      FgmresInverseOperator
  *   AffineGalerkinOperator, NewtonParameter, NewtonInverseOperator, FemScheme
  *                      R(u) = S[u] - F and the Newton solve of R(w) = u
  *                      (dune/fem/solver/newtoninverseoperator.hh:101-802,
@@ -426,6 +427,18 @@ public:
     explicit FgmresInverseOperator(Context &ctx, const SolverParameter &p = SolverParameter())
         : InverseOperator(ctx, p, FEM_SOLVER_FGMRES) {}
 };
+class BicgstabInverseOperator : public InverseOperator {
+public:
+    static constexpr int fixedMethod = FEM_SOLVER_BICGSTAB;
+    explicit BicgstabInverseOperator(Context &ctx, const SolverParameter &p = SolverParameter())
+        : InverseOperator(ctx, p, FEM_SOLVER_BICGSTAB) {}
+};
+// the reference's other names for these (krylovinverseoperators.hh:302-317)
+using ParDGGeneralizedMinResInverseOperator = GmresInverseOperator;
+using ParDGBiCGStabInverseOperator = BicgstabInverseOperator;
+using OEMCGOp = CgInverseOperator;
+using OEMBICGSTABOp = BicgstabInverseOperator;
+using OEMGMRESOp = GmresInverseOperator;
 
 // R(u) = S[u] - F: a Galerkin operator with its source term, as the
 // reference's GalerkinOperator evaluates it (galerkin.hh:333-360); nonlinear

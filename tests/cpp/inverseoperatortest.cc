@@ -365,6 +365,7 @@ int main()
         algorithm<CgInverseOperator>(ctx, " === CgInverseOperator === ", nullptr);
         algorithm<GmresInverseOperator>(ctx, " === GmresInverseOperator === ", nullptr);
         algorithm<FgmresInverseOperator>(ctx, " === FgmresInverseOperator === ", nullptr);
+        algorithm<BicgstabInverseOperator>(ctx, " === BicgstabInverseOperator === ", nullptr);
         SolverParameter p;
         p.set("fem.solver.errormeasure", "relative").set("fem.solver.tolerance", "1e-12");
         algorithm<CgInverseOperator>(ctx, " === CgInverseOperator + SolverParameter === ", &p);
