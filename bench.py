@@ -401,9 +401,9 @@ def batch_section(fa, wl, ctx, count=256):
     log("batch: autotuned tile", tx, ty)
     F = B.rhs(np.concatenate([d["fq"] for d in inst]))
     X = B.vector()
-    B.solve(F, X, tol=1e-8)          # warm-up (captures the graph)
+    B.solve(F, X, tol=1e-8, use_graph=bool(GRAPH))          # warm-up (captures the graph)
     X.clear()
-    info = B.solve(F, X, tol=1e-8)
+    info = B.solve(F, X, tol=1e-8, use_graph=bool(GRAPH))
     # per-instance FP64 loop: one operator + CG(Jacobi) solve per problem
     Fh = F.download()
     t1 = time.perf_counter()

@@ -255,32 +255,56 @@ __global__ void __launch_bounds__(256) k_mse(int V, const float *__restrict__ y,
 }
 
 // ---- weight gradients ----------------------------------------------------------------
-// hidden layers: block (tap, chunk), 4 waves = (ci tile, co tile); K = voxels
+// hidden layers: block (tap, chunk), 4 waves = (ci tile, co tile); K =
+// voxels.  Per step of 32 voxels the block stages the tap-shifted input rows
+// (32 channels, BF16) and the output-gradient rows (rounded to BF16) in LDS,
+// transposed to [channel][voxel], so that a lane's MFMA operand (8
+// consecutive voxels of one channel) is one 16-byte LDS read; the four
+// wavefronts take their 16 x 16 tiles from the shared stage.  Each staged
+// row is one coalesced 8- and 16-byte load per thread instead of 16 scalar
+// gathers per lane and step.
+constexpr int WGS = 40;   // LDS row stride in BF16: 32 voxels + 8 (16-byte aligned rows)
 __global__ void __launch_bounds__(256) k_hid_wgrad(Grid g, int dil, const u16 *__restrict__ ain,
                                                    const float *__restrict__ gout, float *__restrict__ wpart,
                                                    int nchunks)
 {
+    __shared__ __attribute__((aligned(16))) u16 At[CH * WGS];
+    __shared__ __attribute__((aligned(16))) u16 Bt[CH * WGS];
     const int t = blockIdx.x, chunk = blockIdx.y;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int tci = wave >> 1, tco = wave & 1;
     const int i = lane & 15, kb = lane >> 4;
+    // staging role of the thread: voxel r of the step, channels 4p..4p+3
+    const int r = threadIdx.x >> 3, p = threadIdx.x & 7;
     const int c0 = chunk * WG_CHUNK;
     f32x4_t acc = { 0.f, 0.f, 0.f, 0.f };
     for (int k0 = c0; k0 < c0 + WG_CHUNK; k0 += 32) {
-        bf16x8_t A, B;
-        Vox pv = vox(g, k0 + 8 * kb);
-#pragma unroll
-        for (int r = 0; r < 8; ++r, vox_next(g, pv)) {
-            const int v = pv.v;
-            float av = 0.f, gv = 0.f;
-            if (v < g.V) {
-                const int w = nbr(g, pv, t, dil, 1);
-                if (w >= 0) av = bf2f(ain[(size_t)w * CH + 16 * tci + i]);
-                gv = gout[(size_t)v * CH + 16 * tco + i];
+        const int v = k0 + r;
+        u16 a4[4] = { 0, 0, 0, 0 }, b4[4] = { 0, 0, 0, 0 };
+        if (v < g.V) {
+            const int w = nbr(g, v, t, dil, 1);
+            if (w >= 0) {
+                const uint2 q = *reinterpret_cast<const uint2 *>(ain + (size_t)w * CH + 4 * p);
+                a4[0] = (u16)(q.x & 0xffffu);
+                a4[1] = (u16)(q.x >> 16);
+                a4[2] = (u16)(q.y & 0xffffu);
+                a4[3] = (u16)(q.y >> 16);
             }
-            A[r] = __builtin_bit_cast(__bf16, f2bf(av));
-            B[r] = f2b(gv);
+            const float4 gv = *reinterpret_cast<const float4 *>(gout + (size_t)v * CH + 4 * p);
+            b4[0] = f2bf(gv.x);
+            b4[1] = f2bf(gv.y);
+            b4[2] = f2bf(gv.z);
+            b4[3] = f2bf(gv.w);
         }
+        __syncthreads();   // the previous step's operand reads are done
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            At[(4 * p + j) * WGS + r] = a4[j];
+            Bt[(4 * p + j) * WGS + r] = b4[j];
+        }
+        __syncthreads();
+        const bf16x8_t A = *reinterpret_cast<const bf16x8_t *>(At + (16 * tci + i) * WGS + 8 * kb);
+        const bf16x8_t B = *reinterpret_cast<const bf16x8_t *>(Bt + (16 * tco + i) * WGS + 8 * kb);
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc, 0, 0, 0);
     }
 #pragma unroll
@@ -290,24 +314,50 @@ __global__ void __launch_bounds__(256) k_hid_wgrad(Grid g, int dil, const u16 *_
     }
 }
 
-// thin layers: block (tap, chunk), thread (ci, co); input FP32 (layer 1) or BF16 (layer 4)
-template <bool BF>
-__global__ void k_thin_wgrad(Grid g, int dil, const void *__restrict__ ain_, int ci_n, int co_n,
-                             const float *__restrict__ gout, float *__restrict__ wpart, int nchunks)
+// thin layers (CI x CO <= 64 weights per tap): block (tap, chunk), the 256
+// threads share the chunk's voxels (a quarter of them each, strided), every
+// thread accumulating all CI x CO products of its voxels; then a fixed-order
+// reduction, by shuffles within each wavefront and over the four wavefronts
+// in LDS.  Input FP32 (layer 1) or BF16 (layer 4); the output gradient is
+// rounded to BF16 as the MFMA layers see it.
+template <bool BF, int CI, int CO>
+__global__ void __launch_bounds__(256) k_thin_wgrad(Grid g, int dil, const void *__restrict__ ain_,
+                                                    const float *__restrict__ gout, float *__restrict__ wpart,
+                                                    int nchunks)
 {
+    constexpr int NW = CI * CO;
+    static_assert(NW <= 64, "thin layer: at most 64 weights per tap");
     const int t = blockIdx.x, chunk = blockIdx.y;
-    const int ci = threadIdx.x / co_n, co = threadIdx.x % co_n;
-    if (ci >= ci_n) return;
-    const int c0 = chunk * WG_CHUNK;
-    float acc = 0.f;
-    Vox pv = vox(g, c0);
-    for (int v = c0; v < c0 + WG_CHUNK && v < g.V; ++v, vox_next(g, pv)) {
-        const int w = nbr(g, pv, t, dil, 1);
+    const int c0 = chunk * WG_CHUNK, c1 = min(c0 + WG_CHUNK, g.V);
+    float acc[NW];
+#pragma unroll
+    for (int k = 0; k < NW; ++k) acc[k] = 0.f;
+    for (int v = c0 + (int)threadIdx.x; v < c1; v += 256) {
+        const int w = nbr(g, v, t, dil, 1);
         if (w < 0) continue;
-        const float av = BF ? bf2f(((const u16 *)ain_)[(size_t)w * ci_n + ci]) : ((const float *)ain_)[(size_t)w * ci_n + ci];
-        acc = fmaf(av, bf2f(f2bf(gout[(size_t)v * co_n + co])), acc);
+        float a[CI], gg[CO];
+#pragma unroll
+        for (int ci = 0; ci < CI; ++ci)
+            a[ci] = BF ? bf2f(((const u16 *)ain_)[(size_t)w * CI + ci]) : ((const float *)ain_)[(size_t)w * CI + ci];
+#pragma unroll
+        for (int co = 0; co < CO; ++co) gg[co] = bf2f(f2bf(gout[(size_t)v * CO + co]));
+#pragma unroll
+        for (int ci = 0; ci < CI; ++ci)
+#pragma unroll
+            for (int co = 0; co < CO; ++co) acc[ci * CO + co] = fmaf(a[ci], gg[co], acc[ci * CO + co]);
     }
-    wpart[(((size_t)t * nchunks + chunk) * ci_n + ci) * co_n + co] = acc;
+    __shared__ float red[4][NW];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+        float s = acc[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        if (lane == 0) red[wave][k] = s;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < NW; k += 256)
+        wpart[((size_t)t * nchunks + chunk) * NW + k] = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
 }
 
 // grad[t][ci][co] += sum over chunks (fixed order)
@@ -443,7 +493,7 @@ void CNN::backward_from_g4(hipStream_t s)
     Grid g{ n[0], n[1], n[2], (int)V };
     const unsigned gv = (unsigned)((V + 255) / 256), gw = (unsigned)((V + 63) / 64);
     // layer 4 (32 -> 1)
-    hipLaunchKernelGGL((k_thin_wgrad<true>), dim3(27, nchunks), dim3(CH), 0, s, g, dil[3], (const void *)a3.p, CH, 1,
+    hipLaunchKernelGGL((k_thin_wgrad<true, CH, 1>), dim3(27, nchunks), dim3(256), 0, s, g, dil[3], (const void *)a3.p,
                        g4.as<float>(), wpart.as<float>(), nchunks);
     hipLaunchKernelGGL(k_wgrad_reduce, dim3((27 * CH + 255) / 256), dim3(256), 0, s, wpart.as<float>(), nchunks, CH,
                        grad.as<float>() + woff[3]);
@@ -464,8 +514,8 @@ void CNN::backward_from_g4(hipStream_t s)
     hipLaunchKernelGGL(k_hid_bwd, dim3(gw), dim3(256), 0, s, g, dil[1], g2.as<float>(), wbwd.as<u16>(), a1.as<u16>(),
                        g1.as<float>());
     // layer 1 (2 -> 32)
-    hipLaunchKernelGGL((k_thin_wgrad<false>), dim3(27, nchunks), dim3(2 * CH), 0, s, g, dil[0], (const void *)x0.p, 2,
-                       CH, g1.as<float>(), wpart.as<float>(), nchunks);
+    hipLaunchKernelGGL((k_thin_wgrad<false, 2, CH>), dim3(27, nchunks), dim3(256), 0, s, g, dil[0], (const void *)x0.p,
+                       g1.as<float>(), wpart.as<float>(), nchunks);
     hipLaunchKernelGGL(k_wgrad_reduce, dim3((27 * 2 * CH + 255) / 256), dim3(256), 0, s, wpart.as<float>(), nchunks,
                        2 * CH, grad.as<float>() + woff[0]);
     FEM_HIP(hipGetLastError());
