@@ -813,21 +813,54 @@ void cnn_train(CNN &c, int epochs, int batch, float lr, float val_frac, unsigned
               "train: epochs >= 0, batch >= 1, 0 <= val_frac < 1");
     hipStream_t s = c.ctx->stream;
     const int E = (int)c.V, P = c.ds_n;
-    // targets c*_p = A_c^-1 R r_p by coarse CG to a relative (r, z) of 1e-12
+    // targets c*_p = A_c^-1 R r_p by coarse CG to a relative (r, z) of 1e-12.
+    // The pairs' solves are independent chains of small latency-bound
+    // launches: NS of them run at a time, each on a stream of its own
+    // (default priority), each checked every 32 steps as one solve alone.
     DevBuf tg;
     tg.alloc(sizeof(float) * (size_t)P * E);
-    CoarseCG cg;
-    cg.init(&c);
-    for (int p = 0; p < P; ++p) {
-        cg.start<float>(c.ds_r.as<float>() + (size_t)p * E, 1, nullptr, nullptr, 0.f, s, nullptr);
-        for (int it = 0; it < 100000; it += 32) {
-            for (int k = 0; k < 32; ++k) cg.step(s, nullptr);
-            double gnow, gfirst;
-            cg.gammas(s, gnow, gfirst);
-            if (!(gnow > 1e-12 * gfirst)) break;
+    constexpr int NS = 4;
+    struct Streams {
+        hipStream_t h[NS] = {};
+        ~Streams()
+        {
+            for (hipStream_t x : h)
+                if (x) {
+                    (void)hipStreamSynchronize(x);
+                    (void)hipStreamDestroy(x);
+                }
         }
-        FEM_HIP(hipMemcpyAsync(tg.as<float>() + (size_t)p * E, cg.x.p, sizeof(float) * E, hipMemcpyDeviceToDevice, s));
+    } ss;
+    CoarseCG cgs[NS];
+    for (int i = 0; i < NS; ++i) {
+        FEM_HIP(hipStreamCreate(&ss.h[i]));
+        cgs[i].init(&c);
     }
+    FEM_HIP(hipStreamSynchronize(s));   // the pairs' residuals are in place
+    for (int p0 = 0; p0 < P; p0 += NS) {
+        const int nb = std::min(NS, P - p0);
+        bool done[NS] = {};
+        for (int i = 0; i < nb; ++i)
+            cgs[i].start<float>(c.ds_r.as<float>() + (size_t)(p0 + i) * E, 1, nullptr, nullptr, 0.f, ss.h[i], nullptr);
+        for (int it = 0; it < 100000; it += 32) {
+            for (int i = 0; i < nb; ++i)
+                if (!done[i])
+                    for (int k = 0; k < 32; ++k) cgs[i].step(ss.h[i], nullptr);
+            bool all = true;
+            for (int i = 0; i < nb; ++i) {
+                if (done[i]) continue;
+                double gnow, gfirst;
+                cgs[i].gammas(ss.h[i], gnow, gfirst);
+                if (!(gnow > 1e-12 * gfirst)) done[i] = true;
+                else all = false;
+            }
+            if (all) break;
+        }
+        for (int i = 0; i < nb; ++i)
+            FEM_HIP(hipMemcpyAsync(tg.as<float>() + (size_t)(p0 + i) * E, cgs[i].x.p, sizeof(float) * E,
+                                   hipMemcpyDeviceToDevice, ss.h[i]));
+    }
+    for (int i = 0; i < NS; ++i) FEM_HIP(hipStreamSynchronize(ss.h[i]));
     DevBuf st;
     st.alloc(sizeof(double) * 3 * P);
     hipLaunchKernelGGL(k_pair_stats, dim3(P), dim3(256), 0, s, c.ds_r.as<float>(), tg.as<float>(), E, st.as<double>());
