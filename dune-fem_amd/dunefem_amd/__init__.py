@@ -952,3 +952,10 @@ class Scheme:
         i = self.inv.info()
         return {"converged": i["converged"], "iterations": i["iterations"],
                 "linear_iterations": i["linear_iterations"], "timing": i["timing"]}
+
+
+def galerkin(form, space, solver="cg", parameters=None, build=True):
+    """dune.fem.scheme.galerkin for the integrands of dunefem_amd.forms.Form
+    on a dunefem_amd.forms.Space (see forms.galerkin)"""
+    from .forms import galerkin as _galerkin
+    return _galerkin(form, space, solver, parameters, build)

@@ -185,3 +185,33 @@ def test_gpu_newton_with_compiled_reaction_returns_manufactured_solution(ctx):
     R = op.vector()
     scheme(Us, R)
     assert si["converged"] and np.sqrt(R.normSquaredDofs()) < 1e-10 * np.sqrt(Fx.normSquaredDofs())
+
+
+@pytest.mark.gpu
+def test_gpu_galerkin_scheme_from_symbolic_integrands(ctx):
+    # dune.fem.scheme.galerkin's path: integrands (cellwise kappa, reaction
+    # exp(u), a source expression) on a DG Q2 space, scheme.solve(target)
+    # with the reference's parameter keys; the solution's residual, and the
+    # load vector against the oracle's at the same points
+    import dunefem_amd as fa
+    n, p = (5, 4, 6), 2
+    kc = np.random.default_rng(13).uniform(0.5, 2.0, size=120)
+    space = forms.Space(ctx, n, p)
+    form = forms.Form(kappa=kc, reaction="exp(u)", source="10*sin(pi*x)*sin(pi*y)*sin(pi*z)")
+    scheme = fa.galerkin(form, space, solver="cg",
+                         parameters={"fem.solver.nonlinear.tolerance": 1e-9,
+                                     "fem.solver.nonlinear.linear.tolerance": 1e-11,
+                                     "fem.solver.nonlinear.linear.errormeasure": "relative",
+                                     "fem.solver.nonlinear.linear.preconditioning.method": "blockjacobi"})
+    U = scheme.op.vector()
+    info = scheme.solve(U)
+    print("galerkin scheme solve", info, flush=True)
+    assert set(info) == {"converged", "iterations", "linear_iterations", "timing"}
+    assert info["converged"] and 2 <= info["iterations"] <= 10
+    R = scheme.op.vector()
+    scheme(U, R)
+    assert np.sqrt(R.normSquaredDofs()) < 1e-9
+    oop = ora.DGOperator(n, p, 10.0 * p * p, kappa=("cell", kc))
+    X, Y, Z = scheme.op.qp_coordinates()
+    fq = 10 * np.sin(np.pi * X) * np.sin(np.pi * Y) * np.sin(np.pi * Z)
+    assert rel(scheme.F.download(), oop.rhs(fq)) < 1e-13
