@@ -345,6 +345,10 @@ def nonlinear_section(fa, wl, ctx, n=48, p=2, c3=1.0e4, amp=50.0, cpu=True):
     run("newton_cg_mg_diffusion", {}, dict(exact, graph=GRAPH), mg64)
     run("newton_ew_cg_mg32_diffusion", {"nonlinear.forcing": "eisenstatwalker"},
         dict(method=fa.CG, maxiterations=20000, graph=GRAPH), mg32)
+    # the multigrid built on each Jacobian (reaction taken into its levels)
+    run("newton_cg_mg_jacobian", {}, dict(exact, preconditioning=fa.PRE_MULTIGRID, graph=GRAPH))
+    run("newton_ew_cg_mg_jacobian", {"nonlinear.forcing": "eisenstatwalker"},
+        dict(method=fa.CG, maxiterations=20000, preconditioning=fa.PRE_MULTIGRID, graph=GRAPH))
     # each Newton system by FP64 refinement around FP32 inner CG on an FP32
     # twin of the Jacobian, the FP32 multigrid of the diffusion part inside
     run("newton_ew_ir_fp32cg_mg32_diffusion", {"nonlinear.forcing": "eisenstatwalker"},

@@ -1297,7 +1297,8 @@ __device__ void sym_eig(int K, double (*A)[KMAX], double (*Q)[KMAX])
     }
 }
 
-__global__ void k_dg_block_setup(const DGArgs<double> a, int K, double *__restrict__ Qout, double *__restrict__ lam)
+__global__ void k_dg_block_setup(const DGArgs<double> a, int K, double *__restrict__ Qout, double *__restrict__ lam,
+                                 bool bound)
 {
     const int64_t E = (int64_t)a.n[0] * a.n[1] * a.n[2];
     const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -1340,16 +1341,23 @@ __global__ void k_dg_block_setup(const DGArgs<double> a, int K, double *__restri
     sym_eig(K, M, Q);
     // the reaction c + cvol (orthonormal basis: mass block |K| I) shifts
     // every eigenvalue sum; taken into the x direction, with the element's
-    // weighted mean of cvol (exact for an element-constant reaction)
+    // weighted mean of cvol (exact for an element-constant reaction), or
+    // (bound) its largest value: then the block bounds the element's true
+    // diagonal block from above (int cvol v^2 <= max cvol int v^2 under the
+    // positive quadrature weights), which keeps a damped sweep with it
+    // convergent however cvol varies inside the element
     double shift = 0;
     if (d == 0) {
-        double cm = a.c;
+        double cm = 0, cmax = -INFINITY;
         if (a.cvol)
             for (int qz = 0; qz < K; ++qz)
                 for (int qy = 0; qy < K; ++qy)
-                    for (int qx = 0; qx < K; ++qx)
-                        cm += tb.w[qx] * tb.w[qy] * tb.w[qz] * a.cvol[e * K * K * K + qx + K * (qy + K * qz)];
-        shift = cm * a.h[0] * a.h[1] * a.h[2];
+                    for (int qx = 0; qx < K; ++qx) {
+                        const double v = a.cvol[e * K * K * K + qx + K * (qy + K * qz)];
+                        cm += tb.w[qx] * tb.w[qy] * tb.w[qz] * v;
+                        cmax = fmax(cmax, v);
+                    }
+        shift = (a.c + (a.cvol && bound ? cmax : cm)) * a.h[0] * a.h[1] * a.h[2];
     }
     for (int r = 0; r < K; ++r) {
         lam[(e * 3 + d) * K + r] = M[r][r] + shift;
@@ -1357,10 +1365,10 @@ __global__ void k_dg_block_setup(const DGArgs<double> a, int K, double *__restri
     }
 }
 
-void dg_block_setup(const DGArgs<double> &a, int k, double *Q, double *lam, hipStream_t s)
+void dg_block_setup(const DGArgs<double> &a, int k, double *Q, double *lam, hipStream_t s, bool bound)
 {
     const int64_t E = (int64_t)a.n[0] * a.n[1] * a.n[2];
-    hipLaunchKernelGGL(k_dg_block_setup, dim3(grid_for(E * 3, 128)), dim3(128), 0, s, a, k, Q, lam);
+    hipLaunchKernelGGL(k_dg_block_setup, dim3(grid_for(E * 3, 128)), dim3(128), 0, s, a, k, Q, lam, bound);
     FEM_HIP(hipGetLastError());
 }
 

@@ -1,5 +1,9 @@
 // Multigrid preconditioner for the SIPG DG operator (cellwise or constant
-// kappa, no advection, no reaction).  No reference counterpart: dune-fem
+// kappa, no advection; a reaction c and a pointwise reaction cvol at the
+// volume quadrature points, as the Jacobians of the nonlinear operators
+// have, are taken into the levels: c as it is, cvol by its element means on
+// the coarse DG levels, and from there by the Galerkin products into the
+// vertex levels).  No reference counterpart: dune-fem
 // reaches multigrid only through external packages (the "amg-jacobi"
 // preconditioning method of dune/fem/solver/parameter.hh:48-59, an assembled-
 // matrix AMG of dune-istl or PETSc).  This one is matrix-free and geometric.
@@ -21,7 +25,8 @@
 //   * Geometric vertex levels: every second vertex (and the last one) is
 //     kept per direction, trilinear interpolation, Galerkin stencils by
 //     probing, down to at most 512 vertices, which are solved with a dense
-//     inverse computed once on the host.
+//     inverse (Gauss-Jordan on the device; on the host, with pivoting, for
+//     an indefinite operator).
 // The conforming coarse space is what makes the method robust for the
 // high-contrast coefficients of the configurations: element means (Q_0) as
 // the coarse space of DG Q1 leave far more iterations (measured on the CPU
@@ -361,6 +366,29 @@ __global__ void __launch_bounds__(256)
 // ---- set-up by probing (FP64) -------------------------------------------------------------
 __device__ __forceinline__ int vcolor(int x, int y, int z) { return (x % 3) + 3 * ((y % 3) + 3 * (z % 3)); }
 
+// the Gauss-weighted element means of a pointwise coefficient [E][K^3]
+// (grid-stride: the level kernels' grid is capped)
+__global__ void k_mg_cvol_mean(int64_t E, int K, const Tab1D<double> *__restrict__ tab,
+                               const double *__restrict__ cvol, double *__restrict__ mean)
+{
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < E; e += (int64_t)gridDim.x * blockDim.x) {
+        const double *c = cvol + e * K * K * K;
+        double s = 0;
+        for (int qz = 0; qz < K; ++qz)
+            for (int qy = 0; qy < K; ++qy)
+                for (int qx = 0; qx < K; ++qx)
+                    s += tab->w[qx] * tab->w[qy] * tab->w[qz] * c[qx + K * (qy + K * qz)];
+        mean[e] = s;
+    }
+}
+
+// a coarse level's pointwise coefficient: the element mean at each of its points
+__global__ void k_mg_cvol_fill(int64_t E, int K3, const double *__restrict__ mean, double *__restrict__ cvol)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < E * K3; i += (int64_t)gridDim.x * blockDim.x)
+        cvol[i] = mean[i / K3];
+}
+
 __global__ void k_mg_probe(VDims d, double *__restrict__ x, int col)
 {
     const int64_t V = (int64_t)d.n[0] * d.n[1] * d.n[2];
@@ -388,10 +416,63 @@ __global__ void k_mg_extract(VDims d, const double *__restrict__ y, double *__re
     }
 }
 
-__global__ void k_mg_dinv(int64_t V, const double *__restrict__ A, double *__restrict__ dinv)
+// the Jacobi scaling of a vertex level: 1 / max(a_vv, lim sum_w |a_vw|).  By
+// Gershgorin the eigenvalues of diag(d)^-1 A are then at most 1 / lim, so a
+// sweep damped by omega = 1.9 lim stays convergent.  A diffusion stencil
+// (off-diagonal sum -a_vv) keeps its diagonal; a reaction-dominated one (the
+// Q1 mass stencil has row sum 3.375 a_vv) is scaled down instead of
+// diverging.
+__global__ void k_mg_dinv(int64_t V, const double *__restrict__ A, double lim, double *__restrict__ dinv)
 {
-    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < V; v += (int64_t)gridDim.x * blockDim.x)
-        dinv[v] = 1.0 / A[v * 27 + 13];
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < V; v += (int64_t)gridDim.x * blockDim.x) {
+        double rs = 0;
+        for (int j = 0; j < 27; ++j) rs += fabs(A[v * 27 + j]);
+        dinv[v] = 1.0 / fmax(A[v * 27 + 13], lim * rs);
+    }
+}
+
+// the coarsest level's operator as a dense row-major V x V matrix from its
+// 27-point stencil (offset s = sx + 3 sy + 9 sz, 0..2 each, about the vertex)
+__global__ void k_mg_dense_fill(VDims d, const double *__restrict__ A, double *__restrict__ M)
+{
+    const int64_t V = (int64_t)d.n[0] * d.n[1] * d.n[2];
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < V * V; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t v = i / V, w = i - v * V;
+        const int dx = (int)(w % d.n[0]) - (int)(v % d.n[0]);
+        const int dy = (int)((w / d.n[0]) % d.n[1]) - (int)((v / d.n[0]) % d.n[1]);
+        const int dz = (int)(w / ((int64_t)d.n[0] * d.n[1])) - (int)(v / ((int64_t)d.n[0] * d.n[1]));
+        const bool in = dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1;
+        M[i] = in ? A[v * 27 + (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)] : 0.0;
+    }
+}
+
+// one pivot step of the in-place Gauss-Jordan inversion without pivoting
+// (X -> Y, pivot c): Y_cc = 1/p, Y_ck = X_ck / p, Y_rc = -X_rc / p, Y_rk =
+// X_rk - X_rc X_ck / p, p = X_cc.  After the steps c = 0..V-1 the matrix is
+// the inverse.  The pivots are the leading Schur complements' diagonals:
+// all positive exactly when the matrix is symmetric positive definite, for
+// which no pivoting is needed; they are recorded for the host to check.
+__global__ void k_mg_gj_step(int V, int c, const double *__restrict__ X, double *__restrict__ Y,
+                             double *__restrict__ piv)
+{
+    const int64_t V2 = (int64_t)V * V;
+    const double pc = X[(int64_t)c * V + c];
+    const double ip = 1.0 / pc;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < V2; i += (int64_t)gridDim.x * blockDim.x) {
+        const int r = (int)(i / V), k = (int)(i - (int64_t)r * V);
+        double y;
+        if (r == c && k == c) {
+            y = ip;
+            piv[c] = pc;
+        } else if (r == c) {
+            y = X[i] * ip;
+        } else if (k == c) {
+            y = -X[i] * ip;
+        } else {
+            y = X[i] - X[(int64_t)r * V + c] * X[(int64_t)c * V + k] * ip;
+        }
+        Y[i] = y;
+    }
 }
 
 int coarse1(int nf) { return (nf - 1 + 1) / 2 + 1; }   // ceil((nf - 1) / 2) + 1
@@ -418,9 +499,9 @@ void Multigrid::init(Operator *fine)
     op = fine;
     FEM_CHECK(fine->is_dg(), FEM_ERR_UNSUPPORTED, "multigrid: DG operators only");
     const fem_op_desc &d0 = fine->desc;
-    FEM_CHECK(d0.b[0] == 0 && d0.b[1] == 0 && d0.b[2] == 0 && d0.c == 0, FEM_ERR_UNSUPPORTED,
-              "multigrid: diffusion operators only (b = 0, c = 0)");
-    FEM_CHECK(!fine->has_reaction(), FEM_ERR_UNSUPPORTED, "multigrid: no pointwise or nonlinear reaction");
+    FEM_CHECK(d0.b[0] == 0 && d0.b[1] == 0 && d0.b[2] == 0, FEM_ERR_UNSUPPORTED,
+              "multigrid: operators without advection (b = 0)");
+    FEM_CHECK(!fine->nonlinear(), FEM_ERR_STATE, "multigrid: the operator is nonlinear (build it on the Jacobian)");
     FEM_CHECK(d0.kappa_mode == FEM_KAPPA_CELL || d0.kappa_mode == FEM_KAPPA_CONST, FEM_ERR_UNSUPPORTED,
               "multigrid: cellwise or constant kappa");
     FEM_CHECK(fine->a64.kappa_mode >= 0, FEM_ERR_STATE, "coefficient data not set");
@@ -435,6 +516,14 @@ void Multigrid::init(Operator *fine)
         kc.resize(E);
         FEM_HIP(hipMemcpy(kc.data(), fine->kcell64.p, sizeof(double) * E, hipMemcpyDeviceToHost));
     }
+    // a pointwise reaction: its element means serve the coarse DG levels
+    DevBuf cmean;
+    if (fine->cvol64.p) {
+        cmean.alloc(sizeof(double) * E);
+        hipLaunchKernelGGL(k_mg_cvol_mean, dim3(mg_grid(E)), dim3(MGT), 0, s, E, fine->k, fine->a64.tab,
+                           fine->cvol64.as<double>(), cmean.as<double>());
+        FEM_HIP(hipGetLastError());
+    }
     auto make_op = [&](int order, int precision) {
         std::unique_ptr<Operator> o(new Operator());
         fem_op_desc d = d0;
@@ -442,6 +531,25 @@ void Multigrid::init(Operator *fine)
         d.precision = precision;
         o->init(ctx, d);
         if (d0.kappa_mode == FEM_KAPPA_CELL) o->set_kappa_cell(kc.data());
+        if (cmean.p) {
+            const int64_t nq = E * o->k * o->k * o->k;
+            o->cvol64.alloc(sizeof(double) * nq);
+            o->cvol32.alloc(sizeof(float) * nq);
+            hipLaunchKernelGGL(k_mg_cvol_fill, dim3(mg_grid(nq)), dim3(MGT), 0, s, E, o->k * o->k * o->k,
+                               cmean.as<double>(), o->cvol64.as<double>());
+            FEM_HIP(hipGetLastError());
+            Vec lo, hi;
+            lo.ctx = hi.ctx = ctx;
+            lo.n = hi.n = nq;
+            lo.prec = FEM_F32;
+            hi.prec = FEM_F64;
+            lo.data = o->cvol32.p;
+            hi.data = o->cvol64.p;
+            lo.owner = hi.owner = false;
+            vec_copy(lo, hi);
+            o->a64.cvol = o->cvol64.as<double>();
+            o->a32.cvol = o->cvol32.as<float>();
+        }
         return o;
     };
     const size_t es = lp == FEM_F64 ? 8 : 4;
@@ -464,7 +572,11 @@ void Multigrid::init(Operator *fine)
         }
         L->k = p + 1;
         L->t.alloc(L->op->N * es);
-        L->bj.init(L->op, FEM_PRE_BLOCK_JACOBI);
+        // a varying pointwise reaction enters the smoother's blocks by its
+        // largest value per element: the damped sweep stays convergent (a
+        // block below the true one can make it diverge, and the V-cycle
+        // indefinite)
+        L->bj.init(L->op, FEM_PRE_BLOCK_JACOBI, true);
         dg.push_back(std::move(L));
         if (p == 1) break;
     }
@@ -545,7 +657,11 @@ void Multigrid::init(Operator *fine)
         MGVertexLevel &L = *vx[l];
         DevBuf d64;
         d64.alloc(L.V * 8);
-        hipLaunchKernelGGL(k_mg_dinv, dim3(mg_grid(L.V)), dim3(MGT), 0, s, L.V, A64[l]->as<double>(), d64.as<double>());
+        // a reaction-free level keeps the plain diagonal (its stencils have
+        // row sums of at most twice the diagonal away from the boundary)
+        const double lim = (d0.c != 0 || cmean.p) ? omega_vertex / 1.9 : 0.0;
+        hipLaunchKernelGGL(k_mg_dinv, dim3(mg_grid(L.V)), dim3(MGT), 0, s, L.V, A64[l]->as<double>(), lim,
+                           d64.as<double>());
         FEM_HIP(hipGetLastError());
         DevBuf As64;
         As64.alloc(L.V * 27 * 8);
@@ -558,11 +674,38 @@ void Multigrid::init(Operator *fine)
         L.r.alloc(L.V * es);
         L.t.alloc(L.V * es);
     }
-    // dense inverse of the coarsest level (host, FP64, Gauss-Jordan with
-    // partial pivoting), stored row-major
+    // dense inverse of the coarsest level, stored row-major: Gauss-Jordan on
+    // the device (one launch per pivot, no pivoting) for the symmetric
+    // positive definite operators of diffusion and non-negative reaction; a
+    // non-positive pivot (an indefinite operator) sends the inversion to the
+    // host, with partial pivoting
+    const MGVertexLevel &LC = *vx.back();
+    const int VC = (int)LC.V;
+    bool spd = true;
     {
-        const MGVertexLevel &L = *vx.back();
-        const int V = (int)L.V;
+        DevBuf X, Y, pv;
+        X.alloc((size_t)VC * VC * 8);
+        Y.alloc((size_t)VC * VC * 8);
+        pv.alloc((size_t)VC * 8);
+        VDims dc{ { LC.nv[0], LC.nv[1], LC.nv[2] } };
+        const unsigned G = mg_grid((int64_t)VC * VC);
+        hipLaunchKernelGGL(k_mg_dense_fill, dim3(G), dim3(MGT), 0, s, dc, A64.back()->as<double>(), X.as<double>());
+        for (int c = 0; c < VC; ++c) {
+            const DevBuf &src = (c & 1) ? Y : X;
+            DevBuf &dst = (c & 1) ? X : Y;
+            hipLaunchKernelGGL(k_mg_gj_step, dim3(G), dim3(MGT), 0, s, VC, c, src.as<double>(), dst.as<double>(),
+                               pv.as<double>());
+        }
+        FEM_HIP(hipGetLastError());
+        std::vector<double> piv(VC);
+        FEM_HIP(hipMemcpyAsync(piv.data(), pv.p, sizeof(double) * VC, hipMemcpyDeviceToHost, s));
+        FEM_HIP(hipStreamSynchronize(s));
+        for (int c = 0; c < VC; ++c) spd = spd && piv[c] > 0 && std::isfinite(piv[c]);
+        if (spd) store_as(lp, coarse_inv, (VC & 1) ? Y : X, (int64_t)VC * VC, ctx);
+    }
+    if (!spd) {
+        const MGVertexLevel &L = LC;
+        const int V = VC;
         std::vector<double> st(L.V * 27);
         FEM_HIP(hipMemcpyAsync(st.data(), A64.back()->p, sizeof(double) * st.size(), hipMemcpyDeviceToHost, s));
         FEM_HIP(hipStreamSynchronize(s));
@@ -667,7 +810,8 @@ template <typename T>
 static void mg_sweep(const MGDGLevel &L, const T *r, const T *in, T *out, T *t, T omega, const int *skip, hipStream_t s)
 {
     const Operator &o = *L.op;
-    if (o.desc.precision == FEM_BF16MFMA) {
+    // the fused sweep has no pointwise reaction: apply, then the sweep
+    if (o.desc.precision == FEM_BF16MFMA || o.cvol64.p) {
         L.op->apply_raw(in, t, skip);
         mg_bj<T>(L, r, t, out, omega, 1, skip, s);
         return;
@@ -694,7 +838,7 @@ void Multigrid::cycle_dg(size_t l, const T *r, T *x, const int *skip)
     // the fused sweeps write a second buffer (a sweep reads its input's
     // neighbours): cur is the buffer holding the current iterate, and the
     // first sweep is arranged so that the last one ends in x
-    const bool fused = L.op->desc.precision != FEM_BF16MFMA;
+    const bool fused = L.op->desc.precision != FEM_BF16MFMA && !L.op->cvol64.p;
     T *cur = x;
     auto sweep = [&]() {
         if (!fused) {

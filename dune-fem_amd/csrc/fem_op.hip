@@ -587,7 +587,7 @@ void Operator::block_diagonal(Vec &blocks) const
     FEM_HIP(hipStreamSynchronize(s));
 }
 
-void Precond::init(Operator *o, int k_)
+void Precond::init(Operator *o, int k_, bool bound)
 {
     op = o;
     kind = k_;
@@ -616,7 +616,7 @@ void Precond::init(Operator *o, int k_)
         DevBuf Q64, l64;
         Q64.alloc((size_t)o->E * 3 * k * k * sizeof(double));
         l64.alloc((size_t)o->E * 3 * k * sizeof(double));
-        dg_block_setup(o->a64, k, Q64.as<double>(), l64.as<double>(), s);
+        dg_block_setup(o->a64, k, Q64.as<double>(), l64.as<double>(), s, bound);
         if (prec == FEM_F64) {
             Q.alloc(Q64.bytes); lam.alloc(l64.bytes);
             FEM_HIP(hipMemcpyAsync(Q.p, Q64.p, Q64.bytes, hipMemcpyDeviceToDevice, s));

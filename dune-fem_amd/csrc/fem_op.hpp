@@ -125,7 +125,9 @@ struct Precond {
     // widened back; cx, cy hold the FP32 vectors
     const Precond *lowp = nullptr;
     DevBuf cx, cy;
-    void init(Operator *o, int kind);
+    // bound (block Jacobi): a pointwise reaction enters the element blocks by
+    // its largest value, not its mean (the multigrid smoothers)
+    void init(Operator *o, int kind, bool bound = false);
     void init_widening(Operator *o64, const Precond *p32);
     Learned *learned_part() const { return lowp ? lowp->learned : learned; }
     // `dot` (block Jacobi only, may be null): per-workgroup FP64 partials of

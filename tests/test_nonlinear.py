@@ -20,6 +20,8 @@ CPU tests pin the oracle's reaction term and quadrature-point evaluation:
 GPU tests compare the HIP path with the oracle (FP64 within 1e-12, FP32
 within 2e-5) and check Newton against a dense Newton on the oracle.
 """
+import sys
+
 import numpy as np
 import pytest
 from numpy.polynomial import legendre as npleg
@@ -410,6 +412,57 @@ def test_gpu_newton_with_fp32_refinement_solves(ctx):
                    dict(method=fa.CG, refinement=1, inner_tolerance=1e-3, inner_maxiterations=2000), mg32)
     print("eisenstat-walker, fp32 refinement, fp32 multigrid of the diffusion part", mg, flush=True)
     assert mg["converged"] and mg["residual"] < tol and rel(w_mg, ustar.download()) < 1e-6
+
+
+@pytest.mark.gpu
+def test_gpu_multigrid_with_reaction(ctx):
+    # the V-cycle of an operator with c and a pointwise reaction (a Jacobian's
+    # form): CG + multigrid reaches the solution CG + block Jacobi reaches,
+    # in far fewer iterations; and Newton with the multigrid rebuilt on each
+    # Jacobian (preconditioning.method multigrid) takes the steps Newton with
+    # block Jacobi takes
+    import dunefem_amd as fa
+    n, p = (12, 12, 12), 2
+    rng = np.random.default_rng(14)
+    kc = 10.0 ** rng.uniform(-1, 1, size=12 ** 3)
+    op = _dg(ctx, fa, n, p, kc, c=0.5)
+    op.set_reaction_qp(rng.uniform(0.0, 200.0, size=op.N))
+    b = op.vector(rng.normal(size=op.N))
+    sols = {}
+    for name, pre, maxit in (("bj", fa.PRE_BLOCK_JACOBI, 5000), ("mg", fa.PRE_MULTIGRID, 500)):
+        inv = fa.InverseOperator(ctx, method=fa.CG, tolerance=1e-11, errormeasure=fa.RELATIVE, maxiterations=maxit)
+        inv.bind(op, fa.Preconditioner(op, pre))
+        X = op.vector()
+        inv(b, X)
+        sols[name] = (inv.info(), X.download())
+        sys.stderr.write("%s %s\n" % (name, inv.info()))
+        sys.stderr.flush()
+    (ibj, xbj), (img, xmg) = sols["bj"], sols["mg"]
+    assert ibj["converged"] and img["converged"] and img["true_rel_residual"] < 1e-10
+    assert img["iterations"] * 5 < ibj["iterations"]
+    assert rel(xmg, xbj) < 1e-8
+    # Newton on the cubic law with the multigrid of each Jacobian
+    S = _dg(ctx, fa, n, p, kc)
+    S.set_reaction_law(20.0, 2)
+    ustar = S.vector(rng.normal(size=S.N))
+    F = S.vector()
+    S(ustar, F)
+    tol = 1e-9 * np.sqrt(F.normSquaredDofs())
+    res = {}
+    for name, pre in (("bj", fa.PRE_BLOCK_JACOBI), ("mg", fa.PRE_MULTIGRID)):
+        # bounded: at most 30 Newton steps and 60000 linear iterations in all
+        inv = fa.NewtonInverseOperator(ctx, {"nonlinear.tolerance": tol, "nonlinear.maxiterations": 30},
+                                       linear=dict(method=fa.CG, tolerance=1e-11, errormeasure=fa.RELATIVE,
+                                                   maxiterations=60000, preconditioning=pre))
+        inv.bind(S)
+        W = S.vector()
+        inv(F, W)
+        res[name] = (inv.info(), W.download())
+        sys.stderr.write("newton %s %s\n" % (name, inv.info()))
+        sys.stderr.flush()
+    assert res["mg"][0]["converged"] and res["mg"][0]["iterations"] == res["bj"][0]["iterations"]
+    assert res["mg"][0]["linear_iterations"] * 4 < res["bj"][0]["linear_iterations"]
+    assert rel(res["mg"][1], ustar.download()) < 1e-8
 
 
 @pytest.mark.gpu
