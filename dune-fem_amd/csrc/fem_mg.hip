@@ -585,6 +585,9 @@ void Multigrid::init(Operator *fine)
     Operator *q1 = dg.back()->op;
     if (q1->desc.precision != FEM_F64) {
         q164 = make_op(1, FEM_F64);
+        // a Q1 finest level carries its pointwise cvol, not the means
+        if (q1 == fine && fine->cvol64.p)
+            FEM_HIP(hipMemcpyAsync(q164->cvol64.p, fine->cvol64.p, fine->cvol64.bytes, hipMemcpyDeviceToDevice, s));
         q1 = q164.get();
     }
     corners.alloc(E * 8 * sizeof(double));

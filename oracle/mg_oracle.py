@@ -16,7 +16,19 @@ external AMG, dune/fem/solver/parameter.hh:48-59); what pins it: the DG
 Galerkin identity P^T A_p P = A_{p'} for the embedded Legendre modes, the
 conforming Galerkin stencil against the closed-form trilinear stiffness
 stencil, the symmetry of the V-cycle, and its convergence as a PCG
-preconditioner (tests/test_oracle.py)."""
+preconditioner (tests/test_oracle.py).
+
+A reaction (the Jacobians of nonlinear operators): a constant c enters every
+level as it is; a pointwise cvol at the volume quadrature points enters the
+finest level as it is and the coarse DG levels by its Gauss-weighted element
+means (then the vertex levels through P^T A P).  The finest level's smoother
+blocks take cvol by its largest value in the element (an upper bound of the
+true element block: the damped sweep stays convergent), and with any
+reaction the vertex levels' Jacobi scaling is 1 / max(a_vv, omega/1.9
+sum_w |a_vw|) (convergent by Gershgorin).  Pinned by: the bound block equals
+the exact block for an element-constant cvol, the finest-level sweep and the
+vertex sweeps stay contractions for a strongly varying cvol, and the V-cycle
+stays symmetric positive definite there (tests/test_oracle.py)."""
 import numpy as np
 import scipy.sparse as sp
 
@@ -110,28 +122,56 @@ def embed_modes(E, kf, kc):
     return sp.csr_matrix((np.ones(rows.size), (rows, np.arange(rows.size))), shape=(E * kf ** 3, E * kc ** 3))
 
 
+def element_means(cvol, E, k):
+    """Gauss-weighted means of a pointwise coefficient [E][k^3] (x fastest)
+    over each element (the weights on [0, 1] sum to one)"""
+    _, w = ora.gauss(k)
+    W = (w[None, None, :] * w[None, :, None] * w[:, None, None]).ravel()   # [qz][qy][qx]
+    return np.asarray(cvol, dtype=np.float64).reshape(E, k ** 3) @ W
+
+
 class MGOracle:
-    def __init__(self, n, p, beta, kappa, nu_dg=2, omega_dg=0.8, nu_v=1, omega_v=0.7):
+    def __init__(self, n, p, beta, kappa, nu_dg=2, omega_dg=0.8, nu_v=1, omega_v=0.7, c=0.0, cvol=None):
         self.n, self.p = tuple(n), p
         self.nu_dg, self.omega_dg, self.nu_v, self.omega_v = nu_dg, omega_dg, nu_v, omega_v
         E = n[0] * n[1] * n[2]
+        vol = 1.0 / E   # element volume on the unit cube
+        mean = None if cvol is None else element_means(cvol, E, p + 1)
         self.dg = []
         q = p
         while True:
-            op = ora.DGOperator(n, q, beta, (0, 0, 0), 0.0, kappa=kappa)
-            binv = np.linalg.inv(op.block_diag())
+            op = ora.DGOperator(n, q, beta, (0, 0, 0), c, kappa=kappa)
+            # the blocks of the operator without cvol, shifted by the
+            # element's largest cvol (finest level) or its mean (coarse
+            # levels, where cvol is that mean); the Legendre mass block is
+            # vol * I
+            B = op.block_diag()
+            if cvol is not None:
+                k3 = (q + 1) ** 3
+                if not self.dg:
+                    op.set_reaction_qp(cvol)
+                    shift = np.asarray(cvol, dtype=np.float64).reshape(E, k3).max(axis=1)
+                else:
+                    op.set_reaction_qp(np.repeat(mean, k3))
+                    shift = mean
+                B = B + (shift * vol)[:, None, None] * np.eye(k3)[None]
+            binv = np.linalg.inv(B)
             self.dg.append((op, q + 1, binv))
             if q == 1:
                 break
             q = max(1, q // 2)
         self.E = E
+        reaction = c != 0 or cvol is not None
         A1 = assemble_dg(self.dg[-1][0], n, 1)
         self.Pc = conforming_P(n)
         A = (self.Pc.T @ A1 @ self.Pc).tocsr()
         nv = [n[0] + 1, n[1] + 1, n[2] + 1]
         self.vx = []
         while True:
-            lev = {"A": A, "nv": list(nv), "dinv": 1.0 / A.diagonal()}
+            d = A.diagonal()
+            if reaction:
+                d = np.maximum(d, omega_v / 1.9 * np.asarray(abs(A).sum(axis=1)).ravel())
+            lev = {"A": A, "nv": list(nv), "dinv": 1.0 / d}
             self.vx.append(lev)
             if A.shape[0] <= DENSE_MAX:
                 break

@@ -65,6 +65,49 @@ def test_multigrid_vcycle_matches_oracle(ctx, n, p, kind):
     assert e32 < 1e-4
 
 
+REACTION_CASES = [
+    # the reaction in every level kind: pointwise cvol on a Q1 finest level
+    # (probed into the vertex levels as it is), element means on coarse DG
+    # levels, one and two vertex h-levels
+    ((3, 2, 4), 1),
+    ((4, 3, 5), 2),
+    ((5, 4, 3), 3),
+    ((4, 4, 3), 4),
+    ((9, 7, 8), 2),
+    ((20, 20, 20), 1),
+]
+
+
+@pytest.mark.parametrize("n,p", REACTION_CASES)
+def test_multigrid_vcycle_with_reaction_matches_oracle(ctx, n, p):
+    # c and a pointwise cvol (the form of a Jacobian S'(w)): the finest
+    # smoother's blocks by the element's largest cvol, the coarse DG levels
+    # with its element means, the bounded vertex Jacobi scaling
+    rng = np.random.default_rng(7 * sum(n) + p)
+    E = n[0] * n[1] * n[2]
+    kc = 10.0 ** rng.uniform(-1, 1, size=E)
+    cvol = 10.0 ** rng.uniform(0, 3, size=E * (p + 1) ** 3)
+    beta, c = 10.0 * p * p, 0.3
+    M = mgo.MGOracle(n, p, beta, ("cell", kc), c=c, cvol=cvol)
+    zs = {}
+    for prec in (fa.F64, fa.F32):
+        op = fa.GalerkinOperator(ctx, fa.OP_DG_SIPG_3D, n, p, beta, (0, 0, 0), c, fa.KAPPA_CELL, precision=prec)
+        op.set_kappa_cell(kc)
+        op.set_reaction_qp(cvol)
+        pre = fa.Preconditioner(op, fa.PRE_MULTIGRID)
+        r = np.random.default_rng(3).normal(size=op.N)
+        R, Z = op.vector(r), op.vector()
+        Z.fill(np.nan)
+        pre(R, Z)
+        zs[prec] = Z.download()
+    zref = M.apply(np.random.default_rng(3).normal(size=E * (p + 1) ** 3))
+    err = np.abs(zs[fa.F64] - zref).max() / np.abs(zref).max()
+    e32 = np.linalg.norm(zs[fa.F32] - zref) / np.linalg.norm(zref)
+    print("multigrid with reaction", n, p, "vertex levels", [L["A"].shape[0] for L in M.vx], "rel err", err,
+          "fp32 rel l2 err", e32, flush=True)
+    assert err < 1e-12 and e32 < 1e-4
+
+
 def test_multigrid_is_symmetric_positive(ctx):
     P = wl.problem3(n=12)
     op = dg(ctx, P["n"], P["p"], P["beta"], ("cell", P["kcell"]))
