@@ -21,10 +21,12 @@
 //     DG Q1 (per element the 8 Legendre modes of the trilinear interpolant).
 //     Its Galerkin operator P_c^T A_Q1 P_c is a 27-point stencil (interior
 //     face terms vanish on continuous functions); it is computed on the
-//     device by probing with 27 vertex colours.  Smoother: damped Jacobi.
+//     device in closed form from the Q1 level's coefficients.  Smoother:
+//     damped Jacobi.
 //   * Geometric vertex levels: every second vertex (and the last one) is
-//     kept per direction, trilinear interpolation, Galerkin stencils by
-//     probing, down to at most 512 vertices, which are solved with a dense
+//     kept per direction, trilinear interpolation, Galerkin stencils
+//     P^T A P formed directly from the finer stencil, down to at most 512
+//     vertices, which are solved with a dense
 //     inverse (Gauss-Jordan on the device; on the host, with pivoting, for
 //     an indefinite operator).
 // The conforming coarse space is what makes the method robust for the
@@ -363,8 +365,7 @@ __global__ void __launch_bounds__(256)
     if (lane == 0) x[row] = (T)acc;
 }
 
-// ---- set-up by probing (FP64) -------------------------------------------------------------
-__device__ __forceinline__ int vcolor(int x, int y, int z) { return (x % 3) + 3 * ((y % 3) + 3 * (z % 3)); }
+// ---- set-up (FP64) -----------------------------------------------------------------------
 
 // the Gauss-weighted element means of a pointwise coefficient [E][K^3]
 // (grid-stride: the level kernels' grid is capped)
@@ -389,30 +390,126 @@ __global__ void k_mg_cvol_fill(int64_t E, int K3, const double *__restrict__ mea
         cvol[i] = mean[i / K3];
 }
 
-__global__ void k_mg_probe(VDims d, double *__restrict__ x, int col)
-{
-    const int64_t V = (int64_t)d.n[0] * d.n[1] * d.n[2];
-    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < V; v += (int64_t)gridDim.x * blockDim.x) {
-        const int vx = (int)(v % d.n[0]), vy = (int)((v / d.n[0]) % d.n[1]), vz = (int)(v / ((int64_t)d.n[0] * d.n[1]));
-        x[v] = vcolor(vx, vy, vz) == col ? 1.0 : 0.0;
-    }
-}
 
 // row v of the stencil: the probe of colour col answers for the one stencil
 // neighbour of that colour (distinct colours within distance 1)
-__global__ void k_mg_extract(VDims d, const double *__restrict__ y, double *__restrict__ A, int col)
+
+// the conforming level's stencil in closed form: a(phi_W, phi_v) of the DG Q1
+// operator `a` on the trilinear hat functions of the (n+1)^3 vertices.  The
+// interior face terms of SIPG vanish on continuous functions (no jumps; the
+// flux terms against [v] cancel between the two sides), which leaves per
+// element the stiffness kappa sum_d (h_t1 h_t2 / h_d) k(a_d, b_d) m(a_t1,
+// b_t1) m(a_t2, b_t2) (k = +-1, m = 1/3, 1/6: the 1D linear stiffness and mass
+// on [0, 1]), the reaction (c + cvol) at the 2^3 Gauss points, and on the
+// domain boundary the Nitsche terms kappa (beta/h_d phi_a phi_b - n_d
+// (d_d phi_a phi_b + phi_a d_d phi_b)) over the face.  One thread per vertex
+// v and stencil offset s (W = v + d(s)); a, b are the local corners of W and
+// v in each element that holds both.
+__device__ __forceinline__ double mg_m1(int a, int b) { return a == b ? (1.0 / 3.0) : (1.0 / 6.0); }
+
+__global__ void k_mg_conforming(const DGArgs<double> a, VDims vd, double *__restrict__ A)
 {
-    const int64_t V = (int64_t)d.n[0] * d.n[1] * d.n[2];
-    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < V; v += (int64_t)gridDim.x * blockDim.x) {
-        const int vx = (int)(v % d.n[0]), vy = (int)((v / d.n[0]) % d.n[1]), vz = (int)(v / ((int64_t)d.n[0] * d.n[1]));
-        for (int s = 0; s < 27; ++s) {
-            const int wx = vx + s % 3 - 1, wy = vy + (s / 3) % 3 - 1, wz = vz + s / 9 - 1;
-            if (wx < 0 || wy < 0 || wz < 0 || wx >= d.n[0] || wy >= d.n[1] || wz >= d.n[2]) {
-                if (col == 0) A[v * 27 + s] = 0.0;
-                continue;
-            }
-            if (vcolor(wx, wy, wz) == col) A[v * 27 + s] = y[v];
+    const int64_t V = (int64_t)vd.n[0] * vd.n[1] * vd.n[2];
+    const Tab1D<double> &tb = *a.tab;
+    const double detJ = a.h[0] * a.h[1] * a.h[2];
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < V * 27; g += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t v = g / 27;
+        const int so = (int)(g - v * 27);
+        const int P[3] = { (int)(v % vd.n[0]), (int)((v / vd.n[0]) % vd.n[1]), (int)(v / ((int64_t)vd.n[0] * vd.n[1])) };
+        const int W[3] = { P[0] + so % 3 - 1, P[1] + (so / 3) % 3 - 1, P[2] + so / 9 - 1 };
+        double acc = 0;
+        if (W[0] >= 0 && W[1] >= 0 && W[2] >= 0 && W[0] < vd.n[0] && W[1] < vd.n[1] && W[2] < vd.n[2]) {
+            for (int ez = P[2] - 1; ez <= P[2]; ++ez)
+                for (int ey = P[1] - 1; ey <= P[1]; ++ey)
+                    for (int ex = P[0] - 1; ex <= P[0]; ++ex) {
+                        const int ec[3] = { ex, ey, ez };
+                        int ca[3], cb[3];
+                        bool in = true;
+                        for (int d = 0; d < 3; ++d) {
+                            cb[d] = P[d] - ec[d];
+                            ca[d] = W[d] - ec[d];
+                            in = in && ec[d] >= 0 && ec[d] < a.n[d] && ca[d] >= 0 && ca[d] <= 1;
+                        }
+                        if (!in) continue;
+                        const int64_t e = ex + (int64_t)a.n[0] * (ey + (int64_t)a.n[1] * ez);
+                        const double kap = a.kappa_mode == FEM_KAPPA_CELL ? a.kcell[e] : a.kappa_const;
+                        for (int d = 0; d < 3; ++d) {
+                            const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+                            const double mt = mg_m1(ca[t1], cb[t1]) * mg_m1(ca[t2], cb[t2]);
+                            const double area = a.h[t1] * a.h[t2];
+                            acc += kap * area / a.h[d] * (ca[d] == cb[d] ? 1.0 : -1.0) * mt;
+                            for (int sd = 0; sd < 2; ++sd) {
+                                if (ec[d] != (sd ? a.n[d] - 1 : 0)) continue;   // boundary faces only
+                                const double nrm = sd ? 1.0 : -1.0;
+                                const double va = ca[d] == sd ? 1.0 : 0.0, vb = cb[d] == sd ? 1.0 : 0.0;
+                                const double da = ca[d] ? 1.0 : -1.0, db = cb[d] ? 1.0 : -1.0;
+                                acc += area * mt * kap * (a.beta / a.h[d] * va * vb - nrm / a.h[d] * (da * vb + va * db));
+                            }
+                        }
+                        for (int qz = 0; qz < 2; ++qz)
+                            for (int qy = 0; qy < 2; ++qy)
+                                for (int qx = 0; qx < 2; ++qx) {
+                                    const int q[3] = { qx, qy, qz };
+                                    double pa = 1, pb = 1;
+                                    for (int d = 0; d < 3; ++d) {
+                                        const double xq = tb.x[q[d]];
+                                        pa *= ca[d] ? xq : 1.0 - xq;
+                                        pb *= cb[d] ? xq : 1.0 - xq;
+                                    }
+                                    const double r = a.c + (a.cvol ? a.cvol[e * 8 + qx + 2 * qy + 4 * qz] : 0.0);
+                                    acc += tb.w[qx] * tb.w[qy] * tb.w[qz] * detJ * r * pa * pb;
+                                }
+                    }
         }
+        A[g] = acc;
+    }
+}
+
+// the Galerkin product P^T A_f P of a vertex level, directly: one thread per
+// coarse vertex V and stencil offset s (W = V + d(s)) sums
+// P(i, V) A_f(i, j) P(j, W) over the fine vertices i in the support of V
+// (per direction 2V-1 .. 2V+1, which holds the last fine vertex when it
+// belongs to V) and the 27 stencil neighbours j of i.  Offsets that leave
+// the coarse lattice hold 0.
+__global__ void k_mg_galerkin(VDims f, VDims cd, const double *__restrict__ Af, double *__restrict__ Ac)
+{
+    const int64_t Vc = (int64_t)cd.n[0] * cd.n[1] * cd.n[2];
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < Vc * 27; g += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t v = g / 27;
+        const int so = (int)(g - v * 27);
+        const int V[3] = { (int)(v % cd.n[0]), (int)((v / cd.n[0]) % cd.n[1]), (int)(v / ((int64_t)cd.n[0] * cd.n[1])) };
+        const int W[3] = { V[0] + so % 3 - 1, V[1] + (so / 3) % 3 - 1, V[2] + so / 9 - 1 };
+        double acc = 0;
+        if (W[0] >= 0 && W[1] >= 0 && W[2] >= 0 && W[0] < cd.n[0] && W[1] < cd.n[1] && W[2] < cd.n[2]) {
+            int lo[3], hi[3];
+            for (int d = 0; d < 3; ++d) {
+                lo[d] = max(0, 2 * V[d] - 1);
+                hi[d] = min(f.n[d] - 1, 2 * V[d] + 1);
+            }
+            for (int iz = lo[2]; iz <= hi[2]; ++iz) {
+                const double wz = w1(iz, V[2], f.n[2], cd.n[2]);
+                if (wz == 0.0) continue;
+                for (int iy = lo[1]; iy <= hi[1]; ++iy) {
+                    const double wy = w1(iy, V[1], f.n[1], cd.n[1]);
+                    if (wy == 0.0) continue;
+                    for (int ix = lo[0]; ix <= hi[0]; ++ix) {
+                        const double wx = w1(ix, V[0], f.n[0], cd.n[0]);
+                        if (wx == 0.0) continue;
+                        const double *a = Af + (ix + (int64_t)f.n[0] * (iy + (int64_t)f.n[1] * iz)) * 27;
+                        double row = 0;
+                        for (int t = 0; t < 27; ++t) {
+                            const int jx = ix + t % 3 - 1, jy = iy + (t / 3) % 3 - 1, jz = iz + t / 9 - 1;
+                            if (jx < 0 || jy < 0 || jz < 0 || jx >= f.n[0] || jy >= f.n[1] || jz >= f.n[2]) continue;
+                            const double pj = w1(jx, W[0], f.n[0], cd.n[0]) * w1(jy, W[1], f.n[1], cd.n[1]) *
+                                              w1(jz, W[2], f.n[2], cd.n[2]);
+                            row = fma(a[t], pj, row);
+                        }
+                        acc = fma(wx * wy * wz, row, acc);
+                    }
+                }
+            }
+        }
+        Ac[g] = acc;
     }
 }
 
@@ -580,16 +677,9 @@ void Multigrid::init(Operator *fine)
         dg.push_back(std::move(L));
         if (p == 1) break;
     }
-    // conforming vertex level from the Q1 operator (FP64 for the set-up)
-    std::unique_ptr<Operator> q164;
-    Operator *q1 = dg.back()->op;
-    if (q1->desc.precision != FEM_F64) {
-        q164 = make_op(1, FEM_F64);
-        // a Q1 finest level carries its pointwise cvol, not the means
-        if (q1 == fine && fine->cvol64.p)
-            FEM_HIP(hipMemcpyAsync(q164->cvol64.p, fine->cvol64.p, fine->cvol64.bytes, hipMemcpyDeviceToDevice, s));
-        q1 = q164.get();
-    }
+    // conforming vertex level from the Q1 level's FP64 coefficient data
+    // (every operator precision keeps it)
+    const Operator *q1 = dg.back()->op;
     corners.alloc(E * 8 * sizeof(double));
     std::vector<std::unique_ptr<DevBuf>> A64;   // FP64 stencils per vertex level
     {
@@ -598,25 +688,12 @@ void Multigrid::init(Operator *fine)
         V0->nv[1] = n[1] + 1;
         V0->nv[2] = n[2] + 1;
         V0->V = (int64_t)V0->nv[0] * V0->nv[1] * V0->nv[2];
-        DevBuf xp, y, dgx, dgy;
-        xp.alloc(V0->V * 8);
-        y.alloc(V0->V * 8);
-        dgx.alloc(E * 8 * 8);
-        dgy.alloc(E * 8 * 8);
         A64.emplace_back(new DevBuf());
         A64.back()->alloc(V0->V * 27 * 8);
         VDims dv{ { V0->nv[0], V0->nv[1], V0->nv[2] } };
-        for (int col = 0; col < 27; ++col) {
-            hipLaunchKernelGGL(k_mg_probe, dim3(mg_grid(V0->V)), dim3(MGT), 0, s, dv, xp.as<double>(), col);
-            FEM_HIP(hipMemsetAsync(dgx.p, 0, dgx.bytes, s));
-            hipLaunchKernelGGL(k_mg_vertex_to_dg1<double>, dim3(mg_grid(E * 8)), dim3(MGT), 0, s, n[0], n[1], n[2],
-                               xp.as<double>(), dgx.as<double>(), (const int *)nullptr);
-            q1->apply_raw(dgx.p, dgy.p, nullptr);
-            dg1_to_vertex<double>(n, dgy.as<double>(), nullptr, corners.as<double>(), y.as<double>(), nullptr, s);
-            hipLaunchKernelGGL(k_mg_extract, dim3(mg_grid(V0->V)), dim3(MGT), 0, s, dv, y.as<double>(),
-                               A64.back()->as<double>(), col);
-            FEM_HIP(hipGetLastError());
-        }
+        hipLaunchKernelGGL(k_mg_conforming, dim3(mg_grid(V0->V * 27)), dim3(MGT), 0, s, q1->a64, dv,
+                           A64.back()->as<double>());
+        FEM_HIP(hipGetLastError());
         vx.push_back(std::move(V0));
     }
     // geometric vertex levels
@@ -627,31 +704,11 @@ void Multigrid::init(Operator *fine)
         C->V = (int64_t)C->nv[0] * C->nv[1] * C->nv[2];
         if (C->V == F.V) break;   // nothing left to coarsen
         VDims df{ { F.nv[0], F.nv[1], F.nv[2] } }, dc{ { C->nv[0], C->nv[1], C->nv[2] } };
-        DevBuf xc, xf, yf, yc;
-        xc.alloc(C->V * 8);
-        yc.alloc(C->V * 8);
-        xf.alloc(F.V * 8);
-        yf.alloc(F.V * 8);
-        DevBuf Afs;   // the fine level's stencil by offset
-        Afs.alloc(F.V * 27 * 8);
-        hipLaunchKernelGGL(k_mg_by_offset, dim3(mg_grid(F.V * 27)), dim3(MGT), 0, s, F.V, A64.back()->as<double>(),
-                           Afs.as<double>());
         A64.emplace_back(new DevBuf());
         A64.back()->alloc(C->V * 27 * 8);
-        const double *Af = Afs.as<double>();
-        for (int col = 0; col < 27; ++col) {
-            hipLaunchKernelGGL(k_mg_probe, dim3(mg_grid(C->V)), dim3(MGT), 0, s, dc, xc.as<double>(), col);
-            FEM_HIP(hipMemsetAsync(xf.p, 0, xf.bytes, s));
-            hipLaunchKernelGGL(k_mg_v_prolong<double>, dim3(mg_grid(F.V)), dim3(MGT), 0, s, df, dc, xc.as<double>(),
-                               xf.as<double>(), (const int *)nullptr);
-            hipLaunchKernelGGL(k_mg_v_apply<double>, dim3(mg_grid(F.V)), dim3(MGT), 0, s, df, Af, xf.as<double>(),
-                               (const double *)nullptr, yf.as<double>(), 1, (const int *)nullptr);
-            hipLaunchKernelGGL(k_mg_v_restrict<double>, dim3(mg_grid(C->V)), dim3(MGT), 0, s, df, dc, yf.as<double>(),
-                               yc.as<double>(), (const int *)nullptr);
-            hipLaunchKernelGGL(k_mg_extract, dim3(mg_grid(C->V)), dim3(MGT), 0, s, dc, yc.as<double>(),
-                               A64.back()->as<double>(), col);
-            FEM_HIP(hipGetLastError());
-        }
+        hipLaunchKernelGGL(k_mg_galerkin, dim3(mg_grid(C->V * 27)), dim3(MGT), 0, s, df, dc,
+                           A64[A64.size() - 2]->as<double>(), A64.back()->as<double>());
+        FEM_HIP(hipGetLastError());
         vx.push_back(std::move(C));
     }
     FEM_CHECK(vx.back()->V <= 4096, FEM_ERR_UNSUPPORTED, "multigrid: coarsest vertex level too large");
