@@ -1297,7 +1297,8 @@ __device__ void sym_eig(int K, double (*A)[KMAX], double (*Q)[KMAX])
     }
 }
 
-__global__ void k_dg_block_setup(const DGArgs<double> a, int K, double *__restrict__ Qout, double *__restrict__ lam,
+template <typename TO>
+__global__ void k_dg_block_setup(const DGArgs<double> a, int K, TO *__restrict__ Qout, TO *__restrict__ lam,
                                  bool bound)
 {
     const int64_t E = (int64_t)a.n[0] * a.n[1] * a.n[2];
@@ -1360,17 +1361,20 @@ __global__ void k_dg_block_setup(const DGArgs<double> a, int K, double *__restri
         shift = (a.c + (a.cvol && bound ? cmax : cm)) * a.h[0] * a.h[1] * a.h[2];
     }
     for (int r = 0; r < K; ++r) {
-        lam[(e * 3 + d) * K + r] = M[r][r] + shift;
-        for (int c = 0; c < K; ++c) Qout[((e * 3 + d) * K + r) * K + c] = Q[r][c];
+        lam[(e * 3 + d) * K + r] = (TO)(M[r][r] + shift);
+        for (int c = 0; c < K; ++c) Qout[((e * 3 + d) * K + r) * K + c] = (TO)Q[r][c];
     }
 }
 
-void dg_block_setup(const DGArgs<double> &a, int k, double *Q, double *lam, hipStream_t s, bool bound)
+template <typename TO>
+void dg_block_setup(const DGArgs<double> &a, int k, TO *Q, TO *lam, hipStream_t s, bool bound)
 {
     const int64_t E = (int64_t)a.n[0] * a.n[1] * a.n[2];
-    hipLaunchKernelGGL(k_dg_block_setup, dim3(grid_for(E * 3, 128)), dim3(128), 0, s, a, k, Q, lam, bound);
+    hipLaunchKernelGGL(k_dg_block_setup<TO>, dim3(grid_for(E * 3, 128)), dim3(128), 0, s, a, k, Q, lam, bound);
     FEM_HIP(hipGetLastError());
 }
+template void dg_block_setup<double>(const DGArgs<double> &, int, double *, double *, hipStream_t, bool);
+template void dg_block_setup<float>(const DGArgs<double> &, int, float *, float *, hipStream_t, bool);
 
 // one wavefront per block; lanes beyond EPB*K2 shadow element 0 (identical
 // LDS writes, no global writes)

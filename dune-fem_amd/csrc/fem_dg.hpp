@@ -139,8 +139,10 @@ void dg_rhs_bnd(const DGArgs<double> &a, int k, const double *const g[3], double
 // and eigenvalues lam_d with A_KK = sum_d (I..A_d..I), A_d = Q_d diag(lam_d) Q_d^T.
 // A pointwise reaction cvol enters by its element mean, or (bound) by its
 // largest value in the element, which makes the block an upper bound of the
-// true one (the form a smoother needs).
-void dg_block_setup(const DGArgs<double> &a, int k, double *Q /*[E][3][K][K]*/, double *lam /*[E][3][K]*/,
+// true one (the form a smoother needs).  Computed in FP64, stored as TO
+// (double or float).
+template <typename TO>
+void dg_block_setup(const DGArgs<double> &a, int k, TO *Q /*[E][3][K][K]*/, TO *lam /*[E][3][K]*/,
                     hipStream_t s, bool bound = false);
 // `dot` (may be null): each workgroup writes its FP64 partials of <x, y> to
 // dot[workgroup] and of <x, x> to dot[dg_bj_blocks(k, E) + workgroup]

@@ -577,7 +577,7 @@ int coarse1(int nf) { return (nf - 1 + 1) / 2 + 1; }   // ceil((nf - 1) / 2) + 1
 // the level's values in precision T
 void store_as(int prec, DevBuf &dst, const DevBuf &src64, int64_t n, Context *ctx)
 {
-    dst.alloc(n * (prec == FEM_F64 ? 8 : 4));
+    dst.ensure(n * (prec == FEM_F64 ? 8 : 4));
     Vec a, b;
     a.ctx = b.ctx = ctx;
     a.n = b.n = n;
@@ -603,47 +603,32 @@ void Multigrid::init(Operator *fine)
               "multigrid: cellwise or constant kappa");
     FEM_CHECK(fine->a64.kappa_mode >= 0, FEM_ERR_STATE, "coefficient data not set");
     Context *ctx = fine->ctx;
-    hipStream_t s = ctx->stream;
+    desc = d0;
+    reaction = fine->cvol64.p != nullptr;
     prec = fine->vec_prec;
     const int lp = prec == FEM_F64 ? FEM_F64 : FEM_F32;
     const int n[3] = { d0.n[0], d0.n[1], d0.n[2] };
     const int64_t E = fine->E;
-    std::vector<double> kc;
-    if (d0.kappa_mode == FEM_KAPPA_CELL) {
-        kc.resize(E);
-        FEM_HIP(hipMemcpy(kc.data(), fine->kcell64.p, sizeof(double) * E, hipMemcpyDeviceToHost));
-    }
-    // a pointwise reaction: its element means serve the coarse DG levels
-    DevBuf cmean;
-    if (fine->cvol64.p) {
-        cmean.alloc(sizeof(double) * E);
-        hipLaunchKernelGGL(k_mg_cvol_mean, dim3(mg_grid(E)), dim3(MGT), 0, s, E, fine->k, fine->a64.tab,
-                           fine->cvol64.as<double>(), cmean.as<double>());
-        FEM_HIP(hipGetLastError());
-    }
+    // a coarse operator: the finest one's description at another order and
+    // precision, with buffers for the coefficients update() fills (cellwise
+    // kappa; cvol as the element means)
     auto make_op = [&](int order, int precision) {
         std::unique_ptr<Operator> o(new Operator());
         fem_op_desc d = d0;
         d.order = order;
         d.precision = precision;
         o->init(ctx, d);
-        if (d0.kappa_mode == FEM_KAPPA_CELL) o->set_kappa_cell(kc.data());
-        if (cmean.p) {
+        if (d0.kappa_mode == FEM_KAPPA_CELL) {
+            o->kcell64.alloc(sizeof(double) * E);
+            o->kcell32.alloc(sizeof(float) * E);
+            o->a64.kcell = o->kcell64.as<double>();
+            o->a32.kcell = o->kcell32.as<float>();
+            o->a64.kappa_mode = o->a32.kappa_mode = FEM_KAPPA_CELL;
+        }
+        if (reaction) {
             const int64_t nq = E * o->k * o->k * o->k;
             o->cvol64.alloc(sizeof(double) * nq);
             o->cvol32.alloc(sizeof(float) * nq);
-            hipLaunchKernelGGL(k_mg_cvol_fill, dim3(mg_grid(nq)), dim3(MGT), 0, s, E, o->k * o->k * o->k,
-                               cmean.as<double>(), o->cvol64.as<double>());
-            FEM_HIP(hipGetLastError());
-            Vec lo, hi;
-            lo.ctx = hi.ctx = ctx;
-            lo.n = hi.n = nq;
-            lo.prec = FEM_F32;
-            hi.prec = FEM_F64;
-            lo.data = o->cvol32.p;
-            hi.data = o->cvol64.p;
-            lo.owner = hi.owner = false;
-            vec_copy(lo, hi);
             o->a64.cvol = o->cvol64.as<double>();
             o->a32.cvol = o->cvol32.as<float>();
         }
@@ -669,70 +654,131 @@ void Multigrid::init(Operator *fine)
         }
         L->k = p + 1;
         L->t.alloc(L->op->N * es);
-        // a varying pointwise reaction enters the smoother's blocks by its
-        // largest value per element: the damped sweep stays convergent (a
-        // block below the true one can make it diverge, and the V-cycle
-        // indefinite)
-        L->bj.init(L->op, FEM_PRE_BLOCK_JACOBI, true);
         dg.push_back(std::move(L));
         if (p == 1) break;
     }
-    // conforming vertex level from the Q1 level's FP64 coefficient data
-    // (every operator precision keeps it)
-    const Operator *q1 = dg.back()->op;
     corners.alloc(E * 8 * sizeof(double));
-    std::vector<std::unique_ptr<DevBuf>> A64;   // FP64 stencils per vertex level
+    // vertex levels: the conforming one, then every second vertex per
+    // direction down to at most MG_DENSE_MAX vertices
     {
         std::unique_ptr<MGVertexLevel> V0(new MGVertexLevel());
-        V0->nv[0] = n[0] + 1;
-        V0->nv[1] = n[1] + 1;
-        V0->nv[2] = n[2] + 1;
+        for (int d = 0; d < 3; ++d) V0->nv[d] = n[d] + 1;
         V0->V = (int64_t)V0->nv[0] * V0->nv[1] * V0->nv[2];
-        A64.emplace_back(new DevBuf());
-        A64.back()->alloc(V0->V * 27 * 8);
-        VDims dv{ { V0->nv[0], V0->nv[1], V0->nv[2] } };
-        hipLaunchKernelGGL(k_mg_conforming, dim3(mg_grid(V0->V * 27)), dim3(MGT), 0, s, q1->a64, dv,
-                           A64.back()->as<double>());
-        FEM_HIP(hipGetLastError());
         vx.push_back(std::move(V0));
     }
-    // geometric vertex levels
     while (vx.back()->V > MG_DENSE_MAX) {
         const MGVertexLevel &F = *vx.back();
         std::unique_ptr<MGVertexLevel> C(new MGVertexLevel());
         for (int d = 0; d < 3; ++d) C->nv[d] = coarse1(F.nv[d]);
         C->V = (int64_t)C->nv[0] * C->nv[1] * C->nv[2];
         if (C->V == F.V) break;   // nothing left to coarsen
-        VDims df{ { F.nv[0], F.nv[1], F.nv[2] } }, dc{ { C->nv[0], C->nv[1], C->nv[2] } };
-        A64.emplace_back(new DevBuf());
-        A64.back()->alloc(C->V * 27 * 8);
-        hipLaunchKernelGGL(k_mg_galerkin, dim3(mg_grid(C->V * 27)), dim3(MGT), 0, s, df, dc,
-                           A64[A64.size() - 2]->as<double>(), A64.back()->as<double>());
-        FEM_HIP(hipGetLastError());
         vx.push_back(std::move(C));
     }
     FEM_CHECK(vx.back()->V <= 4096, FEM_ERR_UNSUPPORTED, "multigrid: coarsest vertex level too large");
-    // level data in the preconditioner's precision
+    for (auto &L : vx) {
+        A64.emplace_back(new DevBuf());
+        A64.back()->alloc(L->V * 27 * 8);
+        L->x.alloc(L->V * es);
+        L->r.alloc(L->V * es);
+        L->t.alloc(L->V * es);
+    }
+    d64.alloc(vx[0]->V * 8);
+    As64.alloc(vx[0]->V * 27 * 8);
+    const int64_t VC = vx.back()->V;
+    gjX.alloc((size_t)VC * VC * 8);
+    gjY.alloc((size_t)VC * VC * 8);
+    gjp.alloc((size_t)VC * 8);
+    if (reaction) cmean.alloc(sizeof(double) * E);
+    update();
+}
+
+bool Multigrid::updatable_for(const Operator *o) const
+{
+    return o == op && (o->cvol64.p != nullptr) == reaction && !o->nonlinear();
+}
+
+void Multigrid::update()
+{
+    Operator *fine = op;
+    Context *ctx = fine->ctx;
+    hipStream_t s = ctx->stream;
+    const int lp = prec == FEM_F64 ? FEM_F64 : FEM_F32;
+    const int64_t E = fine->E;
+    FEM_CHECK(fine->a64.kappa_mode >= 0, FEM_ERR_STATE, "coefficient data not set");
+    FEM_CHECK((fine->cvol64.p != nullptr) == reaction, FEM_ERR_STATE,
+              "multigrid: the pointwise reaction was added or removed since the set-up");
+    auto round_into = [&](void *lo, const void *hi, int64_t nv) {
+        Vec a, b;
+        a.ctx = b.ctx = ctx;
+        a.n = b.n = nv;
+        a.prec = FEM_F32;
+        b.prec = FEM_F64;
+        a.data = lo;
+        b.data = const_cast<void *>(hi);
+        a.owner = b.owner = false;
+        vec_copy(a, b);
+    };
+    // coarse coefficients: the cellwise kappa as it is, cvol by its
+    // Gauss-weighted element means
+    if (reaction) {
+        hipLaunchKernelGGL(k_mg_cvol_mean, dim3(mg_grid(E)), dim3(MGT), 0, s, E, fine->k, fine->a64.tab,
+                           fine->cvol64.as<double>(), cmean.as<double>());
+        FEM_HIP(hipGetLastError());
+    }
+    auto coefficients = [&](Operator *o) {
+        if (desc.kappa_mode == FEM_KAPPA_CELL) {
+            FEM_HIP(hipMemcpyAsync(o->kcell64.p, fine->kcell64.p, sizeof(double) * E, hipMemcpyDeviceToDevice, s));
+            round_into(o->kcell32.p, o->kcell64.p, E);
+        }
+        if (reaction) {
+            const int64_t nq = E * o->k * o->k * o->k;
+            hipLaunchKernelGGL(k_mg_cvol_fill, dim3(mg_grid(nq)), dim3(MGT), 0, s, E, o->k * o->k * o->k,
+                               cmean.as<double>(), o->cvol64.as<double>());
+            FEM_HIP(hipGetLastError());
+            round_into(o->cvol32.p, o->cvol64.p, nq);
+        }
+    };
+    for (auto &L : dg) {
+        if (L->own) coefficients(L->own.get());
+        if (L->resop) coefficients(L->resop.get());
+    }
+    // smoother blocks; a varying pointwise reaction enters them by its
+    // largest value per element: the damped sweep stays convergent (a block
+    // below the true one can make it diverge, and the V-cycle indefinite)
+    for (auto &L : dg) {
+        if (L->bj.op) L->bj.update();
+        else L->bj.init(L->op, FEM_PRE_BLOCK_JACOBI, true);
+    }
+    // vertex stencils: the conforming level from the Q1 level's FP64
+    // coefficient data (every operator precision keeps it), then the
+    // Galerkin products
+    {
+        const MGVertexLevel &V0 = *vx[0];
+        VDims dv{ { V0.nv[0], V0.nv[1], V0.nv[2] } };
+        hipLaunchKernelGGL(k_mg_conforming, dim3(mg_grid(V0.V * 27)), dim3(MGT), 0, s, dg.back()->op->a64, dv,
+                           A64[0]->as<double>());
+        FEM_HIP(hipGetLastError());
+    }
+    for (size_t l = 1; l < vx.size(); ++l) {
+        const MGVertexLevel &F = *vx[l - 1], &C = *vx[l];
+        VDims df{ { F.nv[0], F.nv[1], F.nv[2] } }, dc{ { C.nv[0], C.nv[1], C.nv[2] } };
+        hipLaunchKernelGGL(k_mg_galerkin, dim3(mg_grid(C.V * 27)), dim3(MGT), 0, s, df, dc, A64[l - 1]->as<double>(),
+                           A64[l]->as<double>());
+        FEM_HIP(hipGetLastError());
+    }
+    // level data in the preconditioner's precision; a reaction-free level
+    // keeps the plain diagonal (its stencils have row sums of at most twice
+    // the diagonal away from the boundary)
+    const double lim = (desc.c != 0 || reaction) ? omega_vertex / 1.9 : 0.0;
     for (size_t l = 0; l < vx.size(); ++l) {
         MGVertexLevel &L = *vx[l];
-        DevBuf d64;
-        d64.alloc(L.V * 8);
-        // a reaction-free level keeps the plain diagonal (its stencils have
-        // row sums of at most twice the diagonal away from the boundary)
-        const double lim = (d0.c != 0 || cmean.p) ? omega_vertex / 1.9 : 0.0;
         hipLaunchKernelGGL(k_mg_dinv, dim3(mg_grid(L.V)), dim3(MGT), 0, s, L.V, A64[l]->as<double>(), lim,
                            d64.as<double>());
-        FEM_HIP(hipGetLastError());
-        DevBuf As64;
-        As64.alloc(L.V * 27 * 8);
         hipLaunchKernelGGL(k_mg_by_offset, dim3(mg_grid(L.V * 27)), dim3(MGT), 0, s, L.V, A64[l]->as<double>(),
                            As64.as<double>());
         FEM_HIP(hipGetLastError());
         store_as(lp, L.A, As64, L.V * 27, ctx);
         store_as(lp, L.dinv, d64, L.V, ctx);
-        L.x.alloc(L.V * es);
-        L.r.alloc(L.V * es);
-        L.t.alloc(L.V * es);
     }
     // dense inverse of the coarsest level, stored row-major: Gauss-Jordan on
     // the device (one launch per pivot, no pivoting) for the symmetric
@@ -743,25 +789,21 @@ void Multigrid::init(Operator *fine)
     const int VC = (int)LC.V;
     bool spd = true;
     {
-        DevBuf X, Y, pv;
-        X.alloc((size_t)VC * VC * 8);
-        Y.alloc((size_t)VC * VC * 8);
-        pv.alloc((size_t)VC * 8);
         VDims dc{ { LC.nv[0], LC.nv[1], LC.nv[2] } };
         const unsigned G = mg_grid((int64_t)VC * VC);
-        hipLaunchKernelGGL(k_mg_dense_fill, dim3(G), dim3(MGT), 0, s, dc, A64.back()->as<double>(), X.as<double>());
+        hipLaunchKernelGGL(k_mg_dense_fill, dim3(G), dim3(MGT), 0, s, dc, A64.back()->as<double>(), gjX.as<double>());
         for (int c = 0; c < VC; ++c) {
-            const DevBuf &src = (c & 1) ? Y : X;
-            DevBuf &dst = (c & 1) ? X : Y;
+            const DevBuf &src = (c & 1) ? gjY : gjX;
+            DevBuf &dst = (c & 1) ? gjX : gjY;
             hipLaunchKernelGGL(k_mg_gj_step, dim3(G), dim3(MGT), 0, s, VC, c, src.as<double>(), dst.as<double>(),
-                               pv.as<double>());
+                               gjp.as<double>());
         }
         FEM_HIP(hipGetLastError());
         std::vector<double> piv(VC);
-        FEM_HIP(hipMemcpyAsync(piv.data(), pv.p, sizeof(double) * VC, hipMemcpyDeviceToHost, s));
+        FEM_HIP(hipMemcpyAsync(piv.data(), gjp.p, sizeof(double) * VC, hipMemcpyDeviceToHost, s));
         FEM_HIP(hipStreamSynchronize(s));
         for (int c = 0; c < VC; ++c) spd = spd && piv[c] > 0 && std::isfinite(piv[c]);
-        if (spd) store_as(lp, coarse_inv, (VC & 1) ? Y : X, (int64_t)VC * VC, ctx);
+        if (spd) store_as(lp, coarse_inv, (VC & 1) ? gjY : gjX, (int64_t)VC * VC, ctx);
     }
     if (!spd) {
         const MGVertexLevel &L = LC;

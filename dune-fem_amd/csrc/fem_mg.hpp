@@ -38,8 +38,25 @@ struct Multigrid {
     DevBuf corners;                  // [E][8] FP64: Q1 modes -> vertex functionals
     int nu_dg = 2, nu_vertex = 1;    // smoothing sweeps before and after the coarse correction
     double omega_dg = 0.8, omega_vertex = 0.7;
+    fem_op_desc desc{};              // the finest operator's, at init
+    bool reaction = false;           // built with a pointwise reaction cvol
+    // set-up data kept for update(): the FP64 stencils per vertex level, the
+    // element means of cvol, and scratch for the level data and the
+    // coarsest inverse
+    std::vector<std::unique_ptr<DevBuf>> A64;
+    DevBuf cmean, d64, As64, gjX, gjY, gjp;
 
+    // builds the hierarchy (levels, operators, buffers), then update()
     void init(Operator *fine);
+    // computes everything that depends on the finest operator's coefficients
+    // (kappa, c, cvol) in the existing buffers: the coarse operators'
+    // coefficients, the smoother blocks, the vertex stencils and Jacobi
+    // scalings, the coarsest inverse.  No allocation after the first call
+    // (the host fallback of an indefinite coarsest level aside).
+    void update();
+    // the operator the hierarchy was built on, with the same structure
+    // (a pointwise reaction present or not)
+    bool updatable_for(const Operator *o) const;
     // z = V-cycle(r); `skip` (device flag, may be null) skips all work
     void apply_raw(const void *r, void *z, const int *skip);
     // one damped block-Jacobi sweep of the finest level: x += omega B^-1 (r - t)

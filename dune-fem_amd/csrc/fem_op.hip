@@ -587,51 +587,30 @@ void Operator::block_diagonal(Vec &blocks) const
     FEM_HIP(hipStreamSynchronize(s));
 }
 
-void Precond::init(Operator *o, int k_, bool bound)
+static bool same_desc(const fem_op_desc &a, const fem_op_desc &b)
+{
+    return a.kind == b.kind && a.n[0] == b.n[0] && a.n[1] == b.n[1] && a.n[2] == b.n[2] && a.order == b.order &&
+           a.beta == b.beta && a.b[0] == b.b[0] && a.b[1] == b.b[1] && a.b[2] == b.b[2] && a.c == b.c &&
+           a.kappa_mode == b.kappa_mode && a.kappa_const == b.kappa_const && a.precision == b.precision;
+}
+
+void Precond::init(Operator *o, int k_, bool bound_)
 {
     op = o;
     kind = k_;
     prec = o->vec_prec;
-    hipStream_t s = o->ctx->stream;
+    bound = bound_;
+    desc = o->desc;
     if (kind == FEM_PRE_JACOBI) {
-        DevBuf d;
-        d.alloc(o->N * sizeof(double));
-        Vec dv;
-        dv.ctx = o->ctx; dv.n = o->N; dv.prec = FEM_F64; dv.data = d.p; dv.owner = false;
-        o->diagonal(dv);
-        dinv.alloc(o->N * (prec == FEM_F64 ? 8 : 4));
-        if (prec == FEM_F64)
-            hipLaunchKernelGGL(k_jacobi_inv<double>, dim3(grid_for(o->N, 256, 8192)), dim3(256), 0, s, d.as<double>(), dinv.as<double>(), o->N);
-        else
-            hipLaunchKernelGGL(k_jacobi_inv<float>, dim3(grid_for(o->N, 256, 8192)), dim3(256), 0, s, d.as<double>(), dinv.as<float>(), o->N);
-        FEM_HIP(hipGetLastError());
-        FEM_HIP(hipStreamSynchronize(s));
+        update();
     } else if (kind == FEM_PRE_BLOCK_JACOBI) {
         FEM_CHECK(o->is_dg(), FEM_ERR_UNSUPPORTED, "block Jacobi needs a DG operator");
         FEM_CHECK(o->desc.b[0] == 0 && o->desc.b[1] == 0 && o->desc.b[2] == 0, FEM_ERR_UNSUPPORTED,
                   "block Jacobi: operators without advection (b = 0)");
-        FEM_CHECK(!o->nonlinear(), FEM_ERR_STATE, "block Jacobi: the operator is nonlinear (take its Jacobian)");
-        FEM_CHECK(o->a64.kappa_mode >= 0, FEM_ERR_STATE, "coefficient data not set");
-        const int k = o->k;
-        DevBuf Q64, l64;
-        Q64.alloc((size_t)o->E * 3 * k * k * sizeof(double));
-        l64.alloc((size_t)o->E * 3 * k * sizeof(double));
-        dg_block_setup(o->a64, k, Q64.as<double>(), l64.as<double>(), s, bound);
-        if (prec == FEM_F64) {
-            Q.alloc(Q64.bytes); lam.alloc(l64.bytes);
-            FEM_HIP(hipMemcpyAsync(Q.p, Q64.p, Q64.bytes, hipMemcpyDeviceToDevice, s));
-            FEM_HIP(hipMemcpyAsync(lam.p, l64.p, l64.bytes, hipMemcpyDeviceToDevice, s));
-        } else {
-            Q.alloc(Q64.bytes / 2); lam.alloc(l64.bytes / 2);
-            Vec a, b;
-            a.ctx = b.ctx = o->ctx;
-            a.prec = FEM_F32; b.prec = FEM_F64; a.owner = b.owner = false;
-            a.n = b.n = (int64_t)o->E * 3 * k * k; a.data = Q.p; b.data = Q64.p;
-            vec_copy(a, b);
-            a.n = b.n = (int64_t)o->E * 3 * k; a.data = lam.p; b.data = l64.p;
-            vec_copy(a, b);
-        }
-        FEM_HIP(hipStreamSynchronize(s));
+        const size_t es = prec == FEM_F64 ? 8 : 4;
+        Q.alloc((size_t)o->E * 3 * o->k * o->k * es);
+        lam.alloc((size_t)o->E * 3 * o->k * es);
+        update();
     } else if (kind == FEM_PRE_MULTIGRID) {
         mg = new Multigrid();
         mg->init(o);
@@ -639,6 +618,47 @@ void Precond::init(Operator *o, int k_, bool bound)
     } else {
         throw Error(FEM_ERR_UNSUPPORTED, "preconditioner kind (the learned one is made by fem_precond_create_learned)");
     }
+}
+
+void Precond::update()
+{
+    Operator *o = op;
+    hipStream_t s = o->ctx->stream;
+    if (kind == FEM_PRE_JACOBI) {
+        DevBuf d;
+        d.alloc(o->N * sizeof(double));
+        Vec dv;
+        dv.ctx = o->ctx; dv.n = o->N; dv.prec = FEM_F64; dv.data = d.p; dv.owner = false;
+        o->diagonal(dv);
+        dinv.ensure(o->N * (prec == FEM_F64 ? 8 : 4));
+        if (prec == FEM_F64)
+            hipLaunchKernelGGL(k_jacobi_inv<double>, dim3(grid_for(o->N, 256, 8192)), dim3(256), 0, s, d.as<double>(), dinv.as<double>(), o->N);
+        else
+            hipLaunchKernelGGL(k_jacobi_inv<float>, dim3(grid_for(o->N, 256, 8192)), dim3(256), 0, s, d.as<double>(), dinv.as<float>(), o->N);
+        FEM_HIP(hipGetLastError());
+        FEM_HIP(hipStreamSynchronize(s));
+    } else if (kind == FEM_PRE_BLOCK_JACOBI) {
+        FEM_CHECK(!o->nonlinear(), FEM_ERR_STATE, "block Jacobi: the operator is nonlinear (take its Jacobian)");
+        FEM_CHECK(o->a64.kappa_mode >= 0, FEM_ERR_STATE, "coefficient data not set");
+        // the eigen data are computed in FP64 and stored in the
+        // preconditioner's precision
+        if (prec == FEM_F64)
+            dg_block_setup<double>(o->a64, o->k, Q.as<double>(), lam.as<double>(), s, bound);
+        else
+            dg_block_setup<float>(o->a64, o->k, Q.as<float>(), lam.as<float>(), s, bound);
+        FEM_HIP(hipStreamSynchronize(s));
+    } else if (kind == FEM_PRE_MULTIGRID) {
+        mg->update();
+    } else {
+        FEM_CHECK(kind == FEM_PRE_NONE, FEM_ERR_UNSUPPORTED, "preconditioner update: Jacobi, block Jacobi, multigrid");
+    }
+}
+
+bool Precond::updatable_for(const Operator *o, int k_) const
+{
+    if (o != op || k_ != kind || lowp != nullptr || o->vec_prec != prec || !same_desc(o->desc, desc)) return false;
+    if (kind == FEM_PRE_MULTIGRID) return mg && mg->updatable_for(o);
+    return kind == FEM_PRE_JACOBI || kind == FEM_PRE_BLOCK_JACOBI;
 }
 
 // y (FP64) = widen(M32 round(x)): the FP32 preconditioner inside an FP64

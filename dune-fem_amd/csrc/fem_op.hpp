@@ -18,6 +18,11 @@ struct DevBuf {
         }
         bytes = b;
     }
+    // allocate unless it already holds exactly b bytes (contents kept then)
+    void ensure(size_t b)
+    {
+        if (b != bytes || (b && !p)) alloc(b);
+    }
     void free()
     {
         if (p) (void)hipFree(p);
@@ -125,9 +130,18 @@ struct Precond {
     // widened back; cx, cy hold the FP32 vectors
     const Precond *lowp = nullptr;
     DevBuf cx, cy;
+    bool bound = false;   // block Jacobi: a pointwise reaction by its largest value per element
+    fem_op_desc desc{};   // the operator's, at init
     // bound (block Jacobi): a pointwise reaction enters the element blocks by
     // its largest value, not its mean (the multigrid smoothers)
     void init(Operator *o, int kind, bool bound = false);
+    // recompute the data built from the operator's coefficients in place
+    // (block Jacobi, Jacobi, multigrid), for new coefficient values on the
+    // same operator
+    void update();
+    // whether update() serves operator o as preconditioner kind k: the same
+    // operator object with the same description and structure
+    bool updatable_for(const Operator *o, int k) const;
     void init_widening(Operator *o64, const Precond *p32);
     Learned *learned_part() const { return lowp ? lowp->learned : learned; }
     // `dot` (block Jacobi only, may be null): per-workgroup FP64 partials of

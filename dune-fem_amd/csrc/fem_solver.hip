@@ -623,25 +623,30 @@ void Solver::bind(Operator *o, Precond *p, Operator *io, Precond *ip)
     }
     // fem.solver.preconditioning.method: built here when no preconditioner
     // object is passed (KrylovInverseOperator::bind creates its own the same
-    // way, krylovinverseoperators.hh:86-97)
-    own_pre.reset();
-    own_ipre.reset();
+    // way, krylovinverseoperators.hh:86-97).  Binding the same operator again
+    // (a Newton solve binds each Jacobian S'(w) to the one solver) updates
+    // the one built before in place: the same result as a new one, without
+    // building the hierarchy and its buffers again.
     widened.reset();
     if (p && p->prec == FEM_F32 && o->vec_prec == FEM_F64) {
         widened.reset(new Precond());
         widened->init_widening(o, p);
         p = widened.get();
     }
-    if (!p && prm.preconditioning != FEM_PRE_NONE && !prm.refinement) {
-        own_pre.reset(new Precond());
-        own_pre->init(o, prm.preconditioning);
-        p = own_pre.get();
-    }
-    if (prm.refinement && !ip && prm.preconditioning != FEM_PRE_NONE) {
-        own_ipre.reset(new Precond());
-        own_ipre->init(io, prm.preconditioning);
-        ip = own_ipre.get();
-    }
+    auto own = [&](std::unique_ptr<Precond> &mine, Operator *target) {
+        if (mine && mine->updatable_for(target, prm.preconditioning)) {
+            mine->update();
+        } else {
+            mine.reset();
+            mine.reset(new Precond());
+            mine->init(target, prm.preconditioning);
+        }
+        return mine.get();
+    };
+    if (!p && prm.preconditioning != FEM_PRE_NONE && !prm.refinement) p = own(own_pre, o);
+    else own_pre.reset();
+    if (prm.refinement && !ip && prm.preconditioning != FEM_PRE_NONE) ip = own(own_ipre, io);
+    else own_ipre.reset();
     op = o;
     pre = p;
     iop = io;

@@ -108,6 +108,45 @@ def test_multigrid_vcycle_with_reaction_matches_oracle(ctx, n, p):
     assert err < 1e-12 and e32 < 1e-4
 
 
+@pytest.mark.parametrize("pre", [fa.PRE_MULTIGRID, fa.PRE_BLOCK_JACOBI])
+@pytest.mark.parametrize("prec", [fa.F64, fa.F32])
+def test_rebinding_updates_the_solvers_preconditioner(ctx, pre, prec):
+    # a solver that builds its preconditioner from
+    # fem.solver.preconditioning.method updates it in place when the same
+    # operator is bound again (the Newton steps do this): after new cvol and
+    # kappa values the solve equals, bit for bit, that of a solver built
+    # afresh; removing cvol changes the structure and rebuilds
+    n, p = (9, 7, 8), 2
+    rng = np.random.default_rng(17)
+    E = n[0] * n[1] * n[2]
+    op = fa.GalerkinOperator(ctx, fa.OP_DG_SIPG_3D, n, p, 40.0, (0, 0, 0), 0.2, fa.KAPPA_CELL, precision=prec)
+    op.set_kappa_cell(10.0 ** rng.uniform(-1, 1, size=E))
+    op.set_reaction_qp(10.0 ** rng.uniform(0, 3, size=op.N))
+    b = op.vector(rng.normal(size=op.N))
+    kw = dict(method=fa.CG, tolerance=1e-6 if prec == fa.F32 else 1e-10, errormeasure=fa.RELATIVE,
+              maxiterations=3000, preconditioning=pre)
+    kept = fa.InverseOperator(ctx, **kw)
+    kept.bind(op)
+    X = op.vector()
+    kept(b, X)
+    changes = [("cvol", lambda: op.set_reaction_qp(10.0 ** rng.uniform(1, 4, size=op.N))),
+               ("kappa", lambda: op.set_kappa_cell(10.0 ** rng.uniform(-2, 2, size=E))),
+               ("cvol removed", lambda: op.set_reaction_qp(None))]
+    for name, change in changes:
+        change()
+        kept.bind(op)
+        X.clear()
+        kept(b, X)
+        fresh = fa.InverseOperator(ctx, **kw)
+        fresh.bind(op)
+        Y = op.vector()
+        fresh(b, Y)
+        ik, jf = kept.info(), fresh.info()
+        print(name, "kept", ik["iterations"], "fresh", jf["iterations"], flush=True)
+        assert ik["converged"] and ik["iterations"] == jf["iterations"]
+        assert np.array_equal(X.download(), Y.download())
+
+
 def test_multigrid_is_symmetric_positive(ctx):
     P = wl.problem3(n=12)
     op = dg(ctx, P["n"], P["p"], P["beta"], ("cell", P["kcell"]))
