@@ -382,7 +382,12 @@ int fem_solver_destroy(fem_solver *s);
  * Without refinement an FP64 operator also takes a preconditioner built on
  * the FP32 (or BF16-MFMA) operator of the same problem: each application
  * rounds its argument to FP32 and widens the result, so the Krylov vectors,
- * dots and the stopping test stay FP64 while the V-cycle runs in FP32. */
+ * dots and the stopping test stay FP64 while the V-cycle runs in FP32.
+ * Without a preconditioner argument the solver builds its own from
+ * fem.solver.preconditioning.method (krylovinverseoperators.hh:86-97);
+ * binding the same operator again, after its coefficients changed,
+ * recomputes that one in place (same result as a new one, no allocation
+ * for block Jacobi and multigrid), as each Newton step does with S'(w). */
 int fem_solver_bind(fem_solver *s, fem_op *op, fem_precond *pre, fem_op *inner_op,
                     fem_precond *inner_pre);
 int fem_solver_unbind(fem_solver *s);
