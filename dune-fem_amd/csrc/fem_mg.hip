@@ -481,30 +481,44 @@ __global__ void k_mg_galerkin(VDims f, VDims cd, const double *__restrict__ Af, 
         const int W[3] = { V[0] + so % 3 - 1, V[1] + (so / 3) % 3 - 1, V[2] + so / 9 - 1 };
         double acc = 0;
         if (W[0] >= 0 && W[1] >= 0 && W[2] >= 0 && W[0] < cd.n[0] && W[1] < cd.n[1] && W[2] < cd.n[2]) {
-            int lo[3], hi[3];
-            for (int d = 0; d < 3; ++d) {
-                lo[d] = max(0, 2 * V[d] - 1);
-                hi[d] = min(f.n[d] - 1, 2 * V[d] + 1);
-            }
-            for (int iz = lo[2]; iz <= hi[2]; ++iz) {
-                const double wz = w1(iz, V[2], f.n[2], cd.n[2]);
-                if (wz == 0.0) continue;
-                for (int iy = lo[1]; iy <= hi[1]; ++iy) {
-                    const double wy = w1(iy, V[1], f.n[1], cd.n[1]);
-                    if (wy == 0.0) continue;
-                    for (int ix = lo[0]; ix <= hi[0]; ++ix) {
-                        const double wx = w1(ix, V[0], f.n[0], cd.n[0]);
-                        if (wx == 0.0) continue;
+            // the 1D weights of V and W at the fine vertices 2V-1 .. 2V+1 and
+            // 2W-1 .. 2W+1 (the supports of both; 0 outside the lattice)
+            double wv[3][3], ww[3][3];
+            for (int d = 0; d < 3; ++d)
+                for (int m = 0; m < 3; ++m) {
+                    const int fv = 2 * V[d] - 1 + m, fw = 2 * W[d] - 1 + m;
+                    wv[d][m] = (fv >= 0 && fv < f.n[d]) ? w1(fv, V[d], f.n[d], cd.n[d]) : 0.0;
+                    ww[d][m] = (fw >= 0 && fw < f.n[d]) ? w1(fw, W[d], f.n[d], cd.n[d]) : 0.0;
+                }
+            const int sh[3] = { 2 * (V[0] - W[0]), 2 * (V[1] - W[1]), 2 * (V[2] - W[2]) };
+            for (int az = 0; az < 3; ++az) {
+                if (wv[2][az] == 0.0) continue;
+                const int iz = 2 * V[2] - 1 + az;
+                for (int ay = 0; ay < 3; ++ay) {
+                    if (wv[1][ay] == 0.0) continue;
+                    const int iy = 2 * V[1] - 1 + ay;
+                    for (int ax = 0; ax < 3; ++ax) {
+                        if (wv[0][ax] == 0.0) continue;
+                        const int ix = 2 * V[0] - 1 + ax;
                         const double *a = Af + (ix + (int64_t)f.n[0] * (iy + (int64_t)f.n[1] * iz)) * 27;
                         double row = 0;
-                        for (int t = 0; t < 27; ++t) {
-                            const int jx = ix + t % 3 - 1, jy = iy + (t / 3) % 3 - 1, jz = iz + t / 9 - 1;
-                            if (jx < 0 || jy < 0 || jz < 0 || jx >= f.n[0] || jy >= f.n[1] || jz >= f.n[2]) continue;
-                            const double pj = w1(jx, W[0], f.n[0], cd.n[0]) * w1(jy, W[1], f.n[1], cd.n[1]) *
-                                              w1(jz, W[2], f.n[2], cd.n[2]);
-                            row = fma(a[t], pj, row);
+                        // neighbour j = i + t - 1 per direction sits at index
+                        // b = j - (2W - 1) = a + sh + t - 1 of W's support
+                        for (int tz = 0; tz < 3; ++tz) {
+                            const int bz = az + sh[2] + tz - 1;
+                            if (bz < 0 || bz > 2 || ww[2][bz] == 0.0) continue;
+                            for (int ty = 0; ty < 3; ++ty) {
+                                const int by = ay + sh[1] + ty - 1;
+                                if (by < 0 || by > 2 || ww[1][by] == 0.0) continue;
+                                const double pzy = ww[2][bz] * ww[1][by];
+                                for (int tx = 0; tx < 3; ++tx) {
+                                    const int bx = ax + sh[0] + tx - 1;
+                                    if (bx < 0 || bx > 2) continue;
+                                    row = fma(a[tx + 3 * ty + 9 * tz], pzy * ww[0][bx], row);
+                                }
+                            }
                         }
-                        acc = fma(wx * wy * wz, row, acc);
+                        acc = fma(wv[2][az] * wv[1][ay] * wv[0][ax], row, acc);
                     }
                 }
             }
@@ -549,24 +563,41 @@ __global__ void k_mg_dense_fill(VDims d, const double *__restrict__ A, double *_
 // the inverse.  The pivots are the leading Schur complements' diagonals:
 // all positive exactly when the matrix is symmetric positive definite, for
 // which no pivoting is needed; they are recorded for the host to check.
-__global__ void k_mg_gj_step(int V, int c, const double *__restrict__ X, double *__restrict__ Y,
+__device__ __forceinline__ double gj_entry(const double *__restrict__ X, int V, int c, double ip, int r, int k)
+{
+    if (r == c) return k == c ? ip : X[(int64_t)c * V + k] * ip;
+    if (k == c) return -X[(int64_t)r * V + c] * ip;
+    return X[(int64_t)r * V + k] - X[(int64_t)r * V + c] * X[(int64_t)c * V + k] * ip;
+}
+
+// `steps` (1 or 2) pivot steps c, c + 1 in one launch: the entries of the
+// intermediate matrix that step c + 1 reads (its own, row c + 1, column
+// c + 1) are formed from X by the same operations as in a launch of their
+// own, so the result equals two single steps bit for bit
+__global__ void k_mg_gj_step(int V, int c, int steps, const double *__restrict__ X, double *__restrict__ Y,
                              double *__restrict__ piv)
 {
     const int64_t V2 = (int64_t)V * V;
     const double pc = X[(int64_t)c * V + c];
     const double ip = 1.0 / pc;
+    const int c1 = c + 1;
+    const double pc1 = steps == 2 ? gj_entry(X, V, c, ip, c1, c1) : 1.0;
+    const double ip1 = 1.0 / pc1;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < V2; i += (int64_t)gridDim.x * blockDim.x) {
         const int r = (int)(i / V), k = (int)(i - (int64_t)r * V);
         double y;
-        if (r == c && k == c) {
-            y = ip;
-            piv[c] = pc;
-        } else if (r == c) {
-            y = X[i] * ip;
-        } else if (k == c) {
-            y = -X[i] * ip;
+        if (steps == 1) {
+            y = gj_entry(X, V, c, ip, r, k);
+        } else if (r == c1) {
+            y = k == c1 ? ip1 : gj_entry(X, V, c, ip, c1, k) * ip1;
+        } else if (k == c1) {
+            y = -gj_entry(X, V, c, ip, r, c1) * ip1;
         } else {
-            y = X[i] - X[(int64_t)r * V + c] * X[(int64_t)c * V + k] * ip;
+            y = gj_entry(X, V, c, ip, r, k) - gj_entry(X, V, c, ip, r, c1) * gj_entry(X, V, c, ip, c1, k) * ip1;
+        }
+        if (i == 0) {
+            piv[c] = pc;
+            if (steps == 2) piv[c1] = pc1;
         }
         Y[i] = y;
     }
@@ -781,7 +812,7 @@ void Multigrid::update()
         store_as(lp, L.dinv, d64, L.V, ctx);
     }
     // dense inverse of the coarsest level, stored row-major: Gauss-Jordan on
-    // the device (one launch per pivot, no pivoting) for the symmetric
+    // the device (two pivots per launch, no pivoting) for the symmetric
     // positive definite operators of diffusion and non-negative reaction; a
     // non-positive pivot (an indefinite operator) sends the inversion to the
     // host, with partial pivoting
@@ -792,18 +823,19 @@ void Multigrid::update()
         VDims dc{ { LC.nv[0], LC.nv[1], LC.nv[2] } };
         const unsigned G = mg_grid((int64_t)VC * VC);
         hipLaunchKernelGGL(k_mg_dense_fill, dim3(G), dim3(MGT), 0, s, dc, A64.back()->as<double>(), gjX.as<double>());
-        for (int c = 0; c < VC; ++c) {
-            const DevBuf &src = (c & 1) ? gjY : gjX;
-            DevBuf &dst = (c & 1) ? gjX : gjY;
-            hipLaunchKernelGGL(k_mg_gj_step, dim3(G), dim3(MGT), 0, s, VC, c, src.as<double>(), dst.as<double>(),
-                               gjp.as<double>());
+        int launches = 0;
+        for (int c = 0; c < VC; c += 2, ++launches) {
+            const DevBuf &src = (launches & 1) ? gjY : gjX;
+            DevBuf &dst = (launches & 1) ? gjX : gjY;
+            hipLaunchKernelGGL(k_mg_gj_step, dim3(G), dim3(MGT), 0, s, VC, c, std::min(2, VC - c), src.as<double>(),
+                               dst.as<double>(), gjp.as<double>());
         }
         FEM_HIP(hipGetLastError());
         std::vector<double> piv(VC);
         FEM_HIP(hipMemcpyAsync(piv.data(), gjp.p, sizeof(double) * VC, hipMemcpyDeviceToHost, s));
         FEM_HIP(hipStreamSynchronize(s));
         for (int c = 0; c < VC; ++c) spd = spd && piv[c] > 0 && std::isfinite(piv[c]);
-        if (spd) store_as(lp, coarse_inv, (VC & 1) ? gjY : gjX, (int64_t)VC * VC, ctx);
+        if (spd) store_as(lp, coarse_inv, (launches & 1) ? gjY : gjX, (int64_t)VC * VC, ctx);
     }
     if (!spd) {
         const MGVertexLevel &L = LC;

@@ -188,11 +188,14 @@ def test_gpu_newton_with_compiled_reaction_returns_manufactured_solution(ctx):
 
 
 @pytest.mark.gpu
-def test_gpu_galerkin_scheme_from_symbolic_integrands(ctx):
+@pytest.mark.parametrize("pre", ["blockjacobi", "multigrid"])
+def test_gpu_galerkin_scheme_from_symbolic_integrands(ctx, pre):
     # dune.fem.scheme.galerkin's path: integrands (cellwise kappa, reaction
     # exp(u), a source expression) on a DG Q2 space, scheme.solve(target)
     # with the reference's parameter keys; the solution's residual, and the
-    # load vector against the oracle's at the same points
+    # load vector against the oracle's at the same points.  With multigrid
+    # the V-cycle is built on each Jacobian (the reaction exp(u_h) in its
+    # levels)
     import dunefem_amd as fa
     n, p = (5, 4, 6), 2
     kc = np.random.default_rng(13).uniform(0.5, 2.0, size=120)
@@ -202,10 +205,10 @@ def test_gpu_galerkin_scheme_from_symbolic_integrands(ctx):
                          parameters={"fem.solver.nonlinear.tolerance": 1e-9,
                                      "fem.solver.nonlinear.linear.tolerance": 1e-11,
                                      "fem.solver.nonlinear.linear.errormeasure": "relative",
-                                     "fem.solver.nonlinear.linear.preconditioning.method": "blockjacobi"})
+                                     "fem.solver.nonlinear.linear.preconditioning.method": pre})
     U = scheme.op.vector()
     info = scheme.solve(U)
-    print("galerkin scheme solve", info, flush=True)
+    print("galerkin scheme solve", pre, info, flush=True)
     assert set(info) == {"converged", "iterations", "linear_iterations", "timing"}
     assert info["converged"] and 2 <= info["iterations"] <= 10
     R = scheme.op.vector()
