@@ -14,6 +14,8 @@
 // the summation order fixed: results are deterministic and need no atomics.
 #include "fem_cg.hpp"
 
+#include <algorithm>
+
 namespace femamd {
 
 CGTab make_cg_tab()
@@ -35,7 +37,7 @@ template <int TX, int TY, typename T>
 __global__ void __launch_bounds__((TX + 1) * (TY + 1))
     k_cg(int mode, const CGInst *__restrict__ insts, int ninst, const CGTab *__restrict__ tabp,
          const T *__restrict__ kvol, const T *__restrict__ in, T *__restrict__ out, const int *__restrict__ skip,
-         const int *__restrict__ inst_done)
+         const int *__restrict__ inst_done, const CGReaction<T> rx)
 {
     constexpr int CG_LX = 2 * TX + 3, CG_LY = 2 * TY + 3;
     if (skip && *skip) return;
@@ -103,7 +105,16 @@ __global__ void __launch_bounds__((TX + 1) * (TY + 1))
                     const T kap = kvol ? kvol[e * 9 + qx + 3 * qy] : (T)P.kappa_const;
                     fx[qy][qx] = W * kap * gx * (T)(1.0 / (hx * hx));
                     fy[qy][qx] = W * kap * gy * (T)(1.0 / (hy * hy));
-                    fm[qy][qx] = W * (T)P.c * uq;
+                    // reaction c + cvol + coef u^pw (the pointwise and
+                    // nonlinear parts of a single instance)
+                    T r = (T)P.c;
+                    if (rx.cvol) r += rx.cvol[e * 9 + qx + 3 * qy];
+                    if (rx.pw) {
+                        T pw = rx.coef;
+                        for (int t = 0; t < rx.pw; ++t) pw *= uq;
+                        r += pw;
+                    }
+                    fm[qy][qx] = W * r * uq;
                 }
             // transpose: y then x
 #pragma unroll
@@ -141,7 +152,8 @@ __global__ void __launch_bounds__((TX + 1) * (TY + 1))
                             const double lx = tb.L[qx][ix], ly = tb.L[qy][iy];
                             if (mode == CG_MODE_DIAG) {
                                 const double gx = tb.Ld[qx][ix] * ly / hx, gy = lx * tb.Ld[qy][iy] / hy;
-                                a += W * (k * (gx * gx + gy * gy) + P.c * lx * lx * ly * ly);
+                                const double r = P.c + (rx.cvol ? (double)rx.cvol[e * 9 + qx + 3 * qy] : 0.0);
+                                a += W * (k * (gx * gx + gy * gy) + r * lx * lx * ly * ly);
                             } else {
                                 a += W * k * lx * ly;  // kvol carries f for the load vector
                             }
@@ -172,19 +184,67 @@ __global__ void __launch_bounds__((TX + 1) * (TY + 1))
         const int64_t node = (int64_t)J * NX + I;
         const bool bnd = I == 0 || J == 0 || I == 2 * P.nx || J == 2 * P.ny;
         T val = lat[(J - J0 + 2) * CG_LX + (I - I0 + 2)];
+        if (mode == CG_MODE_RHS_ADD) {
+            // accumulate into the interior rows; the Dirichlet rows keep what
+            // the apply wrote there
+            if (!bnd) o[node] += val;
+            continue;
+        }
         if (bnd) val = mode == CG_MODE_APPLY ? u[node] : mode == CG_MODE_DIAG ? T(1) : T(0);
         o[node] = val;
     }
 }
 
+// out[e*9 + qx + 3 qy] = base + coef u(x_q)^pw (base may be null: 0): u at
+// the volume quadrature points of a single instance, one thread per element
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_cg_eval_qp(int nx, int ny, const CGTab *__restrict__ tabp, const T *__restrict__ u, const T *__restrict__ base,
+                 T coef, int pw, T *__restrict__ out)
+{
+    const int64_t E = (int64_t)nx * ny;
+    const CGTab &tb = *tabp;
+    const int NX = 2 * nx + 1;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < E; e += (int64_t)gridDim.x * blockDim.x) {
+        const int ex = (int)(e % nx), ey = (int)(e / nx);
+        T ul[3][3];
+        for (int iy = 0; iy < 3; ++iy)
+            for (int ix = 0; ix < 3; ++ix) ul[iy][ix] = u[(int64_t)(2 * ey + iy) * NX + 2 * ex + ix];
+        for (int qy = 0; qy < 3; ++qy)
+            for (int qx = 0; qx < 3; ++qx) {
+                T uq = 0;
+                for (int iy = 0; iy < 3; ++iy)
+                    for (int ix = 0; ix < 3; ++ix) uq += (T)tb.L[qy][iy] * (T)tb.L[qx][ix] * ul[iy][ix];
+                T v = coef;
+                for (int t = 0; t < pw; ++t) v *= uq;
+                out[e * 9 + qx + 3 * qy] = (base ? base[e * 9 + qx + 3 * qy] : T(0)) + v;
+            }
+    }
+}
+
+template <typename T>
+void cg_eval_qp(int nx, int ny, const CGTab *d_tab, const T *u, const T *base, T coef, int pw, T *out, hipStream_t s)
+{
+    const int64_t E = (int64_t)nx * ny;
+    const unsigned G = (unsigned)std::min<int64_t>((E + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_cg_eval_qp<T>, dim3(G), dim3(256), 0, s, nx, ny, d_tab, u, base, coef, pw, out);
+    FEM_HIP(hipGetLastError());
+}
+template void cg_eval_qp<double>(int, int, const CGTab *, const double *, const double *, double, int, double *,
+                                 hipStream_t);
+template void cg_eval_qp<float>(int, int, const CGTab *, const float *, const float *, float, int, float *,
+                                hipStream_t);
+
 template <typename T>
 void cg_launch_shape(int shape, int mode, const CGInst *d_inst, int ninst, int total_tiles, const CGTab *d_tab,
-                     const T *kvol, const T *in, T *out, const int *skip, const int *inst_done, hipStream_t s)
+                     const T *kvol, const T *in, T *out, const int *skip, const int *inst_done, hipStream_t s,
+                     const CGReaction<T> &rx)
 {
+    FEM_CHECK(ninst == 1 || (!rx.cvol && !rx.pw), FEM_ERR_ARG, "pointwise/nonlinear reaction: a single instance");
 #define CGS(ID, TX, TY)                                                                                       \
     case ID:                                                                                                 \
         hipLaunchKernelGGL((k_cg<TX, TY, T>), dim3(total_tiles), dim3((TX + 1) * (TY + 1)), 0, s, mode, d_inst, \
-                           ninst, d_tab, kvol, in, out, skip, inst_done);                                    \
+                           ninst, d_tab, kvol, in, out, skip, inst_done, rx);                                \
         break;
     switch (shape) {
         CGS(0, 15, 15) CGS(1, 7, 7) CGS(2, 15, 7) CGS(3, 31, 7) CGS(4, 7, 31)
@@ -196,18 +256,20 @@ void cg_launch_shape(int shape, int mode, const CGInst *d_inst, int ninst, int t
 
 template <typename T>
 void cg_launch(int mode, const CGInst *d_inst, int ninst, int total_tiles, const CGTab *d_tab, const T *kvol,
-               const T *in, T *out, const int *skip, hipStream_t s)
+               const T *in, T *out, const int *skip, hipStream_t s, const CGReaction<T> &rx)
 {
-    cg_launch_shape<T>(0, mode, d_inst, ninst, total_tiles, d_tab, kvol, in, out, skip, nullptr, s);
+    cg_launch_shape<T>(0, mode, d_inst, ninst, total_tiles, d_tab, kvol, in, out, skip, nullptr, s, rx);
 }
 
 template void cg_launch<double>(int, const CGInst *, int, int, const CGTab *, const double *, const double *,
-                                double *, const int *, hipStream_t);
+                                double *, const int *, hipStream_t, const CGReaction<double> &);
 template void cg_launch<float>(int, const CGInst *, int, int, const CGTab *, const float *, const float *,
-                               float *, const int *, hipStream_t);
+                               float *, const int *, hipStream_t, const CGReaction<float> &);
 template void cg_launch_shape<double>(int, int, const CGInst *, int, int, const CGTab *, const double *,
-                                      const double *, double *, const int *, const int *, hipStream_t);
+                                      const double *, double *, const int *, const int *, hipStream_t,
+                                      const CGReaction<double> &);
 template void cg_launch_shape<float>(int, int, const CGInst *, int, int, const CGTab *, const float *,
-                                     const float *, float *, const int *, const int *, hipStream_t);
+                                     const float *, float *, const int *, const int *, hipStream_t,
+                                     const CGReaction<float> &);
 
 }  // namespace femamd

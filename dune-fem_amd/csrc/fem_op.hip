@@ -180,9 +180,10 @@ static Vec raw_vec(Context *c, void *p, int64_t n, int prec)
 
 static void reaction_checks(const Operator &o)
 {
-    FEM_CHECK(o.is_dg(), FEM_ERR_UNSUPPORTED, "pointwise/nonlinear reaction: DG operators only");
-    FEM_CHECK(o.desc.kappa_mode != FEM_KAPPA_QP, FEM_ERR_UNSUPPORTED,
-              "pointwise/nonlinear reaction: constant or cellwise kappa");
+    FEM_CHECK(o.is_dg() || o.desc.kind == FEM_OP_CG_Q2_2D, FEM_ERR_UNSUPPORTED,
+              "pointwise/nonlinear reaction: DG or Q2 Lagrange operators");
+    FEM_CHECK(!o.is_dg() || o.desc.kappa_mode != FEM_KAPPA_QP, FEM_ERR_UNSUPPORTED,
+              "pointwise/nonlinear reaction on DG operators: constant or cellwise kappa");
     FEM_CHECK(o.desc.precision != FEM_BF16MFMA, FEM_ERR_UNSUPPORTED,
               "pointwise/nonlinear reaction: FP64 or FP32 operators");
 }
@@ -194,7 +195,7 @@ void Operator::set_reaction_qp(const double *cvol)
         cvol64.free();
         cvol32.free();
     } else {
-        const size_t nq = (size_t)E * k * k * k;
+        const size_t nq = (size_t)nqp();
         upload_as<double>(cvol64, cvol, nq, ctx->stream);
         upload_as<float>(cvol32, cvol, nq, ctx->stream);
     }
@@ -210,14 +211,15 @@ Operator::~Operator()
 fem_qp_geometry Operator::qp_geometry() const
 {
     fem_qp_geometry g{};
+    g.dim = is_dg() ? 3 : 2;
     for (int d = 0; d < 3; ++d) {
-        g.n[d] = desc.n[d];
-        g.h[d] = a64.h[d];
+        g.n[d] = d < g.dim ? desc.n[d] : 1;
+        g.h[d] = d < g.dim ? 1.0 / desc.n[d] : 0.0;
     }
     g.m = k;
     double w[KMAX];
     gauss_legendre01(k, g.gx, w);
-    g.nq = E * k * k * k;
+    g.nq = nqp();
     return g;
 }
 
@@ -240,7 +242,7 @@ void Operator::set_reaction_plugin(const char *path)
         dlclose(h);
         throw Error(FEM_ERR_ARG, std::string("compiled reaction: no fem_reaction_eval in ") + path);
     }
-    const size_t nq = (size_t)E * k * k * k;
+    const size_t nq = (size_t)nqp();
     pq.alloc(nq * sizeof(double));
     ps.alloc(nq * sizeof(double));
     plugin_dl = h;
@@ -260,10 +262,18 @@ void Operator::set_reaction_law(double coef, int pw)
 
 void Operator::eval_qp(const Vec &u, Vec &out) const
 {
-    FEM_CHECK(is_dg(), FEM_ERR_UNSUPPORTED, "quadrature-point evaluation: DG operators only");
-    FEM_CHECK(u.n == N && out.n == E * k * k * k, FEM_ERR_ARG, "quadrature-point evaluation: vector sizes");
+    FEM_CHECK(u.n == N && out.n == nqp(), FEM_ERR_ARG, "quadrature-point evaluation: vector sizes");
     FEM_CHECK(u.prec == vec_prec && out.prec == vec_prec, FEM_ERR_ARG,
               "quadrature-point evaluation: vectors of the operator's precision");
+    if (!is_dg()) {
+        if (vec_prec == FEM_F64)
+            cg_eval_qp<double>(desc.n[0], desc.n[1], cgtab.as<CGTab>(), (const double *)u.data, nullptr, 1.0, 1,
+                               (double *)out.data, ctx->stream);
+        else
+            cg_eval_qp<float>(desc.n[0], desc.n[1], cgtab.as<CGTab>(), (const float *)u.data, nullptr, 1.0f, 1,
+                              (float *)out.data, ctx->stream);
+        return;
+    }
     if (vec_prec == FEM_F64)
         dg_eval_qp<double>(a64, k, (const double *)u.data, nullptr, 1.0, 1, (double *)out.data, ctx->stream);
     else
@@ -303,7 +313,7 @@ void Operator::make_jacobian(Operator &jop, int precision) const
 void Operator::jacobian(const Vec &u, Operator &jop) const
 {
     reaction_checks(*this);
-    FEM_CHECK(jop.is_dg() &&
+    FEM_CHECK(jop.desc.kind == desc.kind &&
                   (jop.desc.precision == desc.precision || (desc.precision == FEM_F64 && jop.desc.precision == FEM_F32)) &&
                   jop.k == k && jop.E == E &&
                   jop.desc.n[0] == desc.n[0] && jop.desc.n[1] == desc.n[1] && jop.desc.n[2] == desc.n[2] &&
@@ -312,8 +322,17 @@ void Operator::jacobian(const Vec &u, Operator &jop) const
     FEM_CHECK(&jop != this, FEM_ERR_ARG, "jacobian: an operator other than this one");
     FEM_CHECK(u.n == N && u.prec == vec_prec, FEM_ERR_ARG, "jacobian: vector size or precision");
     hipStream_t s = ctx->stream;
-    const int64_t nq = E * k * k * k;
+    const int64_t nq = nqp();
     FEM_CHECK(!jop.plugin, FEM_ERR_ARG, "jacobian: the Jacobian operator must be linear");
+    // out = base + coef u^pw at the quadrature points, by the operator's kind
+    auto eval64 = [&](const double *uu, const double *base, double coef, int pw, double *out) {
+        if (is_dg()) dg_eval_qp<double>(a64, k, uu, base, coef, pw, out, s);
+        else cg_eval_qp<double>(desc.n[0], desc.n[1], cgtab.as<CGTab>(), uu, base, coef, pw, out, s);
+    };
+    auto eval32 = [&](const float *uu, const float *base, float coef, int pw, float *out) {
+        if (is_dg()) dg_eval_qp<float>(a32, k, uu, base, coef, pw, out, s);
+        else cg_eval_qp<float>(desc.n[0], desc.n[1], cgtab.as<CGTab>(), uu, base, coef, pw, out, s);
+    };
     jop.set_reaction_law(0.0, 0);
     if (nl_pow == 0) {
         // no law: the linear part's cvol, if any
@@ -325,20 +344,18 @@ void Operator::jacobian(const Vec &u, Operator &jop) const
         if (!jop.cvol32.p || jop.cvol32.bytes != (size_t)nq * 4) jop.cvol32.alloc((size_t)nq * 4);
         const double cj = (nl_pow + 1) * nl_coef;
         if (vec_prec == FEM_F64) {
-            dg_eval_qp<double>(a64, k, (const double *)u.data, cvol64.as<double>(), cj, nl_pow, jop.cvol64.as<double>(),
-                               s);
+            eval64((const double *)u.data, cvol64.as<double>(), cj, nl_pow, jop.cvol64.as<double>());
             Vec lo = raw_vec(ctx, jop.cvol32.p, nq, FEM_F32), hi = raw_vec(ctx, jop.cvol64.p, nq, FEM_F64);
             vec_copy(lo, hi);
         } else {
-            dg_eval_qp<float>(a32, k, (const float *)u.data, cvol32.as<float>(), (float)cj, nl_pow,
-                              jop.cvol32.as<float>(), s);
+            eval32((const float *)u.data, cvol32.as<float>(), (float)cj, nl_pow, jop.cvol32.as<float>());
             Vec lo = raw_vec(ctx, jop.cvol32.p, nq, FEM_F32), hi = raw_vec(ctx, jop.cvol64.p, nq, FEM_F64);
             vec_copy(hi, lo);
         }
     }
     if (plugin) {
         // + dr/du(u) from the compiled reaction
-        dg_eval_qp<double>(a64, k, (const double *)u.data, nullptr, 1.0, 1, pq.as<double>(), s);
+        eval64((const double *)u.data, nullptr, 1.0, 1, pq.as<double>());
         const fem_qp_geometry g = qp_geometry();
         FEM_CHECK(plugin(&g, pq.as<double>(), nullptr, ps.as<double>(), (void *)s) == 0, FEM_ERR_HIP,
                   "compiled reaction: launch failed");
@@ -446,14 +463,36 @@ void Operator::apply_raw(const void *x, void *y, const int *skip, double *dot) c
     } else {
         FEM_CHECK(!dot, FEM_ERR_UNSUPPORTED, "fused dot: DG operators only");
         FEM_CHECK(desc.kappa_mode == FEM_KAPPA_CONST || kvol64.p, FEM_ERR_STATE, "coefficient data not set");
-        if (vec_prec == FEM_F64)
+        FEM_CHECK(!plugin || !skip, FEM_ERR_STATE,
+                  "apply: an operator with a compiled reaction is nonlinear (no captured solve)");
+        if (vec_prec == FEM_F64) {
+            CGReaction<double> rx;
+            rx.cvol = cvol64.as<double>();
+            rx.coef = nl_coef;
+            rx.pw = nl_pow;
             cg_launch<double>(CG_MODE_APPLY, inst_dev.as<CGInst>(), 1, tiles, cgtab.as<CGTab>(),
                               desc.kappa_mode == FEM_KAPPA_QP ? kvol64.as<double>() : nullptr, (const double *)x,
-                              (double *)y, skip, s);
-        else
+                              (double *)y, skip, s, rx);
+        } else {
+            CGReaction<float> rx;
+            rx.cvol = cvol32.as<float>();
+            rx.coef = (float)nl_coef;
+            rx.pw = nl_pow;
             cg_launch<float>(CG_MODE_APPLY, inst_dev.as<CGInst>(), 1, tiles, cgtab.as<CGTab>(),
                              desc.kappa_mode == FEM_KAPPA_QP ? kvol32.as<float>() : nullptr, (const float *)x,
-                             (float *)y, skip, s);
+                             (float *)y, skip, s, rx);
+        }
+        if (plugin) {
+            // y += int r(u, x) v on the interior rows (the Dirichlet rows
+            // keep y_D = u_D)
+            cg_eval_qp<double>(desc.n[0], desc.n[1], cgtab.as<CGTab>(), (const double *)x, nullptr, 1.0, 1,
+                               pq.as<double>(), s);
+            const fem_qp_geometry g = qp_geometry();
+            FEM_CHECK(plugin(&g, pq.as<double>(), ps.as<double>(), nullptr, (void *)s) == 0, FEM_ERR_HIP,
+                      "compiled reaction: launch failed");
+            cg_launch<double>(CG_MODE_RHS_ADD, inst_dev.as<CGInst>(), 1, tiles, cgtab.as<CGTab>(), ps.as<double>(),
+                              nullptr, (double *)y, nullptr, s);
+        }
     }
 }
 
@@ -511,8 +550,10 @@ void Operator::diagonal(Vec &d) const
         FEM_CHECK(a64.kappa_mode >= 0, FEM_ERR_STATE, "coefficient data not set");
         dg_diagonal(a64, k, dst, s);
     } else {
+        CGReaction<double> rx;
+        rx.cvol = cvol64.as<double>();
         cg_launch<double>(CG_MODE_DIAG, inst_dev.as<CGInst>(), 1, tiles, cgtab.as<CGTab>(),
-                          desc.kappa_mode == FEM_KAPPA_QP ? kvol64.as<double>() : nullptr, nullptr, dst, nullptr, s);
+                          desc.kappa_mode == FEM_KAPPA_QP ? kvol64.as<double>() : nullptr, nullptr, dst, nullptr, s, rx);
     }
     if (d.prec != FEM_F64) {
         Vec tmp;

@@ -77,6 +77,8 @@ struct Operator {
     fem_reaction_eval_fn plugin = nullptr;
     mutable DevBuf pq, ps;
     bool nonlinear() const { return nl_pow > 0 || plugin != nullptr; }
+    // volume quadrature points: E (p+1)^3 (DG), E 9 (Q2 in 2D)
+    int64_t nqp() const { return is_dg() ? E * k * k * k : E * 9; }
     bool has_reaction() const { return cvol64.p != nullptr || nonlinear(); }
     void set_reaction_plugin(const char *path);
     fem_qp_geometry qp_geometry() const;

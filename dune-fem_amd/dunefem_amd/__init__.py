@@ -411,6 +411,14 @@ class GalerkinOperator:
         self.vec_precision = F64 if precision == F64 else F32
         self.N = _lib.fem_op_ndofs(h)
 
+    @property
+    def nqp(self):
+        """number of volume quadrature points: E (p+1)^3 (DG), E 9 (Q2 in 2D)"""
+        n = self.desc.n
+        if self.desc.kind == OP_CG_Q2_2D:
+            return n[0] * n[1] * 9
+        return n[0] * n[1] * n[2] * (self.desc.order + 1) ** 3
+
     def set_kappa_qp(self, kvol, kfaces=None):
         kvol = np.ascontiguousarray(kvol, dtype=np.float64)
         if kfaces is None:
@@ -469,7 +477,7 @@ class GalerkinOperator:
             _check(_lib.fem_op_set_reaction_qp(self.h, None))
         else:
             cvol = np.ascontiguousarray(cvol, dtype=np.float64)
-            assert cvol.size == self.N, "one value per volume quadrature point"
+            assert cvol.size == self.nqp, "one value per volume quadrature point"
             _check(_lib.fem_op_set_reaction_qp(self.h, _dptr(cvol)))
 
     def set_reaction_law(self, coef, pw):
@@ -481,7 +489,8 @@ class GalerkinOperator:
         """a pointwise reaction r(u, x): the volume term int r(u_h, x) v with
         r a sympy expression (or string) of u, x, y, z, compiled to HIP by
         dunefem_amd.forms (cached; build=False requires the compiled library
-        to exist); None removes it.  FP64 DG operators."""
+        to exist); None removes it.  FP64 DG and Q2 Lagrange operators (z = 0
+        in 2D)."""
         if r is None:
             _check(_lib.fem_op_set_reaction_plugin(self.h, None))
             self.reaction = None
@@ -492,9 +501,12 @@ class GalerkinOperator:
         self.reaction = forms.as_expr(r)
 
     def qp_coordinates(self):
-        """x, y, z of the volume quadrature points [E][m^3] (DG)"""
+        """x, y, z of the volume quadrature points [E][m^3] (DG) or [E][9]
+        (Q2 in 2D, z = 0)"""
         from . import forms
         m = self.desc.order + 1
+        if self.desc.kind == OP_CG_Q2_2D:
+            return forms.qp_coordinates_2d(tuple(self.desc.n[:2]), m, gauss_points(m)[0])
         return forms.qp_coordinates(tuple(self.desc.n), m, gauss_points(m)[0])
 
     def rhs_expr(self, f, F=None):
@@ -513,8 +525,8 @@ class GalerkinOperator:
         return bool(r)
 
     def eval_qp(self, u, out=None):
-        """u at the volume quadrature points [E][m^3] (x fastest)"""
-        out = out if out is not None else DofVector(self.ctx, self.N, self.vec_precision)
+        """u at the volume quadrature points [E][m^dim] (x fastest)"""
+        out = out if out is not None else DofVector(self.ctx, self.nqp, self.vec_precision)
         _check(_lib.fem_op_eval_qp(self.h, u.h, out.h))
         return out
 

@@ -192,26 +192,29 @@ int fem_op_constrain(fem_op *op, fem_vec *x, const fem_vec *add);
  * supported (computed by probing the apply). */
 int fem_op_block_diagonal(fem_op *op, fem_vec *blocks);
 
-/* ---- pointwise and nonlinear reaction (DG operators) -----------------------------
+/* ---- pointwise and nonlinear reaction (DG and Q2 Lagrange operators) -------------
  * Integrands whose source term is pointwise at the volume quadrature points,
  * as the reference's GalerkinOperator evaluates every integrand
  * (dune/fem/schemes/galerkin.hh:333-360, interior term of the integrands,
  * dune/fem/schemes/integrands.hh:412-664): the volume term of the operator
  * becomes int (c + cvol(x) + coef u^pw) u v, so that -div(kappa grad u) +
  * coef u^(pw+1) = f is the nonlinear problem of a Newton solve.  DG operators
- * with constant or cellwise kappa, FP64 or FP32 (FEM_ERR_UNSUPPORTED
- * otherwise).
+ * with constant or cellwise kappa and the 2D Q2 Lagrange operator (any kappa;
+ * its Dirichlet rows keep S[u]_D = u_D, the Jacobian's rows are the identity,
+ * as DirichletWrapperOperator, dune/fem/schemes/dirichletwrapper.hh:101-112),
+ * FP64 or FP32 (FEM_ERR_UNSUPPORTED otherwise).
  *
  * fem_op_set_reaction_qp: cvol at the volume quadrature points [E][m^3]
- *   (x fastest, host array, copied; NULL clears).
+ *   (DG) or [E][9] (Q2: qx + 3 qy), x fastest, host array, copied; NULL
+ *   clears.
  * fem_op_set_reaction_law: coef u^pw, pw in 1..4 (coef = 0 clears).  With a
  *   law set the operator is nonlinear (the reference's Operator::nonlinear(),
  *   dune/fem/operator/common/operator.hh:64): fem_op_apply evaluates S[u],
  *   while the diagonal, the element blocks, the CSR assembly and binding to
  *   a linear solver are refused (FEM_ERR_STATE): they belong to the Jacobian.
  * fem_op_nonlinear: 1 if a law is set, else 0 (negative: error).
- * fem_op_eval_qp: u at the volume quadrature points [E][m^3] (x fastest),
- *   vectors of the operator's precision.
+ * fem_op_eval_qp: u at the volume quadrature points [E][m^3] (DG) or
+ *   [E][9] (Q2), x fastest, vectors of the operator's precision.
  * fem_op_jacobian_create: a linear operator of the same description and
  *   coefficient data (device copies), the Jacobian operator object
  *   (NewtonInverseOperator's jOp, newtoninverseoperator.hh:704).
@@ -238,13 +241,15 @@ int fem_op_set_reaction_law(fem_op *op, double coef, int pw);
  * fem_op_set_reaction_plugin loads it (NULL unloads): the operator is then
  * nonlinear, S[u] = A u + int r(u, x) v (plus any law above), and
  * fem_op_jacobian adds dr/du(u) to the Jacobian's cvol.  FP64 DG operators
- * with constant or cellwise kappa. */
+ * with constant or cellwise kappa, and FP64 Q2 Lagrange operators (2D: dim
+ * = 2, n[2] = 1, h[2] = 0, m^2 points per element, z = 0). */
 typedef struct {
     int n[3];          /* cells per direction                               */
-    int m;             /* points per direction (p + 1)                      */
+    int m;             /* points per direction (p + 1; 3 for Q2)            */
     double h[3];       /* cell size per direction                           */
     double gx[8];      /* Gauss points on [0, 1]                            */
-    int64_t nq;        /* E m^3 quadrature points, [E][m^3], x fastest      */
+    int64_t nq;        /* E m^dim quadrature points, [E][m^dim], x fastest  */
+    int dim;           /* 3 (DG hexahedra) or 2 (Q2 quadrilaterals)         */
 } fem_qp_geometry;
 typedef int (*fem_reaction_eval_fn)(const fem_qp_geometry *g, const double *uq, double *r, double *dr,
                                     void *stream);

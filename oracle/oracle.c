@@ -756,13 +756,29 @@ void ora_cg_apply(const ora_cg_t *op, const double *u, double *w)
                 double kap = op->kappa_mode == ORA_KAPPA_QP ? op->kvol[e * 9 + q] : op->kappa_const;
                 double Wt = T.W[q] * detJ;
                 for (int i = 0; i < 9; ++i)
-                    wl[i] += Wt * (kap * (gx * T.G[0][q][i] / hx + gy * T.G[1][q][i] / hy) + op->c * uq * T.V[q][i]);
+                    wl[i] += Wt * (kap * (gx * T.G[0][q][i] / hx + gy * T.G[1][q][i] / hy) +
+                                   (op->c + (op->cvol ? op->cvol[e * 9 + q] : 0.0)) * uq * T.V[q][i]);
             }
             for (int i = 0; i < 9; ++i) w[cg_node(op, ex, ey, i)] += wl[i];
         }
     /* subConstraints: w_D = u_D (dirichletwrapper.hh:101-105) */
     for (int64_t k = 0; k < N; ++k)
         if (cg_is_boundary(op, k)) w[k] = u[k];
+}
+
+void ora_cg_eval_qp(const ora_cg_t *op, const double *u, double *out)
+{
+    cg_tables T;
+    cg_tables_init(&T);
+    for (int64_t ey = 0; ey < op->n[1]; ++ey)
+        for (int64_t ex = 0; ex < op->n[0]; ++ex) {
+            int64_t e = ex + op->n[0] * ey;
+            for (int q = 0; q < 9; ++q) {
+                double uq = 0;
+                for (int i = 0; i < 9; ++i) uq += T.V[q][i] * u[cg_node(op, ex, ey, i)];
+                out[e * 9 + q] = uq;
+            }
+        }
 }
 
 void ora_cg_rhs(const ora_cg_t *op, const double *fq, double *F)
@@ -799,7 +815,8 @@ void ora_cg_diag(const ora_cg_t *op, double *d)
                 for (int q = 0; q < 9; ++q) {
                     double kap = op->kappa_mode == ORA_KAPPA_QP ? op->kvol[e * 9 + q] : op->kappa_const;
                     double gx = T.G[0][q][i] / hx, gy = T.G[1][q][i] / hy;
-                    a += T.W[q] * detJ * (kap * (gx * gx + gy * gy) + op->c * T.V[q][i] * T.V[q][i]);
+                    double r = op->c + (op->cvol ? op->cvol[e * 9 + q] : 0.0);
+                    a += T.W[q] * detJ * (kap * (gx * gx + gy * gy) + r * T.V[q][i] * T.V[q][i]);
                 }
                 d[cg_node(op, ex, ey, i)] += a;
             }

@@ -114,9 +114,12 @@ typedef struct {
     int kappa_mode;       /* ORA_KAPPA_CONST or ORA_KAPPA_QP */
     double kappa_const;
     const double *kvol;   /* [E][9] */
+    const double *cvol;   /* pointwise reaction at the volume qps [E][9], added to c, or NULL */
 } ora_cg_t;
 
 int64_t ora_cg_ndofs(const ora_cg_t *op);
+/* u at the volume quadrature points: out[e*9 + qx + 3*qy] */
+void ora_cg_eval_qp(const ora_cg_t *op, const double *u, double *out);
 void ora_cg_apply(const ora_cg_t *op, const double *u, double *w);
 /* load vector with f at volume qps; Dirichlet entries set to 0 */
 void ora_cg_rhs(const ora_cg_t *op, const double *fq, double *F);

@@ -27,7 +27,7 @@ class DG(C.Structure):
 
 class CG(C.Structure):
     _fields_ = [("n", C.c_int * 2), ("c", C.c_double), ("kappa_mode", C.c_int),
-                ("kappa_const", C.c_double), ("kvol", _dp)]
+                ("kappa_const", C.c_double), ("kvol", _dp), ("cvol", _dp)]
 
 
 class Diag(C.Structure):
@@ -80,6 +80,7 @@ def lib():
         L.ora_cg_ndofs.restype = C.c_int64
         L.ora_cg_apply.argtypes = [C.POINTER(CG), _dp, _dp]
         L.ora_cg_rhs.argtypes = [C.POINTER(CG), _dp, _dp]
+        L.ora_cg_eval_qp.argtypes = [C.POINTER(CG), _dp, _dp]
         L.ora_cg_diag.argtypes = [C.POINTER(CG), _dp]
         for f in ("ora_cg", "ora_bicgstab"):
             getattr(L, f).argtypes = [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -252,6 +253,24 @@ class CGOperator:
         F = np.empty(self.N)
         lib().ora_cg_rhs(C.byref(self.s), _ptr(fq), _ptr(F))
         return F
+
+    def set_reaction_qp(self, cvol):
+        """pointwise reaction coefficient at the volume quadrature points
+        [E][9] (qx + 3 qy), added to c; None to clear"""
+        if cvol is None:
+            self.s.cvol = None
+            return
+        cv = np.ascontiguousarray(cvol, dtype=np.float64).ravel()
+        assert cv.size == self.n[0] * self.n[1] * 9
+        self._keep.append(cv)
+        self.s.cvol = _ptr(cv)
+
+    def eval_qp(self, u):
+        """u at the volume quadrature points [E][9]"""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        out = np.empty((self.n[0] * self.n[1], 9))
+        lib().ora_cg_eval_qp(C.byref(self.s), _ptr(u), _ptr(out))
+        return out
 
     def diag(self):
         d = np.empty(self.N)
