@@ -218,3 +218,41 @@ def test_gpu_galerkin_scheme_from_symbolic_integrands(ctx, pre):
     X, Y, Z = scheme.op.qp_coordinates()
     fq = 10 * np.sin(np.pi * X) * np.sin(np.pi * Y) * np.sin(np.pi * Z)
     assert rel(scheme.F.download(), oop.rhs(fq)) < 1e-13
+
+
+def test_lagrange_space_arguments():
+    with pytest.raises(ValueError):
+        forms.Space(None, (4, 4), 1, kind="lagrange")
+    with pytest.raises(ValueError):
+        forms.Space(None, (4, 4, 4), 2, kind="lagrange")
+    with pytest.raises(ValueError):
+        forms.Space(None, (4, 4), 2, kind="raviartthomas")
+
+
+@pytest.mark.gpu
+def test_gpu_galerkin_scheme_on_a_lagrange_space(ctx):
+    # the reference's most common path: a Lagrange space of order 2 with
+    # u = 0 on the boundary, cellwise kappa, reaction exp(u) and a source
+    # expression; scheme.solve with Jacobi-preconditioned CG inside Newton,
+    # the solution's residual, and the load vector against the oracle's
+    import dunefem_amd as fa
+    n = (24, 20)
+    kc = np.random.default_rng(31).uniform(0.5, 2.0, size=n[0] * n[1])
+    space = forms.Space(ctx, n, 2, kind="lagrange")
+    form = forms.Form(kappa=kc, reaction="exp(u)", source="10*sin(pi*x)*sin(pi*y)")
+    scheme = fa.galerkin(form, space, solver="cg",
+                         parameters={"fem.solver.nonlinear.tolerance": 1e-9,
+                                     "fem.solver.nonlinear.linear.tolerance": 1e-12,
+                                     "fem.solver.nonlinear.linear.errormeasure": "relative",
+                                     "fem.solver.nonlinear.linear.maxiterations": 20000,
+                                     "fem.solver.nonlinear.linear.preconditioning.method": "jacobi"})
+    U = scheme.op.vector()
+    info = scheme.solve(U)
+    print("lagrange galerkin scheme solve", info, flush=True)
+    assert info["converged"] and 2 <= info["iterations"] <= 10
+    R = scheme.op.vector()
+    scheme(U, R)
+    assert np.sqrt(R.normSquaredDofs()) < 1e-9
+    X, Y, _ = scheme.op.qp_coordinates()
+    oop = ora.CGOperator(n, kappa=np.repeat(kc, 9))
+    assert rel(scheme.F.download(), oop.rhs(10 * np.sin(np.pi * X) * np.sin(np.pi * Y))) < 1e-13
