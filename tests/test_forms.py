@@ -256,3 +256,77 @@ def test_gpu_galerkin_scheme_on_a_lagrange_space(ctx):
     X, Y, _ = scheme.op.qp_coordinates()
     oop = ora.CGOperator(n, kappa=np.repeat(kc, 9))
     assert rel(scheme.F.download(), oop.rhs(10 * np.sin(np.pi * X) * np.sin(np.pi * Y))) < 1e-13
+
+
+def test_kappa_points_match_the_quadrature_coordinates():
+    # the volume values of an expression kappa at the points qp_coordinates
+    # names (3D and 2D), and the face arrays in the face layout of the
+    # workloads (one value per face point of each direction)
+    n, m = (3, 4, 2), 3
+    kv, kf = forms.kappa_points("exp(x)*(1 + y*z)", n, m)
+    X, Y, Z = forms.qp_coordinates(n, m, ora.gauss(m)[0])
+    assert np.abs(kv - np.exp(X) * (1 + Y * Z)).max() < 1e-14
+    assert [a.shape for a in kf] == [((n[0] + 1) * n[1] * n[2], m * m), (n[0] * (n[1] + 1) * n[2], m * m),
+                                     (n[0] * n[1] * (n[2] + 1), m * m)]
+    kv2, none = forms.kappa_points("1 + x - y", (5, 3), 3)
+    X2, Y2, _ = forms.qp_coordinates_2d((5, 3), 3, ora.gauss(3)[0])
+    assert none is None and np.abs(kv2 - (1 + X2 - Y2)).max() < 1e-15
+    with pytest.raises(ValueError):
+        forms.kappa_points("1 + u", n, m)
+
+
+@pytest.mark.gpu
+def test_gpu_galerkin_with_a_kappa_expression(ctx):
+    # kappa = 1 + x y z on a DG Q2 space (pointwise kappa at volume and face
+    # points; a linear problem: one Newton step, no nonlinear iteration) and
+    # kappa = 1 + x - y / 2 with the reaction exp(u) on the Lagrange space;
+    # each operator against the oracle with the same point values
+    import dunefem_amd as fa
+    n, p = (5, 4, 6), 2
+    space = forms.Space(ctx, n, p)
+    form = forms.Form(kappa="1 + x*y*z", c=0.5, source="10*sin(pi*x)*sin(pi*y)*sin(pi*z)")
+    scheme = fa.galerkin(form, space, solver="cg",
+                         parameters={"fem.solver.nonlinear.tolerance": 1e-9,
+                                     "fem.solver.nonlinear.linear.tolerance": 1e-12,
+                                     "fem.solver.nonlinear.linear.errormeasure": "relative",
+                                     "fem.solver.nonlinear.linear.preconditioning.method": "blockjacobi"})
+    kv, kf = forms.kappa_points("1 + x*y*z", n, p + 1)
+    oop = ora.DGOperator(n, p, 10.0 * p * p, c=0.5, kappa=("qp", kv, kf))
+    u = np.random.default_rng(41).normal(size=scheme.op.N)
+    Uv, Wv = scheme.op.vector(u), scheme.op.vector()
+    scheme.op(Uv, Wv)
+    assert rel(Wv.download(), oop.apply(u)) < 1e-12
+    U = scheme.op.vector()
+    info = scheme.solve(U)
+    R = scheme.op.vector()
+    scheme(U, R)
+    print("dg kappa expression", info, np.sqrt(R.normSquaredDofs()), flush=True)
+    assert info["converged"] and info["iterations"] == 0
+    assert np.sqrt(R.normSquaredDofs()) < 1e-10 * np.sqrt(scheme.F.normSquaredDofs())
+    # not together with a reaction on DG (pointwise kappa and the reaction
+    # are separate kernel variants)
+    with pytest.raises(fa.FemError):
+        fa.galerkin(forms.Form(kappa="1 + x", reaction="u**3"), space, build=False)
+    # Lagrange: kappa expression and reaction
+    n2 = (20, 16)
+    space2 = forms.Space(ctx, n2, 2, kind="lagrange")
+    form2 = forms.Form(kappa="1 + x - y/2", reaction="exp(u)", source="10*sin(pi*x)*sin(pi*y)")
+    scheme2 = fa.galerkin(form2, space2, solver="cg",
+                          parameters={"fem.solver.nonlinear.tolerance": 1e-9,
+                                      "fem.solver.nonlinear.linear.tolerance": 1e-12,
+                                      "fem.solver.nonlinear.linear.errormeasure": "relative",
+                                      "fem.solver.nonlinear.linear.maxiterations": 20000,
+                                      "fem.solver.nonlinear.linear.preconditioning.method": "jacobi"})
+    oop2 = ora.CGOperator(n2, kappa=forms.kappa_points("1 + x - y/2", n2, 3)[0].ravel())
+    u2 = np.random.default_rng(42).normal(size=scheme2.op.N)
+    S2 = scheme2.op.vector()
+    scheme2.op.set_reaction(None)
+    scheme2.op(scheme2.op.vector(u2), S2)
+    assert rel(S2.download(), oop2.apply(u2)) < 1e-12
+    scheme2.op.set_reaction("exp(u)", build=False)
+    U2 = scheme2.op.vector()
+    info2 = scheme2.solve(U2)
+    R2 = scheme2.op.vector()
+    scheme2(U2, R2)
+    print("lagrange kappa expression", info2, flush=True)
+    assert info2["converged"] and np.sqrt(R2.normSquaredDofs()) < 1e-9
