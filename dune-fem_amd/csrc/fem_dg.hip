@@ -320,7 +320,7 @@ struct Ctr {
 // SM: the damped block-Jacobi sweep of a multigrid level in the same pass,
 // xout = u + omega B^-1 (r - A u), instead of w = A u (sm carries r, the
 // element blocks' eigen data, omega and xout; w and dot are not used)
-template <int K, typename T, bool MF, bool KQ, bool SM = false>
+template <int K, typename T, bool MF, bool KQ, bool SM = false, bool RX = false>
 __global__ void __launch_bounds__(ApplyShape<K>::WG)
     k_dg_apply(const DGArgs<T> a, const T *__restrict__ u, T *__restrict__ w, const int *__restrict__ skip,
                double *__restrict__ dot, const DGSmooth<T> sm)
@@ -350,9 +350,11 @@ __global__ void __launch_bounds__(ApplyShape<K>::WG)
     // (K = 4 only: the other orders keep the face-trace scratch there)
     constexpr bool KLDS = (K == 4) && KQ;
     // the pointwise/nonlinear reaction is compiled into the plain applies
-    // with constant or cellwise kappa only (not the BF16-MFMA, pointwise
-    // kappa or fused-smoother variants)
-    constexpr bool CV = !KQ && !MF && !SM;
+    // with constant or cellwise kappa, and into a separate variant (RX) of
+    // the pointwise-kappa apply that only operators with a reaction use (not
+    // into the BF16-MFMA or fused-smoother variants): the pointwise-kappa
+    // kernel without it keeps its code and registers
+    constexpr bool CV = (!KQ && !MF && !SM) || RX;
     constexpr int SZ = F64MF_ ? (KLDS ? 244 : 164) : D35 ? (K == 3 ? 107 : 361) : S::SZ;
     static_assert(!F64MF_ || SZ >= (KLDS ? 3 : 2) * K * S1, "tensors per element");
     static_assert(F64MF_ || SZ >= 2 * K * S1 + 4 * SXQ, "tensors and face-trace scratch per element");
@@ -867,7 +869,13 @@ static void launch_apply(const DGArgs<T> &a, const T *u, T *w, const int *skip, 
     const int64_t E = (int64_t)a.n[0] * a.n[1] * a.n[2];
     FEM_CHECK(E * S::K3 < (int64_t)1 << 31, FEM_ERR_UNSUPPORTED, "DG apply: more than 2^31 DoFs");
     const unsigned nb = (unsigned)dg_blocks(K, a.n);
-    if (a.kappa_mode == FEM_KAPPA_QP)
+    bool rx = false;
+    if constexpr (!MF) rx = a.cvol || a.nl_pow;
+    if (a.kappa_mode == FEM_KAPPA_QP && rx) {
+        if constexpr (!MF)
+            hipLaunchKernelGGL((k_dg_apply<K, T, false, true, false, true>), dim3(nb), dim3(A::WG), 0, s, a, u, w,
+                               skip, dot, DGSmooth<T>{});
+    } else if (a.kappa_mode == FEM_KAPPA_QP)
         hipLaunchKernelGGL((k_dg_apply<K, T, MF, true>), dim3(nb), dim3(A::WG), 0, s, a, u, w, skip, dot,
                            DGSmooth<T>{});
     else

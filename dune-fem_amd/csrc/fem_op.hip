@@ -182,8 +182,6 @@ static void reaction_checks(const Operator &o)
 {
     FEM_CHECK(o.is_dg() || o.desc.kind == FEM_OP_CG_Q2_2D, FEM_ERR_UNSUPPORTED,
               "pointwise/nonlinear reaction: DG or Q2 Lagrange operators");
-    FEM_CHECK(!o.is_dg() || o.desc.kappa_mode != FEM_KAPPA_QP, FEM_ERR_UNSUPPORTED,
-              "pointwise/nonlinear reaction on DG operators: constant or cellwise kappa");
     FEM_CHECK(o.desc.precision != FEM_BF16MFMA, FEM_ERR_UNSUPPORTED,
               "pointwise/nonlinear reaction: FP64 or FP32 operators");
 }
@@ -303,8 +301,16 @@ void Operator::make_jacobian(Operator &jop, int precision) const
     jop.has_bnd = has_bnd;
     jop.a64.kappa_mode = a64.kappa_mode;
     jop.a32.kappa_mode = a32.kappa_mode;
+    // the kernel arguments point at the copies (pointwise kappa: volume and
+    // face values)
     jop.a64.kcell = jop.kcell64.as<double>();
     jop.a32.kcell = jop.kcell32.as<float>();
+    jop.a64.kvol = jop.kvol64.as<double>();
+    jop.a32.kvol = jop.kvol32.as<float>();
+    for (int d = 0; d < 3; ++d) {
+        jop.a64.kface[d] = jop.kface64[d].as<double>();
+        jop.a32.kface[d] = jop.kface32[d].as<float>();
+    }
     jop.a64.cvol = jop.cvol64.as<double>();
     jop.a32.cvol = jop.cvol32.as<float>();
     FEM_HIP(hipStreamSynchronize(s));
@@ -445,6 +451,13 @@ void Operator::apply_raw(const void *x, void *y, const int *skip, double *dot) c
     hipStream_t s = ctx->stream;
     if (is_dg()) {
         FEM_CHECK(a64.kappa_mode >= 0, FEM_ERR_STATE, "coefficient data not set");
+        // every coefficient array the kernel of this mode reads is in place
+        const bool f64 = desc.precision == FEM_F64;
+        const bool qp_ok = f64 ? (a64.kvol && a64.kface[0] && a64.kface[1] && a64.kface[2])
+                               : (a32.kvol && a32.kface[0] && a32.kface[1] && a32.kface[2]);
+        FEM_CHECK(a64.kappa_mode != FEM_KAPPA_QP || qp_ok, FEM_ERR_STATE, "pointwise kappa data not set");
+        FEM_CHECK(a64.kappa_mode != FEM_KAPPA_CELL || (f64 ? a64.kcell != nullptr : a32.kcell != nullptr),
+                  FEM_ERR_STATE, "cellwise kappa data not set");
         FEM_CHECK(!dot || desc.precision != FEM_BF16MFMA, FEM_ERR_UNSUPPORTED, "fused dot: FP64 or FP32 operator");
         FEM_CHECK(!plugin || (!dot && !skip), FEM_ERR_STATE,
                   "apply: an operator with a compiled reaction is nonlinear (no fused dot, no captured solve)");

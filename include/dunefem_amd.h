@@ -199,7 +199,7 @@ int fem_op_block_diagonal(fem_op *op, fem_vec *blocks);
  * dune/fem/schemes/integrands.hh:412-664): the volume term of the operator
  * becomes int (c + cvol(x) + coef u^pw) u v, so that -div(kappa grad u) +
  * coef u^(pw+1) = f is the nonlinear problem of a Newton solve.  DG operators
- * with constant or cellwise kappa and the 2D Q2 Lagrange operator (any kappa;
+ * (any kappa mode) and the 2D Q2 Lagrange operator (any kappa;
  * its Dirichlet rows keep S[u]_D = u_D, the Jacobian's rows are the identity,
  * as DirichletWrapperOperator, dune/fem/schemes/dirichletwrapper.hh:101-112),
  * FP64 or FP32 (FEM_ERR_UNSUPPORTED otherwise).
@@ -241,7 +241,7 @@ int fem_op_set_reaction_law(fem_op *op, double coef, int pw);
  * fem_op_set_reaction_plugin loads it (NULL unloads): the operator is then
  * nonlinear, S[u] = A u + int r(u, x) v (plus any law above), and
  * fem_op_jacobian adds dr/du(u) to the Jacobian's cvol.  FP64 DG operators
- * with constant or cellwise kappa, and FP64 Q2 Lagrange operators (2D: dim
+ * and FP64 Q2 Lagrange operators (2D: dim
  * = 2, n[2] = 1, h[2] = 0, m^2 points per element, z = 0). */
 typedef struct {
     int n[3];          /* cells per direction                               */
@@ -409,7 +409,7 @@ int fem_solver_solve(fem_solver *s, const fem_vec *b, fem_vec *x, fem_solver_inf
  * no nonlinear iteration (:786-790).  Stopping and failure follow
  * NewtonInverseOperator::failed (:568-584) with the NewtonFailure codes
  * (:389-400).  With a linear solver in FP64 refinement (`refinement` = 1;
- * FP64 DG operators with constant or cellwise kappa) each Newton system is
+ * FP64 DG operators) each Newton system is
  * solved by FP32 inner solves on an FP32 twin of S'(w), the residual and the
  * update staying FP64; a preconditioner given at bind must then be FP32 and
  * serves the inner solves. */

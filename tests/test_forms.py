@@ -303,10 +303,14 @@ def test_gpu_galerkin_with_a_kappa_expression(ctx):
     print("dg kappa expression", info, np.sqrt(R.normSquaredDofs()), flush=True)
     assert info["converged"] and info["iterations"] == 0
     assert np.sqrt(R.normSquaredDofs()) < 1e-10 * np.sqrt(scheme.F.normSquaredDofs())
-    # not together with a reaction on DG (pointwise kappa and the reaction
-    # are separate kernel variants)
-    with pytest.raises(fa.FemError):
-        fa.galerkin(forms.Form(kappa="1 + x", reaction="u**3"), space, build=False)
+    # with a reaction on DG: the reaction variant of the pointwise-kappa
+    # apply, S[u] against the oracle (u^3 as cvol = u^2 at the points)
+    sr = fa.galerkin(forms.Form(kappa="1 + x", reaction="u**3", c=0.5), space, build=False)
+    kv1, kf1 = forms.kappa_points("1 + x", n, p + 1)
+    oop1 = ora.DGOperator(n, p, 10.0 * p * p, c=0.5, kappa=("qp", kv1, kf1))
+    oop1.set_reaction_qp(oop1.eval_qp(u) ** 2)
+    sr.op(Uv, Wv)
+    assert rel(Wv.download(), oop1.apply(u)) < 1e-12
     # Lagrange: kappa expression and reaction
     n2 = (20, 16)
     space2 = forms.Space(ctx, n2, 2, kind="lagrange")

@@ -221,6 +221,39 @@ def test_gpu_nonlinear_apply_and_jacobian_match_oracle(ctx, p, pw):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("p,pw", [(1, 2), (2, 2), (3, 1), (4, 2)])
+def test_gpu_reaction_with_pointwise_kappa_matches_oracle(ctx, p, pw):
+    # the reaction variant of the pointwise-kappa apply (kappa at volume and
+    # face points, cvol and the law): S[u] and S'(u) w against the oracle,
+    # FP64 and FP32, with advection
+    import dunefem_amd as fa
+    from dunefem_amd import workloads as wl
+    n, rng, coef, b = (4, 3, 5), np.random.default_rng(60 + p), 2.0, (0.5, -0.25, 0.3)
+    P = wl.problem1(n=n, p=p, seed=60 + p, with_u=False)
+    kq = ("qp", P["kvol"], P["kfaces"])
+    cv = rng.uniform(0.0, 2.0, size=n[0] * n[1] * n[2] * (p + 1) ** 3)
+    oop = ora.DGOperator(n, p, 10.0 * p * p, b, 0.2, kappa=kq)
+    u, w = rng.normal(size=oop.N), rng.normal(size=oop.N)
+    uq = oop.eval_qp(u).ravel()
+    oop.set_reaction_qp(cv + coef * uq ** pw)
+    S_ref = oop.apply(u)
+    oop.set_reaction_qp(cv + (pw + 1) * coef * uq ** pw)
+    J_ref = oop.apply(w)
+    for prec, tol in ((fa.F64, TOL64), (fa.F32, TOL32)):
+        op = fa.GalerkinOperator(ctx, fa.OP_DG_SIPG_3D, n, p, 10.0 * p * p, b, 0.2, fa.KAPPA_QP, precision=prec)
+        op.set_kappa_qp(P["kvol"], P["kfaces"])
+        op.set_reaction_qp(cv)
+        op.set_reaction_law(coef, pw)
+        U, Wv, Y = op.vector(u), op.vector(w), op.vector()
+        op(U, Y)
+        errS = rel(Y.download(), S_ref)
+        op.jacobian(U)(Wv, Y)
+        errJ = rel(Y.download(), J_ref)
+        print("pointwise kappa p", p, "prec", prec, "S rel err", errS, "J rel err", errJ, flush=True)
+        assert errS < tol and errJ < tol
+
+
+@pytest.mark.gpu
 def test_gpu_block_jacobi_exact_for_element_constant_reaction(ctx):
     # an element-constant cvol keeps the element block a sum of Kronecker
     # terms: the block-Jacobi inverse (eigen data shifted by the reaction)
