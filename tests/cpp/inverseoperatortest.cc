@@ -346,6 +346,23 @@ static void newton_check(Context &ctx)
                newton.iterations() <= 15 && newton.residual() < 1e-10 * fnorm,
            "NewtonInverseOperator solves S[w] = F (iterations)", (double)newton.iterations());
     report(error(w) < 1e-8, "NewtonInverseOperator returns u*", error(w));
+    // the same Newton solve with the multigrid the linear solver builds on
+    // each Jacobian (the reaction in its levels, updated in place per step)
+    {
+        NewtonParameter mp;
+        mp.set("fem.solver.nonlinear.tolerance", std::to_string(1e-10 * fnorm))
+            .set("fem.solver.nonlinear.linear.tolerance", "1e-12")
+            .set("fem.solver.nonlinear.linear.errormeasure", "relative")
+            .set("fem.solver.nonlinear.linear.preconditioning.method", "multigrid");
+        NewtonInverseOperator<CgInverseOperator> mg(ctx, mp);
+        mg.bind(S);
+        DiscreteFunction wm = S.function("wm");
+        mg(F, wm);
+        report(mg.converged() && mg.iterations() == newton.iterations() &&
+                   mg.linearIterations() * 2 < newton.linearIterations(),
+               "Newton with the multigrid of each Jacobian (linear iterations)", (double)mg.linearIterations());
+        report(error(wm) < 1e-8, "Newton with multigrid returns u*", error(wm));
+    }
     // FemScheme over R(u) = S[u] - F
     CgInverseOperator inv(ctx, np.linear());
     NewtonParameter nl;
