@@ -472,7 +472,7 @@ class GalerkinOperator:
 
     def set_reaction_qp(self, cvol):
         """pointwise reaction coefficient at the volume quadrature points
-        [E][m^3] (x fastest), added to c; None clears (DG operators)"""
+        [E][m^3] (DG) or [E][9] (Q2), x fastest, added to c; None clears"""
         if cvol is None:
             _check(_lib.fem_op_set_reaction_qp(self.h, None))
         else:

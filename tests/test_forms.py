@@ -334,3 +334,55 @@ def test_gpu_galerkin_with_a_kappa_expression(ctx):
     scheme2(U2, R2)
     print("lagrange kappa expression", info2, flush=True)
     assert info2["converged"] and np.sqrt(R2.normSquaredDofs()) < 1e-9
+
+
+def test_boundary_values_layouts():
+    # Q2: the nodal values of g on the node lattice; DG: the face arrays of
+    # the pointwise-kappa layout
+    space = forms.Space(None, (4, 3), 2, kind="lagrange")
+    g = forms.boundary_values("1 + x + 2*y", space)
+    xs, ys = np.linspace(0, 1, 9), np.linspace(0, 1, 7)
+    assert g.shape == (63,) and abs(g[0] - 1) < 1e-15 and abs(g[8] - 2) < 1e-15 and abs(g[-1] - 4) < 1e-15
+    assert np.abs(g.reshape(7, 9) - (1 + xs[None, :] + 2 * ys[:, None])).max() < 1e-15
+    dg = forms.Space(None, (3, 2, 2), 2)
+    gf = forms.boundary_values("x*y + z", dg)
+    assert [a.shape for a in gf] == [(4 * 2 * 2, 9), (3 * 3 * 2, 9), (3 * 2 * 3, 9)]
+    with pytest.raises(ValueError):
+        forms.boundary_values("u + 1", dg)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["lagrange", "dglegendre"])
+def test_gpu_galerkin_with_dirichlet_data_reproduces_a_polynomial_solution(ctx, kind):
+    # u = 1 + x + 2y (+ 3z) with u = g on the boundary, -div grad u + u/2 +
+    # u^3 = f: the quadrature rules integrate every term exactly for this u,
+    # so the discrete nonlinear problem has u itself as its solution (Q2
+    # nodal values; DG values at the quadrature points), reached by Newton
+    # with the preconditioner built on each Jacobian
+    import dunefem_amd as fa
+    if kind == "lagrange":
+        space, ue, pre = forms.Space(ctx, (16, 12), 2, kind="lagrange"), "1 + x + 2*y", "jacobi"
+    else:
+        space, ue, pre = forms.Space(ctx, (6, 5, 4), 2), "1 + x + 2*y + 3*z", "multigrid"
+    form = forms.Form(kappa=1.0, c=0.5, reaction="u**3", source="0.5*(%s) + (%s)**3" % (ue, ue), dirichlet=ue)
+    scheme = fa.galerkin(form, space, solver="cg", build=False,
+                         parameters={"fem.solver.nonlinear.tolerance": 1e-11,
+                                     "fem.solver.nonlinear.linear.tolerance": 1e-13,
+                                     "fem.solver.nonlinear.linear.errormeasure": "relative",
+                                     "fem.solver.nonlinear.linear.maxiterations": 20000,
+                                     "fem.solver.nonlinear.linear.preconditioning.method": pre})
+    U = scheme.op.vector()
+    info = scheme.solve(U)
+    print("dirichlet data", kind, info, flush=True)
+    assert info["converged"] and info["iterations"] >= 2
+    if kind == "lagrange":
+        xs, ys = np.linspace(0, 1, 33), np.linspace(0, 1, 25)
+        ref = (1 + xs[None, :] + 2 * ys[:, None]).ravel()
+        got = U.download()
+    else:
+        X, Y, Z = scheme.op.qp_coordinates()
+        ref = (1 + X + 2 * Y + 3 * Z).ravel()
+        got = scheme.op.eval_qp(U).download()
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print("  max rel error against u", err, flush=True)
+    assert err < 1e-9
