@@ -386,3 +386,35 @@ def test_gpu_galerkin_with_dirichlet_data_reproduces_a_polynomial_solution(ctx, 
     err = np.abs(got - ref).max() / np.abs(ref).max()
     print("  max rel error against u", err, flush=True)
     assert err < 1e-9
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("method", ["gmres", "bicgstab"])
+def test_gpu_galerkin_advection_reaction_with_a_nonsymmetric_solver(ctx, method):
+    # -div grad u + b . grad u + u^3 = f with upwind advection on DG Q2: a
+    # nonsymmetric Jacobian, Newton with GMRES or BiCGStab and Jacobi inside;
+    # the solution's residual, and S[u] against the oracle (advection with
+    # the reaction as cvol = u^2 at the points)
+    import dunefem_amd as fa
+    n, p, b = (6, 5, 4), 2, (1.0, 0.5, -0.25)
+    space = forms.Space(ctx, n, p)
+    form = forms.Form(kappa=1.0, b=b, reaction="u**3", source="10*sin(pi*x)*sin(pi*y)*sin(pi*z)")
+    scheme = fa.galerkin(form, space, solver=method, build=False,
+                         parameters={"fem.solver.nonlinear.tolerance": 1e-9,
+                                     "fem.solver.nonlinear.linear.tolerance": 1e-12,
+                                     "fem.solver.nonlinear.linear.errormeasure": "relative",
+                                     "fem.solver.nonlinear.linear.maxiterations": 5000,
+                                     "fem.solver.nonlinear.linear.preconditioning.method": "jacobi"})
+    u = np.random.default_rng(51).normal(size=scheme.op.N)
+    oop = ora.DGOperator(n, p, 10.0 * p * p, b, 0.0, kappa=1.0)
+    oop.set_reaction_qp(oop.eval_qp(u) ** 2)
+    S = scheme.op.vector()
+    scheme.op(scheme.op.vector(u), S)
+    assert rel(S.download(), oop.apply(u)) < 1e-12
+    U = scheme.op.vector()
+    info = scheme.solve(U)
+    R = scheme.op.vector()
+    scheme(U, R)
+    print("advection-reaction", method, info, np.sqrt(R.normSquaredDofs()), flush=True)
+    assert info["converged"] and 2 <= info["iterations"] <= 10
+    assert np.sqrt(R.normSquaredDofs()) < 1e-9
