@@ -509,6 +509,24 @@ class GalerkinOperator:
             return forms.qp_coordinates_2d(tuple(self.desc.n[:2]), m, gauss_points(m)[0])
         return forms.qp_coordinates(tuple(self.desc.n), m, gauss_points(m)[0])
 
+    def l2_error(self, u, g=0):
+        """||u_h - g||_L2 by the operator's volume quadrature (u_h at the
+        points from eval_qp, g an expression of x, y, z or a number): the
+        discrete counterpart of the reference's integrate(grid, (u_h - g)^2)
+        used in convergence studies"""
+        from . import forms
+        m = self.desc.order + 1
+        _, w = gauss_points(m)
+        X, Y, Z = self.qp_coordinates()
+        gq, _ = forms.evaluate(g, np.zeros_like(X), X, Y, Z)
+        uq = self.eval_qp(u).download().reshape(X.shape)
+        n = self.desc.n
+        if self.desc.kind == OP_CG_Q2_2D:
+            W = (w[None, :] * w[:, None]).ravel() / (n[0] * n[1])                       # [qy][qx]
+        else:
+            W = (w[None, None, :] * w[None, :, None] * w[:, None, None]).ravel() / (n[0] * n[1] * n[2])
+        return float(np.sqrt(np.sum(W[None, :] * (uq - gq) ** 2)))
+
     def rhs_expr(self, f, F=None):
         """load vector of a source f(x, y, z) (a sympy expression or string
         of x, y, z), evaluated at the volume quadrature points"""

@@ -418,3 +418,35 @@ def test_gpu_galerkin_advection_reaction_with_a_nonsymmetric_solver(ctx, method)
     print("advection-reaction", method, info, np.sqrt(R.normSquaredDofs()), flush=True)
     assert info["converged"] and 2 <= info["iterations"] <= 10
     assert np.sqrt(R.normSquaredDofs()) < 1e-9
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["dglegendre", "lagrange"])
+def test_gpu_newton_solutions_converge_at_the_expected_order(ctx, kind):
+    # what the mathematics fixes: for -lap u + u^3 = f with the smooth
+    # manufactured u = prod sin(pi x_d) (zero on the boundary), the L2 error
+    # of the Newton solution on Q2 spaces falls as h^3 (SIPG DG Q2 on n^3
+    # hexes, Lagrange Q2 on n^2 quads): halving h divides it by about 8
+    import dunefem_amd as fa
+    if kind == "dglegendre":
+        ue, d, sizes, pre = "sin(pi*x)*sin(pi*y)*sin(pi*z)", 3, (4, 8), "multigrid"
+    else:
+        ue, d, sizes, pre = "sin(pi*x)*sin(pi*y)", 2, (8, 16), "jacobi"
+    source = "%d*pi**2*(%s) + (%s)**3" % (d, ue, ue)
+    errs = []
+    for n in sizes:
+        space = forms.Space(ctx, (n,) * d, 2, kind=kind)
+        scheme = fa.galerkin(forms.Form(kappa=1.0, reaction="u**3", source=source), space, solver="cg",
+                             build=False,
+                             parameters={"fem.solver.nonlinear.tolerance": 1e-10,
+                                         "fem.solver.nonlinear.linear.tolerance": 1e-12,
+                                         "fem.solver.nonlinear.linear.errormeasure": "relative",
+                                         "fem.solver.nonlinear.linear.maxiterations": 20000,
+                                         "fem.solver.nonlinear.linear.preconditioning.method": pre})
+        U = scheme.op.vector()
+        info = scheme.solve(U)
+        assert info["converged"]
+        errs.append(scheme.op.l2_error(U, ue))
+    rate = np.log2(errs[0] / errs[1])
+    print(kind, "L2 errors", errs, "rate", rate, flush=True)
+    assert 2.7 < rate < 3.5 and errs[1] < 1e-3
