@@ -450,3 +450,34 @@ def test_gpu_newton_solutions_converge_at_the_expected_order(ctx, kind):
     rate = np.log2(errs[0] / errs[1])
     print(kind, "L2 errors", errs, "rate", rate, flush=True)
     assert 2.7 < rate < 3.5 and errs[1] < 1e-3
+
+
+@pytest.mark.gpu
+def test_gpu_sipg_q3_with_pointwise_kappa_and_advection_converges_at_order_four(ctx):
+    # the discretisation of the headline apply (Q3, kappa at the volume and
+    # face points, upwind advection): -div(kappa grad u) + b . grad u = f
+    # with kappa = 1 + x y z and the manufactured u = prod sin(pi x_d); GMRES
+    # + Jacobi; the L2 error falls as h^4 (p + 1)
+    import sympy as sp
+    import dunefem_amd as fa
+    b = (1.0, 0.5, 0.25)
+    X, Y, Z = forms.x, forms.y, forms.z
+    ue = sp.sin(sp.pi * X) * sp.sin(sp.pi * Y) * sp.sin(sp.pi * Z)
+    kap = 1 + X * Y * Z
+    f = -sum(sp.diff(kap * sp.diff(ue, v), v) for v in (X, Y, Z)) + sum(bb * sp.diff(ue, v) for bb, v in zip(b, (X, Y, Z)))
+    errs = []
+    for n in (3, 6):
+        space = forms.Space(ctx, (n, n, n), 3)
+        scheme = fa.galerkin(forms.Form(kappa=kap, b=b, source=f), space, solver="gmres",
+                             parameters={"fem.solver.nonlinear.tolerance": 1e-10,
+                                         "fem.solver.nonlinear.linear.tolerance": 1e-13,
+                                         "fem.solver.nonlinear.linear.errormeasure": "relative",
+                                         "fem.solver.nonlinear.linear.maxiterations": 20000,
+                                         "fem.solver.nonlinear.linear.preconditioning.method": "jacobi"})
+        U = scheme.op.vector()
+        info = scheme.solve(U)
+        assert info["converged"]
+        errs.append(scheme.op.l2_error(U, ue))
+    rate = np.log2(errs[0] / errs[1])
+    print("SIPG Q3 pointwise kappa + advection: L2 errors", errs, "rate", rate, flush=True)
+    assert 3.5 < rate < 4.8
