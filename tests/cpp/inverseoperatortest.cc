@@ -126,6 +126,14 @@ static void algorithm(Context &ctx, const std::string &designation, const Solver
            designation + " NewtonInvOp", dist);
 }
 
+// a parameter value in full precision (std::to_string prints six decimals)
+static std::string num(double v)
+{
+    char b[64];
+    std::snprintf(b, sizeof(b), "%.17g", v);
+    return b;
+}
+
 static fem_op_desc dg_desc(int n, int p, int precision, bool advection = true)
 {
     fem_op_desc d{};
@@ -335,7 +343,7 @@ static void newton_check(Context &ctx)
     report(!J.nonlinear(), "its Jacobian operator is linear", 0.0);
 
     NewtonParameter np;
-    np.set("fem.solver.nonlinear.tolerance", std::to_string(1e-10 * fnorm))
+    np.set("fem.solver.nonlinear.tolerance", num(1e-10 * fnorm))
         .set("fem.solver.nonlinear.linear.tolerance", "1e-12")
         .set("fem.solver.nonlinear.linear.errormeasure", "relative")
         .set("fem.solver.nonlinear.linear.preconditioning.method", "blockjacobi");
@@ -350,7 +358,7 @@ static void newton_check(Context &ctx)
     // each Jacobian (the reaction in its levels, updated in place per step)
     {
         NewtonParameter mp;
-        mp.set("fem.solver.nonlinear.tolerance", std::to_string(1e-10 * fnorm))
+        mp.set("fem.solver.nonlinear.tolerance", num(1e-10 * fnorm))
             .set("fem.solver.nonlinear.linear.tolerance", "1e-12")
             .set("fem.solver.nonlinear.linear.errormeasure", "relative")
             .set("fem.solver.nonlinear.linear.preconditioning.method", "multigrid");
@@ -375,6 +383,51 @@ static void newton_check(Context &ctx)
     report(error(v) < 1e-8, "FemScheme::solve returns u*", error(v));
 }
 
+// Newton on the continuous Q2 Lagrange operator (2D, strong u = 0): the
+// reaction law 20 u^2 u, a manufactured u* zero on the boundary, CG with the
+// Jacobi preconditioner the solver builds on each Jacobian
+static void newton_q2_check(Context &ctx)
+{
+    const int n = 24;
+    fem_op_desc d{};
+    d.kind = FEM_OP_CG_Q2_2D;
+    d.n[0] = d.n[1] = n;
+    d.n[2] = 1;
+    d.order = 2;
+    d.c = 0.5;
+    d.kappa_mode = FEM_KAPPA_CONST;
+    d.kappa_const = 1.0;
+    d.precision = FEM_F64;
+    GalerkinOperator S(ctx, d);
+    S.setReactionLaw(20.0, 2);
+    const int W = 2 * n + 1;
+    std::vector<double> us(S.size(), 0.0);
+    for (int j = 1; j < W - 1; ++j)
+        for (int i = 1; i < W - 1; ++i) us[(size_t)j * W + i] = std::sin(0.37 * (i + 3 * j)) + 0.5;
+    DiscreteFunction ustar = S.function("ustar"), F = S.function("F"), w = S.function("w");
+    ustar.upload(us);
+    S(ustar, F);
+    const double fnorm = std::sqrt(F.normSquaredDofs());
+    NewtonParameter np;
+    np.set("fem.solver.nonlinear.tolerance", num(1e-10 * fnorm))
+        .set("fem.solver.nonlinear.maxiterations", "30")
+        .set("fem.solver.nonlinear.linear.tolerance", "1e-12")
+        .set("fem.solver.nonlinear.linear.errormeasure", "relative")
+        .set("fem.solver.nonlinear.linear.preconditioning.method", "jacobi");
+    NewtonInverseOperator<CgInverseOperator> newton(ctx, np);
+    newton.bind(S);
+    newton(F, w);
+    const std::vector<double> h = w.download();
+    double e2 = 0, n2 = 0;
+    for (size_t i = 0; i < h.size(); ++i) {
+        e2 += (h[i] - us[i]) * (h[i] - us[i]);
+        n2 += us[i] * us[i];
+    }
+    report(newton.converged() && newton.iterations() >= 2 && newton.iterations() <= 15,
+           "Newton on the Q2 Lagrange operator (iterations)", (double)newton.iterations());
+    report(std::sqrt(e2 / n2) < 1e-8, "Newton on Q2 returns u*", std::sqrt(e2 / n2));
+}
+
 int main()
 {
     try {
@@ -390,6 +443,7 @@ int main()
         dg_checks(ctx);
         scheme_check(ctx);
         newton_check(ctx);
+        newton_q2_check(ctx);
     } catch (const Exception &e) {
         std::printf("FAIL unexpected exception %d: %s\n", e.code(), e.what());
         ++failures;
