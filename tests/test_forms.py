@@ -481,3 +481,27 @@ def test_gpu_sipg_q3_with_pointwise_kappa_and_advection_converges_at_order_four(
     rate = np.log2(errs[0] / errs[1])
     print("SIPG Q3 pointwise kappa + advection: L2 errors", errs, "rate", rate, flush=True)
     assert 3.5 < rate < 4.8
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("p,sizes,lo,hi", [(1, (4, 8), 1.8, 2.4), (4, (3, 6), 4.3, 5.8)])
+def test_gpu_sipg_converges_at_order_p_plus_one(ctx, p, sizes, lo, hi):
+    # the other orders' kernels (Q1, and Q4 of the C4 workload) against the
+    # h^(p+1) the theory gives: -2 lap u = f, u = prod sin(pi x_d), constant
+    # kappa 2, CG + block Jacobi
+    import dunefem_amd as fa
+    ue = "sin(pi*x)*sin(pi*y)*sin(pi*z)"
+    errs = []
+    for n in sizes:
+        space = forms.Space(ctx, (n, n, n), p)
+        scheme = fa.galerkin(forms.Form(kappa=2.0, source="6*pi**2*" + ue), space, solver="cg",
+                             parameters={"fem.solver.nonlinear.linear.tolerance": 1e-13,
+                                         "fem.solver.nonlinear.linear.errormeasure": "relative",
+                                         "fem.solver.nonlinear.linear.maxiterations": 20000,
+                                         "fem.solver.nonlinear.linear.preconditioning.method": "blockjacobi"})
+        U = scheme.op.vector()
+        assert scheme.solve(U)["converged"]
+        errs.append(scheme.op.l2_error(U, ue))
+    rate = np.log2(errs[0] / errs[1])
+    print("SIPG Q%d" % p, "L2 errors", errs, "rate", rate, flush=True)
+    assert lo < rate < hi
